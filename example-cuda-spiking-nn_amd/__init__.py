@@ -1,0 +1,19 @@
+"""MI355X-native per-step engine for the conductance-based EIF spiking network of
+paulazoo/example-cuda-spiking-nn.
+
+The product is the C++/HIP library ``lib/libburyn_amd.so`` with the C ABI of
+``include/buryn_amd.h``; this package is its Python view (ctypes).  There is no Python or CPU
+fallback: every call goes to the HIP library, and importing the binding raises if the library
+was not built.
+"""
+from .engine import (  # noqa: F401
+    BURYN_FLAGS,
+    LIB_PATH,
+    BnError,
+    Simulation,
+    build_library,
+    config_dump,
+    generate_weight_matrix,
+    lib,
+    poisson_table,
+)

@@ -1,0 +1,441 @@
+// Host engine: the reference's component API (same class names, methods and step order) on
+// top of the extern "C" launchers, plus the fused, graph-captured step.
+//
+// reference: cpp_src/gpu_core/gpu_simulation_state.h, cpp_src/neuron_groups/neuron_group.h,
+// cpp_src/connections/connection.h, cpp_src/stimuli/stimulus.h, cpp_src/plasticities/*.h,
+// cpp_src/monitors/*.h, cpp_src/core/system.h
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <functional>
+#include <fstream>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../bn_step.h"
+#include "buryn_amd.h"
+#include "generation.h"
+
+namespace bn {
+
+constexpr uint32_t kRecordingIntervalSteps = 10000;  // buryn_definitions.h:13
+
+// --------------------------------------------------------------------------------------
+// GpuSimulationState — owns every device buffer (gpu_simulation_state.cu:35-104)
+// --------------------------------------------------------------------------------------
+struct StateOptions {
+    bool reference_histories = true;   // [neuron][slot] spike/voltage histories (component path)
+    bool fused_rings = false;          // spike-bit ring + voltage ring of recorded neurons
+    std::vector<uint32_t> vrec_global_ids;
+    bool lazy_normalize = false;
+    uint32_t lazy_kmax = 256;
+    uint32_t recording_interval_steps = kRecordingIntervalSteps;
+    std::array<uint32_t, 4> gemv_row_block = {0, 0, 0, 0};  // 0 = heuristic
+};
+
+class GpuSimulationState {
+public:
+    GpuSimulationState(uint32_t total_simulation_length, uint32_t num_excitatory_neurons,
+                       uint32_t num_inhibitory_neurons, uint32_t num_stimulus_events,
+                       uint32_t num_plasticities, StateOptions opt = StateOptions());
+    ~GpuSimulationState();
+    GpuSimulationState(const GpuSimulationState&) = delete;
+    GpuSimulationState& operator=(const GpuSimulationState&) = delete;
+
+    uint64_t total_allocated_bytes() const { return allocated_; }
+    std::string memory_report() const;
+    hipStream_t stream() const { return stream_; }
+    const StateOptions& options() const { return opt_; }
+
+    float* d_membrane_potentials() const { return v_; }
+    float* d_g_excitatory() const { return ge_; }
+    float* d_g_inhibitory() const { return gi_; }
+    float* d_i_stimulus() const { return istim_; }
+    uint32_t* d_refractory_counter() const { return refr_; }
+    uint32_t* d_spikes() const { return spikes_; }
+    float* d_background_g_excitatory_synaptic_conductances() const { return bg_ge_; }
+    float* d_background_g_inhibitory_synaptic_conductances() const { return bg_gi_; }
+    float* d_background_excitatory_postsynaptic_r_traces() const { return bre_; }
+    float* d_background_excitatory_postsynaptic_d_traces() const { return bde_; }
+    float* d_background_inhibitory_postsynaptic_r_traces() const { return bri_; }
+    float* d_background_inhibitory_postsynaptic_d_traces() const { return bdi_; }
+    bn_device_clock* d_clock() const { return clock_; }
+    uint32_t* d_status() const { return status_; }
+
+    float* d_w(uint32_t idx) const { return w_[idx]; }
+    float* d_w_ee() const { return w_[0]; }
+    float* d_w_ei() const { return w_[1]; }
+    float* d_w_ie() const { return w_[2]; }
+    float* d_w_ii() const { return w_[3]; }
+    uint32_t weight_ld(uint32_t idx) const { return ld_[idx]; }
+    uint32_t weight_pre(uint32_t idx) const { return pre_[idx]; }
+    uint32_t weight_post(uint32_t idx) const { return post_[idx]; }
+    float* d_postsynaptic_r_traces(uint32_t idx) const { return tr_r_[idx]; }
+    float* d_postsynaptic_d_traces(uint32_t idx) const { return tr_d_[idx]; }
+    float* d_summed_postsynaptic_conductances(uint32_t idx) const { return tr_x_[idx]; }
+    double* d_gemv_partials(uint32_t idx) const { return partials_[idx]; }
+    uint32_t gemv_row_block(uint32_t idx) const { return row_block_[idx]; }
+
+    uint32_t* d_timestep_event_count() const { return stim_count_; }
+    uint32_t* d_stimulus_event_neuron_ids() const { return stim_ids_; }
+    uint32_t* d_stimulus_event_post_group_offsets() const { return stim_offs_; }
+    float* d_stimulus_event_amplitude_changes() const { return stim_amps_; }
+
+    float* d_pre_traces() const { return pre_tr_; }
+    float* d_post_traces() const { return post_tr_; }
+    float* d_triplet_pre_traces() const { return tpre_tr_; }
+    float* d_triplet_post_traces() const { return tpost_tr_; }
+    float* d_pre_weight_sums() const { return pre_wsum_; }
+
+    uint32_t* d_spike_history() const { return spike_hist_; }
+    float* d_membrane_potential_history() const { return v_hist_; }
+    uint32_t* d_spike_bits() const { return spike_bits_; }
+    uint32_t* d_spike_bits_ring() const { return spike_ring_; }
+    float* d_v_ring() const { return v_ring_; }
+    const int32_t* d_vrec_index() const { return vrec_index_; }
+    uint32_t spike_words() const { return words_; }
+    uint32_t n_vrec() const { return (uint32_t)opt_.vrec_global_ids.size(); }
+
+    // lazy normalisation of the triplet matrix (plasticity_offset 0)
+    float* d_row_scale() const { return row_scale_; }
+    double* d_row_sum() const { return row_sum_; }
+    double* d_ltp_deltas() const { return ltp_delta_; }
+
+    // Poisson tables per group (0 E, 1 I) and kind (0 exc, 1 inh)
+    const uint32_t* d_poisson_table(uint32_t group, uint32_t kind) const { return poisson_[group][kind]; }
+    uint32_t poisson_table_len(uint32_t group, uint32_t kind) const { return poisson_len_[group][kind]; }
+    void set_poisson_table(uint32_t group, uint32_t kind, float mean);
+
+    void upload_neuron_group_view(const NeuronGroupView& v, const struct NeuronGroupInitHostData& h);
+    void upload_connection_view(const ConnectionView& v, const struct ConnectionInitHostData& h);
+    void upload_stimulus_view(const StimulusView& v, const struct StimulusInitHostData& h);
+    void upload_plasticity_view(const PlasticityView& v, const struct PlasticityInitHostData& h);
+
+    std::vector<uint32_t> download_and_flush_spike_history(uint32_t offset, uint32_t num_neurons, uint32_t num_timesteps);
+    std::vector<float> download_and_flush_membrane_potential_history(uint32_t offset, uint32_t num_neurons,
+                                                                     uint32_t num_timesteps);
+    // Fused-path rings: [interval][words] spike bits (downloaded once per record step and
+    // zeroed), and [interval][n_vrec] voltages.
+    const std::vector<uint32_t>& spike_ring_snapshot(uint32_t clock_step);
+    std::vector<float> download_and_flush_v_ring();
+    // Dense [pre][post] true weights (lazy normalisation: fl32(s_i * M_ij)).
+    std::vector<float> download_weight_matrix(uint32_t weight_matrix_idx) const;
+    std::string lookup_weight_matrix_name(uint32_t weight_matrix_idx) const;
+    void setup_lazy_normalization(uint32_t weight_matrix_idx);  // after W_EE upload
+    bool lazy_active() const { return lazy_active_; }
+
+    uint32_t total_simulation_length() const { return total_simulation_length_; }
+    uint32_t num_excitatory_neurons() const { return ne_; }
+    uint32_t num_inhibitory_neurons() const { return ni_; }
+    uint32_t total_neurons() const { return n_; }
+    uint32_t num_stimulus_events() const { return num_stimulus_events_; }
+    uint32_t num_plasticities() const { return num_plasticities_; }
+    void check_status();  // throws if a kernel flagged an error
+
+private:
+    template <typename T>
+    T* alloc(size_t n, const char* what);
+    void require_upload_bounds(const std::string& field, uint32_t offset, uint32_t count) const;
+
+    StateOptions opt_;
+    uint32_t total_simulation_length_, ne_, ni_, n_, num_stimulus_events_, num_plasticities_;
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocations_;
+    uint64_t allocated_ = 0;
+
+    float *v_, *ge_, *gi_, *istim_, *bg_ge_, *bg_gi_, *bre_, *bde_, *bri_, *bdi_;
+    uint32_t *refr_, *spikes_;
+    bn_device_clock* clock_;
+    uint32_t* status_;
+    float* w_[4];
+    uint32_t ld_[4], pre_[4], post_[4], row_block_[4];
+    float *tr_r_[4], *tr_d_[4], *tr_x_[4];
+    double* partials_[4];
+    uint32_t *stim_count_ = nullptr, *stim_ids_ = nullptr, *stim_offs_ = nullptr;
+    float* stim_amps_ = nullptr;
+    float *pre_tr_, *post_tr_, *tpre_tr_, *tpost_tr_, *pre_wsum_;
+    uint32_t* spike_hist_ = nullptr;
+    float* v_hist_ = nullptr;
+    uint32_t* spike_bits_ = nullptr;
+    uint32_t* spike_ring_ = nullptr;
+    float* v_ring_ = nullptr;
+    int32_t* vrec_index_ = nullptr;
+    uint32_t words_ = 0;
+    float* row_scale_ = nullptr;
+    double* row_sum_ = nullptr;
+    double* ltp_delta_ = nullptr;
+    bool lazy_active_ = false;
+    uint32_t lazy_matrix_ = 0;
+    uint32_t* poisson_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    uint32_t poisson_len_[2][2] = {{0, 0}, {0, 0}};
+    std::vector<uint32_t> ring_snapshot_;
+    int64_t ring_snapshot_step_ = -1;
+};
+
+// --------------------------------------------------------------------------------------
+// Components
+// --------------------------------------------------------------------------------------
+struct NeuronGroupParams {  // neuron_group_params.h:7-33
+    float timestep = 1e-4f, g_leak = 10.0f, membrane_capacitance = 0.1f, depolarization_slope_factor = 2.0f;
+    float v_effective_spike_threshold = -50.0f, v_excitatory_reversal = 0.0f, v_inhibitory_reversal = -80.0f;
+    float v_threshold = -45.0f, v_reset = -55.0f, v_rest = -65.0f;
+    uint32_t refractory_steps = 0;
+    bool is_inhibitory = false;
+    float mean_background_excitatory_spikes = 0.0f, mean_background_inhibitory_spikes = 0.0f;
+    float background_excitatory_postsynaptic_r_trace_decay = 0.0f, background_excitatory_postsynaptic_d_trace_decay = 0.0f;
+    float background_inhibitory_postsynaptic_r_trace_decay = 0.0f, background_inhibitory_postsynaptic_d_trace_decay = 0.0f;
+    float background_excitatory_scaling_factor = 0.0f, background_inhibitory_scaling_factor = 0.0f;
+    uint32_t offset = 0, num_neurons = 0;
+};
+
+struct NeuronGroupInitHostData {
+    std::vector<float> membrane_potentials, g_excitatory, g_inhibitory, i_stimulus;
+    std::vector<uint32_t> refractory_counter, spikes;
+    std::vector<float> background_g_excitatory_synaptic_conductances, background_g_inhibitory_synaptic_conductances;
+    std::vector<float> background_excitatory_postsynaptic_r_traces, background_excitatory_postsynaptic_d_traces;
+    std::vector<float> background_inhibitory_postsynaptic_r_traces, background_inhibitory_postsynaptic_d_traces;
+};
+
+class NeuronGroup {
+public:
+    NeuronGroup(std::vector<Location> neuron_locations, NeuronGroupParams params, GpuSimulationState& gpu_state);
+    NeuronGroupInitHostData initialize_data();
+    void make_gpu_view();
+    void upload_data(const NeuronGroupInitHostData& h);
+    void evolve();
+    uint32_t offset() const { return params_.offset; }
+    uint32_t num_neurons() const { return params_.num_neurons; }
+    bool is_inhibitory() const { return params_.is_inhibitory; }
+    const NeuronGroupView& gpu_view() const { return gpu_view_; }
+    const std::vector<Location>& neuron_locations() const { return neuron_locations_; }
+
+private:
+    GpuSimulationState& gpu_state_;
+    NeuronGroupParams params_;
+    std::vector<Location> neuron_locations_;
+    NeuronGroupView gpu_view_{};
+};
+
+struct ConnectionParams {  // connection_params.h:7-15
+    float timestep = 0.0f, decay_time = 0.0f, rise_time = 0.0f, postsynaptic_kernel_normalization_value = 0.0f;
+    uint32_t weight_matrix_idx = 0;
+};
+
+struct ConnectionInitHostData {
+    std::vector<float> weight_matrix;  // dense [pre][post]
+    std::vector<float> postsynaptic_r_traces, postsynaptic_d_traces, summed_postsynaptic_conductances;
+};
+
+class Connection {
+public:
+    // weight_matrix: dense row-major [pre][post]
+    Connection(NeuronGroup& pre_group, NeuronGroup& post_group, const std::vector<float>& weight_matrix,
+               ConnectionParams params, GpuSimulationState& gpu_state);
+    void make_gpu_view();
+    void propagate();
+    uint32_t weight_matrix_idx() const { return params_.weight_matrix_idx; }
+    uint32_t pre_group_offset() const { return pre_group_.offset(); }
+    uint32_t post_group_offset() const { return post_group_.offset(); }
+    uint32_t pre_group_num_neurons() const { return pre_group_.num_neurons(); }
+    uint32_t post_group_num_neurons() const { return post_group_.num_neurons(); }
+    const ConnectionView& gpu_view() const { return gpu_view_; }
+    const NeuronGroup& pre_group() const { return pre_group_; }
+    const NeuronGroup& post_group() const { return post_group_; }
+
+private:
+    NeuronGroup& pre_group_;
+    NeuronGroup& post_group_;
+    ConnectionParams params_;
+    ConnectionView gpu_view_{};
+    GpuSimulationState& gpu_state_;
+};
+
+struct StimulusInitHostData {
+    std::vector<uint32_t> timestep_event_count, stimulus_event_neuron_ids, stimulus_event_post_group_offsets;
+    std::vector<float> stimulus_event_amplitude_changes;
+};
+
+class Stimulus {
+public:
+    Stimulus(std::vector<uint32_t> timestep_event_count, std::vector<uint32_t> stimulus_event_neuron_ids,
+             std::vector<uint32_t> stimulus_event_post_group_offsets, std::vector<float> stimulus_event_amplitude_changes,
+             GpuSimulationState& gpu_state);
+    void apply();  // host cursor, as stimulus.cpp:46-60
+    const StimulusView& gpu_view() const { return gpu_view_; }
+
+private:
+    std::vector<uint32_t> timestep_event_count_;
+    uint32_t next_event_idx_ = 0, current_timestep_ = 0, total_stimulus_events_ = 0;
+    StimulusView gpu_view_{};
+    GpuSimulationState& gpu_state_;
+};
+
+struct PlasticityInitHostData {
+    std::vector<float> pre_traces, post_traces, triplet_pre_traces, triplet_post_traces, pre_weight_sums;
+};
+
+class Plasticity {
+public:
+    Plasticity(Connection& connection, PlasticityParams params, GpuSimulationState& gpu_state, uint32_t kind);
+    virtual ~Plasticity() = default;
+    virtual void update() = 0;
+    uint32_t kind() const { return kind_; }
+    virtual bool normalizes() const { return false; }
+    const PlasticityView& gpu_view() const { return gpu_view_; }
+    const PlasticityParams& params() const { return params_; }
+    const Connection& connection() const { return connection_; }
+
+protected:
+    void make_gpu_view();
+    Connection& connection_;
+    PlasticityParams params_;
+    PlasticityView gpu_view_{};
+    GpuSimulationState& gpu_state_;
+    uint32_t kind_;
+};
+
+class TripletPlasticity : public Plasticity {  // triplet_plasticity.cpp
+public:
+    TripletPlasticity(Connection& connection, PlasticityParams params, GpuSimulationState& gpu_state);
+    void update() override;
+    bool normalizes() const override { return true; }
+};
+
+class IstdpPlasticity : public Plasticity {  // istdp_plasticity.cpp
+public:
+    IstdpPlasticity(Connection& connection, PlasticityParams params, GpuSimulationState& gpu_state);
+    void update() override;
+};
+
+class StdpPlasticity : public Plasticity {  // stdp_plasticity.cpp
+public:
+    StdpPlasticity(Connection& connection, PlasticityParams params, GpuSimulationState& gpu_state);
+    void update() override;
+};
+
+// --------------------------------------------------------------------------------------
+// Monitors (monitor.h, spike_recorder.h, voltage_recorder.h, weight_recorder.h)
+// --------------------------------------------------------------------------------------
+class Monitor {
+public:
+    explicit Monitor(GpuSimulationState& s) : gpu_state_(s) {}
+    virtual ~Monitor() = default;
+    virtual void historicize(uint32_t clock_step) = 0;
+    virtual void record(uint32_t clock_step) = 0;
+    bool fused = false;  // rings are written by the fused step
+    bool quiet = false;
+
+protected:
+    GpuSimulationState& gpu_state_;
+};
+
+class SpikeRecorder : public Monitor {
+public:
+    SpikeRecorder(const NeuronGroup& group, const std::string& output_filepath, uint32_t recording_interval_steps,
+                  GpuSimulationState& gpu_state);
+    void historicize(uint32_t clock_step) override;
+    void record(uint32_t clock_step) override;
+    bool good() const { return output_.good(); }
+
+private:
+    const NeuronGroup& group_;
+    std::ofstream output_;
+    uint32_t recording_interval_steps_;
+    SpikeRecorderView gpu_view_{};
+};
+
+class VoltageRecorder : public Monitor {
+public:
+    VoltageRecorder(const NeuronGroup& group, const std::string& output_filepath, std::vector<uint32_t> selected_neurons,
+                    uint32_t recording_interval_steps, GpuSimulationState& gpu_state);
+    void historicize(uint32_t clock_step) override;
+    void record(uint32_t clock_step) override;
+    const std::vector<uint32_t>& selected_neurons() const { return selected_neurons_; }
+
+private:
+    const NeuronGroup& group_;
+    std::string output_filepath_;
+    std::vector<uint32_t> selected_neurons_;
+    uint32_t recording_interval_steps_;
+    VoltageRecorderView gpu_view_{};
+};
+
+class WeightRecorder : public Monitor {
+public:
+    WeightRecorder(const Connection& connection, const std::string& output_filepath, GpuSimulationState& gpu_state);
+    void historicize(uint32_t) override {}
+    void record(uint32_t clock_step) override;
+
+private:
+    const Connection& connection_;
+    std::string output_filepath_;
+};
+
+// --------------------------------------------------------------------------------------
+// System (system.h) + the fused step
+// --------------------------------------------------------------------------------------
+class System {
+public:
+    explicit System(GpuSimulationState& s) : state_(s) {}
+    ~System();
+    void add_neuron_group(std::unique_ptr<NeuronGroup> g);
+    void add_connection(std::unique_ptr<Connection> c);
+    void add_stimulus(std::unique_ptr<Stimulus> s);
+    void add_plasticity(std::unique_ptr<Plasticity> p);
+    void add_monitor(std::unique_ptr<Monitor> m);
+    void add_step_callback(std::function<void(uint32_t)> cb);
+
+    // Component path: the reference's order, one launcher per component (system.cpp:43-63).
+    void step_the_clock(uint32_t clock_step);
+    // Reference run loop (system.cpp:65-83); uses the fused step when enabled.
+    void run(uint32_t total_clock_steps);
+
+    // Fused step: build the plans, optionally capture `steps_per_graph` steps in a hipGraph.
+    void enable_fused(bool use_graph, uint32_t steps_per_graph);
+    bool fused() const { return fused_; }
+    void launch_fused_steps(uint32_t n);           // enqueue n steps (graph or eager)
+    void launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end);
+    uint32_t kernels_per_step() const { return kernels_per_step_; }
+    // Conductances after the last propagate, for state downloads (fused path keeps them as
+    // base + pending GEMV partials).
+    void materialize_g(float* d_ge_out, float* d_gi_out);
+    // Trace set used by the fused step for connection idx (EE/EI share one, IE/II another).
+    const TraceSet* trace_set_for_connection(uint32_t idx) const;
+
+    uint32_t clock_step() const { return clock_step_; }
+    bool quiet = false;
+    const std::vector<std::unique_ptr<NeuronGroup>>& neuron_groups() const { return neuron_groups_; }
+    const std::vector<std::unique_ptr<Connection>>& connections() const { return connections_; }
+    const std::vector<std::unique_ptr<Plasticity>>& plasticities() const { return plasticities_; }
+    const std::vector<std::unique_ptr<Monitor>>& monitors() const { return monitors_; }
+
+private:
+    void build_plans();
+    void enqueue_fused_step(hipEvent_t gemv_begin, hipEvent_t gemv_end);
+
+    GpuSimulationState& state_;
+    std::vector<std::unique_ptr<NeuronGroup>> neuron_groups_;
+    std::vector<std::unique_ptr<Connection>> connections_;
+    std::vector<std::unique_ptr<Stimulus>> stimuli_;
+    std::vector<std::unique_ptr<Plasticity>> plasticities_;
+    std::vector<std::unique_ptr<Monitor>> monitors_;
+    std::vector<std::function<void(uint32_t)>> step_callbacks_;
+    uint32_t clock_step_ = 0;
+
+    bool fused_ = false, use_graph_ = false;
+    uint32_t steps_per_graph_ = 1;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    EvolvePlan evolve_{};
+    GemvPlan gemv_{};
+    PlastPlan plast_{};
+    TailPlan tail_{};
+    std::vector<int> conn_trace_set_;
+    uint32_t kernels_per_step_ = 0;
+};
+
+}  // namespace bn

@@ -1,0 +1,310 @@
+// System: the reference's step orchestrator (cpp_src/core/system.cpp:43-83) plus the fused,
+// graph-captured step of the whole network.
+#include <chrono>
+#include <cstring>
+#include <iostream>
+
+#include "../bn_errors.h"
+#include "engine.h"
+
+namespace bn {
+
+System::~System() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+}
+
+void System::add_neuron_group(std::unique_ptr<NeuronGroup> g) {
+    if (g) neuron_groups_.push_back(std::move(g));
+}
+void System::add_connection(std::unique_ptr<Connection> c) {
+    if (c) connections_.push_back(std::move(c));
+}
+void System::add_stimulus(std::unique_ptr<Stimulus> s) {
+    if (s) stimuli_.push_back(std::move(s));
+}
+void System::add_plasticity(std::unique_ptr<Plasticity> p) {
+    if (p) plasticities_.push_back(std::move(p));
+}
+void System::add_monitor(std::unique_ptr<Monitor> m) {
+    if (m) monitors_.push_back(std::move(m));
+}
+void System::add_step_callback(std::function<void(uint32_t)> cb) {
+    if (cb) step_callbacks_.push_back(std::move(cb));
+}
+
+// system.cpp:43-63, then the device clock advances (counter-based RNG).
+void System::step_the_clock(uint32_t clock_step) {
+    if (fused_) throw Error(BN_ERR_RUNTIME, "step_the_clock: the fused step is enabled for this system");
+    for (auto& g : neuron_groups_) g->evolve();
+    for (auto& c : connections_) c->propagate();
+    for (auto& s : stimuli_) s->apply();
+    for (auto& p : plasticities_) p->update();
+    for (auto& m : monitors_) m->historicize(clock_step);
+    BN_CALL(bn_clock_advance(state_.d_clock(), state_.stream()));
+    clock_step_ = clock_step + 1;
+}
+
+// system.cpp:65-83
+void System::run(uint32_t total_clock_steps) {
+    using clock = std::chrono::steady_clock;
+    const auto start = clock::now();
+    const uint32_t interval = state_.options().recording_interval_steps;
+    uint32_t step = clock_step_;
+    const uint32_t end = clock_step_ + total_clock_steps;
+    while (step < end) {
+        uint32_t n = 1;
+        if (fused_ && step_callbacks_.empty()) {
+            // Run up to and including the next record step in one batch.
+            const uint32_t next_record = (step % interval == 0) ? step : (step / interval + 1) * interval;
+            n = std::min(end, next_record + 1) - step;
+        }
+        for (uint32_t k = 0; k < n; ++k)
+            for (auto& cb : step_callbacks_) cb(step + k);
+        if (fused_) launch_fused_steps(n);
+        else step_the_clock(step);
+        step += n;
+        clock_step_ = step;
+        const uint32_t last = step - 1;
+        if (last % interval == 0) {
+            BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "step sync");
+            state_.check_status();
+            for (auto& m : monitors_) m->record(last);
+            const std::chrono::duration<float> elapsed = clock::now() - start;
+            if (!quiet) std::cout << "Completed clock step: " << last << " | elapsed: " << elapsed.count() << "s\n";
+        }
+    }
+}
+
+const TraceSet* System::trace_set_for_connection(uint32_t idx) const {
+    if (!fused_ || idx >= conn_trace_set_.size()) return nullptr;
+    return &evolve_.ts[conn_trace_set_[idx]];
+}
+
+void System::build_plans() {
+    GpuSimulationState& S = state_;
+    if (neuron_groups_.size() > (size_t)kMaxGroups) throw Error(BN_ERR_INVALID_ARGUMENT, "too many neuron groups");
+    if (connections_.size() > (size_t)kMaxConns) throw Error(BN_ERR_INVALID_ARGUMENT, "too many connections");
+    if (plasticities_.size() > (size_t)kMaxPlast) throw Error(BN_ERR_INVALID_ARGUMENT, "too many plasticities");
+    if (stimuli_.size() > 1) throw Error(BN_ERR_INVALID_ARGUMENT, "the fused step supports one stimulus");
+    evolve_ = EvolvePlan{};
+    gemv_ = GemvPlan{};
+    plast_ = PlastPlan{};
+    tail_ = TailPlan{};
+    conn_trace_set_.assign(connections_.size(), -1);
+
+    // Groups and the GEMV partial sources that finalise their conductances.
+    for (size_t g = 0; g < neuron_groups_.size(); ++g) evolve_.g[g].v = neuron_groups_[g]->gpu_view();
+    evolve_.n_groups = (uint32_t)neuron_groups_.size();
+    evolve_.total_neurons = S.total_neurons();
+    for (size_t c = 0; c < connections_.size(); ++c) {
+        const ConnectionView& v = connections_[c]->gpu_view();
+        uint32_t pg = 0;
+        while (pg < evolve_.n_groups && evolve_.g[pg].v.offset != v.post_group_offset) ++pg;
+        if (pg == evolve_.n_groups) throw Error(BN_ERR_INVALID_ARGUMENT, "connection post group not found");
+        EvolveGroup& G = evolve_.g[pg];
+        const FinalizeSource src{v.gemv_partials, (v.pre_group_num_neurons + v.gemv_row_block - 1) / v.gemv_row_block,
+                                 v.weight_matrix_ld};
+        if (v.pre_group_is_inhibtory) {
+            if (G.n_fin_i >= (uint32_t)kMaxSources) throw Error(BN_ERR_INVALID_ARGUMENT, "too many inhibitory sources");
+            G.fin_i[G.n_fin_i++] = src;
+        } else {
+            if (G.n_fin_e >= (uint32_t)kMaxSources) throw Error(BN_ERR_INVALID_ARGUMENT, "too many excitatory sources");
+            G.fin_e[G.n_fin_e++] = src;
+        }
+        // Connections with the same pre group and trace constants share one trace set.
+        int ts = -1;
+        for (uint32_t t = 0; t < evolve_.n_ts; ++t) {
+            const TraceSet& T = evolve_.ts[t];
+            if (T.pre_offset == v.pre_group_offset && T.n_pre == v.pre_group_num_neurons &&
+                T.decay_r == v.postsynaptic_r_trace_decay && T.decay_d == v.postsynaptic_d_trace_decay &&
+                T.K == v.postsynaptic_kernel_normalization_value)
+                ts = (int)t;
+        }
+        if (ts < 0) {
+            if (evolve_.n_ts >= (uint32_t)kMaxTraceSets) throw Error(BN_ERR_INVALID_ARGUMENT, "too many trace sets");
+            ts = (int)evolve_.n_ts++;
+            evolve_.ts[ts] = TraceSet{v.pre_group_offset,         v.pre_group_num_neurons,
+                                      v.postsynaptic_r_trace_decay, v.postsynaptic_d_trace_decay,
+                                      v.postsynaptic_kernel_normalization_value,
+                                      v.postsynaptic_r_traces,      v.postsynaptic_d_traces,
+                                      v.summed_postsynaptic_conductances};
+        }
+        conn_trace_set_[c] = ts;
+    }
+    // Plasticity trace decays and the plasticity plan.
+    int lazy_idx = -1;
+    for (size_t k = 0; k < plasticities_.size(); ++k) {
+        const Plasticity& P = *plasticities_[k];
+        const PlasticityView& v = P.gpu_view();
+        const PlasticityParams& pp = P.params();
+        PlastTraces& Q = evolve_.pt[k];
+        Q.kind = P.kind();
+        Q.pre_offset = v.pre_group_offset;
+        Q.n_pre = v.pre_group_num_neurons;
+        Q.post_offset = v.post_group_offset;
+        Q.n_post = v.post_group_num_neurons;
+        Q.pre_decay = pp.pre_trace_decay;
+        Q.post_decay = pp.post_trace_decay;
+        Q.trip_pre_decay = pp.triplet_pre_trace_decay;
+        Q.trip_post_decay = pp.triplet_post_trace_decay;
+        Q.pre = v.pre_traces;
+        Q.post = v.post_traces;
+        Q.trip_pre = v.triplet_pre_traces;
+        Q.trip_post = v.triplet_post_traces;
+        PlastDev& D = plast_.p[k];
+        D.kind = P.kind();
+        D.normalize = P.normalizes() ? 1u : 0u;
+        D.p = pp;
+        D.W = v.weight_matrix;
+        D.ld = v.weight_matrix_ld;
+        D.n_pre = v.pre_group_num_neurons;
+        D.n_post = v.post_group_num_neurons;
+        D.pre_offset = v.pre_group_offset;
+        D.post_offset = v.post_group_offset;
+        D.pre_spikes = v.pre_spikes;
+        D.post_spikes = v.post_spikes;
+        D.pre = v.pre_traces;
+        D.post = v.post_traces;
+        D.trip_pre = v.triplet_pre_traces;
+        D.trip_post = v.triplet_post_traces;
+        D.target = v.pre_weight_sums;
+        D.bits = S.d_spike_bits();
+        if (D.normalize && D.n_post > 64u * kNormThreads)
+            throw Error(BN_ERR_INVALID_ARGUMENT, "post group larger than the normalisation kernel supports");
+        if (D.normalize && S.options().lazy_normalize && lazy_idx < 0) {
+            if (v.weight_matrix_idx != 0 || D.n_pre != S.num_excitatory_neurons())
+                throw Error(BN_ERR_INVALID_ARGUMENT, "lazy normalisation supports the E->E matrix");
+            D.lazy = 1;
+            D.kmax = S.options().lazy_kmax;
+            D.s = S.d_row_scale();
+            D.R = S.d_row_sum();
+            D.delta = S.d_ltp_deltas();
+            lazy_idx = (int)k;
+            S.setup_lazy_normalization(v.weight_matrix_idx);
+        }
+        plast_.items_post += D.n_post;
+        plast_.items_pre += D.n_pre;
+    }
+    evolve_.n_pt = plast_.n = (uint32_t)plasticities_.size();
+    evolve_.clock = S.d_clock();
+    evolve_.spike_bits = S.d_spike_bits();
+    evolve_.spike_bits_ring = S.d_spike_bits_ring();
+    evolve_.v_ring = S.d_v_ring();
+    evolve_.vrec_index = S.d_vrec_index();
+    evolve_.n_vrec = S.n_vrec();
+    evolve_.interval = S.options().recording_interval_steps;
+    evolve_.words = S.spike_words();
+
+    // GEMV over every connection, one wave per (row chunk, 256 columns) tile.
+    uint32_t tiles = 0;
+    for (size_t c = 0; c < connections_.size(); ++c) {
+        const ConnectionView& v = connections_[c]->gpu_view();
+        GemvConn& C = gemv_.c[c];
+        C.W = v.weight_matrix;
+        C.x = evolve_.ts[conn_trace_set_[c]].x;
+        C.row_scale = nullptr;
+        if (lazy_idx >= 0 && plast_.p[lazy_idx].W == v.weight_matrix) C.row_scale = S.d_row_scale();
+        C.partials = v.gemv_partials;
+        C.ld = v.weight_matrix_ld;
+        C.n_pre = v.pre_group_num_neurons;
+        C.n_post = v.post_group_num_neurons;
+        C.row_block = v.gemv_row_block;
+        C.n_chunks = (C.n_pre + C.row_block - 1) / C.row_block;
+        C.n_coltiles = (C.n_post + kGemvTileCols - 1) / kGemvTileCols;
+        C.tile_begin = tiles;
+        tiles += C.n_chunks * C.n_coltiles;
+        C.tile_end = tiles;
+    }
+    gemv_.n_conn = (uint32_t)connections_.size();
+    gemv_.total_tiles = tiles;
+
+    // Tail: faithful normalisation rows, trace increments, stimulus, clock.
+    tail_.pl = plast_;
+    tail_.norm_rows = 0;
+    for (uint32_t k = 0; k < tail_.pl.n; ++k) {
+        PlastDev& D = tail_.pl.p[k];
+        if (D.lazy) D.normalize = 0;
+        if (D.normalize) tail_.norm_rows += D.n_pre;
+    }
+    tail_.total_neurons = S.total_neurons();
+    tail_.increments = 1;
+    if (!stimuli_.empty()) {
+        const StimulusView& sv = stimuli_[0]->gpu_view();
+        tail_.i_stimulus = sv.i_stimulus;
+        tail_.timestep_event_count = sv.timestep_event_count;
+        tail_.event_ids = sv.stimulus_event_neuron_ids;
+        tail_.event_offsets = sv.stimulus_event_post_group_offsets;
+        tail_.event_amplitudes = sv.stimulus_event_amplitude_changes;
+        tail_.num_timesteps = sv.num_timesteps;
+        tail_.num_events = sv.num_events;
+    }
+    tail_.clock = S.d_clock();
+    tail_.status = S.d_status();
+    tail_.advance_clock = 1;
+
+    kernels_per_step_ = 1 + (gemv_.total_tiles ? 1 : 0) + (plast_.n ? 2 : 0) + (lazy_idx >= 0 ? 1 : 0) + 1;
+    if (lazy_idx >= 0) BN_HIP_CHECK((hipError_t)launch_lazy_init(plast_.p[lazy_idx], S.stream()), "lazy normalisation init");
+}
+
+void System::enqueue_fused_step(hipEvent_t gemv_begin, hipEvent_t gemv_end) {
+    hipStream_t s = state_.stream();
+    BN_HIP_CHECK((hipError_t)launch_evolve(evolve_, s), "fused evolve launch");
+    if (gemv_begin) BN_HIP_CHECK(hipEventRecord(gemv_begin, s), "event record");
+    BN_HIP_CHECK((hipError_t)launch_gemv(gemv_, s), "fused gemv launch");
+    if (gemv_end) BN_HIP_CHECK(hipEventRecord(gemv_end, s), "event record");
+    if (plast_.n) {
+        BN_HIP_CHECK((hipError_t)launch_ltp(plast_, s), "fused ltp launch");
+        BN_HIP_CHECK((hipError_t)launch_ltd(plast_, s), "fused ltd launch");
+        BN_HIP_CHECK((hipError_t)launch_lazy_rows(plast_, s), "fused lazy rows launch");
+    }
+    BN_HIP_CHECK((hipError_t)launch_norm_tail(tail_, s), "fused tail launch");
+}
+
+void System::enable_fused(bool use_graph, uint32_t steps_per_graph) {
+    for (auto& cb : step_callbacks_) (void)cb;
+    build_plans();
+    fused_ = true;
+    for (auto& m : monitors_) m->fused = true;
+    use_graph_ = use_graph;
+    steps_per_graph_ = std::max<uint32_t>(1, steps_per_graph);
+    if (use_graph_) {
+        hipStream_t s = state_.stream();
+        BN_HIP_CHECK(hipStreamSynchronize(s), "pre-capture sync");
+        BN_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        for (uint32_t k = 0; k < steps_per_graph_; ++k) enqueue_fused_step(nullptr, nullptr);
+        BN_HIP_CHECK(hipStreamEndCapture(s, &graph_), "hipStreamEndCapture");
+        BN_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+}
+
+void System::launch_fused_steps(uint32_t n) {
+    hipStream_t s = state_.stream();
+    uint32_t k = 0;
+    if (use_graph_)
+        for (; k + steps_per_graph_ <= n; k += steps_per_graph_)
+            BN_HIP_CHECK(hipGraphLaunch(graph_exec_, s), "hipGraphLaunch");
+    for (; k < n; ++k) enqueue_fused_step(nullptr, nullptr);
+    clock_step_ += n;
+}
+
+void System::launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end) {
+    enqueue_fused_step(gemv_begin, gemv_end);
+    clock_step_ += 1;
+}
+
+void System::materialize_g(float* d_ge_out, float* d_gi_out) {
+    hipStream_t s = state_.stream();
+    for (uint32_t g = 0; g < evolve_.n_groups || (!fused_ && g < neuron_groups_.size()); ++g) {
+        const NeuronGroupView& v = fused_ ? evolve_.g[g].v : neuron_groups_[g]->gpu_view();
+        const uint32_t ne = fused_ ? evolve_.g[g].n_fin_e : 0, ni = fused_ ? evolve_.g[g].n_fin_i : 0;
+        BN_HIP_CHECK((hipError_t)launch_materialize_g(d_ge_out + v.offset, v.g_excitatory,
+                                                      fused_ ? evolve_.g[g].fin_e : nullptr, ne, v.num_neurons, s),
+                     "materialize g_e");
+        BN_HIP_CHECK((hipError_t)launch_materialize_g(d_gi_out + v.offset, v.g_inhibitory,
+                                                      fused_ ? evolve_.g[g].fin_i : nullptr, ni, v.num_neurons, s),
+                     "materialize g_i");
+    }
+}
+
+}  // namespace bn

@@ -1,0 +1,912 @@
+// Hand-written HIP kernels of the per-step hot path for CDNA4 (gfx950), and the extern "C"
+// launcher layer of include/buryn_amd.h.
+//
+// Reference semantics followed (paths relative to the reference repository):
+//   background + EIF      cpp_src/neuron_groups/kernel_neuron_group_evolve.cu:11-130
+//   connection traces     cpp_src/connections/kernel_connection_propagate.cu:8-46
+//   GEMV (cublasSgemv)    cpp_src/connections/kernel_connection_propagate.cu:79-94
+//   stimulus              cpp_src/stimuli/kernel_stimulus_apply.cu:7-21, stimulus.cpp:46-60
+//   triplet STDP          cpp_src/plasticities/kernel_triplet_plasticity_update.cu:16-216
+//   iSTDP                 cpp_src/plasticities/kernel_istdp_plasticity_update.cu:16-135
+//   pair STDP             cpp_src/plasticities/kernel_stdp_plasticity_update.cu:16-137
+//   monitors              cpp_src/monitors/kernel_historicize_{spikes,membrane_potentials}.cu
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+
+#include <algorithm>
+#include <string>
+
+#include "bn_device.h"
+#include "bn_step.h"
+#include "bn_errors.h"
+
+namespace bn {
+
+// =======================================================================================
+// Per-neuron device functions (one neuron, reference order of operations)
+// =======================================================================================
+
+// kernel_neuron_group_evolve.cu:85-123 — refractory pin, or forward-Euler EIF step.
+__device__ __forceinline__ void eif_step(const NeuronGroupView& v, float ge, float gi, float istim,
+                                         float& V, uint32_t& refr, uint32_t& spike) {
+    if (refr > 0) {
+        refr = refr - 1;
+        spike = 0;
+        V = v.v_reset;
+        return;
+    }
+    const float vv = V;
+    const float linear_leak_term = -1.0f * v.g_leak * (vv - v.v_rest);
+    const float exponential_depolarization_term =
+        v.g_leak * v.depolarization_slope_factor *
+        bn_expf((vv - v.v_effective_spike_threshold) / v.depolarization_slope_factor);
+    const float excitatory_synaptic_term = -1.0f * ge * (vv - v.v_excitatory_reversal);
+    const float inhibitory_synaptic_term = -1.0f * gi * (vv - v.v_inhibitory_reversal);
+    const float dv = (v.timestep / v.membrane_capacitance) *
+                     (linear_leak_term + exponential_depolarization_term +
+                      excitatory_synaptic_term + inhibitory_synaptic_term + istim);
+    const float v_next = vv + dv;
+    if (v_next >= v.v_threshold) {
+        refr = v.refractory_steps;
+        spike = 1;
+        V = v.v_threshold;
+    } else {
+        spike = 0;
+        V = v_next;
+    }
+}
+
+// kernel_neuron_group_evolve.cu:11-83 — background traces, Poisson shot noise, bg conductance.
+__device__ __forceinline__ void background_step(const NeuronGroupView& v, uint32_t lid,
+                                                uint32_t step, const bn_device_clock& clk,
+                                                float& bg_ge, float& bg_gi) {
+    float re = flush_trace(v.background_excitatory_postsynaptic_r_traces[lid] *
+                           v.background_excitatory_postsynaptic_r_trace_decay);
+    float ri = flush_trace(v.background_inhibitory_postsynaptic_r_traces[lid] *
+                           v.background_inhibitory_postsynaptic_r_trace_decay);
+    float de = flush_trace(v.background_excitatory_postsynaptic_d_traces[lid] *
+                           v.background_excitatory_postsynaptic_d_trace_decay);
+    float di = flush_trace(v.background_inhibitory_postsynaptic_d_traces[lid] *
+                           v.background_inhibitory_postsynaptic_d_trace_decay);
+    uint32_t ke = 0, ki = 0;
+    if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
+        const U32x4 r = background_draws(clk, step, v.offset + lid, v.rng_instance);
+        if (v.mean_background_excitatory_spikes > 0.0f)
+            ke = poisson_from_table(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len);
+        if (v.mean_background_inhibitory_spikes > 0.0f)
+            ki = poisson_from_table(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len);
+    }
+    if (ke > 0u) {
+        const float inc = (float)ke;
+        re += inc;
+        de += inc;
+    }
+    bg_ge = (de - re) * v.background_excitatory_scaling_factor;
+    if (ki > 0u) {
+        const float inc = (float)ki;
+        ri += inc;
+        di += inc;
+    }
+    bg_gi = (di - ri) * v.background_inhibitory_scaling_factor;
+    v.background_excitatory_postsynaptic_r_traces[lid] = re;
+    v.background_inhibitory_postsynaptic_r_traces[lid] = ri;
+    v.background_excitatory_postsynaptic_d_traces[lid] = de;
+    v.background_inhibitory_postsynaptic_d_traces[lid] = di;
+    v.background_g_excitatory_synaptic_conductances[lid] = bg_ge;
+    v.background_g_inhibitory_synaptic_conductances[lid] = bg_gi;
+}
+
+// kernel_connection_propagate.cu:8-46 for one pre neuron.
+__device__ __forceinline__ void connection_trace_step(float* r, float* d, float* x, float decay_r,
+                                                      float decay_d, float K, uint32_t i,
+                                                      uint32_t spike) {
+    float dd = flush_trace(d[i] * decay_d);
+    float rr = flush_trace(r[i] * decay_r);
+    if (spike == 1) {
+        rr += K;
+        dd += K;
+    }
+    d[i] = dd;
+    r[i] = rr;
+    x[i] = dd - rr;
+}
+
+// Sum of FP64 GEMV partials into an FP32 conductance: fl32((double)g + p_0 + p_1 + ...).
+__device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, uint32_t j) {
+    double acc = (double)g;
+    const double* p = s.partials + j;
+    for (uint32_t c = 0; c < s.chunks; ++c) acc += p[(size_t)c * s.ld];
+    return (float)acc;
+}
+
+// =======================================================================================
+// Fused evolve (every neuron of every group in one launch)
+// =======================================================================================
+__global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bn_device_clock clk = *P.clock;
+    const uint32_t step = clk.clock_step;
+    uint32_t spike = 0;
+    float V = 0.0f;
+    if (gid < P.total_neurons) {
+        for (uint32_t gi = 0; gi < P.n_groups; ++gi) {
+            const EvolveGroup& G = P.g[gi];
+            if (gid < G.v.offset || gid >= G.v.offset + G.v.num_neurons) continue;
+            const NeuronGroupView& v = G.v;
+            const uint32_t lid = gid - v.offset;
+            float ge = v.g_excitatory[lid];
+            float gi_ = v.g_inhibitory[lid];
+            for (uint32_t s = 0; s < G.n_fin_e; ++s) ge = finalize_g(ge, G.fin_e[s], lid);
+            for (uint32_t s = 0; s < G.n_fin_i; ++s) gi_ = finalize_g(gi_, G.fin_i[s], lid);
+            V = v.membrane_potentials[lid];
+            uint32_t refr = v.refractory_counter[lid];
+            eif_step(v, ge, gi_, v.i_stimulus[lid], V, refr, spike);
+            v.membrane_potentials[lid] = V;
+            v.refractory_counter[lid] = refr;
+            v.spikes[lid] = spike;
+            float bg_ge, bg_gi;
+            background_step(v, lid, step, clk, bg_ge, bg_gi);
+            v.g_excitatory[lid] = bg_ge;
+            v.g_inhibitory[lid] = bg_gi;
+        }
+        for (uint32_t t = 0; t < P.n_ts; ++t) {
+            const TraceSet& T = P.ts[t];
+            if (gid < T.pre_offset || gid >= T.pre_offset + T.n_pre) continue;
+            connection_trace_step(T.r, T.d, T.x, T.decay_r, T.decay_d, T.K, gid - T.pre_offset, spike);
+        }
+        if (!P.skip_plast_decay) {
+            for (uint32_t q = 0; q < P.n_pt; ++q) {
+                const PlastTraces& Q = P.pt[q];
+                if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
+                    const uint32_t i = gid - Q.pre_offset;
+                    Q.pre[i] = flush_trace(Q.pre[i] * Q.pre_decay);
+                    if (Q.kind == kPlastTriplet) Q.trip_pre[i] = flush_trace(Q.trip_pre[i] * Q.trip_pre_decay);
+                }
+                if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
+                    const uint32_t j = gid - Q.post_offset;
+                    Q.post[j] = flush_trace(Q.post[j] * Q.post_decay);
+                    if (Q.kind == kPlastTriplet) Q.trip_post[j] = flush_trace(Q.trip_post[j] * Q.trip_post_decay);
+                }
+            }
+        }
+    }
+    // Spike bits of this step (64 global ids per wave → two 32-bit words).
+    const uint64_t ballot = __ballot(spike == 1);
+    const uint32_t lane = threadIdx.x & 63;
+    if (lane == 0 || lane == 32) {
+        const uint32_t word = (gid >> 5);
+        if (word < P.words) {
+            const uint32_t bits = lane == 0 ? (uint32_t)ballot : (uint32_t)(ballot >> 32);
+            P.spike_bits[word] = bits;
+            if (P.spike_bits_ring) P.spike_bits_ring[(size_t)(step % P.interval) * P.words + word] = bits;
+        }
+    }
+    if (P.v_ring && gid < P.total_neurons) {
+        const int32_t slot = P.vrec_index[gid];
+        if (slot >= 0) P.v_ring[(size_t)(step % P.interval) * P.n_vrec + slot] = V;
+    }
+}
+
+// =======================================================================================
+// GEMV partials: one wave per tile of (row chunk, 256 columns).
+//   partial[chunk][j] = sum_{i in chunk, increasing} (double)x'_i * (double)W[i][j]
+// Products of two binary32 numbers are exact in binary64, so the only rounding is the
+// FP64 accumulation in this fixed order, then one rounding to FP32 when g is finalised.
+// =======================================================================================
+__device__ __forceinline__ float row_x(const GemvConn& C, uint32_t i) {
+    const float xi = C.x[i];
+    return C.row_scale ? xi * C.row_scale[i] : xi;
+}
+
+__global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (wave >= P.total_tiles) return;
+    uint32_t ci = 0;
+    while (ci + 1 < P.n_conn && wave >= P.c[ci].tile_end) ++ci;
+    const GemvConn& C = P.c[ci];
+    const uint32_t t = wave - C.tile_begin;
+    const uint32_t chunk = t / C.n_coltiles;
+    const uint32_t q = t - chunk * C.n_coltiles;
+    const uint32_t j0 = q * kGemvTileCols + 4 * lane;
+    const uint32_t i0 = chunk * C.row_block;
+    const uint32_t i1 = min(i0 + C.row_block, C.n_pre);
+    if (j0 >= C.n_post) return;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const float* wp = C.W + (size_t)i0 * C.ld + j0;
+    uint32_t i = i0;
+    for (; i + 8 <= i1; i += 8) {
+        float4 w[8];
+        float xs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            w[k] = *reinterpret_cast<const float4*>(wp + (size_t)k * C.ld);
+            xs[k] = row_x(C, i + k);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double xd = (double)xs[k];
+            a0 += xd * (double)w[k].x;
+            a1 += xd * (double)w[k].y;
+            a2 += xd * (double)w[k].z;
+            a3 += xd * (double)w[k].w;
+        }
+        wp += (size_t)8 * C.ld;
+    }
+    for (; i < i1; ++i) {
+        const float4 w = *reinterpret_cast<const float4*>(wp);
+        const double xd = (double)row_x(C, i);
+        a0 += xd * (double)w.x;
+        a1 += xd * (double)w.y;
+        a2 += xd * (double)w.z;
+        a3 += xd * (double)w.w;
+        wp += C.ld;
+    }
+    double* out = C.partials + (size_t)chunk * C.ld + j0;
+    out[0] = a0;
+    out[1] = a1;
+    out[2] = a2;
+    out[3] = a3;
+}
+
+// Scalar-load variant for row strides that are not 16-byte aligned (reference layout with
+// post % 4 != 0).  Same tiles, same summation order.
+__global__ __launch_bounds__(256) void k_gemv_scalar(GemvPlan P) {
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (wave >= P.total_tiles) return;
+    uint32_t ci = 0;
+    while (ci + 1 < P.n_conn && wave >= P.c[ci].tile_end) ++ci;
+    const GemvConn& C = P.c[ci];
+    const uint32_t t = wave - C.tile_begin;
+    const uint32_t chunk = t / C.n_coltiles;
+    const uint32_t q = t - chunk * C.n_coltiles;
+    const uint32_t i0 = chunk * C.row_block;
+    const uint32_t i1 = min(i0 + C.row_block, C.n_pre);
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t j = q * kGemvTileCols + 4 * lane + k;
+        if (j >= C.n_post) return;
+        double a = 0.0;
+        for (uint32_t i = i0; i < i1; ++i) a += (double)row_x(C, i) * (double)C.W[(size_t)i * C.ld + j];
+        C.partials[(size_t)chunk * C.ld + j] = a;
+    }
+}
+
+// g_dst[j] = fl32((double)g_base[j] + partials...) for each source in order.
+__global__ __launch_bounds__(256) void k_materialize_g(float* g_dst, const float* g_base,
+                                                      FinalizeSource s0, FinalizeSource s1,
+                                                      uint32_t n_src, uint32_t n) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float g = g_base[j];
+    if (n_src > 0) g = finalize_g(g, s0, j);
+    if (n_src > 1) g = finalize_g(g, s1, j);
+    g_dst[j] = g;
+}
+
+// =======================================================================================
+// Plasticity: LTP column walks, LTD row walks, row normalisation, trace increments
+// =======================================================================================
+__device__ __forceinline__ bool locate_item(const PlastPlan& P, uint32_t item, bool post,
+                                            uint32_t& pi, uint32_t& idx) {
+    for (uint32_t k = 0; k < P.n; ++k) {
+        const uint32_t n = post ? P.p[k].n_post : P.p[k].n_pre;
+        if (item < n) {
+            pi = k;
+            idx = item;
+            return true;
+        }
+        item -= n;
+    }
+    return false;
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Lazy normalisation of the triplet matrix: W_true = s_i * M.
+// ---------------------------------------------------------------------------------------
+// Number of set bits in global bit range [lo, hi) (uniform across the block).
+__device__ __forceinline__ uint32_t count_bits(const uint32_t* bits, uint32_t lo, uint32_t hi) {
+    uint32_t c = 0;
+    for (uint32_t g = lo; g < hi;) {
+        const uint32_t w = g >> 5, b = g & 31;
+        const uint32_t n = min(32u - b, hi - g);
+        const uint32_t mask = n == 32 ? 0xFFFFFFFFu : (((1u << n) - 1u) << b);
+        c += __popc(bits[w] & mask);
+        g += n;
+    }
+    return c;
+}
+
+// LTP on column j of the lazy matrix; the change of each M entry goes to delta[rank][i].
+__device__ void ltp_lazy_column(const PlastDev& D, uint32_t j) {
+    const uint32_t rank = count_bits(D.bits, D.post_offset, D.post_offset + j);
+    const bool keep = rank < D.kmax;
+    const float fac = D.p.ltp_A + (D.p.ltp_B * D.trip_post[j]);
+    for (uint32_t i = threadIdx.x; i < D.n_pre; i += blockDim.x) {
+        double dm = 0.0;
+        const float si = D.s[i];
+        if (i != j && si != 0.0f) {
+            float* mp = D.W + (size_t)i * D.ld + j;
+            const float m = *mp;
+            const float w = clamp_weight(si * m + D.pre[i] * fac, D.p.weight_min, D.p.weight_max);
+            const float mn = w / si;
+            *mp = mn;
+            dm = (double)mn - (double)m;
+        }
+        if (keep) D.delta[(size_t)rank * D.n_pre + i] = dm;
+    }
+}
+
+// Canonical row sum (same order as normalize_row) of a row held in registers.
+__device__ __forceinline__ double block_row_sum(const float* vals, uint32_t n, double* lds4) {
+    const uint32_t t = threadIdx.x;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNormRegs; ++k)
+        if (t + (uint32_t)k * kNormThreads < n) acc += (double)vals[k];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((t & 63) == 0) lds4[t >> 6] = acc;
+    __syncthreads();
+    const double total = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+    __syncthreads();
+    return total;
+}
+
+// Block per row: LTD of a spiking pre row (in true units), then the canonical re-sum.
+__device__ void lazy_resum_row(const PlastDev& D, uint32_t i, bool ltd, double* lds4) {
+    float* row = D.W + (size_t)i * D.ld;
+    const float si = D.s[i];
+    const float fac = D.p.ltd_A + (D.p.ltd_B * D.trip_pre[i]);
+    float vals[kNormRegs];
+#pragma unroll
+    for (int k = 0; k < kNormRegs; ++k) {
+        const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
+        vals[k] = 0.0f;
+        if (j < D.n_post) {
+            float m = row[j];
+            if (ltd && si != 0.0f && j != i) {
+                const float w = clamp_weight(si * m + -1.0f * D.post[j] * fac, D.p.weight_min, D.p.weight_max);
+                m = w / si;
+                row[j] = m;
+            }
+            vals[k] = m;
+        }
+    }
+    const double R = block_row_sum(vals, D.n_post, lds4);
+    if (threadIdx.x == 0) {
+        D.R[i] = R;
+        if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
+    }
+}
+
+// Blocks [0, nt): one thread per row adds the LTP deltas (rows whose pre did not spike).
+// Blocks [nt, grid): grid-stride over rows, whole block per spiking row (LTD + re-sum), or
+// over every row when more than kmax posts spiked.
+__global__ __launch_bounds__(256) void k_lazy_rows(PlastPlan P, uint32_t nt) {
+    __shared__ double lds4[4];
+    for (uint32_t k = 0; k < P.n; ++k) {
+        const PlastDev& D = P.p[k];
+        if (!D.lazy) continue;
+        const uint32_t K = count_bits(D.bits, D.post_offset, D.post_offset + D.n_post);
+        const bool overflow = K > D.kmax;
+        if (blockIdx.x < nt) {
+            const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (overflow || i >= D.n_pre || D.pre_spikes[i] == 1) continue;
+            double dsum = 0.0;
+            for (uint32_t r = 0; r < K; ++r) dsum += D.delta[(size_t)r * D.n_pre + i];
+            const double R = D.R[i] + dsum;
+            D.R[i] = R;
+            if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
+        } else {
+            for (uint32_t i = blockIdx.x - nt; i < D.n_pre; i += gridDim.x - nt) {
+                const bool spiked = D.pre_spikes[i] == 1;
+                if (spiked || overflow) lazy_resum_row(D, i, spiked, lds4);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lazy_init(PlastDev D) {
+    __shared__ double lds4[4];
+    const uint32_t i = blockIdx.x;
+    if (i >= D.n_pre) return;
+    if (threadIdx.x == 0) D.s[i] = 1.0f;
+    __syncthreads();
+    float vals[kNormRegs];
+    const float* row = D.W + (size_t)i * D.ld;
+#pragma unroll
+    for (int k = 0; k < kNormRegs; ++k) {
+        const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
+        vals[k] = j < D.n_post ? row[j] : 0.0f;
+    }
+    const double R = block_row_sum(vals, D.n_post, lds4);
+    if (threadIdx.x == 0) D.R[i] = R;
+}
+
+// kernel_triplet_plasticity_update.cu:54-72, kernel_istdp_plasticity_update.cu:42-60,
+// kernel_stdp_plasticity_update.cu:42-60 (index-based self skip kept).
+__global__ __launch_bounds__(256) void k_ltp(PlastPlan P) {
+    for (uint32_t item = blockIdx.x; item < P.items_post; item += gridDim.x) {
+        uint32_t pi, j;
+        if (!locate_item(P, item, true, pi, j)) return;
+        const PlastDev& D = P.p[pi];
+        if (D.post_spikes[j] != 1) continue;
+        if (D.lazy) {
+            ltp_lazy_column(D, j);
+            continue;
+        }
+        float fac = D.p.ltp_A;
+        if (D.kind == kPlastTriplet) fac = D.p.ltp_A + (D.p.ltp_B * D.trip_post[j]);
+        for (uint32_t i = threadIdx.x; i < D.n_pre; i += blockDim.x) {
+            if (i == j) continue;
+            float* w = D.W + (size_t)i * D.ld + j;
+            const float delta = D.kind == kPlastTriplet ? D.pre[i] * fac : D.p.ltp_A * D.pre[i];
+            *w = clamp_weight(*w + delta, D.p.weight_min, D.p.weight_max);
+        }
+    }
+}
+
+// kernel_triplet_plasticity_update.cu:74-93, kernel_istdp_plasticity_update.cu:62-81,
+// kernel_stdp_plasticity_update.cu:62-81.
+__global__ __launch_bounds__(256) void k_ltd(PlastPlan P) {
+    for (uint32_t item = blockIdx.x; item < P.items_pre; item += gridDim.x) {
+        uint32_t pi, i;
+        if (!locate_item(P, item, false, pi, i)) return;
+        const PlastDev& D = P.p[pi];
+        if (D.lazy || D.pre_spikes[i] != 1) continue;
+        const float fac = D.p.ltd_A + (D.p.ltd_B * (D.kind == kPlastTriplet ? D.trip_pre[i] : 0.0f));
+        float* row = D.W + (size_t)i * D.ld;
+        for (uint32_t j = threadIdx.x; j < D.n_post; j += blockDim.x) {
+            if (i == j) continue;
+            float delta;
+            if (D.kind == kPlastTriplet) delta = -1.0f * D.post[j] * fac;
+            else if (D.kind == kPlastIstdp) delta = -1.0f * D.p.ltd_A * (D.post[j] - D.p.ltd_B);
+            else delta = -1.0f * D.p.ltd_A * D.post[j];
+            row[j] = clamp_weight(row[j] + delta, D.p.weight_min, D.p.weight_max);
+        }
+    }
+}
+
+// Row sum of the normalisation in a fixed order: thread t (of kNormThreads) sums columns
+// t, t+256, ... sequentially in FP64; each wave combines its 64 lanes by an xor butterfly
+// (offsets 32,16,8,4,2,1); the four wave sums are added in wave order.
+// Then as kernel_triplet_plasticity_update.cu:117-145: skip if sum <= 0, else scale the row
+// by target/sum (target = pre_weight_sums[i], the initial column sum: triplet_plasticity.cpp:22-30).
+
+__device__ void normalize_row(const PlastDev& D, uint32_t i, double* lds4) {
+    float* row = D.W + (size_t)i * D.ld;
+    const uint32_t t = threadIdx.x;
+    float vals[kNormRegs];
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kNormRegs; ++k) {
+        const uint32_t j = t + (uint32_t)k * kNormThreads;
+        vals[k] = 0.0f;
+        if (j < D.n_post) {
+            vals[k] = row[j];
+            acc += (double)vals[k];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((t & 63) == 0) lds4[t >> 6] = acc;
+    __syncthreads();
+    const double total = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+    __syncthreads();
+    const float current_sum = (float)total;
+    if (current_sum <= 0.0f) return;
+    const float scale = D.target[i] / current_sum;
+#pragma unroll
+    for (int k = 0; k < kNormRegs; ++k) {
+        const uint32_t j = t + (uint32_t)k * kNormThreads;
+        if (j < D.n_post) row[j] = vals[k] * scale;
+    }
+}
+
+__device__ __forceinline__ void increment_traces(const PlastPlan& P, uint32_t gid) {
+    // kernel_triplet_plasticity_update.cu:95-115 (and istdp/stdp equivalents)
+    for (uint32_t k = 0; k < P.n; ++k) {
+        const PlastDev& D = P.p[k];
+        if (gid >= D.pre_offset && gid < D.pre_offset + D.n_pre) {
+            const uint32_t i = gid - D.pre_offset;
+            if (D.pre_spikes[i] == 1) {
+                D.pre[i] += 1.0f;
+                if (D.kind == kPlastTriplet) D.trip_pre[i] += 1.0f;
+            }
+        }
+        if (gid >= D.post_offset && gid < D.post_offset + D.n_post) {
+            const uint32_t j = gid - D.post_offset;
+            if (D.post_spikes[j] == 1) {
+                D.post[j] += 1.0f;
+                if (D.kind == kPlastTriplet) D.trip_post[j] += 1.0f;
+            }
+        }
+    }
+}
+
+// Stimulus events of clock_step applied in list order (stimulus.cpp:46-60), then the clock
+// advances.  One thread: the event count per step is small and sequential order keeps the
+// result independent of scheduling (the reference uses atomicAdd).
+__device__ void stimulus_and_clock(const TailPlan& T) {
+    bn_device_clock* clk = T.clock;
+    const uint32_t s = clk->clock_step;
+    if (T.timestep_event_count && s < T.num_timesteps) {
+        const uint32_t n = T.timestep_event_count[s];
+        const uint32_t cur = clk->next_event_idx;
+        if ((uint64_t)cur + n > T.num_events) {
+            *T.status = 1u;
+        } else {
+            for (uint32_t e = cur; e < cur + n; ++e)
+                T.i_stimulus[T.event_ids[e] + T.event_offsets[e]] += T.event_amplitudes[e];
+            clk->next_event_idx = cur + n;
+        }
+    }
+    if (T.advance_clock) clk->clock_step = s + 1;
+}
+
+// Blocks [0, norm_rows): one row each.  Blocks after: trace increments for 256 neurons each.
+// The last block also applies the stimulus and advances the clock (no other block reads the
+// clock in this kernel).
+__global__ __launch_bounds__(256) void k_norm_tail(TailPlan T) {
+    __shared__ double lds4[4];
+    const uint32_t b = blockIdx.x;
+    if (b < T.norm_rows) {
+        uint32_t rem = b;
+        for (uint32_t k = 0; k < T.pl.n; ++k) {
+            const PlastDev& D = T.pl.p[k];
+            if (!D.normalize) continue;
+            if (rem < D.n_pre) {
+                normalize_row(D, rem, lds4);
+                return;
+            }
+            rem -= D.n_pre;
+        }
+        return;
+    }
+    const uint32_t gid = (b - T.norm_rows) * blockDim.x + threadIdx.x;
+    if (T.increments && gid < T.total_neurons) increment_traces(T.pl, gid);
+    if (b == gridDim.x - 1 && threadIdx.x == 0) stimulus_and_clock(T);
+}
+
+// =======================================================================================
+// Component (drop-in) kernels
+// =======================================================================================
+__global__ __launch_bounds__(256) void k_group_evolve(NeuronGroupView v) {
+    const uint32_t lid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lid >= v.num_neurons) return;
+    const bn_device_clock clk = *v.rng_clock;
+    float V = v.membrane_potentials[lid];
+    uint32_t refr = v.refractory_counter[lid];
+    uint32_t spike = 0;
+    eif_step(v, v.g_excitatory[lid], v.g_inhibitory[lid], v.i_stimulus[lid], V, refr, spike);
+    v.membrane_potentials[lid] = V;
+    v.refractory_counter[lid] = refr;
+    v.spikes[lid] = spike;
+    float bg_ge, bg_gi;
+    background_step(v, lid, clk.clock_step, clk, bg_ge, bg_gi);
+    v.g_excitatory[lid] = bg_ge;
+    v.g_inhibitory[lid] = bg_gi;
+}
+
+__global__ __launch_bounds__(256) void k_connection_traces(ConnectionView v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.pre_group_num_neurons) return;
+    connection_trace_step(v.postsynaptic_r_traces, v.postsynaptic_d_traces,
+                          v.summed_postsynaptic_conductances, v.postsynaptic_r_trace_decay,
+                          v.postsynaptic_d_trace_decay, v.postsynaptic_kernel_normalization_value,
+                          i, v.spikes[i]);
+}
+
+__global__ __launch_bounds__(256) void k_plast_decay(PlastPlan P) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t k = 0; k < P.n; ++k) {
+        const PlastDev& D = P.p[k];
+        if (idx < D.n_pre) {
+            D.pre[idx] = flush_trace(D.pre[idx] * D.p.pre_trace_decay);
+            if (D.kind == kPlastTriplet)
+                D.trip_pre[idx] = flush_trace(D.trip_pre[idx] * D.p.triplet_pre_trace_decay);
+        }
+        if (idx < D.n_post) {
+            D.post[idx] = flush_trace(D.post[idx] * D.p.post_trace_decay);
+            if (D.kind == kPlastTriplet)
+                D.trip_post[idx] = flush_trace(D.trip_post[idx] * D.p.triplet_post_trace_decay);
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_plast_increment(PlastPlan P, uint32_t total_neurons) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < total_neurons) increment_traces(P, gid);
+}
+
+__global__ void k_stimulus_apply(StimulusView v, uint32_t first, uint32_t n) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (uint32_t e = first; e < first + n; ++e)
+        v.i_stimulus[v.stimulus_event_neuron_ids[e] + v.stimulus_event_post_group_offsets[e]] +=
+            v.stimulus_event_amplitude_changes[e];
+}
+
+__global__ void k_stimulus_device_cursor(StimulusView v, bn_device_clock* clk, uint32_t* status) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t s = clk->clock_step;
+    if (s >= v.num_timesteps) return;
+    const uint32_t n = v.timestep_event_count[s];
+    const uint32_t cur = clk->next_event_idx;
+    if ((uint64_t)cur + n > v.num_events) {
+        if (status) *status = 1u;
+        return;
+    }
+    for (uint32_t e = cur; e < cur + n; ++e)
+        v.i_stimulus[v.stimulus_event_neuron_ids[e] + v.stimulus_event_post_group_offsets[e]] +=
+            v.stimulus_event_amplitude_changes[e];
+    clk->next_event_idx = cur + n;
+}
+
+__global__ void k_clock_advance(bn_device_clock* clk) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) clk->clock_step = clk->clock_step + 1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_historicize(const T* src, T* hist, uint32_t n,
+                                                     uint32_t interval, uint32_t step) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    hist[(size_t)id * interval + step % interval] = src[id];
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t value, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// =======================================================================================
+// Host-side launch helpers
+// =======================================================================================
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int launch_evolve(const EvolvePlan& plan, bn_stream_t s) {
+    static_assert(sizeof(EvolvePlan) < 3800, "kernel argument too large");
+    if (plan.total_neurons == 0) return 0;
+    hipLaunchKernelGGL(k_evolve, dim3(cdiv(plan.total_neurons, 256)), dim3(256), 0, s, plan);
+    return (int)hipGetLastError();
+}
+
+static bool gemv_vector_ok(const GemvPlan& plan) {
+    for (uint32_t c = 0; c < plan.n_conn; ++c) {
+        if (plan.c[c].ld % 4 != 0) return false;
+        if (((uintptr_t)plan.c[c].W) % 16 != 0) return false;
+    }
+    return true;
+}
+
+int launch_gemv(const GemvPlan& plan, bn_stream_t s) {
+    if (plan.total_tiles == 0) return 0;
+    if (gemv_vector_ok(plan))
+        hipLaunchKernelGGL(k_gemv, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
+    else
+        hipLaunchKernelGGL(k_gemv_scalar, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
+    return (int)hipGetLastError();
+}
+
+static uint32_t plast_grid(uint32_t items) { return items < 2048 ? (items ? items : 1) : 2048; }
+
+int launch_ltp(const PlastPlan& plan, bn_stream_t s) {
+    if (plan.n == 0 || plan.items_post == 0) return 0;
+    hipLaunchKernelGGL(k_ltp, dim3(plast_grid(plan.items_post)), dim3(256), 0, s, plan);
+    return (int)hipGetLastError();
+}
+
+int launch_ltd(const PlastPlan& plan, bn_stream_t s) {
+    if (plan.n == 0 || plan.items_pre == 0) return 0;
+    hipLaunchKernelGGL(k_ltd, dim3(plast_grid(plan.items_pre)), dim3(256), 0, s, plan);
+    return (int)hipGetLastError();
+}
+
+int launch_norm_tail(const TailPlan& plan, bn_stream_t s) {
+    static_assert(sizeof(TailPlan) < 3800, "kernel argument too large");
+    const uint32_t blocks = plan.norm_rows + cdiv(plan.total_neurons, 256) + 0;
+    hipLaunchKernelGGL(k_norm_tail, dim3(blocks ? blocks : 1), dim3(256), 0, s, plan);
+    return (int)hipGetLastError();
+}
+
+static uint32_t lazy_thread_blocks(const PlastPlan& plan) {
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < plan.n; ++k)
+        if (plan.p[k].lazy) n = std::max(n, plan.p[k].n_pre);
+    return cdiv(n, 256);
+}
+
+int launch_lazy_rows(const PlastPlan& plan, bn_stream_t s) {
+    const uint32_t nt = lazy_thread_blocks(plan);
+    if (nt == 0) return 0;
+    hipLaunchKernelGGL(k_lazy_rows, dim3(nt + 256), dim3(256), 0, s, plan, nt);
+    return (int)hipGetLastError();
+}
+
+int launch_lazy_init(const PlastDev& d, bn_stream_t s) {
+    if (d.n_pre == 0) return 0;
+    hipLaunchKernelGGL(k_lazy_init, dim3(d.n_pre), dim3(256), 0, s, d);
+    return (int)hipGetLastError();
+}
+
+int launch_materialize_g(float* g_dst, const float* g_base, const FinalizeSource* src,
+                         uint32_t n_src, uint32_t n, bn_stream_t s) {
+    if (n == 0) return 0;
+    FinalizeSource s0{}, s1{};
+    if (n_src > 0) s0 = src[0];
+    if (n_src > 1) s1 = src[1];
+    hipLaunchKernelGGL(k_materialize_g, dim3(cdiv(n, 256)), dim3(256), 0, s, g_dst, g_base, s0, s1,
+                       n_src, n);
+    return (int)hipGetLastError();
+}
+
+int launch_fill_u32(uint32_t* p, uint32_t value, size_t n, bn_stream_t s) {
+    if (n == 0) return 0;
+    const uint32_t blocks = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, value, n);
+    return (int)hipGetLastError();
+}
+
+// Single-plasticity plan from the reference's view/params pair.
+static PlastPlan plan_from_view(const PlasticityView& v, const PlasticityParams& p, uint32_t kind,
+                                bool normalize) {
+    PlastPlan P{};
+    PlastDev& D = P.p[0];
+    D.kind = kind;
+    D.normalize = normalize ? 1u : 0u;
+    D.p = p;
+    D.W = v.weight_matrix;
+    D.ld = v.weight_matrix_ld ? v.weight_matrix_ld : v.post_group_num_neurons;
+    D.n_pre = v.pre_group_num_neurons;
+    D.n_post = v.post_group_num_neurons;
+    D.pre_offset = v.pre_group_offset;
+    D.post_offset = v.post_group_offset;
+    D.pre_spikes = v.pre_spikes;
+    D.post_spikes = v.post_spikes;
+    D.pre = v.pre_traces;
+    D.post = v.post_traces;
+    D.trip_pre = v.triplet_pre_traces;
+    D.trip_post = v.triplet_post_traces;
+    D.target = v.pre_weight_sums;
+    P.n = 1;
+    P.items_post = D.n_post;
+    P.items_pre = D.n_pre;
+    return P;
+}
+
+static int plasticity_update(const PlasticityView& v, const PlasticityParams& p, uint32_t kind,
+                             bool normalize, bn_stream_t s, const char* what) {
+    if (kind == kPlastTriplet && normalize && v.post_group_num_neurons > kNormRegs * kNormThreads)
+        return set_error(BN_ERR_INVALID_ARGUMENT, std::string(what) +
+                                                      ": post group larger than the normalisation kernel supports");
+    PlastPlan P = plan_from_view(v, p, kind, normalize);
+    const uint32_t nmax = std::max(P.p[0].n_pre, P.p[0].n_post);
+    if (nmax == 0) return BN_OK;
+    hipLaunchKernelGGL(k_plast_decay, dim3(cdiv(nmax, 256)), dim3(256), 0, s, P);
+    BN_HIP_TRY(hipGetLastError(), what);
+    BN_HIP_TRY((hipError_t)launch_ltp(P, s), what);
+    BN_HIP_TRY((hipError_t)launch_ltd(P, s), what);
+    // The reference increments with the pre/post group slices only, so the neuron range for
+    // the increment kernel is [min offset, max end).
+    const uint32_t lo = std::min(P.p[0].pre_offset, P.p[0].post_offset);
+    const uint32_t hi = std::max(P.p[0].pre_offset + P.p[0].n_pre, P.p[0].post_offset + P.p[0].n_post);
+    if (normalize) {
+        TailPlan T{};
+        T.pl = P;
+        T.norm_rows = P.p[0].n_pre;
+        T.total_neurons = 0;
+        T.increments = 0;
+        T.clock = nullptr;
+        T.advance_clock = 0;
+        // Normalisation only (no tail blocks): reuse k_norm_tail with zero neurons.
+        hipLaunchKernelGGL(k_norm_tail, dim3(T.norm_rows), dim3(256), 0, s, T);
+        BN_HIP_TRY(hipGetLastError(), what);
+    }
+    hipLaunchKernelGGL(k_plast_increment, dim3(cdiv(hi, 256)), dim3(256), 0, s, P, hi);
+    BN_HIP_TRY(hipGetLastError(), what);
+    (void)lo;
+    return BN_OK;
+}
+
+}  // namespace bn
+
+// =======================================================================================
+// extern "C" launcher layer
+// =======================================================================================
+using namespace bn;
+
+extern "C" int bn_neuron_group_evolve(NeuronGroupView v, bn_stream_t s) {
+    if (v.num_neurons == 0) return BN_OK;
+    if (!v.rng_clock) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_neuron_group_evolve: rng_clock is null");
+    hipLaunchKernelGGL(k_group_evolve, dim3(cdiv(v.num_neurons, 256)), dim3(256), 0, s, v);
+    BN_HIP_TRY(hipGetLastError(), "kernel_neuron_group_evolve launch");
+    return BN_OK;
+}
+
+extern "C" int bn_connection_propagate(ConnectionView v, bn_stream_t s) {
+    if (v.pre_group_num_neurons == 0 || v.post_group_num_neurons == 0) return BN_OK;
+    if (!v.gemv_partials || v.gemv_row_block == 0)
+        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_connection_propagate: gemv_partials/gemv_row_block not set");
+    hipLaunchKernelGGL(k_connection_traces, dim3(cdiv(v.pre_group_num_neurons, 256)), dim3(256), 0, s, v);
+    BN_HIP_TRY(hipGetLastError(), "kernel_connection_traces launch");
+    GemvPlan P{};
+    GemvConn& C = P.c[0];
+    C.W = v.weight_matrix;
+    C.x = v.summed_postsynaptic_conductances;
+    C.row_scale = nullptr;
+    C.partials = v.gemv_partials;
+    C.ld = v.weight_matrix_ld ? v.weight_matrix_ld : v.post_group_num_neurons;
+    C.n_pre = v.pre_group_num_neurons;
+    C.n_post = v.post_group_num_neurons;
+    C.row_block = v.gemv_row_block;
+    C.n_chunks = cdiv(C.n_pre, C.row_block);
+    C.n_coltiles = cdiv(C.n_post, kGemvTileCols);
+    C.tile_begin = 0;
+    C.tile_end = C.n_chunks * C.n_coltiles;
+    P.n_conn = 1;
+    P.total_tiles = C.tile_end;
+    BN_HIP_TRY((hipError_t)launch_gemv(P, s), "gemv connection accumulate");
+    float* target = v.pre_group_is_inhibtory ? v.post_group_g_inhibitory : v.post_group_g_excitatory;
+    FinalizeSource src{v.gemv_partials, C.n_chunks, C.ld};
+    BN_HIP_TRY((hipError_t)launch_materialize_g(target, target, &src, 1, C.n_post, s),
+               "gemv finalize");
+    return BN_OK;
+}
+
+extern "C" int bn_stimulus_apply(StimulusView v, uint32_t next_event_idx, uint32_t n, bn_stream_t s) {
+    if (n == 0) return BN_OK;
+    hipLaunchKernelGGL(k_stimulus_apply, dim3(1), dim3(64), 0, s, v, next_event_idx, n);
+    BN_HIP_TRY(hipGetLastError(), "kernel_stimulus_apply launch");
+    return BN_OK;
+}
+
+extern "C" int bn_stimulus_apply_device_cursor(StimulusView v, bn_device_clock* clock, bn_stream_t s) {
+    if (!clock) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_stimulus_apply_device_cursor: clock is null");
+    hipLaunchKernelGGL(k_stimulus_device_cursor, dim3(1), dim3(64), 0, s, v, clock, (uint32_t*)nullptr);
+    BN_HIP_TRY(hipGetLastError(), "kernel_stimulus_apply launch");
+    return BN_OK;
+}
+
+extern "C" int bn_triplet_plasticity_update(PlasticityView v, PlasticityParams p, bn_stream_t s) {
+    return plasticity_update(v, p, kPlastTriplet, false, s, "kernel_triplet_plasticity_update");
+}
+extern "C" int bn_triplet_plasticity_update_normalize(PlasticityView v, PlasticityParams p, bn_stream_t s) {
+    return plasticity_update(v, p, kPlastTriplet, true, s, "kernel_triplet_plasticity_update");
+}
+extern "C" int bn_istdp_plasticity_update(PlasticityView v, PlasticityParams p, bn_stream_t s) {
+    return plasticity_update(v, p, kPlastIstdp, false, s, "kernel_istdp_plasticity_update");
+}
+extern "C" int bn_stdp_plasticity_update(PlasticityView v, PlasticityParams p, bn_stream_t s) {
+    return plasticity_update(v, p, kPlastStdp, false, s, "kernel_stdp_plasticity_update");
+}
+
+extern "C" int bn_historicize_spikes(const uint32_t* d_spikes, uint32_t* hist, uint32_t n,
+                                     uint32_t interval, uint32_t step, bn_stream_t s) {
+    if (n == 0) return BN_OK;
+    if (interval == 0) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_historicize_spikes: interval is 0");
+    hipLaunchKernelGGL(k_historicize<uint32_t>, dim3(cdiv(n, 256)), dim3(256), 0, s, d_spikes, hist, n,
+                       interval, step);
+    BN_HIP_TRY(hipGetLastError(), "kernel_historicize_spikes launch");
+    return BN_OK;
+}
+
+extern "C" int bn_historicize_membrane_potentials(const float* d_v, float* hist, uint32_t n,
+                                                  uint32_t interval, uint32_t step, bn_stream_t s) {
+    if (n == 0) return BN_OK;
+    if (interval == 0)
+        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_historicize_membrane_potentials: interval is 0");
+    hipLaunchKernelGGL(k_historicize<float>, dim3(cdiv(n, 256)), dim3(256), 0, s, d_v, hist, n, interval,
+                       step);
+    BN_HIP_TRY(hipGetLastError(), "kernel_historicize_membrane_potentials launch");
+    return BN_OK;
+}
+
+extern "C" int bn_clock_advance(bn_device_clock* clock, bn_stream_t s) {
+    hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, s, clock);
+    BN_HIP_TRY(hipGetLastError(), "clock advance launch");
+    return BN_OK;
+}

@@ -1,0 +1,230 @@
+"""ctypes view of the engine's C ABI (include/buryn_amd.h).
+
+``Simulation`` mirrors what the reference's driver does with a config
+(cpp_src/main_buryn.cpp:98-343) and exposes the step loop (System::run,
+cpp_src/core/system.cpp:65-83) plus state downloads for parity checks.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import subprocess
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libburyn_amd.so")
+
+BURYN_FLAGS = {
+    "write_files": 1 << 0,
+    "no_monitors": 1 << 1,
+    "lazy_normalize": 1 << 2,
+    "component_path": 1 << 3,
+    "no_graph": 1 << 4,
+    "quiet": 1 << 5,
+}
+
+
+class BnError(RuntimeError):
+    """An engine call failed; carries the C-ABI code and the engine's message."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class SimInfo(C.Structure):
+    _fields_ = [
+        ("num_excitatory_neurons", C.c_uint32), ("num_inhibitory_neurons", C.c_uint32),
+        ("total_neurons", C.c_uint32), ("clock_step", C.c_uint32), ("simulation_steps", C.c_uint32),
+        ("num_plasticities", C.c_uint32), ("weight_matrix_ld", C.c_uint32 * 4),
+        ("gemv_row_block", C.c_uint32 * 4), ("weight_pre", C.c_uint32 * 4), ("weight_post", C.c_uint32 * 4),
+        ("flags", C.c_uint32), ("kernels_per_step", C.c_uint32), ("device_bytes", C.c_uint64),
+    ]
+
+
+_STATE_FIELDS = [
+    ("membrane_potentials", np.float32), ("g_excitatory", np.float32), ("g_inhibitory", np.float32),
+    ("i_stimulus", np.float32), ("refractory_counter", np.uint32), ("spikes", np.uint32),
+    ("background_g_excitatory_synaptic_conductances", np.float32),
+    ("background_g_inhibitory_synaptic_conductances", np.float32),
+    ("background_excitatory_postsynaptic_r_traces", np.float32),
+    ("background_excitatory_postsynaptic_d_traces", np.float32),
+    ("background_inhibitory_postsynaptic_r_traces", np.float32),
+    ("background_inhibitory_postsynaptic_d_traces", np.float32),
+]
+
+
+class NeuronStateHost(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n, _ in _STATE_FIELDS]
+
+
+EXPORTED_SYMBOLS = [
+    "bn_last_error", "bn_poisson_table", "bn_neuron_group_evolve", "bn_connection_propagate",
+    "bn_stimulus_apply", "bn_stimulus_apply_device_cursor", "bn_triplet_plasticity_update",
+    "bn_triplet_plasticity_update_normalize", "bn_istdp_plasticity_update", "bn_stdp_plasticity_update",
+    "bn_historicize_spikes", "bn_historicize_membrane_potentials", "bn_clock_advance", "bn_sim_create",
+    "bn_sim_create_from_json_text", "bn_sim_destroy", "bn_sim_run", "bn_sim_step_components",
+    "bn_sim_synchronize", "bn_sim_get_info", "bn_sim_get_neuron_state", "bn_sim_get_weights",
+    "bn_sim_get_connection_traces", "bn_sim_get_plasticity_traces", "bn_sim_run_timed",
+    "bn_sim_time_propagate_kernel", "bn_config_dump_json", "bn_generate_weight_matrix",
+]
+
+_lib = None
+
+
+def build_library() -> None:
+    """Compiles the engine for gfx950 in-tree (csrc/Makefile)."""
+    subprocess.run(["make", "-C", os.path.join(PKG, "csrc"), "-j8"], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P, U, I = C.c_void_p, C.c_uint32, C.c_int
+    fp = C.POINTER(C.c_float)
+    up = C.POINTER(C.c_uint32)
+    L.bn_last_error.restype = C.c_char_p
+    L.bn_poisson_table.argtypes = [C.c_float, up, U, up]
+    L.bn_sim_create.argtypes = [C.c_char_p, U, C.POINTER(P)]
+    L.bn_sim_create_from_json_text.argtypes = [C.c_char_p, U, C.POINTER(P)]
+    L.bn_sim_destroy.argtypes = [P]
+    L.bn_sim_destroy.restype = None
+    for f in ("bn_sim_run", "bn_sim_step_components"):
+        getattr(L, f).argtypes = [P, U]
+    L.bn_sim_synchronize.argtypes = [P]
+    L.bn_sim_get_info.argtypes = [P, C.POINTER(SimInfo)]
+    L.bn_sim_get_neuron_state.argtypes = [P, C.POINTER(NeuronStateHost)]
+    L.bn_sim_get_weights.argtypes = [P, U, fp]
+    L.bn_sim_get_connection_traces.argtypes = [P, U, fp, fp, fp]
+    L.bn_sim_get_plasticity_traces.argtypes = [P, U, fp, fp, fp, fp]
+    L.bn_sim_run_timed.argtypes = [P, U, fp]
+    L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
+    L.bn_config_dump_json.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.bn_generate_weight_matrix.argtypes = [C.c_char_p, U, fp, up, up]
+    for f in EXPORTED_SYMBOLS:
+        if f not in ("bn_last_error", "bn_sim_destroy"):
+            getattr(L, f).restype = I
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise BnError(rc, lib().bn_last_error().decode())
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def poisson_table(mean: float, capacity: int = 256) -> np.ndarray:
+    t = np.zeros(capacity, np.uint32)
+    n = C.c_uint32(0)
+    _check(lib().bn_poisson_table(mean, _up(t), capacity, C.byref(n)))
+    return t[: n.value]
+
+
+def config_dump(path: str) -> dict:
+    """The engine's parsed config (defaults, overrides and path prefixing applied)."""
+    need = C.c_size_t(0)
+    _check(lib().bn_config_dump_json(path.encode(), None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value)
+    _check(lib().bn_config_dump_json(path.encode(), buf, need.value, C.byref(need)))
+    return json.loads(buf.value.decode())
+
+
+def generate_weight_matrix(path: str, idx: int) -> np.ndarray:
+    pre, post = C.c_uint32(0), C.c_uint32(0)
+    _check(lib().bn_generate_weight_matrix(path.encode(), idx, None, C.byref(pre), C.byref(post)))
+    out = np.zeros((pre.value, post.value), np.float32)
+    _check(lib().bn_generate_weight_matrix(path.encode(), idx, _fp(out), C.byref(pre), C.byref(post)))
+    return out
+
+
+class Simulation:
+    """One network on the GPU.  ``flags``: names from BURYN_FLAGS."""
+
+    def __init__(self, config: str | dict, flags=("no_monitors", "quiet")):
+        self.flag_bits = 0
+        for f in flags:
+            self.flag_bits |= BURYN_FLAGS[f]
+        h = C.c_void_p()
+        if isinstance(config, dict):
+            _check(lib().bn_sim_create_from_json_text(json.dumps(config).encode(), self.flag_bits, C.byref(h)))
+        else:
+            _check(lib().bn_sim_create(config.encode(), self.flag_bits, C.byref(h)))
+        self.h = h
+        self.info = self.get_info()
+        self.N = self.info.total_neurons
+        self.NE = self.info.num_excitatory_neurons
+        self.NI = self.info.num_inhibitory_neurons
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bn_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_info(self) -> SimInfo:
+        info = SimInfo()
+        _check(lib().bn_sim_get_info(self.h, C.byref(info)))
+        return info
+
+    def run(self, n: int) -> None:
+        _check(lib().bn_sim_run(self.h, n))
+
+    def step_components(self, n: int) -> None:
+        _check(lib().bn_sim_step_components(self.h, n))
+
+    def synchronize(self) -> None:
+        _check(lib().bn_sim_synchronize(self.h))
+
+    def run_timed(self, n: int) -> float:
+        ms = C.c_float(0)
+        _check(lib().bn_sim_run_timed(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def time_propagate_kernel(self, n: int):
+        k, s = C.c_float(0), C.c_float(0)
+        _check(lib().bn_sim_time_propagate_kernel(self.h, n, C.byref(k), C.byref(s)))
+        return k.value, s.value
+
+    def state(self) -> dict:
+        arrays = {n: np.zeros(self.N, dt) for n, dt in _STATE_FIELDS}
+        st = NeuronStateHost(*[arrays[n].ctypes.data for n, _ in _STATE_FIELDS])
+        _check(lib().bn_sim_get_neuron_state(self.h, C.byref(st)))
+        return arrays
+
+    def weights(self, idx: int) -> np.ndarray:
+        out = np.zeros((self.info.weight_pre[idx], self.info.weight_post[idx]), np.float32)
+        _check(lib().bn_sim_get_weights(self.h, idx, _fp(out)))
+        return out
+
+    def connection_traces(self, idx: int):
+        n = self.info.weight_pre[idx]
+        r, d, x = (np.zeros(n, np.float32) for _ in range(3))
+        _check(lib().bn_sim_get_connection_traces(self.h, idx, _fp(r), _fp(d), _fp(x)))
+        return r, d, x
+
+    def plasticity_traces(self, off: int):
+        a = [np.zeros(self.N, np.float32) for _ in range(4)]
+        _check(lib().bn_sim_get_plasticity_traces(self.h, off, *[_fp(x) for x in a]))
+        return a

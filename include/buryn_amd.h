@@ -1,0 +1,361 @@
+/*
+ * buryn_amd.h — C ABI of the MI355X-native per-step engine for the conductance-based EIF
+ * spiking network of paulazoo/example-cuda-spiking-nn ("buryn").
+ *
+ * Everything here is plain C: POD structs, plain pointers, sizes, and an opaque HIP stream
+ * handle.  No C++ or torch type crosses this boundary, and no exception either: every entry
+ * point returns 0 on success or a nonzero code, and bn_last_error() returns the message.
+ * The C++ host layer (csrc/host) rethrows these as std::runtime_error("<what>: <msg>"),
+ * the same convention as the reference's cuda_check (cpp_src/gpu_core/cuda_utils.h:10-20).
+ *
+ * Layer map (reference → here):
+ *   kernel_*_launch(XView)            → bn_*(XView, bn_stream_t)          (drop-in launchers)
+ *   GpuSimulationState + components   → csrc/host C++ classes (same names) + bn_sim_* below
+ *   System::step_the_clock / run      → bn_sim_run (fused, graph-captured step) and
+ *                                       bn_sim_step_components (component-by-component step)
+ *
+ * Field names of the *View structs follow the reference exactly, including the misspelled
+ * ConnectionView::pre_group_is_inhibtory.  Fields marked "extension" are additions this
+ * engine needs (row stride, scratch for the deterministic GEMV, counter-based RNG clock).
+ */
+#ifndef BURYN_AMD_H
+#define BURYN_AMD_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* hipStream_t is `struct ihipStream_t*`; this alias keeps the header free of HIP includes. */
+typedef struct ihipStream_t* bn_stream_t;
+
+/* ----------------------------------------------------------------------------------------
+ * Error handling
+ * ------------------------------------------------------------------------------------- */
+enum {
+    BN_OK = 0,
+    BN_ERR_INVALID_ARGUMENT = 1001, /* std::invalid_argument in the reference */
+    BN_ERR_OUT_OF_RANGE = 1002,     /* std::out_of_range in the reference */
+    BN_ERR_RUNTIME = 1003,          /* std::runtime_error in the reference */
+    BN_ERR_HIP = 1004               /* a HIP runtime call failed; message carries hipGetErrorString */
+};
+/* Message of the last failing call on this thread ("<what>: <detail>"). */
+const char* bn_last_error(void);
+
+/* ----------------------------------------------------------------------------------------
+ * Device clock (replaces the per-neuron curandStatePhilox4_32_10_t array,
+ * reference: cpp_src/gpu_core/gpu_simulation_state.cu:50, kernel_init_neuron_rng_states.cu:6-26)
+ *
+ * The background noise is drawn from a stateless Philox4x32-10 keyed by rng_key (the run's
+ * rng_seed) with counter (clock_step, stream, global neuron id, instance).  The clock lives
+ * in device memory so a whole step can be captured in a hipGraph and replayed: kernels read
+ * clock_step, the step-tail kernel advances it.  next_event_idx is the stimulus cursor that
+ * the reference keeps on the host (stimulus.h:42-44).
+ * ------------------------------------------------------------------------------------- */
+typedef struct bn_device_clock {
+    uint32_t clock_step;      /* index of the step being executed */
+    uint32_t next_event_idx;  /* first stimulus event not yet applied */
+    uint32_t rng_key[2];      /* Philox key: low and high word of (uint64)rng_seed */
+} bn_device_clock;
+
+/* Philox counter word 1 ("stream") values. */
+enum { BN_RNG_STREAM_BACKGROUND = 0 };
+
+/* ----------------------------------------------------------------------------------------
+ * Views — reference cpp_src/<component>/<name>_view.h and <name>_params.h
+ * ------------------------------------------------------------------------------------- */
+
+/* reference: cpp_src/neuron_groups/neuron_group_view.h:8-49 */
+typedef struct NeuronGroupView {
+    float timestep;
+    float g_leak;
+    float membrane_capacitance;
+    float depolarization_slope_factor;
+    float v_effective_spike_threshold;
+    float v_excitatory_reversal;
+    float v_inhibitory_reversal;
+    float v_threshold;
+    float v_reset;
+    float v_rest;
+    uint32_t refractory_steps;
+    bool is_inhibitory;
+    float mean_background_excitatory_spikes;
+    float mean_background_inhibitory_spikes;
+    float background_excitatory_postsynaptic_r_trace_decay;
+    float background_excitatory_postsynaptic_d_trace_decay;
+    float background_inhibitory_postsynaptic_r_trace_decay;
+    float background_inhibitory_postsynaptic_d_trace_decay;
+    float background_excitatory_scaling_factor;
+    float background_inhibitory_scaling_factor;
+
+    uint32_t num_neurons;
+    uint32_t offset;
+
+    float* membrane_potentials;
+    float* g_excitatory;
+    float* g_inhibitory;
+    float* i_stimulus;
+    uint32_t* refractory_counter;
+    uint32_t* spikes;
+    float* background_g_excitatory_synaptic_conductances;
+    float* background_g_inhibitory_synaptic_conductances;
+    float* background_excitatory_postsynaptic_r_traces;
+    float* background_excitatory_postsynaptic_d_traces;
+    float* background_inhibitory_postsynaptic_r_traces;
+    float* background_inhibitory_postsynaptic_d_traces;
+    /* extension: replaces `void* neuron_rng_states` (counter-based RNG, see bn_device_clock) */
+    const bn_device_clock* rng_clock;
+    /* extension: Poisson inverse-CDF tables built by bn_poisson_table() (length 0 = mean 0) */
+    const uint32_t* poisson_table_excitatory;
+    const uint32_t* poisson_table_inhibitory;
+    uint32_t poisson_table_excitatory_len;
+    uint32_t poisson_table_inhibitory_len;
+    /* extension: Philox counter word 3; 0 for a single network, the instance id when batched */
+    uint32_t rng_instance;
+} NeuronGroupView;
+
+/* reference: cpp_src/connections/connection_view.h:8-31 (cublas_handle dropped) */
+typedef struct ConnectionView {
+    float postsynaptic_r_trace_decay;
+    float postsynaptic_d_trace_decay;
+    float postsynaptic_kernel_normalization_value;
+    bool pre_group_is_inhibtory; /* sic — reference field name */
+
+    uint32_t weight_matrix_idx;
+    uint32_t pre_group_offset;
+    uint32_t post_group_offset;
+    uint32_t pre_group_num_neurons;
+    uint32_t post_group_num_neurons;
+
+    float* weight_matrix;  /* row-major [pre][post], row stride weight_matrix_ld */
+    float* postsynaptic_r_traces;
+    float* postsynaptic_d_traces;
+    float* summed_postsynaptic_conductances;
+    float* post_group_g_excitatory;
+    float* post_group_g_inhibitory;
+    uint32_t* spikes;
+    /* extension: row stride in floats (0 = post_group_num_neurons, the reference layout).
+     * Must be a multiple of 4 with 16-byte aligned rows for the vectorised GEMV, else the
+     * launcher falls back to a scalar-load GEMV. */
+    uint32_t weight_matrix_ld;
+    /* extension: deterministic GEMV scratch, ceil(pre/gemv_row_block) * ld doubles */
+    double* gemv_partials;
+    uint32_t gemv_row_block;
+} ConnectionView;
+
+/* reference: cpp_src/stimuli/stimulus_view.h:8-16 */
+typedef struct StimulusView {
+    float* g_excitatory;
+    float* g_inhibitory;
+    float* i_stimulus;
+    uint32_t* timestep_event_count;
+    uint32_t* stimulus_event_neuron_ids;
+    uint32_t* stimulus_event_post_group_offsets;
+    float* stimulus_event_amplitude_changes;
+    /* extension: lengths of the two tables above, used by the device-side cursor */
+    uint32_t num_timesteps;
+    uint32_t num_events;
+} StimulusView;
+
+/* reference: cpp_src/plasticities/plasticity_view.h:8-38 */
+typedef struct PlasticityView {
+    float ltp_A;
+    float ltd_A;
+    float ltp_B;
+    float ltd_B;
+    float pre_trace_decay;
+    float post_trace_decay;
+    float triplet_pre_trace_decay;
+    float triplet_post_trace_decay;
+    float weight_min;
+    float weight_max;
+
+    uint32_t weight_matrix_idx;
+    uint32_t plasticity_offset;
+    uint32_t pre_group_offset;
+    uint32_t post_group_offset;
+    uint32_t pre_group_num_neurons;
+    uint32_t post_group_num_neurons;
+
+    float* pre_traces;
+    float* post_traces;
+    float* triplet_pre_traces;
+    float* triplet_post_traces;
+    uint32_t* pre_spikes;
+    uint32_t* post_spikes;
+    float* weight_matrix;
+    float* pre_weight_sums;
+    /* extension: row stride of weight_matrix in floats (0 = post_group_num_neurons) */
+    uint32_t weight_matrix_ld;
+} PlasticityView;
+
+/* reference: cpp_src/plasticities/plasticity_params.h:7-20 */
+typedef struct PlasticityParams {
+    float ltp_A;
+    float ltd_A;
+    float ltp_B;
+    float ltd_B;
+    float pre_trace_decay;
+    float post_trace_decay;
+    float triplet_pre_trace_decay;
+    float triplet_post_trace_decay;
+    float weight_min;
+    float weight_max;
+    uint32_t plasticity_offset;
+} PlasticityParams;
+
+/* reference: cpp_src/monitors/spike_recorder_view.h:8-16 */
+typedef struct SpikeRecorderView {
+    uint32_t num_neurons;
+    uint32_t offset;
+    uint32_t* spikes;
+    uint32_t* spike_history;
+} SpikeRecorderView;
+
+/* reference: cpp_src/monitors/voltage_recorder_view.h:8-16 */
+typedef struct VoltageRecorderView {
+    uint32_t num_neurons;
+    uint32_t offset;
+    float* membrane_potentials;
+    float* membrane_potential_history;
+} VoltageRecorderView;
+
+/* ----------------------------------------------------------------------------------------
+ * Drop-in launchers.  Same work and order as the reference launchers; they are asynchronous
+ * on `stream` (the reference ends each one in cudaDeviceSynchronize).
+ * ------------------------------------------------------------------------------------- */
+
+/* Builds the inverse-CDF table of Poisson(mean) used by the background noise: entry k is
+ * floor(2^32 * P(X <= k)); a 32-bit uniform u maps to min{k : u < table[k]} (or len).
+ * *len = 0 when mean <= 0 (no draw, as the reference skips curand_poisson).
+ * Supported means: 0 <= mean <= 64. */
+int bn_poisson_table(float mean, uint32_t* table, uint32_t capacity, uint32_t* len);
+
+/* Replaces kernel_neuron_group_evolve_launch (neuron_groups/kernel_neuron_group_evolve.h:10):
+ * background shot-noise conductances then the EIF step, fused in one kernel. */
+int bn_neuron_group_evolve(NeuronGroupView view, bn_stream_t stream);
+
+/* Replaces kernel_connection_propagate_launch (connections/kernel_connection_propagate.h:10):
+ * per-pre r/d traces, x = d - r, then g_post += W^T x with a hand-written GEMV
+ * (FP64 accumulation of exact FP32 products: per column, rows are summed sequentially inside
+ * chunks of gemv_row_block rows, then the chunk sums are added to g in chunk order). */
+int bn_connection_propagate(ConnectionView view, bn_stream_t stream);
+
+/* Replaces kernel_stimulus_apply_launch (stimuli/kernel_stimulus_apply.h:8-10). */
+int bn_stimulus_apply(StimulusView view, uint32_t next_event_idx, uint32_t num_events_to_apply,
+                      bn_stream_t stream);
+/* Same with the cursor on the device: applies the events of clock->clock_step and advances
+ * clock->next_event_idx (graph-capturable). */
+int bn_stimulus_apply_device_cursor(StimulusView view, bn_device_clock* clock, bn_stream_t stream);
+
+/* Replace kernel_{triplet,istdp,stdp}_plasticity_update*_launch
+ * (plasticities/kernel_triplet_plasticity_update.h:9-10, kernel_istdp_plasticity_update.h:9,
+ *  kernel_stdp_plasticity_update.h:9). */
+int bn_triplet_plasticity_update(PlasticityView view, PlasticityParams params, bn_stream_t stream);
+int bn_triplet_plasticity_update_normalize(PlasticityView view, PlasticityParams params,
+                                           bn_stream_t stream);
+int bn_istdp_plasticity_update(PlasticityView view, PlasticityParams params, bn_stream_t stream);
+int bn_stdp_plasticity_update(PlasticityView view, PlasticityParams params, bn_stream_t stream);
+
+/* Replace kernel_historicize_{spikes,membrane_potentials}_launch
+ * (monitors/kernel_historicize_spikes.h:7-12, kernel_historicize_membrane_potentials.h:7-12).
+ * Layout as the reference: history[neuron * recording_interval_steps + step % interval]. */
+int bn_historicize_spikes(const uint32_t* d_spikes, uint32_t* spikes_history, uint32_t total_neurons,
+                          uint32_t recording_interval_steps, uint32_t clock_step, bn_stream_t stream);
+int bn_historicize_membrane_potentials(const float* d_membrane_potentials,
+                                       float* membrane_potential_history, uint32_t total_neurons,
+                                       uint32_t recording_interval_steps, uint32_t clock_step,
+                                       bn_stream_t stream);
+
+/* Advances clock->clock_step by one (the last node of a captured step). */
+int bn_clock_advance(bn_device_clock* clock, bn_stream_t stream);
+
+/* ----------------------------------------------------------------------------------------
+ * Whole-simulation API (what the reference's main_buryn.cpp + System::run provide).
+ * ------------------------------------------------------------------------------------- */
+typedef struct bn_sim bn_sim;
+
+enum {
+    BN_SIM_WRITE_FILES = 1u << 0,       /* write locations/weights/schedules/recordings like the
+                                           reference (output_directory is cleaned first) */
+    BN_SIM_NO_MONITORS = 1u << 1,       /* no spike/voltage/weight recorders */
+    BN_SIM_LAZY_NORMALIZE = 1u << 2,    /* triplet normalisation kept as per-row scale factors
+                                           (W = s_i * M) instead of rewriting W_EE every step */
+    BN_SIM_COMPONENT_PATH = 1u << 3,    /* step through the component launchers (drop-in path) */
+    BN_SIM_NO_GRAPH = 1u << 4,          /* launch the fused step eagerly instead of a hipGraph */
+    BN_SIM_QUIET = 1u << 5              /* no progress prints */
+};
+
+typedef struct bn_sim_info {
+    uint32_t num_excitatory_neurons;
+    uint32_t num_inhibitory_neurons;
+    uint32_t total_neurons;
+    uint32_t clock_step;             /* steps executed so far */
+    uint32_t simulation_steps;
+    uint32_t num_plasticities;
+    uint32_t weight_matrix_ld[4];    /* EE, EI, IE, II row strides (floats) */
+    uint32_t gemv_row_block[4];      /* rows per GEMV partial chunk (FP64 summation order) */
+    uint32_t weight_pre[4];
+    uint32_t weight_post[4];
+    uint32_t flags;
+    uint32_t kernels_per_step;       /* launches in one fused step */
+    uint64_t device_bytes;           /* device memory held by the simulation */
+} bn_sim_info;
+
+/* Host pointers to fill; any may be NULL.  Arrays are indexed by global neuron id (length
+ * total_neurons).  g_* are the conductances after the last step's propagate. */
+typedef struct bn_neuron_state_host {
+    float* membrane_potentials;
+    float* g_excitatory;
+    float* g_inhibitory;
+    float* i_stimulus;
+    uint32_t* refractory_counter;
+    uint32_t* spikes;
+    float* background_g_excitatory_synaptic_conductances;
+    float* background_g_inhibitory_synaptic_conductances;
+    float* background_excitatory_postsynaptic_r_traces;
+    float* background_excitatory_postsynaptic_d_traces;
+    float* background_inhibitory_postsynaptic_r_traces;
+    float* background_inhibitory_postsynaptic_d_traces;
+} bn_neuron_state_host;
+
+int bn_sim_create(const char* config_path, uint32_t flags, bn_sim** out);
+int bn_sim_create_from_json_text(const char* json_text, uint32_t flags, bn_sim** out);
+void bn_sim_destroy(bn_sim* sim);
+/* Runs `num_steps` steps with the reference's System::run semantics (record() at
+ * step % recording_interval_steps == 0 when monitors are on). Asynchronous unless monitors
+ * record. */
+int bn_sim_run(bn_sim* sim, uint32_t num_steps);
+/* Same, through System::step_the_clock and the component launchers. */
+int bn_sim_step_components(bn_sim* sim, uint32_t num_steps);
+int bn_sim_synchronize(bn_sim* sim);
+int bn_sim_get_info(bn_sim* sim, bn_sim_info* info);
+int bn_sim_get_neuron_state(bn_sim* sim, const bn_neuron_state_host* dst);
+/* True weights of matrix idx (0 EE, 1 EI, 2 IE, 3 II) as dense row-major [pre][post]. */
+int bn_sim_get_weights(bn_sim* sim, uint32_t weight_matrix_idx, float* dst);
+/* Per-pre connection traces of connection idx: r, d, x (length pre); any may be NULL. */
+int bn_sim_get_connection_traces(bn_sim* sim, uint32_t idx, float* r, float* d, float* x);
+/* Plasticity traces (length total_neurons each, global ids); any may be NULL. */
+int bn_sim_get_plasticity_traces(bn_sim* sim, uint32_t plasticity_offset, float* pre, float* post,
+                                 float* triplet_pre, float* triplet_post);
+/* Times `num_steps` steps between two HIP events on the simulation stream; *ms = elapsed. */
+int bn_sim_run_timed(bn_sim* sim, uint32_t num_steps, float* ms);
+/* Runs `num_steps` steps launched eagerly with HIP events around the propagate (GEMV) kernel
+ * of every step; *kernel_ms = summed kernel time, *step_ms = summed wall time of the steps. */
+int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_ms, float* step_ms);
+/* Writes the JSON of the parsed config (all fields, after defaults/overrides/prefixing). */
+int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, size_t* needed);
+
+/* Host-side inputs (reference cpp_src/generation/generate_weights.cpp): dense row-major [pre][post] matrix of
+ * matrix idx for the given config, without writing files. */
+int bn_generate_weight_matrix(const char* config_path, uint32_t weight_matrix_idx, float* dst,
+                              uint32_t* pre, uint32_t* post);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BURYN_AMD_H */

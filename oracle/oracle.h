@@ -1,0 +1,115 @@
+/*
+ * ORACLE — test infrastructure only.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker / CPU baseline; the product
+ * never links or calls it.
+ *
+ * A CPU (C++/OpenMP) restatement of the reference's per-step semantics
+ * (cpp_src/core/system.cpp:43-63 order: evolve -> propagate -> stimulus -> plasticity -> monitors)
+ * with the engine's documented choices where the reference is unpinned (cuRAND, cuBLAS):
+ *   - background noise: Philox4x32-10 counter (step, 0, global id, instance), key = rng_seed,
+ *     Poisson by inversion of a floor(2^32 * CDF) table;
+ *   - expf: Cody–Waite reduction + degree-7 Taylor (same algorithm as the kernels);
+ *   - GEMV: FP64 accumulation of exact FP32 products, rows summed sequentially inside chunks of
+ *     gemv_row_block rows, chunk sums added to (double)g in chunk order, one rounding to FP32;
+ *   - triplet row normalisation sum: 256 interleaved FP64 partials, xor butterfly inside each
+ *     64-lane group (32,16,8,4,2,1), the four group sums added in order.
+ *   - lazy normalisation (lazy_normalize = 1): the triplet matrix is kept as W = diag(s) M.  A
+ *     touched entry is updated in true units, w' = clamp(s_i*M_ij + delta), M_ij = w'/s_i (rows
+ *     with s_i = 0 stay untouched); R_i = sum_j M_ij is kept in FP64: rows whose pre neuron
+ *     spiked (LTD) are re-summed in the normalisation order above, other rows add the LTP
+ *     deltas (double)M'_ij - (double)M_ij in increasing post order (all rows are re-summed when
+ *     more than lazy_kmax posts spiked); then s_i = fl32(target_i / R_i) when R_i > 0.
+ * precision 32 reproduces the engine bit for bit; precision 64 runs every quantity in FP64
+ * (libm exp, plain sequential sums) as an independent accuracy reference.
+ *
+ * Parity pins (tests/test_oracle_*.py): Philox known-answer vectors of the published method and
+ * rocRAND's engine; expf against libm/FP64 within 2 ulp; Poisson table against scipy's CDF;
+ * traces against the closed form K(d^k - r^k); EIF single step against an FP64 hand
+ * computation; GEMV against FP64 numpy; plasticity deltas for isolated pre/post pairs; row
+ * normalisation against numpy column sums; connectivity against the reference generator.
+ */
+#ifndef BURYN_ORACLE_H
+#define BURYN_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_group_params {
+    float mean_background_excitatory_spikes;
+    float mean_background_inhibitory_spikes;
+    float background_excitatory_postsynaptic_r_trace_decay;
+    float background_excitatory_postsynaptic_d_trace_decay;
+    float background_inhibitory_postsynaptic_r_trace_decay;
+    float background_inhibitory_postsynaptic_d_trace_decay;
+    float background_excitatory_scaling_factor;
+    float background_inhibitory_scaling_factor;
+} oracle_group_params;
+
+typedef struct oracle_params {
+    uint32_t num_excitatory_neurons;
+    uint32_t num_inhibitory_neurons;
+    int32_t rng_seed;
+    uint32_t rng_instance;
+    float timestep;
+    float g_leak;
+    float membrane_capacitance;
+    float depolarization_slope_factor;
+    float v_effective_spike_threshold;
+    float v_excitatory_reversal;
+    float v_inhibitory_reversal;
+    float v_threshold;
+    float v_reset;
+    float v_rest;
+    uint32_t refractory_steps;
+    oracle_group_params group[2]; /* 0 = E, 1 = I */
+    float tau_decay_e, tau_rise_e, tau_decay_i, tau_rise_i;
+    float postsynaptic_kernel_normalization_value;
+    uint32_t e_e_plasticity_enabled;
+    float e_e_ltp_A, e_e_ltd_A, e_e_ltp_B, e_e_ltd_B;
+    float e_e_tau_ltp, e_e_tau_ltd, e_e_triplet_tau_ltp, e_e_triplet_tau_ltd;
+    float e_e_weight_min, e_e_weight_max;
+    uint32_t i_e_plasticity_enabled;
+    float i_e_ltp_A, i_e_ltd_A, i_e_ltd_B, i_e_tau_ltp, i_e_tau_ltd, i_e_weight_min, i_e_weight_max;
+    uint32_t gemv_row_block[4];   /* EE, EI, IE, II */
+    uint32_t lazy_normalize;      /* 1: W_EE = diag(s) M with per-row scale factors */
+    uint32_t lazy_kmax;           /* spiking-post capacity of the LTP row-sum delta buffer */
+} oracle_params;
+
+typedef struct oracle_sim oracle_sim;
+
+/* Weights: dense row-major [pre][post] (EE N_E x N_E, EI N_E x N_I, IE N_I x N_E, II N_I x N_I).
+ * Stimulus arrays as produced by the schedule generator (may be NULL when num_events == 0 and
+ * stim_counts == NULL). precision: 32 or 64. */
+oracle_sim* oracle_create(const oracle_params* p, int precision, const float* w_ee, const float* w_ei,
+                          const float* w_ie, const float* w_ii, const uint32_t* stim_counts,
+                          uint32_t num_timesteps, const uint32_t* stim_ids, const uint32_t* stim_offsets,
+                          const float* stim_amps, uint32_t num_events);
+void oracle_destroy(oracle_sim* s);
+/* Runs n steps; returns 0, or 1 when the stimulus cursor runs past the event tables. */
+int oracle_step(oracle_sim* s, uint32_t n);
+uint32_t oracle_clock_step(const oracle_sim* s);
+/* Global-id arrays of length N (any pointer may be NULL). g_* are after the last propagate. */
+void oracle_get_state(const oracle_sim* s, float* v, float* ge, float* gi, float* istim, uint32_t* refr,
+                      uint32_t* spikes, float* bg_ge, float* bg_gi);
+void oracle_get_state64(const oracle_sim* s, double* v, double* ge, double* gi);
+/* True weights (lazy mode: s_i * M), dense [pre][post]. */
+void oracle_get_weights(const oracle_sim* s, uint32_t idx, float* dst);
+void oracle_get_connection_traces(const oracle_sim* s, uint32_t idx, float* r, float* d, float* x);
+/* plasticity_offset 0/1 in enable order; arrays of length N (global ids). */
+void oracle_get_plasticity_traces(const oracle_sim* s, uint32_t plasticity_offset, float* pre, float* post,
+                                  float* trip_pre, float* trip_post);
+
+/* Primitives (for the pinning tests). */
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+float oracle_expf(float x);
+/* Returns the table length (0 for mean <= 0); fills table[0..len). */
+uint32_t oracle_poisson_table(float mean, uint32_t* table, uint32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

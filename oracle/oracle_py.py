@@ -1,0 +1,223 @@
+"""ORACLE binding — test infrastructure only.
+
+ctypes wrapper of oracle/liboracle.so (the CPU restatement of the reference step, see
+oracle/oracle.h).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+and only as the checker / CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class GroupParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "mean_background_excitatory_spikes", "mean_background_inhibitory_spikes",
+        "background_excitatory_postsynaptic_r_trace_decay", "background_excitatory_postsynaptic_d_trace_decay",
+        "background_inhibitory_postsynaptic_r_trace_decay", "background_inhibitory_postsynaptic_d_trace_decay",
+        "background_excitatory_scaling_factor", "background_inhibitory_scaling_factor")]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("num_excitatory_neurons", C.c_uint32), ("num_inhibitory_neurons", C.c_uint32),
+        ("rng_seed", C.c_int32), ("rng_instance", C.c_uint32),
+        ("timestep", C.c_float), ("g_leak", C.c_float), ("membrane_capacitance", C.c_float),
+        ("depolarization_slope_factor", C.c_float), ("v_effective_spike_threshold", C.c_float),
+        ("v_excitatory_reversal", C.c_float), ("v_inhibitory_reversal", C.c_float),
+        ("v_threshold", C.c_float), ("v_reset", C.c_float), ("v_rest", C.c_float),
+        ("refractory_steps", C.c_uint32), ("group", GroupParams * 2),
+        ("tau_decay_e", C.c_float), ("tau_rise_e", C.c_float), ("tau_decay_i", C.c_float), ("tau_rise_i", C.c_float),
+        ("postsynaptic_kernel_normalization_value", C.c_float),
+        ("e_e_plasticity_enabled", C.c_uint32),
+        ("e_e_ltp_A", C.c_float), ("e_e_ltd_A", C.c_float), ("e_e_ltp_B", C.c_float), ("e_e_ltd_B", C.c_float),
+        ("e_e_tau_ltp", C.c_float), ("e_e_tau_ltd", C.c_float), ("e_e_triplet_tau_ltp", C.c_float),
+        ("e_e_triplet_tau_ltd", C.c_float), ("e_e_weight_min", C.c_float), ("e_e_weight_max", C.c_float),
+        ("i_e_plasticity_enabled", C.c_uint32),
+        ("i_e_ltp_A", C.c_float), ("i_e_ltd_A", C.c_float), ("i_e_ltd_B", C.c_float), ("i_e_tau_ltp", C.c_float),
+        ("i_e_tau_ltd", C.c_float), ("i_e_weight_min", C.c_float), ("i_e_weight_max", C.c_float),
+        ("gemv_row_block", C.c_uint32 * 4), ("lazy_normalize", C.c_uint32), ("lazy_kmax", C.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.run(["make", "-C", HERE, "liboracle.so"], check=True, capture_output=True)
+        L = C.CDLL(LIB_PATH)
+        P, U, F, I = C.c_void_p, C.c_uint32, C.c_float, C.c_int
+        fp = C.POINTER(C.c_float)
+        up = C.POINTER(C.c_uint32)
+        L.oracle_create.restype = P
+        L.oracle_create.argtypes = [C.POINTER(Params), I, fp, fp, fp, fp, up, U, up, up, fp, U]
+        L.oracle_destroy.argtypes = [P]
+        L.oracle_step.argtypes = [P, U]
+        L.oracle_step.restype = I
+        L.oracle_clock_step.argtypes = [P]
+        L.oracle_clock_step.restype = U
+        L.oracle_get_state.argtypes = [P, fp, fp, fp, fp, up, up, fp, fp]
+        dp = C.POINTER(C.c_double)
+        L.oracle_get_state64.argtypes = [P, dp, dp, dp]
+        L.oracle_get_weights.argtypes = [P, U, fp]
+        L.oracle_get_connection_traces.argtypes = [P, U, fp, fp, fp]
+        L.oracle_get_plasticity_traces.argtypes = [P, U, fp, fp, fp, fp]
+        L.oracle_philox4x32_10.argtypes = [up, up, up]
+        L.oracle_expf.argtypes = [F]
+        L.oracle_expf.restype = F
+        L.oracle_poisson_table.argtypes = [F, up, U]
+        L.oracle_poisson_table.restype = U
+        _lib = L
+    return _lib
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _up(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=False, lazy_kmax=256,
+                       rng_instance=0) -> Params:
+    """Builds oracle parameters from a parsed config (the engine's config dump)."""
+    p = Params()
+    p.num_excitatory_neurons = cfg["num_excitatory_neurons"]
+    p.num_inhibitory_neurons = cfg["num_inhibitory_neurons"]
+    p.rng_seed = cfg["rng_seed"]
+    p.rng_instance = rng_instance
+    for k in ("timestep", "g_leak", "membrane_capacitance", "depolarization_slope_factor",
+              "v_effective_spike_threshold", "v_excitatory_reversal", "v_inhibitory_reversal",
+              "v_threshold", "v_reset", "v_rest", "tau_decay_e", "tau_rise_e", "tau_decay_i", "tau_rise_i",
+              "postsynaptic_kernel_normalization_value"):
+        setattr(p, k, cfg[k])
+    p.refractory_steps = cfg["refractory_steps"]
+    for gi, g in enumerate(("e", "i")):
+        for f, _ in GroupParams._fields_:
+            setattr(p.group[gi], f, cfg[f"group_{g}_{f}"])
+    p.e_e_plasticity_enabled = int(cfg["e_e_plasticity_enabled"])
+    p.i_e_plasticity_enabled = int(cfg["i_e_plasticity_enabled"])
+    for a, b in (("e_e_ltp_A", "e_e_plasticity_ltp_A"), ("e_e_ltd_A", "e_e_plasticity_ltd_A"),
+                 ("e_e_ltp_B", "e_e_plasticity_ltp_B"), ("e_e_ltd_B", "e_e_plasticity_ltd_B"),
+                 ("e_e_tau_ltp", "e_e_plasticity_tau_ltp"), ("e_e_tau_ltd", "e_e_plasticity_tau_ltd"),
+                 ("e_e_triplet_tau_ltp", "e_e_plasticity_triplet_tau_ltp"),
+                 ("e_e_triplet_tau_ltd", "e_e_plasticity_triplet_tau_ltd"),
+                 ("e_e_weight_min", "e_e_plasticity_weight_min"), ("e_e_weight_max", "e_e_plasticity_weight_max"),
+                 ("i_e_ltp_A", "i_e_plasticity_ltp_A"), ("i_e_ltd_A", "i_e_plasticity_ltd_A"),
+                 ("i_e_ltd_B", "i_e_plasticity_ltd_B"), ("i_e_tau_ltp", "i_e_plasticity_tau_ltp"),
+                 ("i_e_tau_ltd", "i_e_plasticity_tau_ltd"), ("i_e_weight_min", "i_e_plasticity_weight_min"),
+                 ("i_e_weight_max", "i_e_plasticity_weight_max")):
+        setattr(p, a, cfg[b])
+    for k in range(4):
+        p.gemv_row_block[k] = int(gemv_row_block[k])
+    p.lazy_normalize = int(lazy_normalize)
+    p.lazy_kmax = int(lazy_kmax)
+    return p
+
+
+class OracleSim:
+    """One network on the CPU oracle. Weights: list of 4 dense [pre][post] float32 arrays."""
+
+    def __init__(self, params: Params, weights, precision=32, stim=None):
+        self.p = params
+        self.NE = params.num_excitatory_neurons
+        self.NI = params.num_inhibitory_neurons
+        self.N = self.NE + self.NI
+        self._w = [np.ascontiguousarray(w, dtype=np.float32) for w in weights]
+        if stim is None:
+            counts = ids = offs = amps = None
+            nt = ne = 0
+        else:
+            counts, ids, offs, amps = [np.ascontiguousarray(a) for a in stim]
+            counts = counts.astype(np.uint32)
+            ids = ids.astype(np.uint32)
+            offs = offs.astype(np.uint32)
+            amps = amps.astype(np.float32)
+            nt, ne = len(counts), len(ids)
+            if ne == 0:
+                ids = offs = None
+                amps = None
+        self._stim = (counts, ids, offs, amps)
+        self.h = lib().oracle_create(C.byref(params), precision, *[_fp(w) for w in self._w],
+                                     _up(counts), nt, _up(ids), _up(offs), _fp(amps), ne)
+        self.precision = precision
+
+    def close(self):
+        if self.h:
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, n=1):
+        rc = lib().oracle_step(self.h, n)
+        if rc:
+            raise RuntimeError("Stimulus event indexing exceeded uploaded event buffers")
+
+    @property
+    def clock_step(self):
+        return lib().oracle_clock_step(self.h)
+
+    def state(self):
+        N = self.N
+        out = {k: np.zeros(N, np.float32) for k in ("V", "ge", "gi", "istim", "bg_ge", "bg_gi")}
+        refr = np.zeros(N, np.uint32)
+        spikes = np.zeros(N, np.uint32)
+        lib().oracle_get_state(self.h, _fp(out["V"]), _fp(out["ge"]), _fp(out["gi"]), _fp(out["istim"]),
+                               _up(refr), _up(spikes), _fp(out["bg_ge"]), _fp(out["bg_gi"]))
+        out["refr"] = refr
+        out["spikes"] = spikes
+        return out
+
+    def state64(self):
+        N = self.N
+        v, ge, gi = (np.zeros(N, np.float64) for _ in range(3))
+        dp = C.POINTER(C.c_double)
+        lib().oracle_get_state64(self.h, v.ctypes.data_as(dp), ge.ctypes.data_as(dp), gi.ctypes.data_as(dp))
+        return {"V": v, "ge": ge, "gi": gi}
+
+    def weights(self, idx):
+        pre, post = self._w[idx].shape
+        out = np.zeros((pre, post), np.float32)
+        lib().oracle_get_weights(self.h, idx, _fp(out))
+        return out
+
+    def connection_traces(self, idx):
+        n = self._w[idx].shape[0]
+        r, d, x = (np.zeros(n, np.float32) for _ in range(3))
+        lib().oracle_get_connection_traces(self.h, idx, _fp(r), _fp(d), _fp(x))
+        return r, d, x
+
+    def plasticity_traces(self, off):
+        a = [np.zeros(self.N, np.float32) for _ in range(4)]
+        lib().oracle_get_plasticity_traces(self.h, off, *[_fp(x) for x in a])
+        return a
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().oracle_philox4x32_10(_up(c), _up(k), _up(out))
+    return out
+
+
+def expf(x: float) -> float:
+    return lib().oracle_expf(x)
+
+
+def poisson_table(mean: float, cap=256):
+    t = np.zeros(cap, np.uint32)
+    n = lib().oracle_poisson_table(mean, _up(t), cap)
+    return t[:n]
