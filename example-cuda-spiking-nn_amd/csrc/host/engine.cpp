@@ -148,6 +148,7 @@ void GpuSimulationState::set_poisson_table(uint32_t group, uint32_t kind, float 
         BN_HIP_CHECK(hipMalloc(&p, len * sizeof(uint32_t)), "hipMalloc poisson table");
         BN_HIP_CHECK(hipMemcpyAsync(p, t.data(), len * sizeof(uint32_t), hipMemcpyHostToDevice, stream_),
                      "H2D poisson table");
+        BN_HIP_CHECK(hipStreamSynchronize(stream_), "poisson table sync");  // `t` is pageable and local
         poisson_[group][kind] = static_cast<uint32_t*>(p);
     }
 }

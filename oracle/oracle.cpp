@@ -349,7 +349,7 @@ struct Sim {
                     for (uint32_t j = j0; j < j1; ++j) part[j - j0] = 0.0;
                     for (uint32_t i = i0; i < i1; ++i) {
                         const double xi = (double)xs[i];
-                        if (xi == 0.0) continue;  // adds +0 in the engine: no effect
+                        if (xi == 0.0 && !p.dense_gemv) continue;  // adds +0 in the engine: no effect
                         const R* row = &C.W[(size_t)i * C.n_post];
                         for (uint32_t j = j0; j < j1; ++j) part[j - j0] += xi * (double)row[j];
                     }
@@ -358,7 +358,7 @@ struct Sim {
             } else {
                 for (uint32_t i = 0; i < C.n_pre; ++i) {
                     const double xi = (double)xs[i];
-                    if (xi == 0.0) continue;
+                    if (xi == 0.0 && !p.dense_gemv) continue;
                     const R* row = &C.W[(size_t)i * C.n_post];
                     for (uint32_t j = j0; j < j1; ++j) acc[j - j0] += xi * (double)row[j];
                 }

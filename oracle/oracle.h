@@ -76,6 +76,7 @@ typedef struct oracle_params {
     uint32_t gemv_row_block[4];   /* EE, EI, IE, II */
     uint32_t lazy_normalize;      /* 1: W_EE = diag(s) M with per-row scale factors */
     uint32_t lazy_kmax;           /* spiking-post capacity of the LTP row-sum delta buffer */
+    uint32_t dense_gemv;          /* 1: visit every pre row (no skip of x = 0 rows; same result) */
 } oracle_params;
 
 typedef struct oracle_sim oracle_sim;

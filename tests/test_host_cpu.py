@@ -1,0 +1,138 @@
+"""CPU-only tests of the engine library: C-ABI exports, config loader and connectivity/stimulus
+generation against the REFERENCE's own generator (golden digests in tests/golden), the
+engine's Poisson table against the oracle's, and the loader's error behaviour.
+No kernel is launched here."""
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from helpers import REPO, TESTS, engine, stimulus_schedule
+
+GOLDEN = json.load(open(os.path.join(TESTS, "golden", "hostgen_golden.json")))
+CFG_PATHS = {
+    "config01": os.path.join(TESTS, "golden", "configs", "config01.json"),
+    "defaults": os.path.join(TESTS, "golden", "configs", "defaults.json"),
+    "mixed_keys": os.path.join(TESTS, "golden", "configs", "mixed_keys.json"),
+    "small_offgrid": os.path.join(TESTS, "golden", "configs", "small_offgrid.json"),
+    "net3125_stdp": os.path.join(REPO, "configs", "net3125_stdp.json"),
+    "net10125_stdp": os.path.join(REPO, "configs", "net10125_stdp.json"),
+}
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(REPO, "include", "buryn_amd.h")).read()
+    declared = set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]*?\b(bn_\w+)\s*\(", header, re.M))
+    assert len(declared) >= 25
+    L = engine().lib()
+    missing = [s for s in sorted(declared) if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(engine().engine.EXPORTED_SYMBOLS) <= declared
+
+
+@pytest.mark.parametrize("name", sorted(CFG_PATHS))
+def test_config_loader_matches_reference(name):
+    got = engine().config_dump(CFG_PATHS[name])
+    ref = GOLDEN[name]["config"]
+    assert set(ref) <= set(got)
+    diff = {k: (got[k], ref[k]) for k in ref if got[k] != ref[k]}
+    assert not diff
+
+
+@pytest.mark.parametrize("name", ["config01", "defaults", "mixed_keys", "small_offgrid", "net3125_stdp",
+                                  "net10125_stdp"])
+def test_connectivity_matches_reference_generator(name):
+    eng = engine()
+    for k in range(4):
+        w = eng.generate_weight_matrix(CFG_PATHS[name], k)
+        ref = GOLDEN[name]["weights"][k]
+        assert list(w.shape) == ref["shape"]
+        assert sha((w != 0).astype(np.uint8)) == ref["mask_sha256"], f"matrix {k} connection mask"
+        assert int((w != 0).sum()) == ref["nnz"]
+        if name.startswith("net"):
+            # uniform(0, w_max) weight values (extension, second RNG stream) on the same mask
+            wmax = [0.5e-9, 1e-9, 2e-9, 2e-9][k]
+            nz = w[w != 0]
+            assert nz.max() <= np.float32(wmax) and abs(nz.mean() / wmax - 0.5) < 0.02
+        else:
+            assert sha(w) == ref["sha256"], f"matrix {k} values"
+
+
+def test_offgrid_distances_follow_the_reference_build_flags():
+    # With non-integer spacing the reference's -march=native build (FMA in the squared distance)
+    # and a build without contraction disagree; the engine reproduces the former.
+    g = GOLDEN["small_offgrid"]
+    assert [m["sha256"] for m in g["weights"]] != g["weights_nofma"]
+    w = engine().generate_weight_matrix(CFG_PATHS["small_offgrid"], 0)
+    assert sha(w) == g["weights"][0]["sha256"]
+
+
+@pytest.mark.parametrize("name", ["config01", "defaults", "mixed_keys", "small_offgrid"])
+def test_stimulus_schedule_restatement_matches_reference(name):
+    cfg = engine().config_dump(CFG_PATHS[name])
+    counts, ids, offs, amps = stimulus_schedule(cfg)
+    ref = GOLDEN[name]["stimulus"]
+    assert counts.size == ref["num_timesteps"] and sha(counts) == ref["counts_sha256"]
+    assert ids.size == ref["num_events"]
+    assert ids.tolist()[:2000] == ref["neuron_ids"]
+    assert offs.tolist()[:2000] == ref["post_group_offsets"]
+    assert np.array_equal(amps[:2000], np.array(ref["amplitudes"], np.float32))
+
+
+@pytest.mark.parametrize("lam", [0.02, 0.1, 1.0, 2.5, 17.0, 64.0])
+def test_poisson_table_engine_equals_oracle(lam):
+    assert np.array_equal(engine().poisson_table(lam), O.poisson_table(lam))
+
+
+def test_poisson_table_rejects_large_means():
+    with pytest.raises(engine().BnError, match="above 64"):
+        engine().poisson_table(65.0)
+    assert engine().poisson_table(0.0).size == 0
+
+
+def _write(tmp_path, obj, name="c.json"):
+    p = os.path.join(tmp_path, name)
+    with open(p, "w") as f:
+        f.write(obj if isinstance(obj, str) else json.dumps(obj))
+    return p
+
+
+@pytest.mark.parametrize("cfg,msg", [
+    ("{not json", "JSON parse error"),
+    ("[1, 2]", "must be an object"),
+    ({"stimulus_selected_input_ids": [1, 2]}, "size must match"),
+    ({"stimulus_selected_input_ids": [1, -2]}, "must not contain negative values"),
+    ({"stimulus_start_clock_steps": [1.5]}, "must contain only integers"),
+    ({"voltage_recording_neuron_ids": "x"}, "must be an array"),
+    ({"num_inhibitory_neurons": 0}, "Neuron counts must be > 0"),
+    ({"membrane_capacitance": -1e-12}, "membrane_capacitance must be > 0"),
+    ({"e_e_plasticity_enabled": 1}, "type must be boolean"),
+    ({"timestep": "0.1"}, "type must be number"),
+])
+def test_config_errors_like_reference(tmp_path, cfg, msg):
+    with pytest.raises(engine().BnError, match=msg):
+        engine().config_dump(_write(tmp_path, cfg))
+
+
+def test_config_missing_file():
+    with pytest.raises(engine().BnError, match="Failed to open config file"):
+        engine().config_dump("/nonexistent/config.json")
+
+
+def test_output_path_prefixing_quirk(tmp_path):
+    # Relative paths are prefixed with output_directory unless their first component equals
+    # the output directory's first component (config.cpp:46-66).
+    p = _write(tmp_path, {"output_directory": "../out/run1", "e_spike_recording_file": "rec/spikes.txt",
+                          "i_spike_recording_file": "../elsewhere/i.txt", "voltage_recording_file": "/abs/v.txt"})
+    c = engine().config_dump(p)
+    assert c["e_spike_recording_file"] == "../out/run1/rec/spikes.txt"
+    assert c["i_spike_recording_file"] == "../elsewhere/i.txt"
+    assert c["voltage_recording_file"] == "/abs/v.txt"
