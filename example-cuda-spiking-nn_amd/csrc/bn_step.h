@@ -1,23 +1,20 @@
 // Internal (C++) interface between the host engine and the fused-step kernels.
 //
 // A step of the whole network is a fixed sequence of launches, each described by a plan
-// struct passed by value as the kernel argument (so the sequence can be captured once in a
-// hipGraph and replayed: nothing in a plan changes between steps; the step index lives in
-// the device clock).
+// struct passed by value as the kernel argument, so the sequence is captured once in a
+// hipGraph and replayed: nothing in a plan changes between steps (the step index lives in the
+// device clock).
 //
-//   faithful step (reference-exact triplet normalisation):
-//     k_evolve      every neuron: finalise g from last step's GEMV partials, EIF step,
-//                   background shot noise, connection pre-traces, plasticity trace decays,
-//                   spike bits and monitor rings
-//     k_gemv        every connection: FP64 partial sums of W^T x per row chunk
-//     k_ltp         LTP column walks for spiking post neurons
-//     k_ltd         LTD row walks for spiking pre neurons
-//     k_norm_tail   triplet row normalisation, plasticity trace increments, stimulus
-//                   events of this step, clock advance
-//
-//   lazy step (W_EE = diag(s) * M; normalisation by per-row scale factors):
-//     k_evolve -> k_fused_ee (GEMV + LTP/LTD of the lazily normalised matrix, fused in one
-//     pass) together with k_gemv for the other connections -> k_rowscale_tail
+//   k_evolve       every neuron: finalise g from the last step's GEMV partials, EIF step,
+//                  background shot noise, connection pre-traces, plasticity trace decay (with
+//                  the previous step's spike increments folded in), spike bits, monitor rings
+//   k_gemv         every connection: FP64 partial sums of W^T x per row chunk
+//   k_plasticity   LTP/LTD of every plasticity, organised by pre row (a thread per row for
+//                  rows whose pre neuron did not spike, a block per spiking row); the lazily
+//                  normalised matrix also updates its row sums and scale factors here; the
+//                  last block applies this step's stimulus events and advances the clock
+//   k_norm_tail    (reference-exact normalisation only) triplet row normalisation, then the
+//                  stimulus and clock advance instead of k_plasticity
 #pragma once
 
 #include <stdint.h>
@@ -32,7 +29,7 @@ constexpr int kMaxSources = 2;
 constexpr int kMaxPlast = 4;
 constexpr int kMaxTraceSets = 4;
 constexpr int kGemvTileCols = 256;   // 64 lanes x float4
-constexpr int kNormThreads = 256;    // threads per row in the normalisation sum
+constexpr int kNormThreads = 256;    // threads per row in the row sums
 constexpr int kNormRegs = 64;        // columns per thread kept in registers (n_post <= 16384)
 
 enum PlastKind : uint32_t { kPlastTriplet = 0, kPlastIstdp = 1, kPlastStdp = 2 };
@@ -82,7 +79,6 @@ struct EvolvePlan {
     float* v_ring;               // [interval][n_vrec] or nullptr
     const int32_t* vrec_index;   // [total_neurons] slot in v_ring or -1
     uint32_t n_vrec, interval, words;
-    uint32_t skip_plast_decay;   // 1: plasticity trace decays are done elsewhere
 };
 
 struct GemvConn {
@@ -112,28 +108,22 @@ struct PlastDev {
     float* trip_pre;
     float* trip_post;
     const float* target;          // pre_weight_sums (triplet normalisation targets)
-    // lazy normalisation (W = diag(s) M): only for a normalising triplet plasticity
+    const uint32_t* bits;         // spike bits of this step (global ids)
+    // lazy normalisation (W = diag(s) M) of a normalising triplet plasticity
     uint32_t lazy;
-    uint32_t kmax;                // capacity of `delta` in spiking posts
     float* s;                     // [n_pre] row scale factors
     double* R;                    // [n_pre] FP64 row sums of M
-    double* delta;                // [kmax][n_pre] LTP deltas of M, by rank of the spiking post
-    const uint32_t* bits;         // spike bits of this step (global ids)
 };
 
 struct PlastPlan {
     PlastDev p[kMaxPlast];
     uint32_t n;
-    uint32_t items_post;  // sum of n_post (LTP work items)
-    uint32_t items_pre;   // sum of n_pre (LTD work items)
+    uint32_t items_post;  // sum of n_post
+    uint32_t items_pre;   // sum of n_pre
 };
 
-struct TailPlan {
-    PlastPlan pl;
-    uint32_t norm_rows;          // rows normalised (sum over normalising plasticities)
-    uint32_t total_neurons;
-    uint32_t increments;         // 1: apply plasticity trace increments here
-    // stimulus (device cursor)
+// Stimulus events of the step being executed (device cursor) and the clock advance.
+struct StimTail {
     float* i_stimulus;           // global
     const uint32_t* timestep_event_count;
     const uint32_t* event_ids;
@@ -142,22 +132,38 @@ struct TailPlan {
     uint32_t num_timesteps, num_events;
     bn_device_clock* clock;
     uint32_t* status;            // device error word (1 = stimulus cursor overflow)
-    uint32_t advance_clock;
+    uint32_t active;             // 0: nothing to do in this kernel
+};
+
+struct PlastStepPlan {
+    PlastPlan pl;
+    uint32_t row_blocks;         // blocks running one thread per row
+    uint32_t spiking_blocks;     // blocks sharing the rows whose pre neuron spiked
+    StimTail tail;
+};
+
+struct TailPlan {
+    PlastPlan pl;
+    uint32_t norm_rows;          // rows normalised (sum over normalising plasticities)
+    uint32_t total_neurons;
+    uint32_t increments;         // 1: apply plasticity trace increments here
+    StimTail tail;
 };
 
 // Launchers used by the host engine (return hipError_t as int).
 int launch_evolve(const EvolvePlan& plan, bn_stream_t s);
 int launch_gemv(const GemvPlan& plan, bn_stream_t s);
+int launch_plasticity(const PlastStepPlan& plan, bn_stream_t s);
 int launch_ltp(const PlastPlan& plan, bn_stream_t s);
 int launch_ltd(const PlastPlan& plan, bn_stream_t s);
 int launch_norm_tail(const TailPlan& plan, bn_stream_t s);
-// Lazy normalisation: LTD rows of the lazy matrix, row-sum bookkeeping and scale factors.
-int launch_lazy_rows(const PlastPlan& plan, bn_stream_t s);
 // R_i = canonical row sum of W, s_i = 1 (set-up of the lazy normalisation).
 int launch_lazy_init(const PlastDev& d, bn_stream_t s);
 // g[j] = fl32((double)g[j] + sum_p partials[p][j]) for every source, in order.
 int launch_materialize_g(float* g_dst, const float* g_base, const FinalizeSource* src,
                          uint32_t n_src, uint32_t n, bn_stream_t s);
+// dst[i] = src[i] + (spikes[i] == 1 ? 1 : 0): plasticity traces with this step's increments.
+int launch_add_increments(float* dst, const float* src, const uint32_t* spikes, uint32_t n, bn_stream_t s);
 int launch_fill_u32(uint32_t* p, uint32_t value, size_t n, bn_stream_t s);
 
 }  // namespace bn

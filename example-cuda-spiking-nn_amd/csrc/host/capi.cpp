@@ -437,7 +437,28 @@ int bn_sim_get_plasticity_traces(bn_sim* sim, uint32_t off, float* pre, float* p
         d2h(post, S.d_post_traces() + base, N, s);
         d2h(tpre, S.d_triplet_pre_traces() + base, N, s);
         d2h(tpost, S.d_triplet_post_traces() + base, N, s);
+        std::vector<uint32_t> spikes(N);
+        d2h(spikes.data(), S.d_spikes(), N, s);
         BN_HIP_CHECK(hipStreamSynchronize(s), "trace download sync");
+        if (sim->system->fused()) {
+            // The fused step keeps the +1 of a spike for the next step's decay; present the
+            // traces as the reference holds them after the step (trace + 1 where spiked).
+            for (const auto& p : sim->system->plasticities()) {
+                const PlasticityView& v = p->gpu_view();
+                if (v.plasticity_offset != off) continue;
+                auto add = [&](float* a, uint32_t lo, uint32_t n) {
+                    if (!a) return;
+                    for (uint32_t g = lo; g < lo + n; ++g)
+                        if (spikes[g] == 1) a[g] = a[g] + 1.0f;
+                };
+                add(pre, v.pre_group_offset, v.pre_group_num_neurons);
+                add(post, v.post_group_offset, v.post_group_num_neurons);
+                if (p->kind() == kPlastTriplet) {
+                    add(tpre, v.pre_group_offset, v.pre_group_num_neurons);
+                    add(tpost, v.post_group_offset, v.post_group_num_neurons);
+                }
+            }
+        }
     });
 }
 

@@ -112,7 +112,6 @@ GpuSimulationState::GpuSimulationState(uint32_t total_simulation_length, uint32_
     if (opt_.lazy_normalize) {
         row_scale_ = alloc<float>(ne_, "row_scale");
         row_sum_ = alloc<double>(ne_, "row_sum");
-        ltp_delta_ = alloc<double>((size_t)opt_.lazy_kmax * ne_, "ltp_deltas");
     }
     bn_device_clock c{};
     BN_HIP_CHECK(hipMemcpyAsync(clock_, &c, sizeof c, hipMemcpyHostToDevice, stream_), "H2D clock");

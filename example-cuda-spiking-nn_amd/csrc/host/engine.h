@@ -32,7 +32,6 @@ struct StateOptions {
     bool fused_rings = false;          // spike-bit ring + voltage ring of recorded neurons
     std::vector<uint32_t> vrec_global_ids;
     bool lazy_normalize = false;
-    uint32_t lazy_kmax = 256;
     uint32_t recording_interval_steps = kRecordingIntervalSteps;
     std::array<uint32_t, 4> gemv_row_block = {0, 0, 0, 0};  // 0 = heuristic
 };
@@ -103,7 +102,6 @@ public:
     // lazy normalisation of the triplet matrix (plasticity_offset 0)
     float* d_row_scale() const { return row_scale_; }
     double* d_row_sum() const { return row_sum_; }
-    double* d_ltp_deltas() const { return ltp_delta_; }
 
     // Poisson tables per group (0 E, 1 I) and kind (0 exc, 1 inh)
     const uint32_t* d_poisson_table(uint32_t group, uint32_t kind) const { return poisson_[group][kind]; }
@@ -167,7 +165,6 @@ private:
     uint32_t words_ = 0;
     float* row_scale_ = nullptr;
     double* row_sum_ = nullptr;
-    double* ltp_delta_ = nullptr;
     bool lazy_active_ = false;
     uint32_t lazy_matrix_ = 0;
     uint32_t* poisson_[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -433,6 +430,7 @@ private:
     EvolvePlan evolve_{};
     GemvPlan gemv_{};
     PlastPlan plast_{};
+    PlastStepPlan plast_step_{};
     TailPlan tail_{};
     std::vector<int> conn_trace_set_;
     uint32_t kernels_per_step_ = 0;

@@ -176,13 +176,13 @@ void System::build_plans() {
             if (v.weight_matrix_idx != 0 || D.n_pre != S.num_excitatory_neurons())
                 throw Error(BN_ERR_INVALID_ARGUMENT, "lazy normalisation supports the E->E matrix");
             D.lazy = 1;
-            D.kmax = S.options().lazy_kmax;
             D.s = S.d_row_scale();
             D.R = S.d_row_sum();
-            D.delta = S.d_ltp_deltas();
             lazy_idx = (int)k;
             S.setup_lazy_normalization(v.weight_matrix_idx);
         }
+        if (D.n_post > 64u * kNormThreads)
+            throw Error(BN_ERR_INVALID_ARGUMENT, "post group larger than the plasticity kernel supports");
         plast_.items_post += D.n_post;
         plast_.items_pre += D.n_pre;
     }
@@ -219,7 +219,22 @@ void System::build_plans() {
     gemv_.n_conn = (uint32_t)connections_.size();
     gemv_.total_tiles = tiles;
 
-    // Tail: faithful normalisation rows, trace increments, stimulus, clock.
+    // Stimulus events + clock advance: in the plasticity kernel, or after the faithful
+    // normalisation when there is one.
+    StimTail st{};
+    if (!stimuli_.empty()) {
+        const StimulusView& sv = stimuli_[0]->gpu_view();
+        st.i_stimulus = sv.i_stimulus;
+        st.timestep_event_count = sv.timestep_event_count;
+        st.event_ids = sv.stimulus_event_neuron_ids;
+        st.event_offsets = sv.stimulus_event_post_group_offsets;
+        st.event_amplitudes = sv.stimulus_event_amplitude_changes;
+        st.num_timesteps = sv.num_timesteps;
+        st.num_events = sv.num_events;
+    }
+    st.clock = S.d_clock();
+    st.status = S.d_status();
+    st.active = 1;
     tail_.pl = plast_;
     tail_.norm_rows = 0;
     for (uint32_t k = 0; k < tail_.pl.n; ++k) {
@@ -228,22 +243,15 @@ void System::build_plans() {
         if (D.normalize) tail_.norm_rows += D.n_pre;
     }
     tail_.total_neurons = S.total_neurons();
-    tail_.increments = 1;
-    if (!stimuli_.empty()) {
-        const StimulusView& sv = stimuli_[0]->gpu_view();
-        tail_.i_stimulus = sv.i_stimulus;
-        tail_.timestep_event_count = sv.timestep_event_count;
-        tail_.event_ids = sv.stimulus_event_neuron_ids;
-        tail_.event_offsets = sv.stimulus_event_post_group_offsets;
-        tail_.event_amplitudes = sv.stimulus_event_amplitude_changes;
-        tail_.num_timesteps = sv.num_timesteps;
-        tail_.num_events = sv.num_events;
-    }
-    tail_.clock = S.d_clock();
-    tail_.status = S.d_status();
-    tail_.advance_clock = 1;
+    tail_.increments = 0;  // folded into the next step's trace decay (k_evolve)
+    tail_.tail = st;
+    plast_step_.pl = plast_;
+    plast_step_.row_blocks = (plast_.items_pre + 255) / 256;
+    plast_step_.spiking_blocks = std::max<uint32_t>(1, std::min<uint32_t>(256, plast_.items_pre));
+    plast_step_.tail = st;
+    plast_step_.tail.active = tail_.norm_rows == 0 ? 1 : 0;
 
-    kernels_per_step_ = 1 + (gemv_.total_tiles ? 1 : 0) + (plast_.n ? 2 : 0) + (lazy_idx >= 0 ? 1 : 0) + 1;
+    kernels_per_step_ = 1 + (gemv_.total_tiles ? 1 : 0) + 1 + (tail_.norm_rows ? 1 : 0);
     if (lazy_idx >= 0) BN_HIP_CHECK((hipError_t)launch_lazy_init(plast_.p[lazy_idx], S.stream()), "lazy normalisation init");
 }
 
@@ -253,12 +261,8 @@ void System::enqueue_fused_step(hipEvent_t gemv_begin, hipEvent_t gemv_end) {
     if (gemv_begin) BN_HIP_CHECK(hipEventRecord(gemv_begin, s), "event record");
     BN_HIP_CHECK((hipError_t)launch_gemv(gemv_, s), "fused gemv launch");
     if (gemv_end) BN_HIP_CHECK(hipEventRecord(gemv_end, s), "event record");
-    if (plast_.n) {
-        BN_HIP_CHECK((hipError_t)launch_ltp(plast_, s), "fused ltp launch");
-        BN_HIP_CHECK((hipError_t)launch_ltd(plast_, s), "fused ltd launch");
-        BN_HIP_CHECK((hipError_t)launch_lazy_rows(plast_, s), "fused lazy rows launch");
-    }
-    BN_HIP_CHECK((hipError_t)launch_norm_tail(tail_, s), "fused tail launch");
+    BN_HIP_CHECK((hipError_t)launch_plasticity(plast_step_, s), "fused plasticity launch");
+    if (tail_.norm_rows) BN_HIP_CHECK((hipError_t)launch_norm_tail(tail_, s), "fused normalisation launch");
 }
 
 void System::enable_fused(bool use_graph, uint32_t steps_per_graph) {

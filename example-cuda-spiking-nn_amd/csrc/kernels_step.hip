@@ -12,14 +12,12 @@
 //   monitors              cpp_src/monitors/kernel_historicize_{spikes,membrane_potentials}.cu
 #include <hip/hip_runtime.h>
 
-#include <stdio.h>
-
 #include <algorithm>
 #include <string>
 
 #include "bn_device.h"
-#include "bn_step.h"
 #include "bn_errors.h"
+#include "bn_step.h"
 
 namespace bn {
 
@@ -120,14 +118,21 @@ __device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, ui
     return (float)acc;
 }
 
+// Plasticity trace: the previous step's +1 increment (kernel_triplet_plasticity_update.cu:95-115)
+// then this step's decay with flush (:16-50), the same two roundings as the reference.
+__device__ __forceinline__ float trace_step(float tr, uint32_t spiked_prev, float decay) {
+    if (spiked_prev == 1) tr += 1.0f;
+    return flush_trace(tr * decay);
+}
+
 // =======================================================================================
-// Fused evolve (every neuron of every group in one launch)
+// k_evolve: every neuron of every group in one launch
 // =======================================================================================
 __global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const bn_device_clock clk = *P.clock;
     const uint32_t step = clk.clock_step;
-    uint32_t spike = 0;
+    uint32_t spike = 0, spike_prev = 0;
     float V = 0.0f;
     if (gid < P.total_neurons) {
         for (uint32_t gi = 0; gi < P.n_groups; ++gi) {
@@ -141,6 +146,7 @@ __global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
             for (uint32_t s = 0; s < G.n_fin_i; ++s) gi_ = finalize_g(gi_, G.fin_i[s], lid);
             V = v.membrane_potentials[lid];
             uint32_t refr = v.refractory_counter[lid];
+            spike_prev = v.spikes[lid];
             eif_step(v, ge, gi_, v.i_stimulus[lid], V, refr, spike);
             v.membrane_potentials[lid] = V;
             v.refractory_counter[lid] = refr;
@@ -155,27 +161,25 @@ __global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
             if (gid < T.pre_offset || gid >= T.pre_offset + T.n_pre) continue;
             connection_trace_step(T.r, T.d, T.x, T.decay_r, T.decay_d, T.K, gid - T.pre_offset, spike);
         }
-        if (!P.skip_plast_decay) {
-            for (uint32_t q = 0; q < P.n_pt; ++q) {
-                const PlastTraces& Q = P.pt[q];
-                if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
-                    const uint32_t i = gid - Q.pre_offset;
-                    Q.pre[i] = flush_trace(Q.pre[i] * Q.pre_decay);
-                    if (Q.kind == kPlastTriplet) Q.trip_pre[i] = flush_trace(Q.trip_pre[i] * Q.trip_pre_decay);
-                }
-                if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
-                    const uint32_t j = gid - Q.post_offset;
-                    Q.post[j] = flush_trace(Q.post[j] * Q.post_decay);
-                    if (Q.kind == kPlastTriplet) Q.trip_post[j] = flush_trace(Q.trip_post[j] * Q.trip_post_decay);
-                }
+        for (uint32_t q = 0; q < P.n_pt; ++q) {
+            const PlastTraces& Q = P.pt[q];
+            if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
+                const uint32_t i = gid - Q.pre_offset;
+                Q.pre[i] = trace_step(Q.pre[i], spike_prev, Q.pre_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_pre[i] = trace_step(Q.trip_pre[i], spike_prev, Q.trip_pre_decay);
+            }
+            if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
+                const uint32_t j = gid - Q.post_offset;
+                Q.post[j] = trace_step(Q.post[j], spike_prev, Q.post_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_post[j] = trace_step(Q.trip_post[j], spike_prev, Q.trip_post_decay);
             }
         }
     }
-    // Spike bits of this step (64 global ids per wave → two 32-bit words).
+    // Spike bits of this step (64 global ids per wave -> two 32-bit words).
     const uint64_t ballot = __ballot(spike == 1);
     const uint32_t lane = threadIdx.x & 63;
     if (lane == 0 || lane == 32) {
-        const uint32_t word = (gid >> 5);
+        const uint32_t word = gid >> 5;
         if (word < P.words) {
             const uint32_t bits = lane == 0 ? (uint32_t)ballot : (uint32_t)(ballot >> 32);
             P.spike_bits[word] = bits;
@@ -189,10 +193,10 @@ __global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
 }
 
 // =======================================================================================
-// GEMV partials: one wave per tile of (row chunk, 256 columns).
+// k_gemv: FP64 partial sums, one wave per tile of (row chunk, 256 columns).
 //   partial[chunk][j] = sum_{i in chunk, increasing} (double)x'_i * (double)W[i][j]
-// Products of two binary32 numbers are exact in binary64, so the only rounding is the
-// FP64 accumulation in this fixed order, then one rounding to FP32 when g is finalised.
+// Products of two binary32 numbers are exact in binary64, so the only rounding is the FP64
+// accumulation in this fixed order, then one rounding to FP32 when g is finalised.
 // =======================================================================================
 __device__ __forceinline__ float row_x(const GemvConn& C, uint32_t i) {
     const float xi = C.x[i];
@@ -285,61 +289,46 @@ __global__ __launch_bounds__(256) void k_materialize_g(float* g_dst, const float
     g_dst[j] = g;
 }
 
+__global__ __launch_bounds__(256) void k_add_increments(float* dst, const float* src, const uint32_t* spikes,
+                                                        uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float t = src[i];
+    if (spikes[i] == 1) t += 1.0f;
+    dst[i] = t;
+}
+
 // =======================================================================================
-// Plasticity: LTP column walks, LTD row walks, row normalisation, trace increments
+// Plasticity
 // =======================================================================================
-__device__ __forceinline__ bool locate_item(const PlastPlan& P, uint32_t item, bool post,
-                                            uint32_t& pi, uint32_t& idx) {
-    for (uint32_t k = 0; k < P.n; ++k) {
-        const uint32_t n = post ? P.p[k].n_post : P.p[k].n_pre;
-        if (item < n) {
-            pi = k;
-            idx = item;
-            return true;
-        }
-        item -= n;
-    }
-    return false;
+
+// LTP increment of entry (i, j): kernel_triplet_plasticity_update.cu:64-66,
+// kernel_istdp_plasticity_update.cu:55, kernel_stdp_plasticity_update.cu:55.
+__device__ __forceinline__ float ltp_factor(const PlastDev& D, uint32_t j) {
+    return D.kind == kPlastTriplet ? D.p.ltp_A + (D.p.ltp_B * D.trip_post[j]) : D.p.ltp_A;
+}
+__device__ __forceinline__ float ltp_delta(const PlastDev& D, uint32_t i, float fac_j) {
+    return D.kind == kPlastTriplet ? D.pre[i] * fac_j : D.p.ltp_A * D.pre[i];
+}
+// LTD increment of entry (i, j): kernel_triplet_plasticity_update.cu:85-87,
+// kernel_istdp_plasticity_update.cu:77, kernel_stdp_plasticity_update.cu:77.
+__device__ __forceinline__ float ltd_factor(const PlastDev& D, uint32_t i) {
+    return D.kind == kPlastTriplet ? D.p.ltd_A + (D.p.ltd_B * D.trip_pre[i]) : 0.0f;
+}
+__device__ __forceinline__ float ltd_delta(const PlastDev& D, uint32_t j, float fac_i) {
+    if (D.kind == kPlastTriplet) return -1.0f * D.post[j] * fac_i;
+    if (D.kind == kPlastIstdp) return -1.0f * D.p.ltd_A * (D.post[j] - D.p.ltd_B);
+    return -1.0f * D.p.ltd_A * D.post[j];
+}
+// One update of a stored value m in true units (lazy: w = s*m, stored back as w'/s).
+__device__ __forceinline__ float apply_delta(const PlastDev& D, float m, float si, float delta) {
+    if (D.lazy) return clamp_weight(si * m + delta, D.p.weight_min, D.p.weight_max) / si;
+    return clamp_weight(m + delta, D.p.weight_min, D.p.weight_max);
 }
 
-
-// ---------------------------------------------------------------------------------------
-// Lazy normalisation of the triplet matrix: W_true = s_i * M.
-// ---------------------------------------------------------------------------------------
-// Number of set bits in global bit range [lo, hi) (uniform across the block).
-__device__ __forceinline__ uint32_t count_bits(const uint32_t* bits, uint32_t lo, uint32_t hi) {
-    uint32_t c = 0;
-    for (uint32_t g = lo; g < hi;) {
-        const uint32_t w = g >> 5, b = g & 31;
-        const uint32_t n = min(32u - b, hi - g);
-        const uint32_t mask = n == 32 ? 0xFFFFFFFFu : (((1u << n) - 1u) << b);
-        c += __popc(bits[w] & mask);
-        g += n;
-    }
-    return c;
-}
-
-// LTP on column j of the lazy matrix; the change of each M entry goes to delta[rank][i].
-__device__ void ltp_lazy_column(const PlastDev& D, uint32_t j) {
-    const uint32_t rank = count_bits(D.bits, D.post_offset, D.post_offset + j);
-    const bool keep = rank < D.kmax;
-    const float fac = D.p.ltp_A + (D.p.ltp_B * D.trip_post[j]);
-    for (uint32_t i = threadIdx.x; i < D.n_pre; i += blockDim.x) {
-        double dm = 0.0;
-        const float si = D.s[i];
-        if (i != j && si != 0.0f) {
-            float* mp = D.W + (size_t)i * D.ld + j;
-            const float m = *mp;
-            const float w = clamp_weight(si * m + D.pre[i] * fac, D.p.weight_min, D.p.weight_max);
-            const float mn = w / si;
-            *mp = mn;
-            dm = (double)mn - (double)m;
-        }
-        if (keep) D.delta[(size_t)rank * D.n_pre + i] = dm;
-    }
-}
-
-// Canonical row sum (same order as normalize_row) of a row held in registers.
+// Canonical row sum: thread t of 256 sums columns t, t+256, ... in FP64, xor butterfly inside
+// each wave (32,16,8,4,2,1), the four wave sums added in wave order.  `vals` holds the
+// thread's columns.
 __device__ __forceinline__ double block_row_sum(const float* vals, uint32_t n, double* lds4) {
     const uint32_t t = threadIdx.x;
     double acc = 0.0;
@@ -355,11 +344,47 @@ __device__ __forceinline__ double block_row_sum(const float* vals, uint32_t n, d
     return total;
 }
 
-// Block per row: LTD of a spiking pre row (in true units), then the canonical re-sum.
-__device__ void lazy_resum_row(const PlastDev& D, uint32_t i, bool ltd, double* lds4) {
+// Row i whose pre neuron did NOT spike: only the LTP entries of this step's spiking posts
+// (increasing post order, from the spike bits).  Lazy: R_i += sum of the changes of M.
+__device__ void plasticity_quiet_row(const PlastDev& D, uint32_t i) {
+    const bool lazy = D.lazy != 0;
+    const float si = lazy ? D.s[i] : 1.0f;
+    double dsum = 0.0;
+    if (!lazy || si != 0.0f) {
+        float* row = D.W + (size_t)i * D.ld;
+        const uint32_t g0 = D.post_offset, g1 = D.post_offset + D.n_post;
+        for (uint32_t w = g0 >> 5; w <= (g1 - 1) >> 5; ++w) {
+            uint32_t bits = D.bits[w];
+            if (w == (g0 >> 5)) bits &= 0xFFFFFFFFu << (g0 & 31);
+            if (w == ((g1 - 1) >> 5)) bits &= 0xFFFFFFFFu >> (31 - ((g1 - 1) & 31));
+            while (bits) {
+                const uint32_t b = (uint32_t)__ffs(bits) - 1;
+                bits &= bits - 1;
+                const uint32_t j = w * 32 + b - g0;
+                if (j == i) continue;
+                const float m = row[j];
+                const float mn = apply_delta(D, m, si, ltp_delta(D, i, ltp_factor(D, j)));
+                row[j] = mn;
+                if (lazy) dsum += (double)mn - (double)m;
+            }
+        }
+    }
+    if (lazy) {
+        const double R = D.R[i] + dsum;
+        D.R[i] = R;
+        if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
+    }
+}
+
+// Row i whose pre neuron spiked, by the whole block: per entry the LTP increment (if the post
+// neuron spiked) then the LTD increment, as the reference's LTP-then-LTD kernel order.
+// Lazy: the row's sum of M is recomputed in the canonical order.
+__device__ void plasticity_spiking_row(const PlastDev& D, uint32_t i, double* lds4) {
+    const bool lazy = D.lazy != 0;
+    const float si = lazy ? D.s[i] : 1.0f;
+    const bool update = !lazy || si != 0.0f;
+    const float fac_i = ltd_factor(D, i);
     float* row = D.W + (size_t)i * D.ld;
-    const float si = D.s[i];
-    const float fac = D.p.ltd_A + (D.p.ltd_B * D.trip_pre[i]);
     float vals[kNormRegs];
 #pragma unroll
     for (int k = 0; k < kNormRegs; ++k) {
@@ -367,141 +392,138 @@ __device__ void lazy_resum_row(const PlastDev& D, uint32_t i, bool ltd, double* 
         vals[k] = 0.0f;
         if (j < D.n_post) {
             float m = row[j];
-            if (ltd && si != 0.0f && j != i) {
-                const float w = clamp_weight(si * m + -1.0f * D.post[j] * fac, D.p.weight_min, D.p.weight_max);
-                m = w / si;
+            if (update && j != i) {
+                if (D.post_spikes[j] == 1) m = apply_delta(D, m, si, ltp_delta(D, i, ltp_factor(D, j)));
+                m = apply_delta(D, m, si, ltd_delta(D, j, fac_i));
                 row[j] = m;
             }
             vals[k] = m;
         }
     }
-    const double R = block_row_sum(vals, D.n_post, lds4);
-    if (threadIdx.x == 0) {
-        D.R[i] = R;
-        if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
-    }
-}
-
-// Blocks [0, nt): one thread per row adds the LTP deltas (rows whose pre did not spike).
-// Blocks [nt, grid): grid-stride over rows, whole block per spiking row (LTD + re-sum), or
-// over every row when more than kmax posts spiked.
-__global__ __launch_bounds__(256) void k_lazy_rows(PlastPlan P, uint32_t nt) {
-    __shared__ double lds4[4];
-    for (uint32_t k = 0; k < P.n; ++k) {
-        const PlastDev& D = P.p[k];
-        if (!D.lazy) continue;
-        const uint32_t K = count_bits(D.bits, D.post_offset, D.post_offset + D.n_post);
-        const bool overflow = K > D.kmax;
-        if (blockIdx.x < nt) {
-            const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-            if (overflow || i >= D.n_pre || D.pre_spikes[i] == 1) continue;
-            double dsum = 0.0;
-            for (uint32_t r = 0; r < K; ++r) dsum += D.delta[(size_t)r * D.n_pre + i];
-            const double R = D.R[i] + dsum;
+    if (lazy) {
+        const double R = block_row_sum(vals, D.n_post, lds4);
+        if (threadIdx.x == 0) {
             D.R[i] = R;
             if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
-        } else {
-            for (uint32_t i = blockIdx.x - nt; i < D.n_pre; i += gridDim.x - nt) {
-                const bool spiked = D.pre_spikes[i] == 1;
-                if (spiked || overflow) lazy_resum_row(D, i, spiked, lds4);
-            }
         }
     }
 }
 
-__global__ __launch_bounds__(256) void k_lazy_init(PlastDev D) {
-    __shared__ double lds4[4];
-    const uint32_t i = blockIdx.x;
-    if (i >= D.n_pre) return;
-    if (threadIdx.x == 0) D.s[i] = 1.0f;
-    __syncthreads();
-    float vals[kNormRegs];
-    const float* row = D.W + (size_t)i * D.ld;
-#pragma unroll
-    for (int k = 0; k < kNormRegs; ++k) {
-        const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
-        vals[k] = j < D.n_post ? row[j] : 0.0f;
+// Stimulus events of clock_step in list order (stimulus.cpp:46-60), then the clock advance.
+// One thread: the event count per step is small, and sequential order makes the sum
+// independent of scheduling (the reference uses atomicAdd).
+__device__ void stimulus_and_clock(const StimTail& T) {
+    bn_device_clock* clk = T.clock;
+    const uint32_t s = clk->clock_step;
+    if (T.timestep_event_count && s < T.num_timesteps) {
+        const uint32_t n = T.timestep_event_count[s];
+        const uint32_t cur = clk->next_event_idx;
+        if ((uint64_t)cur + n > T.num_events) {
+            *T.status = 1u;
+        } else {
+            for (uint32_t e = cur; e < cur + n; ++e)
+                T.i_stimulus[T.event_ids[e] + T.event_offsets[e]] += T.event_amplitudes[e];
+            clk->next_event_idx = cur + n;
+        }
     }
-    const double R = block_row_sum(vals, D.n_post, lds4);
-    if (threadIdx.x == 0) D.R[i] = R;
+    clk->clock_step = s + 1;
 }
 
-// kernel_triplet_plasticity_update.cu:54-72, kernel_istdp_plasticity_update.cu:42-60,
-// kernel_stdp_plasticity_update.cu:42-60 (index-based self skip kept).
+__global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
+    __shared__ double lds4[4];
+    const uint32_t b = blockIdx.x;
+    if (b < P.row_blocks) {
+        uint32_t r = b * blockDim.x + threadIdx.x;
+        for (uint32_t k = 0; k < P.pl.n; ++k) {
+            const PlastDev& D = P.pl.p[k];
+            if (r < D.n_pre) {
+                if (D.pre_spikes[r] != 1) plasticity_quiet_row(D, r);
+                break;
+            }
+            r -= D.n_pre;
+        }
+    } else if (b < P.row_blocks + P.spiking_blocks) {
+        for (uint32_t r = b - P.row_blocks; r < P.pl.items_pre; r += P.spiking_blocks) {
+            uint32_t rr = r;
+            for (uint32_t k = 0; k < P.pl.n; ++k) {
+                const PlastDev& D = P.pl.p[k];
+                if (rr < D.n_pre) {
+                    if (D.pre_spikes[rr] == 1) plasticity_spiking_row(D, rr, lds4);
+                    break;
+                }
+                rr -= D.n_pre;
+            }
+        }
+    }
+    if (P.tail.active && b == gridDim.x - 1 && threadIdx.x == 0) stimulus_and_clock(P.tail);
+}
+
+__device__ __forceinline__ bool locate_item(const PlastPlan& P, uint32_t item, bool post,
+                                            uint32_t& pi, uint32_t& idx) {
+    for (uint32_t k = 0; k < P.n; ++k) {
+        const uint32_t n = post ? P.p[k].n_post : P.p[k].n_pre;
+        if (item < n) {
+            pi = k;
+            idx = item;
+            return true;
+        }
+        item -= n;
+    }
+    return false;
+}
+
+// Component-path LTP: one block per spiking post column, lanes over pre rows
+// (kernel_triplet_plasticity_update.cu:54-72 with the index-based self skip).
 __global__ __launch_bounds__(256) void k_ltp(PlastPlan P) {
     for (uint32_t item = blockIdx.x; item < P.items_post; item += gridDim.x) {
         uint32_t pi, j;
         if (!locate_item(P, item, true, pi, j)) return;
         const PlastDev& D = P.p[pi];
         if (D.post_spikes[j] != 1) continue;
-        if (D.lazy) {
-            ltp_lazy_column(D, j);
-            continue;
-        }
-        float fac = D.p.ltp_A;
-        if (D.kind == kPlastTriplet) fac = D.p.ltp_A + (D.p.ltp_B * D.trip_post[j]);
+        const float fac = ltp_factor(D, j);
         for (uint32_t i = threadIdx.x; i < D.n_pre; i += blockDim.x) {
             if (i == j) continue;
             float* w = D.W + (size_t)i * D.ld + j;
-            const float delta = D.kind == kPlastTriplet ? D.pre[i] * fac : D.p.ltp_A * D.pre[i];
-            *w = clamp_weight(*w + delta, D.p.weight_min, D.p.weight_max);
+            *w = clamp_weight(*w + ltp_delta(D, i, fac), D.p.weight_min, D.p.weight_max);
         }
     }
 }
 
-// kernel_triplet_plasticity_update.cu:74-93, kernel_istdp_plasticity_update.cu:62-81,
-// kernel_stdp_plasticity_update.cu:62-81.
+// Component-path LTD: one block per spiking pre row, lanes over post columns
+// (kernel_triplet_plasticity_update.cu:74-93).
 __global__ __launch_bounds__(256) void k_ltd(PlastPlan P) {
     for (uint32_t item = blockIdx.x; item < P.items_pre; item += gridDim.x) {
         uint32_t pi, i;
         if (!locate_item(P, item, false, pi, i)) return;
         const PlastDev& D = P.p[pi];
-        if (D.lazy || D.pre_spikes[i] != 1) continue;
-        const float fac = D.p.ltd_A + (D.p.ltd_B * (D.kind == kPlastTriplet ? D.trip_pre[i] : 0.0f));
+        if (D.pre_spikes[i] != 1) continue;
+        const float fac = ltd_factor(D, i);
         float* row = D.W + (size_t)i * D.ld;
         for (uint32_t j = threadIdx.x; j < D.n_post; j += blockDim.x) {
             if (i == j) continue;
-            float delta;
-            if (D.kind == kPlastTriplet) delta = -1.0f * D.post[j] * fac;
-            else if (D.kind == kPlastIstdp) delta = -1.0f * D.p.ltd_A * (D.post[j] - D.p.ltd_B);
-            else delta = -1.0f * D.p.ltd_A * D.post[j];
-            row[j] = clamp_weight(row[j] + delta, D.p.weight_min, D.p.weight_max);
+            row[j] = clamp_weight(row[j] + ltd_delta(D, j, fac), D.p.weight_min, D.p.weight_max);
         }
     }
 }
 
-// Row sum of the normalisation in a fixed order: thread t (of kNormThreads) sums columns
-// t, t+256, ... sequentially in FP64; each wave combines its 64 lanes by an xor butterfly
-// (offsets 32,16,8,4,2,1); the four wave sums are added in wave order.
-// Then as kernel_triplet_plasticity_update.cu:117-145: skip if sum <= 0, else scale the row
-// by target/sum (target = pre_weight_sums[i], the initial column sum: triplet_plasticity.cpp:22-30).
-
+// Triplet row normalisation (kernel_triplet_plasticity_update.cu:117-145) with the canonical
+// FP64 row sum: skip if sum <= 0, else scale the row by target/sum (target = pre_weight_sums[i],
+// the initial column sum, triplet_plasticity.cpp:22-30).
 __device__ void normalize_row(const PlastDev& D, uint32_t i, double* lds4) {
     float* row = D.W + (size_t)i * D.ld;
-    const uint32_t t = threadIdx.x;
     float vals[kNormRegs];
-    double acc = 0.0;
 #pragma unroll
     for (int k = 0; k < kNormRegs; ++k) {
-        const uint32_t j = t + (uint32_t)k * kNormThreads;
-        vals[k] = 0.0f;
-        if (j < D.n_post) {
-            vals[k] = row[j];
-            acc += (double)vals[k];
-        }
+        const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
+        vals[k] = j < D.n_post ? row[j] : 0.0f;
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if ((t & 63) == 0) lds4[t >> 6] = acc;
-    __syncthreads();
-    const double total = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
-    __syncthreads();
+    const double total = block_row_sum(vals, D.n_post, lds4);
     const float current_sum = (float)total;
     if (current_sum <= 0.0f) return;
     const float scale = D.target[i] / current_sum;
 #pragma unroll
     for (int k = 0; k < kNormRegs; ++k) {
-        const uint32_t j = t + (uint32_t)k * kNormThreads;
+        const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
         if (j < D.n_post) row[j] = vals[k] * scale;
     }
 }
@@ -527,29 +549,9 @@ __device__ __forceinline__ void increment_traces(const PlastPlan& P, uint32_t gi
     }
 }
 
-// Stimulus events of clock_step applied in list order (stimulus.cpp:46-60), then the clock
-// advances.  One thread: the event count per step is small and sequential order keeps the
-// result independent of scheduling (the reference uses atomicAdd).
-__device__ void stimulus_and_clock(const TailPlan& T) {
-    bn_device_clock* clk = T.clock;
-    const uint32_t s = clk->clock_step;
-    if (T.timestep_event_count && s < T.num_timesteps) {
-        const uint32_t n = T.timestep_event_count[s];
-        const uint32_t cur = clk->next_event_idx;
-        if ((uint64_t)cur + n > T.num_events) {
-            *T.status = 1u;
-        } else {
-            for (uint32_t e = cur; e < cur + n; ++e)
-                T.i_stimulus[T.event_ids[e] + T.event_offsets[e]] += T.event_amplitudes[e];
-            clk->next_event_idx = cur + n;
-        }
-    }
-    if (T.advance_clock) clk->clock_step = s + 1;
-}
-
-// Blocks [0, norm_rows): one row each.  Blocks after: trace increments for 256 neurons each.
-// The last block also applies the stimulus and advances the clock (no other block reads the
-// clock in this kernel).
+// Blocks [0, norm_rows): one row each.  Blocks after: trace increments for 256 neurons each
+// (component path).  The last block also applies the stimulus and advances the clock when the
+// tail is active (no other block reads the clock in this kernel).
 __global__ __launch_bounds__(256) void k_norm_tail(TailPlan T) {
     __shared__ double lds4[4];
     const uint32_t b = blockIdx.x;
@@ -560,15 +562,33 @@ __global__ __launch_bounds__(256) void k_norm_tail(TailPlan T) {
             if (!D.normalize) continue;
             if (rem < D.n_pre) {
                 normalize_row(D, rem, lds4);
-                return;
+                break;
             }
             rem -= D.n_pre;
         }
-        return;
+    } else {
+        const uint32_t gid = (b - T.norm_rows) * blockDim.x + threadIdx.x;
+        if (T.increments && gid < T.total_neurons) increment_traces(T.pl, gid);
     }
-    const uint32_t gid = (b - T.norm_rows) * blockDim.x + threadIdx.x;
-    if (T.increments && gid < T.total_neurons) increment_traces(T.pl, gid);
-    if (b == gridDim.x - 1 && threadIdx.x == 0) stimulus_and_clock(T);
+    if (T.tail.active && b == gridDim.x - 1 && threadIdx.x == 0) stimulus_and_clock(T.tail);
+}
+
+__global__ __launch_bounds__(256) void k_lazy_init(PlastDev D) {
+    __shared__ double lds4[4];
+    const uint32_t i = blockIdx.x;
+    if (i >= D.n_pre) return;
+    float vals[kNormRegs];
+    const float* row = D.W + (size_t)i * D.ld;
+#pragma unroll
+    for (int k = 0; k < kNormRegs; ++k) {
+        const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
+        vals[k] = j < D.n_post ? row[j] : 0.0f;
+    }
+    const double R = block_row_sum(vals, D.n_post, lds4);
+    if (threadIdx.x == 0) {
+        D.R[i] = R;
+        D.s[i] = 1.0f;
+    }
 }
 
 // =======================================================================================
@@ -629,16 +649,13 @@ __global__ void k_stimulus_apply(StimulusView v, uint32_t first, uint32_t n) {
             v.stimulus_event_amplitude_changes[e];
 }
 
-__global__ void k_stimulus_device_cursor(StimulusView v, bn_device_clock* clk, uint32_t* status) {
+__global__ void k_stimulus_device_cursor(StimulusView v, bn_device_clock* clk) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const uint32_t s = clk->clock_step;
     if (s >= v.num_timesteps) return;
     const uint32_t n = v.timestep_event_count[s];
     const uint32_t cur = clk->next_event_idx;
-    if ((uint64_t)cur + n > v.num_events) {
-        if (status) *status = 1u;
-        return;
-    }
+    if ((uint64_t)cur + n > v.num_events) return;
     for (uint32_t e = cur; e < cur + n; ++e)
         v.i_stimulus[v.stimulus_event_neuron_ids[e] + v.stimulus_event_post_group_offsets[e]] +=
             v.stimulus_event_amplitude_changes[e];
@@ -692,6 +709,14 @@ int launch_gemv(const GemvPlan& plan, bn_stream_t s) {
     return (int)hipGetLastError();
 }
 
+int launch_plasticity(const PlastStepPlan& plan, bn_stream_t s) {
+    static_assert(sizeof(PlastStepPlan) < 3800, "kernel argument too large");
+    const uint32_t blocks = plan.row_blocks + plan.spiking_blocks;
+    if (blocks == 0) return 0;
+    hipLaunchKernelGGL(k_plasticity, dim3(blocks), dim3(256), 0, s, plan);
+    return (int)hipGetLastError();
+}
+
 static uint32_t plast_grid(uint32_t items) { return items < 2048 ? (items ? items : 1) : 2048; }
 
 int launch_ltp(const PlastPlan& plan, bn_stream_t s) {
@@ -708,22 +733,10 @@ int launch_ltd(const PlastPlan& plan, bn_stream_t s) {
 
 int launch_norm_tail(const TailPlan& plan, bn_stream_t s) {
     static_assert(sizeof(TailPlan) < 3800, "kernel argument too large");
-    const uint32_t blocks = plan.norm_rows + cdiv(plan.total_neurons, 256) + 0;
-    hipLaunchKernelGGL(k_norm_tail, dim3(blocks ? blocks : 1), dim3(256), 0, s, plan);
-    return (int)hipGetLastError();
-}
-
-static uint32_t lazy_thread_blocks(const PlastPlan& plan) {
-    uint32_t n = 0;
-    for (uint32_t k = 0; k < plan.n; ++k)
-        if (plan.p[k].lazy) n = std::max(n, plan.p[k].n_pre);
-    return cdiv(n, 256);
-}
-
-int launch_lazy_rows(const PlastPlan& plan, bn_stream_t s) {
-    const uint32_t nt = lazy_thread_blocks(plan);
-    if (nt == 0) return 0;
-    hipLaunchKernelGGL(k_lazy_rows, dim3(nt + 256), dim3(256), 0, s, plan, nt);
+    const uint32_t blocks =
+        plan.norm_rows + (plan.increments ? cdiv(plan.total_neurons, 256) : 0) + (plan.tail.active ? 1 : 0);
+    if (blocks == 0) return 0;
+    hipLaunchKernelGGL(k_norm_tail, dim3(blocks), dim3(256), 0, s, plan);
     return (int)hipGetLastError();
 }
 
@@ -741,6 +754,12 @@ int launch_materialize_g(float* g_dst, const float* g_base, const FinalizeSource
     if (n_src > 1) s1 = src[1];
     hipLaunchKernelGGL(k_materialize_g, dim3(cdiv(n, 256)), dim3(256), 0, s, g_dst, g_base, s0, s1,
                        n_src, n);
+    return (int)hipGetLastError();
+}
+
+int launch_add_increments(float* dst, const float* src, const uint32_t* spikes, uint32_t n, bn_stream_t s) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(k_add_increments, dim3(cdiv(n, 256)), dim3(256), 0, s, dst, src, spikes, n);
     return (int)hipGetLastError();
 }
 
@@ -778,11 +797,13 @@ static PlastPlan plan_from_view(const PlasticityView& v, const PlasticityParams&
     return P;
 }
 
+// decay -> LTP -> LTD -> (normalise) -> increments, as the reference launchers
+// (kernel_triplet_plasticity_update.cu:181-216, kernel_istdp_plasticity_update.cu:103-135).
 static int plasticity_update(const PlasticityView& v, const PlasticityParams& p, uint32_t kind,
                              bool normalize, bn_stream_t s, const char* what) {
-    if (kind == kPlastTriplet && normalize && v.post_group_num_neurons > kNormRegs * kNormThreads)
-        return set_error(BN_ERR_INVALID_ARGUMENT, std::string(what) +
-                                                      ": post group larger than the normalisation kernel supports");
+    if (normalize && v.post_group_num_neurons > (uint32_t)(kNormRegs * kNormThreads))
+        return set_error(BN_ERR_INVALID_ARGUMENT,
+                         std::string(what) + ": post group larger than the normalisation kernel supports");
     PlastPlan P = plan_from_view(v, p, kind, normalize);
     const uint32_t nmax = std::max(P.p[0].n_pre, P.p[0].n_post);
     if (nmax == 0) return BN_OK;
@@ -790,25 +811,15 @@ static int plasticity_update(const PlasticityView& v, const PlasticityParams& p,
     BN_HIP_TRY(hipGetLastError(), what);
     BN_HIP_TRY((hipError_t)launch_ltp(P, s), what);
     BN_HIP_TRY((hipError_t)launch_ltd(P, s), what);
-    // The reference increments with the pre/post group slices only, so the neuron range for
-    // the increment kernel is [min offset, max end).
-    const uint32_t lo = std::min(P.p[0].pre_offset, P.p[0].post_offset);
-    const uint32_t hi = std::max(P.p[0].pre_offset + P.p[0].n_pre, P.p[0].post_offset + P.p[0].n_post);
     if (normalize) {
         TailPlan T{};
         T.pl = P;
         T.norm_rows = P.p[0].n_pre;
-        T.total_neurons = 0;
-        T.increments = 0;
-        T.clock = nullptr;
-        T.advance_clock = 0;
-        // Normalisation only (no tail blocks): reuse k_norm_tail with zero neurons.
-        hipLaunchKernelGGL(k_norm_tail, dim3(T.norm_rows), dim3(256), 0, s, T);
-        BN_HIP_TRY(hipGetLastError(), what);
+        BN_HIP_TRY((hipError_t)launch_norm_tail(T, s), what);
     }
+    const uint32_t hi = std::max(P.p[0].pre_offset + P.p[0].n_pre, P.p[0].post_offset + P.p[0].n_post);
     hipLaunchKernelGGL(k_plast_increment, dim3(cdiv(hi, 256)), dim3(256), 0, s, P, hi);
     BN_HIP_TRY(hipGetLastError(), what);
-    (void)lo;
     return BN_OK;
 }
 
@@ -852,8 +863,7 @@ extern "C" int bn_connection_propagate(ConnectionView v, bn_stream_t s) {
     BN_HIP_TRY((hipError_t)launch_gemv(P, s), "gemv connection accumulate");
     float* target = v.pre_group_is_inhibtory ? v.post_group_g_inhibitory : v.post_group_g_excitatory;
     FinalizeSource src{v.gemv_partials, C.n_chunks, C.ld};
-    BN_HIP_TRY((hipError_t)launch_materialize_g(target, target, &src, 1, C.n_post, s),
-               "gemv finalize");
+    BN_HIP_TRY((hipError_t)launch_materialize_g(target, target, &src, 1, C.n_post, s), "gemv finalize");
     return BN_OK;
 }
 
@@ -866,7 +876,7 @@ extern "C" int bn_stimulus_apply(StimulusView v, uint32_t next_event_idx, uint32
 
 extern "C" int bn_stimulus_apply_device_cursor(StimulusView v, bn_device_clock* clock, bn_stream_t s) {
     if (!clock) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_stimulus_apply_device_cursor: clock is null");
-    hipLaunchKernelGGL(k_stimulus_device_cursor, dim3(1), dim3(64), 0, s, v, clock, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_stimulus_device_cursor, dim3(1), dim3(64), 0, s, v, clock);
     BN_HIP_TRY(hipGetLastError(), "kernel_stimulus_apply launch");
     return BN_OK;
 }

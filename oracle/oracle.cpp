@@ -463,16 +463,16 @@ struct Sim {
         std::vector<uint32_t> posts;
         for (uint32_t j = 0; j < nq; ++j)
             if (post_sp[j] == 1) posts.push_back(j);
-        const bool overflow = posts.size() > p.lazy_kmax;
-        std::vector<double> delta(overflow ? 0 : posts.size() * (size_t)np, 0.0);
-        for (size_t k = 0; k < posts.size(); ++k) {
-            const uint32_t j = posts[k];
+        // LTP over every spiking post column (rows of spiking pres included: their sums are
+        // recomputed below), keeping each row's change of M in increasing post order.
+        std::vector<double> dsum(np, 0.0);
+        for (const uint32_t j : posts) {
             for (uint32_t i = 0; i < np; ++i) {
                 if (i == j || P.s[i] == 0.0f) continue;
                 float& m = M[(size_t)i * nq + j];
                 const float w = clampw<float>(P.s[i] * m + (float)ltp_delta(P, i, j), (float)P.wmin, (float)P.wmax);
                 const float mn = w / P.s[i];
-                if (!overflow) delta[k * np + i] = (double)mn - (double)m;
+                dsum[i] += (double)mn - (double)m;
                 m = mn;
             }
         }
@@ -486,12 +486,8 @@ struct Sim {
                         row[j] = w / P.s[i];
                     }
                 P.Rs[i] = canonical_row_sum(row, nq);
-            } else if (overflow) {
-                P.Rs[i] = canonical_row_sum(row, nq);
             } else {
-                double dsum = 0.0;
-                for (size_t k = 0; k < posts.size(); ++k) dsum += delta[k * np + i];
-                P.Rs[i] = P.Rs[i] + dsum;
+                P.Rs[i] = P.Rs[i] + dsum[i];
             }
             if (P.Rs[i] > 0.0) P.s[i] = (float)((double)P.target[i] / P.Rs[i]);
         }

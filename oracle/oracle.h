@@ -16,9 +16,9 @@
  *   - lazy normalisation (lazy_normalize = 1): the triplet matrix is kept as W = diag(s) M.  A
  *     touched entry is updated in true units, w' = clamp(s_i*M_ij + delta), M_ij = w'/s_i (rows
  *     with s_i = 0 stay untouched); R_i = sum_j M_ij is kept in FP64: rows whose pre neuron
- *     spiked (LTD) are re-summed in the normalisation order above, other rows add the LTP
- *     deltas (double)M'_ij - (double)M_ij in increasing post order (all rows are re-summed when
- *     more than lazy_kmax posts spiked); then s_i = fl32(target_i / R_i) when R_i > 0.
+ *     spiked (LTD) are re-summed in the normalisation order above, other rows add the sum of
+ *     their LTP changes (double)M'_ij - (double)M_ij accumulated in increasing post order;
+ *     then s_i = fl32(target_i / R_i) when R_i > 0.
  * precision 32 reproduces the engine bit for bit; precision 64 runs every quantity in FP64
  * (libm exp, plain sequential sums) as an independent accuracy reference.
  *
@@ -75,7 +75,6 @@ typedef struct oracle_params {
     float i_e_ltp_A, i_e_ltd_A, i_e_ltd_B, i_e_tau_ltp, i_e_tau_ltd, i_e_weight_min, i_e_weight_max;
     uint32_t gemv_row_block[4];   /* EE, EI, IE, II */
     uint32_t lazy_normalize;      /* 1: W_EE = diag(s) M with per-row scale factors */
-    uint32_t lazy_kmax;           /* spiking-post capacity of the LTP row-sum delta buffer */
     uint32_t dense_gemv;          /* 1: visit every pre row (no skip of x = 0 rows; same result) */
 } oracle_params;
 

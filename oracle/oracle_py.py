@@ -42,7 +42,7 @@ class Params(C.Structure):
         ("i_e_plasticity_enabled", C.c_uint32),
         ("i_e_ltp_A", C.c_float), ("i_e_ltd_A", C.c_float), ("i_e_ltd_B", C.c_float), ("i_e_tau_ltp", C.c_float),
         ("i_e_tau_ltd", C.c_float), ("i_e_weight_min", C.c_float), ("i_e_weight_max", C.c_float),
-        ("gemv_row_block", C.c_uint32 * 4), ("lazy_normalize", C.c_uint32), ("lazy_kmax", C.c_uint32),
+        ("gemv_row_block", C.c_uint32 * 4), ("lazy_normalize", C.c_uint32),
         ("dense_gemv", C.c_uint32),
     ]
 
@@ -89,7 +89,7 @@ def _up(a):
     return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
-def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=False, lazy_kmax=256,
+def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=False,
                        rng_instance=0, dense_gemv=False) -> Params:
     """Builds oracle parameters from a parsed config (the engine's config dump)."""
     p = Params()
@@ -122,7 +122,6 @@ def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=Fa
     for k in range(4):
         p.gemv_row_block[k] = int(gemv_row_block[k])
     p.lazy_normalize = int(lazy_normalize)
-    p.lazy_kmax = int(lazy_kmax)
     p.dense_gemv = int(dense_gemv)
     return p
 

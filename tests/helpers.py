@@ -94,13 +94,13 @@ def stimulus_schedule(cfg: dict):
             np.array([e[3] for e in ev], np.float32))
 
 
-def oracle_for(config_path: str, sim=None, precision=32, lazy=False, lazy_kmax=256, row_block=None):
+def oracle_for(config_path: str, sim=None, precision=32, lazy=False, row_block=None):
     """Oracle on the same config, connectivity and GEMV chunking as `sim`."""
     eng = engine()
     cfg = eng.config_dump(config_path)
     if row_block is None:
         row_block = tuple(sim.info.gemv_row_block) if sim is not None else (0, 0, 0, 0)
-    p = oracle_py.params_from_config(cfg, gemv_row_block=row_block, lazy_normalize=lazy, lazy_kmax=lazy_kmax)
+    p = oracle_py.params_from_config(cfg, gemv_row_block=row_block, lazy_normalize=lazy)
     weights = [eng.generate_weight_matrix(config_path, k) for k in range(4)]
     stim = stimulus_schedule(cfg)
     return oracle_py.OracleSim(p, weights, precision=precision, stim=stim)

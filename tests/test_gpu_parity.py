@@ -76,8 +76,8 @@ def test_lazy_normalization_bit_exact(tmp_path):
     sim.close()
 
 
-def test_lazy_overflow_path_bit_exact(tmp_path):
-    # lazy_kmax is 256; a synchronous burst of more spiking posts takes the re-sum-all-rows path.
+def test_lazy_burst_bit_exact(tmp_path):
+    # A synchronous burst: hundreds of spiking posts and pres in the same steps.
     cfg = small_config(n_e_side=24, n_i_side=12, steps=400)
     cfg["stimulus_selected_input_ids"] = list(range(300))
     cfg["stimulus_selected_input_post_group_offsets"] = [0] * 300
