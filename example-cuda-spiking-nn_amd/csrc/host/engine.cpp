@@ -113,7 +113,11 @@ GpuSimulationState::GpuSimulationState(uint32_t total_simulation_length, uint32_
         row_scale_ = alloc<float>(ne_, "row_scale");
         row_sum_ = alloc<double>(ne_, "row_sum");
     }
+    // The reference seeds cuRAND with (unsigned long long)rng_seed (kernel_init_neuron_rng_states.cu:20-25).
     bn_device_clock c{};
+    const uint64_t key = (uint64_t)opt_.rng_seed;
+    c.rng_key[0] = (uint32_t)key;
+    c.rng_key[1] = (uint32_t)(key >> 32);
     BN_HIP_CHECK(hipMemcpyAsync(clock_, &c, sizeof c, hipMemcpyHostToDevice, stream_), "H2D clock");
     BN_HIP_CHECK(hipStreamSynchronize(stream_), "state init sync");
 }

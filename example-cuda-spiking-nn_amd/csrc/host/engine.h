@@ -33,6 +33,7 @@ struct StateOptions {
     std::vector<uint32_t> vrec_global_ids;
     bool lazy_normalize = false;
     uint32_t recording_interval_steps = kRecordingIntervalSteps;
+    int64_t rng_seed = 314;  // Philox key (buryn::rng_seed, buryn_definitions.h:11)
     std::array<uint32_t, 4> gemv_row_block = {0, 0, 0, 0};  // 0 = heuristic
 };
 
