@@ -111,10 +111,20 @@ __device__ __forceinline__ void connection_trace_step(float* r, float* d, float*
 }
 
 // Sum of FP64 GEMV partials into an FP32 conductance: fl32((double)g + p_0 + p_1 + ...).
+// The loads are issued 16 at a time; the additions keep chunk order.
 __device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, uint32_t j) {
     double acc = (double)g;
     const double* p = s.partials + j;
-    for (uint32_t c = 0; c < s.chunks; ++c) acc += p[(size_t)c * s.ld];
+    const size_t ld = s.ld;
+    uint32_t c = 0;
+    for (; c + 16 <= s.chunks; c += 16) {
+        double v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = p[(size_t)(c + k) * ld];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc += v[k];
+    }
+    for (; c < s.chunks; ++c) acc += p[(size_t)c * ld];
     return (float)acc;
 }
 
@@ -198,13 +208,55 @@ __global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
 // Products of two binary32 numbers are exact in binary64, so the only rounding is the FP64
 // accumulation in this fixed order, then one rounding to FP32 when g is finalised.
 // =======================================================================================
-__device__ __forceinline__ float row_x(const GemvConn& C, uint32_t i) {
-    const float xi = C.x[i];
-    return C.row_scale ? xi * C.row_scale[i] : xi;
+template <bool kScaled>
+__device__ __forceinline__ float row_x(const float* __restrict__ x, const float* __restrict__ s, uint32_t i) {
+    return kScaled ? x[i] * s[i] : x[i];
+}
+
+// One tile: every index is wave-uniform, so x (and s) come through the scalar cache and the
+// eight float4 row loads of an iteration are all in flight together.
+template <bool kScaled>
+__device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const float* __restrict__ x,
+                                          const float* __restrict__ s, double* __restrict__ partials,
+                                          uint32_t ld, uint32_t i0, uint32_t i1, uint32_t j0, uint32_t chunk) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    const float* wp = W + (size_t)i0 * ld + j0;
+    uint32_t i = i0;
+    for (; i + 8 <= i1; i += 8) {
+        float4 w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = *reinterpret_cast<const float4*>(wp + (size_t)k * ld);
+        float xs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xs[k] = row_x<kScaled>(x, s, i + k);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double xd = (double)xs[k];
+            a0 += xd * (double)w[k].x;
+            a1 += xd * (double)w[k].y;
+            a2 += xd * (double)w[k].z;
+            a3 += xd * (double)w[k].w;
+        }
+        wp += (size_t)8 * ld;
+    }
+    for (; i < i1; ++i) {
+        const float4 w = *reinterpret_cast<const float4*>(wp);
+        const double xd = (double)row_x<kScaled>(x, s, i);
+        a0 += xd * (double)w.x;
+        a1 += xd * (double)w.y;
+        a2 += xd * (double)w.z;
+        a3 += xd * (double)w.w;
+        wp += ld;
+    }
+    double* out = partials + (size_t)chunk * ld + j0;
+    out[0] = a0;
+    out[1] = a1;
+    out[2] = a2;
+    out[3] = a3;
 }
 
 __global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
-    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63;
     if (wave >= P.total_tiles) return;
     uint32_t ci = 0;
@@ -217,41 +269,8 @@ __global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
     const uint32_t i0 = chunk * C.row_block;
     const uint32_t i1 = min(i0 + C.row_block, C.n_pre);
     if (j0 >= C.n_post) return;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    const float* wp = C.W + (size_t)i0 * C.ld + j0;
-    uint32_t i = i0;
-    for (; i + 8 <= i1; i += 8) {
-        float4 w[8];
-        float xs[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            w[k] = *reinterpret_cast<const float4*>(wp + (size_t)k * C.ld);
-            xs[k] = row_x(C, i + k);
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const double xd = (double)xs[k];
-            a0 += xd * (double)w[k].x;
-            a1 += xd * (double)w[k].y;
-            a2 += xd * (double)w[k].z;
-            a3 += xd * (double)w[k].w;
-        }
-        wp += (size_t)8 * C.ld;
-    }
-    for (; i < i1; ++i) {
-        const float4 w = *reinterpret_cast<const float4*>(wp);
-        const double xd = (double)row_x(C, i);
-        a0 += xd * (double)w.x;
-        a1 += xd * (double)w.y;
-        a2 += xd * (double)w.z;
-        a3 += xd * (double)w.w;
-        wp += C.ld;
-    }
-    double* out = C.partials + (size_t)chunk * C.ld + j0;
-    out[0] = a0;
-    out[1] = a1;
-    out[2] = a2;
-    out[3] = a3;
+    if (C.row_scale) gemv_tile<true>(C.W, C.x, C.row_scale, C.partials, C.ld, i0, i1, j0, chunk);
+    else gemv_tile<false>(C.W, C.x, nullptr, C.partials, C.ld, i0, i1, j0, chunk);
 }
 
 // Scalar-load variant for row strides that are not 16-byte aligned (reference layout with
@@ -272,7 +291,10 @@ __global__ __launch_bounds__(256) void k_gemv_scalar(GemvPlan P) {
         const uint32_t j = q * kGemvTileCols + 4 * lane + k;
         if (j >= C.n_post) return;
         double a = 0.0;
-        for (uint32_t i = i0; i < i1; ++i) a += (double)row_x(C, i) * (double)C.W[(size_t)i * C.ld + j];
+        for (uint32_t i = i0; i < i1; ++i) {
+            const float xi = C.row_scale ? C.x[i] * C.row_scale[i] : C.x[i];
+            a += (double)xi * (double)C.W[(size_t)i * C.ld + j];
+        }
         C.partials[(size_t)chunk * C.ld + j] = a;
     }
 }
@@ -329,12 +351,11 @@ __device__ __forceinline__ float apply_delta(const PlastDev& D, float m, float s
 // Canonical row sum: thread t of 256 sums columns t, t+256, ... in FP64, xor butterfly inside
 // each wave (32,16,8,4,2,1), the four wave sums added in wave order.  `vals` holds the
 // thread's columns.
-__device__ __forceinline__ double block_row_sum(const float* vals, uint32_t n, double* lds4) {
+// Combination of the per-thread FP64 partials of a row (thread t of 256 holds columns t, t+256,
+// ... summed in increasing order): xor butterfly inside each wave (32,16,8,4,2,1), then the four
+// wave sums added in wave order.  Every thread returns the same total.
+__device__ __forceinline__ double block_reduce_sum(double acc, double* lds4) {
     const uint32_t t = threadIdx.x;
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < kNormRegs; ++k)
-        if (t + (uint32_t)k * kNormThreads < n) acc += (double)vals[k];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
     if ((t & 63) == 0) lds4[t >> 6] = acc;
@@ -344,28 +365,113 @@ __device__ __forceinline__ double block_row_sum(const float* vals, uint32_t n, d
     return total;
 }
 
-// Row i whose pre neuron did NOT spike: only the LTP entries of this step's spiking posts
-// (increasing post order, from the spike bits).  Lazy: R_i += sum of the changes of M.
-__device__ void plasticity_quiet_row(const PlastDev& D, uint32_t i) {
+// Canonical row sum of a row held in registers (NR columns per thread).
+template <int NR>
+__device__ __forceinline__ double block_row_sum(const float* vals, uint32_t n, double* lds4) {
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+        if (threadIdx.x + (uint32_t)k * kNormThreads < n) acc += (double)vals[k];
+    return block_reduce_sum(acc, lds4);
+}
+
+// Spiking neurons of this step as a list in LDS: the set bits of global range [g0, g1), as
+// offsets from g0 in increasing order, compacted by the whole block (popcount + block scan).
+// Returns the count (the same in every thread); entries past `cap` are not stored.
+constexpr uint32_t kListCap = 4096;
+__device__ __forceinline__ uint32_t block_compact_bits(const uint32_t* bits, uint32_t g0, uint32_t g1,
+                                                       uint32_t* list, uint32_t cap, uint32_t* wsum) {
+    if (g1 <= g0) return 0;
+    const uint32_t w0 = g0 >> 5, nw = ((g1 - 1) >> 5) - w0 + 1;
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    uint32_t total = 0;
+    for (uint32_t base = 0; base < nw; base += blockDim.x) {
+        const uint32_t wi = base + t;
+        const uint32_t w = w0 + wi;
+        uint32_t word = 0;
+        if (wi < nw) {
+            word = bits[w];
+            if (w == (g0 >> 5)) word &= 0xFFFFFFFFu << (g0 & 31);
+            if (w == ((g1 - 1) >> 5)) word &= 0xFFFFFFFFu >> (31 - ((g1 - 1) & 31));
+        }
+        const uint32_t c = (uint32_t)__popc(word);
+        uint32_t v = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t n = __shfl_up(v, off, 64);
+            if (lane >= (uint32_t)off) v += n;
+        }
+        if (lane == 63) wsum[wv] = v;
+        __syncthreads();
+        uint32_t wbase = 0, btot = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+            if (k < wv) wbase += wsum[k];
+            btot += wsum[k];
+        }
+        uint32_t pos = total + wbase + v - c;
+        while (word) {
+            const uint32_t b = (uint32_t)__ffs(word) - 1;
+            word &= word - 1;
+            if (pos < cap) list[pos] = w * 32 + b - g0;
+            ++pos;
+        }
+        total += btot;
+        __syncthreads();
+    }
+    return total;
+}
+
+// LTP entries (i, js[0..n)) of a quiet row, in list order; the loads are issued together.
+constexpr int kLtpBatch = 8;
+__device__ __forceinline__ void quiet_row_batch(const PlastDev& D, uint32_t i, float* row, float si, float pre_i,
+                                                const uint32_t* js, int n, double& dsum) {
+    float m[kLtpBatch];
+#pragma unroll
+    for (int k = 0; k < kLtpBatch; ++k)
+        if (k < n) m[k] = row[js[k]];
+#pragma unroll
+    for (int k = 0; k < kLtpBatch; ++k) {
+        if (k >= n || js[k] == i) continue;
+        const float fac = ltp_factor(D, js[k]);
+        const float delta = D.kind == kPlastTriplet ? pre_i * fac : D.p.ltp_A * pre_i;
+        const float mn = apply_delta(D, m[k], si, delta);
+        row[js[k]] = mn;
+        if (D.lazy) dsum += (double)mn - (double)m[k];
+    }
+}
+
+// Row i whose pre neuron did NOT spike: only the LTP entries of this step's spiking posts, in
+// increasing post order (LDS list; straight from the spike bits when the list overflowed).
+// Lazy: R_i += sum of the changes of M.
+__device__ __forceinline__ void plasticity_quiet_row(const PlastDev& D, uint32_t i, const uint32_t* list,
+                                                     uint32_t count) {
     const bool lazy = D.lazy != 0;
     const float si = lazy ? D.s[i] : 1.0f;
     double dsum = 0.0;
-    if (!lazy || si != 0.0f) {
+    if (count && (!lazy || si != 0.0f)) {
         float* row = D.W + (size_t)i * D.ld;
-        const uint32_t g0 = D.post_offset, g1 = D.post_offset + D.n_post;
-        for (uint32_t w = g0 >> 5; w <= (g1 - 1) >> 5; ++w) {
-            uint32_t bits = D.bits[w];
-            if (w == (g0 >> 5)) bits &= 0xFFFFFFFFu << (g0 & 31);
-            if (w == ((g1 - 1) >> 5)) bits &= 0xFFFFFFFFu >> (31 - ((g1 - 1) & 31));
-            while (bits) {
-                const uint32_t b = (uint32_t)__ffs(bits) - 1;
-                bits &= bits - 1;
-                const uint32_t j = w * 32 + b - g0;
-                if (j == i) continue;
-                const float m = row[j];
-                const float mn = apply_delta(D, m, si, ltp_delta(D, i, ltp_factor(D, j)));
-                row[j] = mn;
-                if (lazy) dsum += (double)mn - (double)m;
+        const float pre_i = D.pre[i];
+        uint32_t js[kLtpBatch];
+        if (count <= kListCap) {
+            for (uint32_t k0 = 0; k0 < count; k0 += kLtpBatch) {
+                const int n = (int)min((uint32_t)kLtpBatch, count - k0);
+#pragma unroll
+                for (int k = 0; k < kLtpBatch; ++k) js[k] = k < n ? list[k0 + k] : 0u;
+                quiet_row_batch(D, i, row, si, pre_i, js, n, dsum);
+            }
+        } else {
+            const uint32_t g0 = D.post_offset, g1 = D.post_offset + D.n_post;
+            for (uint32_t w = g0 >> 5; w <= (g1 - 1) >> 5; ++w) {
+                uint32_t bits = D.bits[w];
+                if (w == (g0 >> 5)) bits &= 0xFFFFFFFFu << (g0 & 31);
+                if (w == ((g1 - 1) >> 5)) bits &= 0xFFFFFFFFu >> (31 - ((g1 - 1) & 31));
+                while (bits) {
+                    const uint32_t b = (uint32_t)__ffs(bits) - 1;
+                    bits &= bits - 1;
+                    js[0] = w * 32 + b - g0;
+                    quiet_row_batch(D, i, row, si, pre_i, js, 1, dsum);
+                }
             }
         }
     }
@@ -379,29 +485,52 @@ __device__ void plasticity_quiet_row(const PlastDev& D, uint32_t i) {
 // Row i whose pre neuron spiked, by the whole block: per entry the LTP increment (if the post
 // neuron spiked) then the LTD increment, as the reference's LTP-then-LTD kernel order.
 // Lazy: the row's sum of M is recomputed in the canonical order.
-__device__ void plasticity_spiking_row(const PlastDev& D, uint32_t i, double* lds4) {
+__device__ __forceinline__ void plasticity_spiking_row(const PlastDev& D, uint32_t i, double* lds4) {
     const bool lazy = D.lazy != 0;
     const float si = lazy ? D.s[i] : 1.0f;
     const bool update = !lazy || si != 0.0f;
     const float fac_i = ltd_factor(D, i);
     float* row = D.W + (size_t)i * D.ld;
-    float vals[kNormRegs];
+    const bool triplet = D.kind == kPlastTriplet;
+    const float pre_i = D.pre[i];
+    double acc = 0.0;
+    // Batches of 8 columns per thread: all loads first, then updates/stores in column order.
+    constexpr int kB = 8;
+    for (uint32_t j0 = threadIdx.x; j0 < D.n_post; j0 += kB * kNormThreads) {
+        float m[kB], post_j[kB], tpost_j[kB];
+        uint32_t sp[kB];
 #pragma unroll
-    for (int k = 0; k < kNormRegs; ++k) {
-        const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
-        vals[k] = 0.0f;
-        if (j < D.n_post) {
-            float m = row[j];
-            if (update && j != i) {
-                if (D.post_spikes[j] == 1) m = apply_delta(D, m, si, ltp_delta(D, i, ltp_factor(D, j)));
-                m = apply_delta(D, m, si, ltd_delta(D, j, fac_i));
-                row[j] = m;
+        for (int k = 0; k < kB; ++k) {
+            const uint32_t j = j0 + (uint32_t)k * kNormThreads;
+            if (j < D.n_post) {
+                m[k] = row[j];
+                sp[k] = D.post_spikes[j];
+                post_j[k] = D.post[j];
+                tpost_j[k] = triplet ? D.trip_post[j] : 0.0f;
             }
-            vals[k] = m;
+        }
+#pragma unroll
+        for (int k = 0; k < kB; ++k) {
+            const uint32_t j = j0 + (uint32_t)k * kNormThreads;
+            if (j >= D.n_post) continue;
+            float v = m[k];
+            if (update && j != i) {
+                if (sp[k] == 1) {
+                    const float fac = triplet ? D.p.ltp_A + (D.p.ltp_B * tpost_j[k]) : D.p.ltp_A;
+                    v = apply_delta(D, v, si, triplet ? pre_i * fac : D.p.ltp_A * pre_i);
+                }
+                float d;
+                if (triplet) d = -1.0f * post_j[k] * fac_i;
+                else if (D.kind == kPlastIstdp) d = -1.0f * D.p.ltd_A * (post_j[k] - D.p.ltd_B);
+                else d = -1.0f * D.p.ltd_A * post_j[k];
+                v = apply_delta(D, v, si, d);
+                row[j] = v;
+            }
+            acc += (double)v;
         }
     }
     if (lazy) {
-        const double R = block_row_sum(vals, D.n_post, lds4);
+        const double R = block_reduce_sum(acc, lds4);
         if (threadIdx.x == 0) {
             D.R[i] = R;
             if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
@@ -412,7 +541,7 @@ __device__ void plasticity_spiking_row(const PlastDev& D, uint32_t i, double* ld
 // Stimulus events of clock_step in list order (stimulus.cpp:46-60), then the clock advance.
 // One thread: the event count per step is small, and sequential order makes the sum
 // independent of scheduling (the reference uses atomicAdd).
-__device__ void stimulus_and_clock(const StimTail& T) {
+__device__ __forceinline__ void stimulus_and_clock(const StimTail& T) {
     bn_device_clock* clk = T.clock;
     const uint32_t s = clk->clock_step;
     if (T.timestep_event_count && s < T.num_timesteps) {
@@ -429,34 +558,46 @@ __device__ void stimulus_and_clock(const StimTail& T) {
     clk->clock_step = s + 1;
 }
 
+// Blocks [0, row_blocks): one thread per row of the concatenated pre rows of all plasticities
+// (rows whose pre did not spike).  Blocks after: the rows whose pre spiked, one block each.
+// The last block also applies the stimulus and advances the clock (no block reads the clock).
 __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
     __shared__ double lds4[4];
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t list[kListCap];
     const uint32_t b = blockIdx.x;
     if (b < P.row_blocks) {
-        uint32_t r = b * blockDim.x + threadIdx.x;
+        const uint32_t r0 = b * blockDim.x;
+        uint32_t base = 0;
         for (uint32_t k = 0; k < P.pl.n; ++k) {
             const PlastDev& D = P.pl.p[k];
-            if (r < D.n_pre) {
-                if (D.pre_spikes[r] != 1) plasticity_quiet_row(D, r);
-                break;
+            if (r0 < base + D.n_pre && r0 + blockDim.x > base) {
+                const uint32_t count =
+                    block_compact_bits(D.bits, D.post_offset, D.post_offset + D.n_post, list, kListCap, wsum);
+                const uint32_t r = r0 + threadIdx.x;
+                if (r >= base && r < base + D.n_pre && D.pre_spikes[r - base] != 1)
+                    plasticity_quiet_row(D, r - base, list, count);
+                __syncthreads();
             }
-            r -= D.n_pre;
+            base += D.n_pre;
         }
     } else if (b < P.row_blocks + P.spiking_blocks) {
-        for (uint32_t r = b - P.row_blocks; r < P.pl.items_pre; r += P.spiking_blocks) {
-            uint32_t rr = r;
-            for (uint32_t k = 0; k < P.pl.n; ++k) {
-                const PlastDev& D = P.pl.p[k];
-                if (rr < D.n_pre) {
-                    if (D.pre_spikes[rr] == 1) plasticity_spiking_row(D, rr, lds4);
-                    break;
-                }
-                rr -= D.n_pre;
+        const uint32_t sb = b - P.row_blocks;
+        for (uint32_t k = 0; k < P.pl.n; ++k) {
+            const PlastDev& D = P.pl.p[k];
+            const uint32_t count = block_compact_bits(D.bits, D.pre_offset, D.pre_offset + D.n_pre, list, kListCap, wsum);
+            if (count <= kListCap) {
+                for (uint32_t e = sb; e < count; e += P.spiking_blocks) plasticity_spiking_row(D, list[e], lds4);
+            } else {
+                for (uint32_t i = sb; i < D.n_pre; i += P.spiking_blocks)
+                    if (D.pre_spikes[i] == 1) plasticity_spiking_row(D, i, lds4);
             }
+            __syncthreads();
         }
     }
     if (P.tail.active && b == gridDim.x - 1 && threadIdx.x == 0) stimulus_and_clock(P.tail);
 }
+
 
 __device__ __forceinline__ bool locate_item(const PlastPlan& P, uint32_t item, bool post,
                                             uint32_t& pi, uint32_t& idx) {
@@ -509,20 +650,21 @@ __global__ __launch_bounds__(256) void k_ltd(PlastPlan P) {
 // Triplet row normalisation (kernel_triplet_plasticity_update.cu:117-145) with the canonical
 // FP64 row sum: skip if sum <= 0, else scale the row by target/sum (target = pre_weight_sums[i],
 // the initial column sum, triplet_plasticity.cpp:22-30).
-__device__ void normalize_row(const PlastDev& D, uint32_t i, double* lds4) {
+template <int NR>
+__device__ __forceinline__ void normalize_row(const PlastDev& D, uint32_t i, double* lds4) {
     float* row = D.W + (size_t)i * D.ld;
-    float vals[kNormRegs];
+    float vals[NR];
 #pragma unroll
-    for (int k = 0; k < kNormRegs; ++k) {
+    for (int k = 0; k < NR; ++k) {
         const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
         vals[k] = j < D.n_post ? row[j] : 0.0f;
     }
-    const double total = block_row_sum(vals, D.n_post, lds4);
+    const double total = block_row_sum<NR>(vals, D.n_post, lds4);
     const float current_sum = (float)total;
     if (current_sum <= 0.0f) return;
     const float scale = D.target[i] / current_sum;
 #pragma unroll
-    for (int k = 0; k < kNormRegs; ++k) {
+    for (int k = 0; k < NR; ++k) {
         const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
         if (j < D.n_post) row[j] = vals[k] * scale;
     }
@@ -552,6 +694,7 @@ __device__ __forceinline__ void increment_traces(const PlastPlan& P, uint32_t gi
 // Blocks [0, norm_rows): one row each.  Blocks after: trace increments for 256 neurons each
 // (component path).  The last block also applies the stimulus and advances the clock when the
 // tail is active (no other block reads the clock in this kernel).
+template <int NR>
 __global__ __launch_bounds__(256) void k_norm_tail(TailPlan T) {
     __shared__ double lds4[4];
     const uint32_t b = blockIdx.x;
@@ -561,7 +704,7 @@ __global__ __launch_bounds__(256) void k_norm_tail(TailPlan T) {
             const PlastDev& D = T.pl.p[k];
             if (!D.normalize) continue;
             if (rem < D.n_pre) {
-                normalize_row(D, rem, lds4);
+                normalize_row<NR>(D, rem, lds4);
                 break;
             }
             rem -= D.n_pre;
@@ -573,18 +716,19 @@ __global__ __launch_bounds__(256) void k_norm_tail(TailPlan T) {
     if (T.tail.active && b == gridDim.x - 1 && threadIdx.x == 0) stimulus_and_clock(T.tail);
 }
 
+template <int NR>
 __global__ __launch_bounds__(256) void k_lazy_init(PlastDev D) {
     __shared__ double lds4[4];
     const uint32_t i = blockIdx.x;
     if (i >= D.n_pre) return;
-    float vals[kNormRegs];
+    float vals[NR];
     const float* row = D.W + (size_t)i * D.ld;
 #pragma unroll
-    for (int k = 0; k < kNormRegs; ++k) {
+    for (int k = 0; k < NR; ++k) {
         const uint32_t j = threadIdx.x + (uint32_t)k * kNormThreads;
         vals[k] = j < D.n_post ? row[j] : 0.0f;
     }
-    const double R = block_row_sum(vals, D.n_post, lds4);
+    const double R = block_row_sum<NR>(vals, D.n_post, lds4);
     if (threadIdx.x == 0) {
         D.R[i] = R;
         D.s[i] = 1.0f;
@@ -709,6 +853,14 @@ int launch_gemv(const GemvPlan& plan, bn_stream_t s) {
     return (int)hipGetLastError();
 }
 
+// Registers per thread for a row of n columns held by 256 threads (n <= 64 * 256).
+static int row_regs(uint32_t n) { return n <= 16u * kNormThreads ? 16 : n <= 32u * kNormThreads ? 32 : 64; }
+static uint32_t max_post(const PlastPlan& p) {
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < p.n; ++k) m = std::max(m, p.p[k].n_post);
+    return m;
+}
+
 int launch_plasticity(const PlastStepPlan& plan, bn_stream_t s) {
     static_assert(sizeof(PlastStepPlan) < 3800, "kernel argument too large");
     const uint32_t blocks = plan.row_blocks + plan.spiking_blocks;
@@ -736,13 +888,21 @@ int launch_norm_tail(const TailPlan& plan, bn_stream_t s) {
     const uint32_t blocks =
         plan.norm_rows + (plan.increments ? cdiv(plan.total_neurons, 256) : 0) + (plan.tail.active ? 1 : 0);
     if (blocks == 0) return 0;
-    hipLaunchKernelGGL(k_norm_tail, dim3(blocks), dim3(256), 0, s, plan);
+    switch (row_regs(max_post(plan.pl))) {
+        case 16: hipLaunchKernelGGL(k_norm_tail<16>, dim3(blocks), dim3(256), 0, s, plan); break;
+        case 32: hipLaunchKernelGGL(k_norm_tail<32>, dim3(blocks), dim3(256), 0, s, plan); break;
+        default: hipLaunchKernelGGL(k_norm_tail<64>, dim3(blocks), dim3(256), 0, s, plan); break;
+    }
     return (int)hipGetLastError();
 }
 
 int launch_lazy_init(const PlastDev& d, bn_stream_t s) {
     if (d.n_pre == 0) return 0;
-    hipLaunchKernelGGL(k_lazy_init, dim3(d.n_pre), dim3(256), 0, s, d);
+    switch (row_regs(d.n_post)) {
+        case 16: hipLaunchKernelGGL(k_lazy_init<16>, dim3(d.n_pre), dim3(256), 0, s, d); break;
+        case 32: hipLaunchKernelGGL(k_lazy_init<32>, dim3(d.n_pre), dim3(256), 0, s, d); break;
+        default: hipLaunchKernelGGL(k_lazy_init<64>, dim3(d.n_pre), dim3(256), 0, s, d); break;
+    }
     return (int)hipGetLastError();
 }
 
