@@ -29,6 +29,8 @@ constexpr int kMaxSources = 2;
 constexpr int kMaxPlast = 4;
 constexpr int kMaxTraceSets = 4;
 constexpr int kGemvTileCols = 256;   // 64 lanes x float4
+constexpr int kFinParts = 8;         // parts the chunk partials of a source are combined in
+constexpr int kEvolveNeurons = 64;   // neurons per evolve block (kFinParts waves each)
 constexpr int kNormThreads = 256;    // threads per row in the row sums
 constexpr int kNormRegs = 64;        // columns per thread kept in registers (n_post <= 16384)
 
@@ -93,6 +95,7 @@ struct GemvPlan {
     GemvConn c[kMaxConns];
     uint32_t n_conn;
     uint32_t total_tiles;    // one wave per tile
+    uint32_t nontemporal;    // stream the weights past the caches (weight set >> MALL)
 };
 
 struct PlastDev {

@@ -173,6 +173,8 @@ void build(bn_sim* sim, const std::string& config_path) {
     }
     opt.lazy_normalize = (flags & BN_SIM_LAZY_NORMALIZE) && c.e_e_plasticity_enabled;
     opt.rng_seed = c.rng_seed;
+    for (int k = 0; k < 4; ++k) opt.gemv_row_block[k] = c.gemv_row_blocks[k];
+    opt.gemv_nontemporal = c.gemv_nontemporal_loads;
     const uint32_t num_stimulus_events =
         (uint32_t)(c.stimulus_selected_input_ids.size() * c.stimulus_start_clock_steps.size() * 2);  // main_buryn.cpp:112
     sim->state.reset(new GpuSimulationState(c.simulation_steps, c.num_excitatory_neurons, c.num_inhibitory_neurons,

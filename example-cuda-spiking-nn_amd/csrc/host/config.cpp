@@ -227,6 +227,9 @@ Config from_value(const Value& j) {
     c.voltage_recording_file = string_or(j, "voltage_recording_file", c.voltage_recording_file);
     read_u32_array(j, "voltage_recording_neuron_ids", c.voltage_recording_neuron_ids);
     c.e_e_weight_recording_file = string_or(j, "e_e_weight_recording_file", c.e_e_weight_recording_file);
+    read_u32_array(j, "gemv_row_blocks", c.gemv_row_blocks);
+    if (c.gemv_row_blocks.size() != 4) throw std::runtime_error("gemv_row_blocks must hold 4 values (EE, EI, IE, II)");
+    c.gemv_nontemporal_loads = value_or<int>(j, "gemv_nontemporal_loads", c.gemv_nontemporal_loads);
 
     const std::string& od = c.output_directory;
     c.excitatory_locations_file = prefix_output_path(od, c.excitatory_locations_file);
@@ -344,6 +347,8 @@ std::string config_to_json(const Config& c) {
     S(e_spike_recording_file) S(i_spike_recording_file) S(voltage_recording_file)
     d.ua("voltage_recording_neuron_ids", c.voltage_recording_neuron_ids);
     S(e_e_weight_recording_file)
+    d.ua("gemv_row_blocks", c.gemv_row_blocks);
+    d.i("gemv_nontemporal_loads", c.gemv_nontemporal_loads);
 #undef F
 #undef U
 #undef S

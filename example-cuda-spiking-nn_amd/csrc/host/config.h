@@ -124,6 +124,14 @@ struct Config {
     std::string voltage_recording_file = "../data/buryn_data/buryn_outputs_/recordings/voltages_step0.txt";
     std::vector<uint32_t> voltage_recording_neuron_ids = {10, 426};
     std::string e_e_weight_recording_file = "../data/buryn_data/buryn_outputs_/weights/weights_e_to_e_recordings.txt";
+
+    // extension (engine tuning): rows of W summed per propagate partial for E->E, E->I, I->E,
+    // I->I; 0 picks the engine's heuristic.  Fixes the summation order, so it is part of the
+    // numerical definition of a run.
+    std::vector<uint32_t> gemv_row_blocks = {0, 0, 0, 0};
+    // extension (engine tuning): weight loads of the propagate kernel bypass the caches (1),
+    // use them (0), or the engine decides from the size of the weight set (-1).
+    int gemv_nontemporal_loads = -1;
 };
 
 Config load_config_json(const std::string& path);

@@ -35,6 +35,7 @@ struct StateOptions {
     uint32_t recording_interval_steps = kRecordingIntervalSteps;
     int64_t rng_seed = 314;  // Philox key (buryn::rng_seed, buryn_definitions.h:11)
     std::array<uint32_t, 4> gemv_row_block = {0, 0, 0, 0};  // 0 = heuristic
+    int gemv_nontemporal = -1;  // weight loads past the caches: 1 yes, 0 no, -1 when W >> MALL
 };
 
 class GpuSimulationState {

@@ -218,6 +218,12 @@ void System::build_plans() {
     }
     gemv_.n_conn = (uint32_t)connections_.size();
     gemv_.total_tiles = tiles;
+    // Weight sets well above the 256 MB MALL are streamed with non-temporal loads; smaller
+    // ones are left to stay cache-resident from step to step.
+    uint64_t w_bytes = 0;
+    for (size_t c = 0; c < connections_.size(); ++c) w_bytes += 4ull * gemv_.c[c].n_pre * gemv_.c[c].ld;
+    const int nt = S.options().gemv_nontemporal;
+    gemv_.nontemporal = nt < 0 ? (w_bytes > (192ull << 20) ? 1u : 0u) : (nt ? 1u : 0u);
 
     // Stimulus events + clock advance: in the plasticity kernel, or after the faithful
     // normalisation when there is one.

@@ -55,25 +55,23 @@ __device__ __forceinline__ void eif_step(const NeuronGroupView& v, float ge, flo
     }
 }
 
-// kernel_neuron_group_evolve.cu:11-83 — background traces, Poisson shot noise, bg conductance.
-__device__ __forceinline__ void background_step(const NeuronGroupView& v, uint32_t lid,
-                                                uint32_t step, const bn_device_clock& clk,
-                                                float& bg_ge, float& bg_gi) {
-    float re = flush_trace(v.background_excitatory_postsynaptic_r_traces[lid] *
-                           v.background_excitatory_postsynaptic_r_trace_decay);
-    float ri = flush_trace(v.background_inhibitory_postsynaptic_r_traces[lid] *
-                           v.background_inhibitory_postsynaptic_r_trace_decay);
-    float de = flush_trace(v.background_excitatory_postsynaptic_d_traces[lid] *
-                           v.background_excitatory_postsynaptic_d_trace_decay);
-    float di = flush_trace(v.background_inhibitory_postsynaptic_d_traces[lid] *
-                           v.background_inhibitory_postsynaptic_d_trace_decay);
+// kernel_neuron_group_evolve.cu:11-83 on values already in registers: decay (+flush) of the
+// four background traces, the Poisson shot counts, the increments, the background conductances.
+__device__ __forceinline__ void background_advance(const NeuronGroupView& v, uint32_t gid, uint32_t step,
+                                                   const bn_device_clock& clk, const uint32_t* tab_e,
+                                                   const uint32_t* tab_i, float& re, float& ri, float& de,
+                                                   float& di, float& bg_ge, float& bg_gi) {
+    re = flush_trace(re * v.background_excitatory_postsynaptic_r_trace_decay);
+    ri = flush_trace(ri * v.background_inhibitory_postsynaptic_r_trace_decay);
+    de = flush_trace(de * v.background_excitatory_postsynaptic_d_trace_decay);
+    di = flush_trace(di * v.background_inhibitory_postsynaptic_d_trace_decay);
     uint32_t ke = 0, ki = 0;
     if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
-        const U32x4 r = background_draws(clk, step, v.offset + lid, v.rng_instance);
+        const U32x4 r = background_draws(clk, step, gid, v.rng_instance);
         if (v.mean_background_excitatory_spikes > 0.0f)
-            ke = poisson_from_table(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len);
+            ke = poisson_from_table(r.x, tab_e, v.poisson_table_excitatory_len);
         if (v.mean_background_inhibitory_spikes > 0.0f)
-            ki = poisson_from_table(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len);
+            ki = poisson_from_table(r.y, tab_i, v.poisson_table_inhibitory_len);
     }
     if (ke > 0u) {
         const float inc = (float)ke;
@@ -87,12 +85,29 @@ __device__ __forceinline__ void background_step(const NeuronGroupView& v, uint32
         di += inc;
     }
     bg_gi = (di - ri) * v.background_inhibitory_scaling_factor;
+}
+
+__device__ __forceinline__ void background_store(const NeuronGroupView& v, uint32_t lid, float re, float ri,
+                                                 float de, float di, float bg_ge, float bg_gi) {
     v.background_excitatory_postsynaptic_r_traces[lid] = re;
     v.background_inhibitory_postsynaptic_r_traces[lid] = ri;
     v.background_excitatory_postsynaptic_d_traces[lid] = de;
     v.background_inhibitory_postsynaptic_d_traces[lid] = di;
     v.background_g_excitatory_synaptic_conductances[lid] = bg_ge;
     v.background_g_inhibitory_synaptic_conductances[lid] = bg_gi;
+}
+
+// The whole background update of one neuron, memory to memory (component path).
+__device__ __forceinline__ void background_step(const NeuronGroupView& v, uint32_t lid,
+                                                uint32_t step, const bn_device_clock& clk,
+                                                float& bg_ge, float& bg_gi) {
+    float re = v.background_excitatory_postsynaptic_r_traces[lid];
+    float ri = v.background_inhibitory_postsynaptic_r_traces[lid];
+    float de = v.background_excitatory_postsynaptic_d_traces[lid];
+    float di = v.background_inhibitory_postsynaptic_d_traces[lid];
+    background_advance(v, v.offset + lid, step, clk, v.poisson_table_excitatory, v.poisson_table_inhibitory,
+                       re, ri, de, di, bg_ge, bg_gi);
+    background_store(v, lid, re, ri, de, di, bg_ge, bg_gi);
 }
 
 // kernel_connection_propagate.cu:8-46 for one pre neuron.
@@ -110,21 +125,46 @@ __device__ __forceinline__ void connection_trace_step(float* r, float* d, float*
     x[i] = dd - rr;
 }
 
-// Sum of FP64 GEMV partials into an FP32 conductance: fl32((double)g + p_0 + p_1 + ...).
-// The loads are issued 16 at a time; the additions keep chunk order.
-__device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, uint32_t j) {
-    double acc = (double)g;
+// Sum of FP64 GEMV partials into an FP32 conductance.  Order of the combination: the chunks of a source are cut into kFinParts contiguous parts
+// of ceil(chunks / kFinParts); each part is summed in increasing chunk order from 0.0, then
+// g' = fl32(g + part_0 + part_1 + ...) over the non-empty parts, in FP64.
+__device__ __forceinline__ uint32_t fin_part_size(uint32_t chunks) { return (chunks + kFinParts - 1) / kFinParts; }
+
+__device__ __forceinline__ double fin_part_sum(const FinalizeSource& s, uint32_t j, uint32_t c0, uint32_t c1) {
     const double* p = s.partials + j;
     const size_t ld = s.ld;
-    uint32_t c = 0;
-    for (; c + 16 <= s.chunks; c += 16) {
-        double v[16];
+    double acc = 0.0;
+    uint32_t c = c0;
+    for (; c + 8 <= c1; c += 8) {
+        double v[8];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = p[(size_t)(c + k) * ld];
+        for (int k = 0; k < 8; ++k) v[k] = p[(size_t)(c + k) * ld];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) acc += v[k];
+        for (int k = 0; k < 8; ++k) acc += v[k];
     }
-    for (; c < s.chunks; ++c) acc += p[(size_t)c * ld];
+    for (; c < c1; ++c) acc += p[(size_t)c * ld];
+    return acc;
+}
+
+// One thread, every part in order (component path and g materialisation).
+__device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, uint32_t j) {
+    const uint32_t per = fin_part_size(s.chunks);
+    double acc = (double)g;
+    for (uint32_t c0 = 0; per && c0 < s.chunks; c0 += per) acc += fin_part_sum(s, j, c0, min(c0 + per, s.chunks));
+    return (float)acc;
+}
+
+// Part sums of one block's 64 neurons: wave w of the block sums part w of every source.
+// fin[slot][w][lane], slots 0..kMaxSources-1 for g_e sources, kMaxSources.. for g_i.
+typedef double FinParts[2 * kMaxSources][kFinParts][kEvolveNeurons];
+
+__device__ __forceinline__ float combine_parts(float g, const FinalizeSource& s, const double (*part)[kEvolveNeurons],
+                                               uint32_t lane) {
+    const uint32_t per = fin_part_size(s.chunks);
+    double acc = (double)g;
+#pragma unroll
+    for (uint32_t w = 0; w < (uint32_t)kFinParts; ++w)
+        if (w * per < s.chunks) acc += part[w][lane];
     return (float)acc;
 }
 
@@ -138,56 +178,157 @@ __device__ __forceinline__ float trace_step(float tr, uint32_t spiked_prev, floa
 // =======================================================================================
 // k_evolve: every neuron of every group in one launch
 // =======================================================================================
-__global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+// A block owns 64 neurons and has kFinParts waves.  Wave 0 first issues every per-neuron load
+// (state, background, connection and plasticity traces); all waves stage the Poisson tables in
+// LDS and sum one part each of the propagate partials of the 64 neurons, so the partial reads
+// are spread over many waves; after the barrier wave 0 combines the parts and updates the
+// neurons from registers.
+constexpr uint32_t kTabCap = 256;  // Poisson table entries staged in LDS per table
+
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
+    __shared__ FinParts fin;
+    __shared__ uint32_t tab[kMaxGroups][2][kTabCap];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t gid = blockIdx.x * kEvolveNeurons + lane;
+    const bool live = gid < P.total_neurons;
     const bn_device_clock clk = *P.clock;
-    const uint32_t step = clk.clock_step;
-    uint32_t spike = 0, spike_prev = 0;
-    float V = 0.0f;
-    if (gid < P.total_neurons) {
-        for (uint32_t gi = 0; gi < P.n_groups; ++gi) {
-            const EvolveGroup& G = P.g[gi];
-            if (gid < G.v.offset || gid >= G.v.offset + G.v.num_neurons) continue;
-            const NeuronGroupView& v = G.v;
+
+    // Per-neuron inputs (wave 0).
+    float V = 0.0f, ge = 0.0f, gi_ = 0.0f, istim = 0.0f, re = 0.0f, ri = 0.0f, de = 0.0f, di = 0.0f;
+    uint32_t refr = 0, spike_prev = 0;
+    int32_t vslot = -1;
+    float tr_r[kMaxTraceSets], tr_d[kMaxTraceSets];
+    float q_pre[kMaxPlast], q_tpre[kMaxPlast], q_post[kMaxPlast], q_tpost[kMaxPlast];
+#pragma unroll
+    for (int t = 0; t < kMaxTraceSets; ++t) tr_r[t] = tr_d[t] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kMaxPlast; ++q) q_pre[q] = q_tpre[q] = q_post[q] = q_tpost[q] = 0.0f;
+    if (wave == 0 && live) {
+        for (uint32_t g = 0; g < P.n_groups; ++g) {
+            const NeuronGroupView& v = P.g[g].v;
+            if (gid < v.offset || gid >= v.offset + v.num_neurons) continue;
             const uint32_t lid = gid - v.offset;
-            float ge = v.g_excitatory[lid];
-            float gi_ = v.g_inhibitory[lid];
-            for (uint32_t s = 0; s < G.n_fin_e; ++s) ge = finalize_g(ge, G.fin_e[s], lid);
-            for (uint32_t s = 0; s < G.n_fin_i; ++s) gi_ = finalize_g(gi_, G.fin_i[s], lid);
             V = v.membrane_potentials[lid];
-            uint32_t refr = v.refractory_counter[lid];
+            refr = v.refractory_counter[lid];
             spike_prev = v.spikes[lid];
-            eif_step(v, ge, gi_, v.i_stimulus[lid], V, refr, spike);
+            ge = v.g_excitatory[lid];
+            gi_ = v.g_inhibitory[lid];
+            istim = v.i_stimulus[lid];
+            re = v.background_excitatory_postsynaptic_r_traces[lid];
+            ri = v.background_inhibitory_postsynaptic_r_traces[lid];
+            de = v.background_excitatory_postsynaptic_d_traces[lid];
+            di = v.background_inhibitory_postsynaptic_d_traces[lid];
+        }
+#pragma unroll
+        for (int t = 0; t < kMaxTraceSets; ++t) {
+            const TraceSet& T = P.ts[t];
+            if ((uint32_t)t < P.n_ts && gid >= T.pre_offset && gid < T.pre_offset + T.n_pre) {
+                tr_r[t] = T.r[gid - T.pre_offset];
+                tr_d[t] = T.d[gid - T.pre_offset];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxPlast; ++q) {
+            const PlastTraces& Q = P.pt[q];
+            if ((uint32_t)q >= P.n_pt) continue;
+            if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
+                q_pre[q] = Q.pre[gid - Q.pre_offset];
+                if (Q.kind == kPlastTriplet) q_tpre[q] = Q.trip_pre[gid - Q.pre_offset];
+            }
+            if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
+                q_post[q] = Q.post[gid - Q.post_offset];
+                if (Q.kind == kPlastTriplet) q_tpost[q] = Q.trip_post[gid - Q.post_offset];
+            }
+        }
+        if (P.v_ring) vslot = P.vrec_index[gid];
+    }
+
+    // Poisson tables to LDS (all waves).
+    for (uint32_t g = 0; g < P.n_groups; ++g) {
+        const NeuronGroupView& v = P.g[g].v;
+        if (v.poisson_table_excitatory_len <= kTabCap)
+            for (uint32_t k = threadIdx.x; k < v.poisson_table_excitatory_len; k += blockDim.x)
+                tab[g][0][k] = v.poisson_table_excitatory[k];
+        if (v.poisson_table_inhibitory_len <= kTabCap)
+            for (uint32_t k = threadIdx.x; k < v.poisson_table_inhibitory_len; k += blockDim.x)
+                tab[g][1][k] = v.poisson_table_inhibitory[k];
+    }
+
+    // Part sums of the propagate partials (all waves).
+    if (live) {
+        for (uint32_t g = 0; g < P.n_groups; ++g) {
+            const EvolveGroup& G = P.g[g];
+            if (gid < G.v.offset || gid >= G.v.offset + G.v.num_neurons) continue;
+            const uint32_t lid = gid - G.v.offset;
+            for (uint32_t s = 0; s < G.n_fin_e + G.n_fin_i; ++s) {
+                const FinalizeSource& F = s < G.n_fin_e ? G.fin_e[s] : G.fin_i[s - G.n_fin_e];
+                const uint32_t slot = s < G.n_fin_e ? s : kMaxSources + (s - G.n_fin_e);
+                const uint32_t per = fin_part_size(F.chunks);
+                const uint32_t c0 = wave * per;
+                if (c0 < F.chunks) fin[slot][wave][lane] = fin_part_sum(F, lid, c0, min(c0 + per, F.chunks));
+            }
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+
+    const uint32_t step = clk.clock_step;
+    uint32_t spike = 0;
+    if (live) {
+        for (uint32_t g = 0; g < P.n_groups; ++g) {
+            const EvolveGroup& G = P.g[g];
+            const NeuronGroupView& v = G.v;
+            if (gid < v.offset || gid >= v.offset + v.num_neurons) continue;
+            const uint32_t lid = gid - v.offset;
+            for (uint32_t s = 0; s < G.n_fin_e; ++s) ge = combine_parts(ge, G.fin_e[s], fin[s], lane);
+            for (uint32_t s = 0; s < G.n_fin_i; ++s) gi_ = combine_parts(gi_, G.fin_i[s], fin[kMaxSources + s], lane);
+            eif_step(v, ge, gi_, istim, V, refr, spike);
+            float bg_ge, bg_gi;
+            const uint32_t* te = v.poisson_table_excitatory_len <= kTabCap ? tab[g][0] : v.poisson_table_excitatory;
+            const uint32_t* ti = v.poisson_table_inhibitory_len <= kTabCap ? tab[g][1] : v.poisson_table_inhibitory;
+            background_advance(v, gid, step, clk, te, ti, re, ri, de, di, bg_ge, bg_gi);
             v.membrane_potentials[lid] = V;
             v.refractory_counter[lid] = refr;
             v.spikes[lid] = spike;
-            float bg_ge, bg_gi;
-            background_step(v, lid, step, clk, bg_ge, bg_gi);
+            background_store(v, lid, re, ri, de, di, bg_ge, bg_gi);
             v.g_excitatory[lid] = bg_ge;
             v.g_inhibitory[lid] = bg_gi;
         }
-        for (uint32_t t = 0; t < P.n_ts; ++t) {
+#pragma unroll
+        for (int t = 0; t < kMaxTraceSets; ++t) {
             const TraceSet& T = P.ts[t];
-            if (gid < T.pre_offset || gid >= T.pre_offset + T.n_pre) continue;
-            connection_trace_step(T.r, T.d, T.x, T.decay_r, T.decay_d, T.K, gid - T.pre_offset, spike);
+            if ((uint32_t)t < P.n_ts && gid >= T.pre_offset && gid < T.pre_offset + T.n_pre) {
+                const uint32_t i = gid - T.pre_offset;
+                float dd = flush_trace(tr_d[t] * T.decay_d);
+                float rr = flush_trace(tr_r[t] * T.decay_r);
+                if (spike == 1) {
+                    rr += T.K;
+                    dd += T.K;
+                }
+                T.d[i] = dd;
+                T.r[i] = rr;
+                T.x[i] = dd - rr;
+            }
         }
-        for (uint32_t q = 0; q < P.n_pt; ++q) {
+#pragma unroll
+        for (int q = 0; q < kMaxPlast; ++q) {
             const PlastTraces& Q = P.pt[q];
+            if ((uint32_t)q >= P.n_pt) continue;
             if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
                 const uint32_t i = gid - Q.pre_offset;
-                Q.pre[i] = trace_step(Q.pre[i], spike_prev, Q.pre_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_pre[i] = trace_step(Q.trip_pre[i], spike_prev, Q.trip_pre_decay);
+                Q.pre[i] = trace_step(q_pre[q], spike_prev, Q.pre_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_pre[i] = trace_step(q_tpre[q], spike_prev, Q.trip_pre_decay);
             }
             if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
                 const uint32_t j = gid - Q.post_offset;
-                Q.post[j] = trace_step(Q.post[j], spike_prev, Q.post_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_post[j] = trace_step(Q.trip_post[j], spike_prev, Q.trip_post_decay);
+                Q.post[j] = trace_step(q_post[q], spike_prev, Q.post_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_post[j] = trace_step(q_tpost[q], spike_prev, Q.trip_post_decay);
             }
         }
     }
     // Spike bits of this step (64 global ids per wave -> two 32-bit words).
     const uint64_t ballot = __ballot(spike == 1);
-    const uint32_t lane = threadIdx.x & 63;
     if (lane == 0 || lane == 32) {
         const uint32_t word = gid >> 5;
         if (word < P.words) {
@@ -196,17 +337,15 @@ __global__ __launch_bounds__(256) void k_evolve(EvolvePlan P) {
             if (P.spike_bits_ring) P.spike_bits_ring[(size_t)(step % P.interval) * P.words + word] = bits;
         }
     }
-    if (P.v_ring && gid < P.total_neurons) {
-        const int32_t slot = P.vrec_index[gid];
-        if (slot >= 0) P.v_ring[(size_t)(step % P.interval) * P.n_vrec + slot] = V;
-    }
+    if (vslot >= 0) P.v_ring[(size_t)(step % P.interval) * P.n_vrec + vslot] = V;
 }
 
 // =======================================================================================
 // k_gemv: FP64 partial sums, one wave per tile of (row chunk, 256 columns).
 //   partial[chunk][j] = sum_{i in chunk, increasing} (double)x'_i * (double)W[i][j]
 // Products of two binary32 numbers are exact in binary64, so the only rounding is the FP64
-// accumulation in this fixed order, then one rounding to FP32 when g is finalised.
+// accumulation in this fixed order, the part sums of the combination, then one rounding to
+// FP32 when g is finalised.
 // =======================================================================================
 template <bool kScaled>
 __device__ __forceinline__ float row_x(const float* __restrict__ x, const float* __restrict__ s, uint32_t i) {
@@ -215,7 +354,18 @@ __device__ __forceinline__ float row_x(const float* __restrict__ x, const float*
 
 // One tile: every index is wave-uniform, so x (and s) come through the scalar cache and the
 // eight float4 row loads of an iteration are all in flight together.
-template <bool kScaled>
+template <bool kNT>
+__device__ __forceinline__ float4 load_w4(const float* p) {
+    if constexpr (kNT) {
+        typedef float f4v __attribute__((ext_vector_type(4)));
+        const f4v t = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+        return make_float4(t.x, t.y, t.z, t.w);
+    } else {
+        return *reinterpret_cast<const float4*>(p);
+    }
+}
+
+template <bool kScaled, bool kNT>
 __device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const float* __restrict__ x,
                                           const float* __restrict__ s, double* __restrict__ partials,
                                           uint32_t ld, uint32_t i0, uint32_t i1, uint32_t j0, uint32_t chunk) {
@@ -225,7 +375,7 @@ __device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const flo
     for (; i + 8 <= i1; i += 8) {
         float4 w[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = *reinterpret_cast<const float4*>(wp + (size_t)k * ld);
+        for (int k = 0; k < 8; ++k) w[k] = load_w4<kNT>(wp + (size_t)k * ld);
         float xs[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) xs[k] = row_x<kScaled>(x, s, i + k);
@@ -240,7 +390,7 @@ __device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const flo
         wp += (size_t)8 * ld;
     }
     for (; i < i1; ++i) {
-        const float4 w = *reinterpret_cast<const float4*>(wp);
+        const float4 w = load_w4<kNT>(wp);
         const double xd = (double)row_x<kScaled>(x, s, i);
         a0 += xd * (double)w.x;
         a1 += xd * (double)w.y;
@@ -255,6 +405,7 @@ __device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const flo
     out[3] = a3;
 }
 
+template <bool kNT>
 __global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
     const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63;
@@ -269,8 +420,8 @@ __global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
     const uint32_t i0 = chunk * C.row_block;
     const uint32_t i1 = min(i0 + C.row_block, C.n_pre);
     if (j0 >= C.n_post) return;
-    if (C.row_scale) gemv_tile<true>(C.W, C.x, C.row_scale, C.partials, C.ld, i0, i1, j0, chunk);
-    else gemv_tile<false>(C.W, C.x, nullptr, C.partials, C.ld, i0, i1, j0, chunk);
+    if (C.row_scale) gemv_tile<true, kNT>(C.W, C.x, C.row_scale, C.partials, C.ld, i0, i1, j0, chunk);
+    else gemv_tile<false, kNT>(C.W, C.x, nullptr, C.partials, C.ld, i0, i1, j0, chunk);
 }
 
 // Scalar-load variant for row strides that are not 16-byte aligned (reference layout with
@@ -422,10 +573,11 @@ __device__ __forceinline__ uint32_t block_compact_bits(const uint32_t* bits, uin
     return total;
 }
 
-// LTP entries (i, js[0..n)) of a quiet row, in list order; the loads are issued together.
-constexpr int kLtpBatch = 8;
+// LTP entries (i, js[0..n)) of a quiet row, in list order, with the post factors fs[0..n)
+// (ltp_factor of each j); the loads are issued together.
+constexpr int kLtpBatch = 16;
 __device__ __forceinline__ void quiet_row_batch(const PlastDev& D, uint32_t i, float* row, float si, float pre_i,
-                                                const uint32_t* js, int n, double& dsum) {
+                                                const uint32_t* js, const float* fs, int n, double& dsum) {
     float m[kLtpBatch];
 #pragma unroll
     for (int k = 0; k < kLtpBatch; ++k)
@@ -433,8 +585,7 @@ __device__ __forceinline__ void quiet_row_batch(const PlastDev& D, uint32_t i, f
 #pragma unroll
     for (int k = 0; k < kLtpBatch; ++k) {
         if (k >= n || js[k] == i) continue;
-        const float fac = ltp_factor(D, js[k]);
-        const float delta = D.kind == kPlastTriplet ? pre_i * fac : D.p.ltp_A * pre_i;
+        const float delta = D.kind == kPlastTriplet ? pre_i * fs[k] : D.p.ltp_A * pre_i;
         const float mn = apply_delta(D, m[k], si, delta);
         row[js[k]] = mn;
         if (D.lazy) dsum += (double)mn - (double)m[k];
@@ -442,23 +593,29 @@ __device__ __forceinline__ void quiet_row_batch(const PlastDev& D, uint32_t i, f
 }
 
 // Row i whose pre neuron did NOT spike: only the LTP entries of this step's spiking posts, in
-// increasing post order (LDS list; straight from the spike bits when the list overflowed).
-// Lazy: R_i += sum of the changes of M.
+// increasing post order (LDS list of posts and their factors; straight from the spike bits when
+// the list overflowed).  Lazy: R_i += sum of the changes of M.
 __device__ __forceinline__ void plasticity_quiet_row(const PlastDev& D, uint32_t i, const uint32_t* list,
-                                                     uint32_t count) {
+                                                     const float* facs, uint32_t count) {
     const bool lazy = D.lazy != 0;
     const float si = lazy ? D.s[i] : 1.0f;
+    const double R0 = lazy ? D.R[i] : 0.0;
+    const float target = lazy ? D.target[i] : 0.0f;
     double dsum = 0.0;
     if (count && (!lazy || si != 0.0f)) {
         float* row = D.W + (size_t)i * D.ld;
         const float pre_i = D.pre[i];
         uint32_t js[kLtpBatch];
+        float fs[kLtpBatch];
         if (count <= kListCap) {
             for (uint32_t k0 = 0; k0 < count; k0 += kLtpBatch) {
                 const int n = (int)min((uint32_t)kLtpBatch, count - k0);
 #pragma unroll
-                for (int k = 0; k < kLtpBatch; ++k) js[k] = k < n ? list[k0 + k] : 0u;
-                quiet_row_batch(D, i, row, si, pre_i, js, n, dsum);
+                for (int k = 0; k < kLtpBatch; ++k) {
+                    js[k] = k < n ? list[k0 + k] : 0u;
+                    fs[k] = k < n ? facs[k0 + k] : 0.0f;
+                }
+                quiet_row_batch(D, i, row, si, pre_i, js, fs, n, dsum);
             }
         } else {
             const uint32_t g0 = D.post_offset, g1 = D.post_offset + D.n_post;
@@ -470,15 +627,16 @@ __device__ __forceinline__ void plasticity_quiet_row(const PlastDev& D, uint32_t
                     const uint32_t b = (uint32_t)__ffs(bits) - 1;
                     bits &= bits - 1;
                     js[0] = w * 32 + b - g0;
-                    quiet_row_batch(D, i, row, si, pre_i, js, 1, dsum);
+                    fs[0] = ltp_factor(D, js[0]);
+                    quiet_row_batch(D, i, row, si, pre_i, js, fs, 1, dsum);
                 }
             }
         }
     }
     if (lazy) {
-        const double R = D.R[i] + dsum;
+        const double R = R0 + dsum;
         D.R[i] = R;
-        if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
+        if (R > 0.0) D.s[i] = (float)((double)target / R);
     }
 }
 
@@ -488,6 +646,7 @@ __device__ __forceinline__ void plasticity_quiet_row(const PlastDev& D, uint32_t
 __device__ __forceinline__ void plasticity_spiking_row(const PlastDev& D, uint32_t i, double* lds4) {
     const bool lazy = D.lazy != 0;
     const float si = lazy ? D.s[i] : 1.0f;
+    const float target = lazy ? D.target[i] : 0.0f;
     const bool update = !lazy || si != 0.0f;
     const float fac_i = ltd_factor(D, i);
     float* row = D.W + (size_t)i * D.ld;
@@ -533,7 +692,7 @@ __device__ __forceinline__ void plasticity_spiking_row(const PlastDev& D, uint32
         const double R = block_reduce_sum(acc, lds4);
         if (threadIdx.x == 0) {
             D.R[i] = R;
-            if (R > 0.0) D.s[i] = (float)((double)D.target[i] / R);
+            if (R > 0.0) D.s[i] = (float)((double)target / R);
         }
     }
 }
@@ -565,6 +724,7 @@ __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
     __shared__ double lds4[4];
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t list[kListCap];
+    __shared__ float facs[kListCap];
     const uint32_t b = blockIdx.x;
     if (b < P.row_blocks) {
         const uint32_t r0 = b * blockDim.x;
@@ -574,9 +734,11 @@ __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
             if (r0 < base + D.n_pre && r0 + blockDim.x > base) {
                 const uint32_t count =
                     block_compact_bits(D.bits, D.post_offset, D.post_offset + D.n_post, list, kListCap, wsum);
+                for (uint32_t e = threadIdx.x; e < min(count, kListCap); e += blockDim.x) facs[e] = ltp_factor(D, list[e]);
+                __syncthreads();
                 const uint32_t r = r0 + threadIdx.x;
                 if (r >= base && r < base + D.n_pre && D.pre_spikes[r - base] != 1)
-                    plasticity_quiet_row(D, r - base, list, count);
+                    plasticity_quiet_row(D, r - base, list, facs, count);
                 __syncthreads();
             }
             base += D.n_pre;
@@ -832,7 +994,7 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 int launch_evolve(const EvolvePlan& plan, bn_stream_t s) {
     static_assert(sizeof(EvolvePlan) < 3800, "kernel argument too large");
     if (plan.total_neurons == 0) return 0;
-    hipLaunchKernelGGL(k_evolve, dim3(cdiv(plan.total_neurons, 256)), dim3(256), 0, s, plan);
+    hipLaunchKernelGGL(k_evolve, dim3(cdiv(plan.total_neurons, kEvolveNeurons)), dim3(kFinParts * 64), 0, s, plan);
     return (int)hipGetLastError();
 }
 
@@ -846,8 +1008,10 @@ static bool gemv_vector_ok(const GemvPlan& plan) {
 
 int launch_gemv(const GemvPlan& plan, bn_stream_t s) {
     if (plan.total_tiles == 0) return 0;
-    if (gemv_vector_ok(plan))
-        hipLaunchKernelGGL(k_gemv, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
+    if (gemv_vector_ok(plan) && plan.nontemporal)
+        hipLaunchKernelGGL(k_gemv<true>, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
+    else if (gemv_vector_ok(plan))
+        hipLaunchKernelGGL(k_gemv<false>, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
     else
         hipLaunchKernelGGL(k_gemv_scalar, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
     return (int)hipGetLastError();

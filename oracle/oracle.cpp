@@ -11,6 +11,9 @@
 
 namespace {
 
+// Parts the propagate chunk partials are combined in (the engine sums one part per wave).
+constexpr uint32_t kFinParts = 8;
+
 // ---------------------------------------------------------------------------------------
 // Primitives
 // ---------------------------------------------------------------------------------------
@@ -344,16 +347,27 @@ struct Sim {
             double acc[NB], part[NB];
             for (uint32_t j = j0; j < j1; ++j) acc[j - j0] = (double)g[C.post_off + j];
             if (kF32) {
-                for (uint32_t i0 = 0; i0 < C.n_pre; i0 += C.row_block) {
-                    const uint32_t i1 = std::min(i0 + C.row_block, C.n_pre);
-                    for (uint32_t j = j0; j < j1; ++j) part[j - j0] = 0.0;
-                    for (uint32_t i = i0; i < i1; ++i) {
-                        const double xi = (double)xs[i];
-                        if (xi == 0.0 && !p.dense_gemv) continue;  // adds +0 in the engine: no effect
-                        const R* row = &C.W[(size_t)i * C.n_post];
-                        for (uint32_t j = j0; j < j1; ++j) part[j - j0] += xi * (double)row[j];
+                // Engine order: chunk partials (row_block rows each, increasing i, from 0.0);
+                // the chunks cut into kFinParts contiguous parts of ceil(chunks/kFinParts),
+                // each part summed in chunk order from 0.0; then g + part_0 + part_1 + ...
+                const uint32_t chunks = (C.n_pre + C.row_block - 1) / C.row_block;
+                const uint32_t per = (chunks + kFinParts - 1) / kFinParts;
+                double qsum[NB];
+                for (uint32_t c0 = 0; c0 < chunks; c0 += per) {
+                    const uint32_t c1 = std::min(c0 + per, chunks);
+                    for (uint32_t j = j0; j < j1; ++j) qsum[j - j0] = 0.0;
+                    for (uint32_t c = c0; c < c1; ++c) {
+                        const uint32_t i0 = c * C.row_block, i1 = std::min(i0 + C.row_block, C.n_pre);
+                        for (uint32_t j = j0; j < j1; ++j) part[j - j0] = 0.0;
+                        for (uint32_t i = i0; i < i1; ++i) {
+                            const double xi = (double)xs[i];
+                            if (xi == 0.0 && !p.dense_gemv) continue;  // adds +0 in the engine: no effect
+                            const R* row = &C.W[(size_t)i * C.n_post];
+                            for (uint32_t j = j0; j < j1; ++j) part[j - j0] += xi * (double)row[j];
+                        }
+                        for (uint32_t j = j0; j < j1; ++j) qsum[j - j0] += part[j - j0];
                     }
-                    for (uint32_t j = j0; j < j1; ++j) acc[j - j0] += part[j - j0];
+                    for (uint32_t j = j0; j < j1; ++j) acc[j - j0] += qsum[j - j0];
                 }
             } else {
                 for (uint32_t i = 0; i < C.n_pre; ++i) {

@@ -58,6 +58,24 @@ def test_fused_step_bit_exact(tmp_path, graph):
     sim.close()
 
 
+@pytest.mark.parametrize("row_blocks,nontemporal", [([8, 8, 16, 8], 1), ([3, 5, 7, 1], 0), ([512, 512, 1, 2], 1)],
+                         ids=["rb8-nt", "rb-odd", "rb-mixed-nt"])
+def test_gemv_tilings_bit_exact(tmp_path, row_blocks, nontemporal):
+    # Row blocks change the number of chunk partials per post neuron (1 to 134 here, up to 17
+    # per combination part) and so the order of the FP64 sums; the oracle follows the same
+    # tiling.  Non-temporal weight loads must not change any value.
+    cfg = small_config(steps=300)
+    cfg["gemv_row_blocks"] = row_blocks
+    cfg["gemv_nontemporal_loads"] = nontemporal
+    path = _write(tmp_path, cfg)
+    sim = engine().Simulation(path, flags=("no_monitors", "quiet", "lazy_normalize"))
+    assert tuple(sim.info.gemv_row_block) == tuple(row_blocks)
+    orc = oracle_for(path, sim, lazy=True)
+    _run_lockstep(sim, orc, 150, check_every=5)
+    _check_traces_and_weights(sim, orc)
+    sim.close()
+
+
 def test_component_path_bit_exact(tmp_path):
     path = _write(tmp_path, small_config(steps=600))
     sim = engine().Simulation(path, flags=("no_monitors", "quiet", "component_path"))
