@@ -31,6 +31,7 @@ constexpr int kMaxTraceSets = 4;
 constexpr int kGemvTileCols = 256;   // 64 lanes x float4
 constexpr int kFinParts = 8;         // parts the chunk partials of a source are combined in
 constexpr int kEvolveNeurons = 64;   // neurons per evolve block (kFinParts waves each)
+constexpr int kPlastQuietRows = 64;  // rows per quiet-row block of the plasticity kernel
 constexpr int kNormThreads = 256;    // threads per row in the row sums
 constexpr int kNormRegs = 64;        // columns per thread kept in registers (n_post <= 16384)
 
@@ -140,7 +141,7 @@ struct StimTail {
 
 struct PlastStepPlan {
     PlastPlan pl;
-    uint32_t row_blocks;         // blocks running one thread per row
+    uint32_t row_blocks;         // blocks of kPlastQuietRows rows (rows whose pre did not spike)
     uint32_t spiking_blocks;     // blocks sharing the rows whose pre neuron spiked
     StimTail tail;
 };

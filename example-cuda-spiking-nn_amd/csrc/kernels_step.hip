@@ -592,52 +592,55 @@ __device__ __forceinline__ void quiet_row_batch(const PlastDev& D, uint32_t i, f
     }
 }
 
-// Row i whose pre neuron did NOT spike: only the LTP entries of this step's spiking posts, in
-// increasing post order (LDS list of posts and their factors; straight from the spike bits when
-// the list overflowed).  Lazy: R_i += sum of the changes of M.
-__device__ __forceinline__ void plasticity_quiet_row(const PlastDev& D, uint32_t i, const uint32_t* list,
-                                                     const float* facs, uint32_t count) {
+// Row i whose pre neuron did NOT spike: the LTP entries of this step's spiking posts.  The list
+// of spiking posts (increasing post order, LDS, with their factors) is cut into kQuietParts
+// contiguous parts of ceil(count / kQuietParts); this call does part q in list order (straight
+// from the spike bits when the list overflowed) and returns the part's sum of the changes of M
+// (lazy), summed from 0.0.
+constexpr uint32_t kQuietParts = 4;
+constexpr uint32_t kQuietRows = kPlastQuietRows;
+__device__ __forceinline__ double plasticity_quiet_part(const PlastDev& D, uint32_t i, const uint32_t* list,
+                                                        const float* facs, uint32_t count, uint32_t q) {
     const bool lazy = D.lazy != 0;
     const float si = lazy ? D.s[i] : 1.0f;
-    const double R0 = lazy ? D.R[i] : 0.0;
-    const float target = lazy ? D.target[i] : 0.0f;
     double dsum = 0.0;
-    if (count && (!lazy || si != 0.0f)) {
-        float* row = D.W + (size_t)i * D.ld;
-        const float pre_i = D.pre[i];
-        uint32_t js[kLtpBatch];
-        float fs[kLtpBatch];
-        if (count <= kListCap) {
-            for (uint32_t k0 = 0; k0 < count; k0 += kLtpBatch) {
-                const int n = (int)min((uint32_t)kLtpBatch, count - k0);
+    const uint32_t per = (count + kQuietParts - 1) / kQuietParts;
+    const uint32_t e0 = q * per, e1 = min(count, e0 + per);
+    if (e0 >= e1 || (lazy && si == 0.0f)) return 0.0;
+    float* row = D.W + (size_t)i * D.ld;
+    const float pre_i = D.pre[i];
+    uint32_t js[kLtpBatch];
+    float fs[kLtpBatch];
+    if (count <= kListCap) {
+        for (uint32_t k0 = e0; k0 < e1; k0 += kLtpBatch) {
+            const int n = (int)min((uint32_t)kLtpBatch, e1 - k0);
 #pragma unroll
-                for (int k = 0; k < kLtpBatch; ++k) {
-                    js[k] = k < n ? list[k0 + k] : 0u;
-                    fs[k] = k < n ? facs[k0 + k] : 0.0f;
-                }
-                quiet_row_batch(D, i, row, si, pre_i, js, fs, n, dsum);
+            for (int k = 0; k < kLtpBatch; ++k) {
+                js[k] = k < n ? list[k0 + k] : 0u;
+                fs[k] = k < n ? facs[k0 + k] : 0.0f;
             }
-        } else {
-            const uint32_t g0 = D.post_offset, g1 = D.post_offset + D.n_post;
-            for (uint32_t w = g0 >> 5; w <= (g1 - 1) >> 5; ++w) {
-                uint32_t bits = D.bits[w];
-                if (w == (g0 >> 5)) bits &= 0xFFFFFFFFu << (g0 & 31);
-                if (w == ((g1 - 1) >> 5)) bits &= 0xFFFFFFFFu >> (31 - ((g1 - 1) & 31));
-                while (bits) {
-                    const uint32_t b = (uint32_t)__ffs(bits) - 1;
-                    bits &= bits - 1;
+            quiet_row_batch(D, i, row, si, pre_i, js, fs, n, dsum);
+        }
+    } else {
+        const uint32_t g0 = D.post_offset, g1 = D.post_offset + D.n_post;
+        uint32_t rank = 0;
+        for (uint32_t w = g0 >> 5; w <= (g1 - 1) >> 5 && rank < e1; ++w) {
+            uint32_t bits = D.bits[w];
+            if (w == (g0 >> 5)) bits &= 0xFFFFFFFFu << (g0 & 31);
+            if (w == ((g1 - 1) >> 5)) bits &= 0xFFFFFFFFu >> (31 - ((g1 - 1) & 31));
+            while (bits) {
+                const uint32_t b = (uint32_t)__ffs(bits) - 1;
+                bits &= bits - 1;
+                if (rank >= e0 && rank < e1) {
                     js[0] = w * 32 + b - g0;
                     fs[0] = ltp_factor(D, js[0]);
                     quiet_row_batch(D, i, row, si, pre_i, js, fs, 1, dsum);
                 }
+                ++rank;
             }
         }
     }
-    if (lazy) {
-        const double R = R0 + dsum;
-        D.R[i] = R;
-        if (R > 0.0) D.s[i] = (float)((double)target / R);
-    }
+    return dsum;
 }
 
 // Row i whose pre neuron spiked, by the whole block: per entry the LTP increment (if the post
@@ -717,7 +720,7 @@ __device__ __forceinline__ void stimulus_and_clock(const StimTail& T) {
     clk->clock_step = s + 1;
 }
 
-// Blocks [0, row_blocks): one thread per row of the concatenated pre rows of all plasticities
+// Blocks [0, row_blocks): kQuietRows rows each of the concatenated pre rows of all plasticities
 // (rows whose pre did not spike).  Blocks after: the rows whose pre spiked, one block each.
 // The last block also applies the stimulus and advances the clock (no block reads the clock).
 __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
@@ -725,20 +728,42 @@ __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t list[kListCap];
     __shared__ float facs[kListCap];
+    __shared__ double qsum[kQuietParts][64];
     const uint32_t b = blockIdx.x;
     if (b < P.row_blocks) {
-        const uint32_t r0 = b * blockDim.x;
+        // kQuietRows rows per block, lane = row; wave q applies part q of the spiking-post list.
+        const uint32_t r0 = b * kQuietRows;
+        const uint32_t q = threadIdx.x >> 6, lane = threadIdx.x & 63;
         uint32_t base = 0;
         for (uint32_t k = 0; k < P.pl.n; ++k) {
             const PlastDev& D = P.pl.p[k];
-            if (r0 < base + D.n_pre && r0 + blockDim.x > base) {
+            if (r0 < base + D.n_pre && r0 + kQuietRows > base) {
                 const uint32_t count =
                     block_compact_bits(D.bits, D.post_offset, D.post_offset + D.n_post, list, kListCap, wsum);
                 for (uint32_t e = threadIdx.x; e < min(count, kListCap); e += blockDim.x) facs[e] = ltp_factor(D, list[e]);
                 __syncthreads();
-                const uint32_t r = r0 + threadIdx.x;
-                if (r >= base && r < base + D.n_pre && D.pre_spikes[r - base] != 1)
-                    plasticity_quiet_row(D, r - base, list, facs, count);
+                const uint32_t r = r0 + lane;
+                const bool mine = r >= base && r < base + D.n_pre && D.pre_spikes[r - base] != 1;
+                double R0 = 0.0;
+                float target = 0.0f;
+                if (D.lazy && q == 0 && mine) {
+                    R0 = D.R[r - base];
+                    target = D.target[r - base];
+                }
+                const double dq = mine && count ? plasticity_quiet_part(D, r - base, list, facs, count, q) : 0.0;
+                if (D.lazy) {
+                    qsum[q][lane] = dq;
+                    __syncthreads();
+                    if (q == 0 && mine) {
+                        // R_i += the part sums in part order (non-empty parts only).
+                        const uint32_t per = (count + kQuietParts - 1) / kQuietParts;
+                        double R = R0;
+                        for (uint32_t p = 0; p < kQuietParts; ++p)
+                            if (p * per < count) R += qsum[p][lane];
+                        D.R[r - base] = R;
+                        if (R > 0.0) D.s[r - base] = (float)((double)target / R);
+                    }
+                }
                 __syncthreads();
             }
             base += D.n_pre;

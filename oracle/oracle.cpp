@@ -13,6 +13,8 @@ namespace {
 
 // Parts the propagate chunk partials are combined in (the engine sums one part per wave).
 constexpr uint32_t kFinParts = 8;
+// Parts the spiking-post list of a non-spiking row's LTP is cut into (one wave each).
+constexpr uint32_t kQuietParts = 4;
 
 // ---------------------------------------------------------------------------------------
 // Primitives
@@ -478,15 +480,20 @@ struct Sim {
         for (uint32_t j = 0; j < nq; ++j)
             if (post_sp[j] == 1) posts.push_back(j);
         // LTP over every spiking post column (rows of spiking pres included: their sums are
-        // recomputed below), keeping each row's change of M in increasing post order.
-        std::vector<double> dsum(np, 0.0);
-        for (const uint32_t j : posts) {
+        // recomputed below).  Engine order of each row's change of M: the spiking-post list
+        // (increasing post order) cut into kQuietParts contiguous parts of ceil(count/4); each
+        // part's changes summed in list order from 0.0; R += part_0 + part_1 + ... (non-empty).
+        const uint32_t count = (uint32_t)posts.size();
+        const uint32_t per = (count + kQuietParts - 1) / kQuietParts;
+        std::vector<double> dq((size_t)kQuietParts * np, 0.0);
+        for (uint32_t e = 0; e < count; ++e) {
+            const uint32_t j = posts[e], q = e / per;
             for (uint32_t i = 0; i < np; ++i) {
                 if (i == j || P.s[i] == 0.0f) continue;
                 float& m = M[(size_t)i * nq + j];
                 const float w = clampw<float>(P.s[i] * m + (float)ltp_delta(P, i, j), (float)P.wmin, (float)P.wmax);
                 const float mn = w / P.s[i];
-                dsum[i] += (double)mn - (double)m;
+                dq[(size_t)q * np + i] += (double)mn - (double)m;
                 m = mn;
             }
         }
@@ -501,7 +508,10 @@ struct Sim {
                     }
                 P.Rs[i] = canonical_row_sum(row, nq);
             } else {
-                P.Rs[i] = P.Rs[i] + dsum[i];
+                double Rn = P.Rs[i];
+                for (uint32_t q = 0; q < kQuietParts; ++q)
+                    if (q * per < count) Rn += dq[(size_t)q * np + i];
+                P.Rs[i] = Rn;
             }
             if (P.Rs[i] > 0.0) P.s[i] = (float)((double)P.target[i] / P.Rs[i]);
         }
