@@ -1,15 +1,16 @@
 """Benchmark of the per-step hot path (one MI355X).
 
-Workload: the 10,125-neuron plastic network (8,100 E + 2,025 I on a 1 mm square, four dense
-[pre][post] FP32 matrices of 410 MB in total, triplet STDP on E->E and iSTDP on I->E, 1 kHz
-Poisson background, seed 1) — configs/net10125_stdp.json.  A "step" is one pass of
-System::step_the_clock (evolve -> propagate -> stimulus -> plasticity -> monitors) over the whole
-network, with the weights resident in HBM.  Metric: simulated steps per second.
+Default workload: the 3,125-neuron plastic network (2,500 E on a 50x50 grid + 625 I on 25x25,
+1 mm square, four dense [pre][post] FP32 matrices of 39 MB in total, triplet STDP on E->E and
+iSTDP on I->E, 1 kHz Poisson background, seed 0) — configs/net3125_stdp.json.  A "step" is one
+pass of System::step_the_clock (evolve -> propagate -> stimulus -> plasticity -> monitors) over
+the whole network, with the weights resident in device memory.  Metric: simulated steps per
+second.  --config configs/net10125_stdp.json runs the 10,125-neuron (410 MB) network instead.
 
 Prints ONE JSON line.  roofline = the propagate (GEMV) kernel's algorithmic bytes per launch
 (the four weight matrices plus the pre-synaptic vectors) over its average duration, measured
-with HIP events on the engine's stream; cpu_baseline = the oracle's FP32 C++/OpenMP step loop
-on this host, timed on a bounded sample of the same workload.
+with HIP events around back-to-back launches on the engine's stream; cpu_baseline = the
+oracle's FP32 C++/OpenMP step loop on this host, timed on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -30,12 +31,16 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=2000)
-    ap.add_argument("--config", default=os.path.join(REPO, "configs", "net10125_stdp.json"))
+    ap.add_argument("--config", default=os.path.join(REPO, "configs", "net3125_stdp.json"))
     ap.add_argument("--normalization", choices=("lazy", "faithful"), default="lazy")
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run 5 plain device copies of this many MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration_mib:
+        return copy_calibration(args.copy_calibration_mib)
     if args.gpus != 1:
         print(f"bench.py: this engine runs one network on one GPU; --gpus {args.gpus} is not supported "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
@@ -51,7 +56,8 @@ def main() -> int:
     ms = sim.run_timed(args.steps)  # synchronises before and after, HIP events on the engine stream
     steps_per_s = args.steps / (ms / 1000.0)
 
-    # Dominant kernel: live HIP-event timing around every propagate (GEMV) launch of real steps.
+    # Dominant kernel: HIP events around back-to-back propagate (GEMV) launches over the current
+    # weights (the simulation's pending partial sums are saved and restored by the engine).
     k_ms, s_ms = sim.time_propagate_kernel(args.kernel_steps)
     k_avg_s = k_ms / 1000.0 / args.kernel_steps
     n_w = sum(int(info.weight_pre[k]) * int(info.weight_post[k]) for k in range(4))
@@ -61,19 +67,22 @@ def main() -> int:
     rate_hz = float(state["spikes"].mean()) / cfg["timestep"]
     sim.close()
 
+    n_total = int(info.total_neurons)
+    name = os.path.splitext(os.path.basename(args.config))[0]
     traffic = None
-    tpath = os.path.join(REPO, "profiles", "gemv_traffic_net10125.json")
+    tpath = os.path.join(REPO, "profiles", f"k_gemv_{name}_hbm_traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
-        traffic = t.get("hbm_bytes_per_launch")
+        if t.get("neurons") == n_total and t.get("weight_entries") == n_w:
+            traffic = t.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.config, cfg, args.cpu_seconds, info)
 
-    n_total = int(info.total_neurons)
+    n_e, n_i = int(cfg["num_excitatory_neurons"]), int(cfg["num_inhibitory_neurons"])
     line = {
-        "metric": "sim steps/s (10,125-nrn STDP net)",
+        "metric": f"sim steps/s ({n_total:,}-nrn STDP net)",
         "value": round(steps_per_s, 2),
         "unit": "steps/s",
         "n_gpus": 1,
@@ -85,8 +94,8 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32 state/weights, f64 GEMV accumulation",
         "data": "synthetic: seeded Philox Poisson background, connectivity from the reference generator",
-        "config": {"workload": "net10125_stdp (8,100 E + 2,025 I, 4 dense FP32 matrices, 410 MB, "
-                               "triplet STDP + iSTDP, seed 1)",
+        "config": {"workload": f"{name} ({n_e:,} E + {n_i:,} I, 4 dense FP32 matrices, {4 * n_w / 1e6:.1f} MB, "
+                               f"triplet STDP + iSTDP, seed {cfg['rng_seed']}, {args.normalization} normalisation)",
                    "neurons": n_total, "weight_entries": n_w, "normalization": args.normalization,
                    "kernels_per_step": int(info.kernels_per_step), "parallelism": "single GPU",
                    "rate_hz_last_step": round(rate_hz, 2)},
@@ -97,6 +106,21 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    return 0
+
+
+def copy_calibration(mib: int) -> int:
+    """Five device-to-device copies of `mib` MiB: the reference point that scales the HBM byte
+    counters (FETCH_SIZE, WRITE_SIZE) of a profiled run to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}), flush=True)
     return 0
 
 

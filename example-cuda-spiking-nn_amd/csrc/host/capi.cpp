@@ -493,23 +493,46 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
         System& sys = *sim->system;
         if (!sys.fused()) throw Error(BN_ERR_RUNTIME, "kernel timing needs the fused step");
         hipStream_t s = sim->state->stream();
-        std::vector<hipEvent_t> ev(2 * (size_t)num_steps + 2);
+        hipEvent_t ev[4];
         for (auto& e : ev) BN_HIP_CHECK(hipEventCreate(&e), "hipEventCreate");
+        // (1) num_steps whole steps launched one by one (no graph).
         BN_HIP_CHECK(hipStreamSynchronize(s), "pre-timing sync");
         BN_HIP_CHECK(hipEventRecord(ev[0], s), "hipEventRecord");
-        for (uint32_t k = 0; k < num_steps; ++k) sys.launch_fused_step_eager(ev[2 + 2 * k], ev[3 + 2 * k]);
+        for (uint32_t k = 0; k < num_steps; ++k) sys.launch_fused_step_eager(nullptr, nullptr);
         BN_HIP_CHECK(hipEventRecord(ev[1], s), "hipEventRecord");
-        BN_HIP_CHECK(hipEventSynchronize(ev[1]), "hipEventSynchronize");
-        float total = 0.0f, kern = 0.0f;
-        BN_HIP_CHECK(hipEventElapsedTime(&total, ev[0], ev[1]), "hipEventElapsedTime");
-        for (uint32_t k = 0; k < num_steps; ++k) {
-            float t = 0.0f;
-            BN_HIP_CHECK(hipEventElapsedTime(&t, ev[2 + 2 * k], ev[3 + 2 * k]), "hipEventElapsedTime");
-            kern += t;
+        // (2) num_steps back-to-back propagate launches over the current weights.  The pending
+        // partial sums belong to the next step: saved before and restored after.
+        const GemvPlan& G = sys.gemv_plan();
+        size_t total = 0;
+        for (uint32_t c = 0; c < G.n_conn; ++c) total += (size_t)G.c[c].n_chunks * G.c[c].ld;
+        double* save = nullptr;
+        if (total) BN_HIP_CHECK(hipMalloc(&save, total * sizeof(double)), "hipMalloc(partials copy)");
+        size_t off = 0;
+        for (uint32_t c = 0; c < G.n_conn; ++c) {
+            const size_t n = (size_t)G.c[c].n_chunks * G.c[c].ld;
+            BN_HIP_CHECK(hipMemcpyAsync(save + off, G.c[c].partials, n * sizeof(double), hipMemcpyDeviceToDevice, s),
+                         "save partials");
+            off += n;
         }
+        BN_HIP_CHECK(hipEventRecord(ev[2], s), "hipEventRecord");
+        for (uint32_t k = 0; k < num_steps; ++k)
+            BN_HIP_CHECK((hipError_t)launch_gemv(G, s), "propagate launch");
+        BN_HIP_CHECK(hipEventRecord(ev[3], s), "hipEventRecord");
+        off = 0;
+        for (uint32_t c = 0; c < G.n_conn; ++c) {
+            const size_t n = (size_t)G.c[c].n_chunks * G.c[c].ld;
+            BN_HIP_CHECK(hipMemcpyAsync(G.c[c].partials, save + off, n * sizeof(double), hipMemcpyDeviceToDevice, s),
+                         "restore partials");
+            off += n;
+        }
+        BN_HIP_CHECK(hipStreamSynchronize(s), "timing sync");
+        if (save) (void)hipFree(save);
+        float total_ms = 0.0f, kern = 0.0f;
+        BN_HIP_CHECK(hipEventElapsedTime(&total_ms, ev[0], ev[1]), "hipEventElapsedTime");
+        BN_HIP_CHECK(hipEventElapsedTime(&kern, ev[2], ev[3]), "hipEventElapsedTime");
         for (auto& e : ev) (void)hipEventDestroy(e);
         *kernel_ms = kern;
-        *step_ms = total;
+        *step_ms = total_ms;
         sim->state->check_status();
     });
 }

@@ -398,6 +398,7 @@ public:
     bool fused() const { return fused_; }
     void launch_fused_steps(uint32_t n);           // enqueue n steps (graph or eager)
     void launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end);
+    const GemvPlan& gemv_plan() const { return gemv_; }
     uint32_t kernels_per_step() const { return kernels_per_step_; }
     // Conductances after the last propagate, for state downloads (fused path keeps them as
     // base + pending GEMV partials).

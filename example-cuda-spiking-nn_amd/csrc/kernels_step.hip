@@ -129,20 +129,20 @@ __device__ __forceinline__ void connection_trace_step(float* r, float* d, float*
 // of ceil(chunks / kFinParts); each part is summed in increasing chunk order from 0.0, then
 // g' = fl32(g + part_0 + part_1 + ...) over the non-empty parts, in FP64.
 __device__ __forceinline__ uint32_t fin_part_size(uint32_t chunks) { return (chunks + kFinParts - 1) / kFinParts; }
+constexpr uint32_t kFinBatch = 16;  // partial loads issued together
 
 __device__ __forceinline__ double fin_part_sum(const FinalizeSource& s, uint32_t j, uint32_t c0, uint32_t c1) {
     const double* p = s.partials + j;
     const size_t ld = s.ld;
     double acc = 0.0;
-    uint32_t c = c0;
-    for (; c + 8 <= c1; c += 8) {
-        double v[8];
+    for (uint32_t c = c0; c < c1; c += kFinBatch) {
+        double v[kFinBatch];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = p[(size_t)(c + k) * ld];
+        for (uint32_t k = 0; k < kFinBatch; ++k) v[k] = c + k < c1 ? p[(size_t)(c + k) * ld] : 0.0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc += v[k];
+        for (uint32_t k = 0; k < kFinBatch; ++k)
+            if (c + k < c1) acc += v[k];
     }
-    for (; c < c1; ++c) acc += p[(size_t)c * ld];
     return acc;
 }
 
@@ -178,32 +178,38 @@ __device__ __forceinline__ float trace_step(float tr, uint32_t spiked_prev, floa
 // =======================================================================================
 // k_evolve: every neuron of every group in one launch
 // =======================================================================================
-// A block owns 64 neurons and has kFinParts waves.  Wave 0 first issues every per-neuron load
-// (state, background, connection and plasticity traces); all waves stage the Poisson tables in
-// LDS and sum one part each of the propagate partials of the 64 neurons, so the partial reads
-// are spread over many waves; after the barrier wave 0 combines the parts and updates the
-// neurons from registers.
-constexpr uint32_t kTabCap = 256;  // Poisson table entries staged in LDS per table
+// A block owns 64 neurons (lane = neuron) and has kFinParts waves.  Before the barrier every
+// wave sums one part of the propagate partials of the 64 neurons (the partial reads are spread
+// over many waves), and in the same phase wave 0 loads the neuron state, wave 1 runs the
+// background update (Philox draws, Poisson counts, the four background traces) and wave 2 the
+// plasticity trace step (it needs only the previous step's spikes).  After the barrier wave 0
+// combines the parts and runs the EIF step, the connection traces and the monitors.
+__device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* table, uint32_t len,
+                                                   const uint32_t (&head)[4]) {
+    // min{k : u < table[k]}; the first four entries come from registers.
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+        if (k >= len || u < head[k]) return k;
+    return poisson_from_table(u, table + 4, len - 4) + 4;
+}
 
 __global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
     __shared__ FinParts fin;
-    __shared__ uint32_t tab[kMaxGroups][2][kTabCap];
+    __shared__ float bgl[2][kEvolveNeurons];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t gid = blockIdx.x * kEvolveNeurons + lane;
     const bool live = gid < P.total_neurons;
     const bn_device_clock clk = *P.clock;
+    const uint32_t step = clk.clock_step;
 
-    // Per-neuron inputs (wave 0).
-    float V = 0.0f, ge = 0.0f, gi_ = 0.0f, istim = 0.0f, re = 0.0f, ri = 0.0f, de = 0.0f, di = 0.0f;
-    uint32_t refr = 0, spike_prev = 0;
+    // Wave 0: per-neuron inputs of the EIF step and the connection traces.
+    float V = 0.0f, ge = 0.0f, gi_ = 0.0f, istim = 0.0f;
+    uint32_t refr = 0;
     int32_t vslot = -1;
     float tr_r[kMaxTraceSets], tr_d[kMaxTraceSets];
-    float q_pre[kMaxPlast], q_tpre[kMaxPlast], q_post[kMaxPlast], q_tpost[kMaxPlast];
 #pragma unroll
     for (int t = 0; t < kMaxTraceSets; ++t) tr_r[t] = tr_d[t] = 0.0f;
-#pragma unroll
-    for (int q = 0; q < kMaxPlast; ++q) q_pre[q] = q_tpre[q] = q_post[q] = q_tpost[q] = 0.0f;
     if (wave == 0 && live) {
         for (uint32_t g = 0; g < P.n_groups; ++g) {
             const NeuronGroupView& v = P.g[g].v;
@@ -211,14 +217,9 @@ __global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
             const uint32_t lid = gid - v.offset;
             V = v.membrane_potentials[lid];
             refr = v.refractory_counter[lid];
-            spike_prev = v.spikes[lid];
             ge = v.g_excitatory[lid];
             gi_ = v.g_inhibitory[lid];
             istim = v.i_stimulus[lid];
-            re = v.background_excitatory_postsynaptic_r_traces[lid];
-            ri = v.background_inhibitory_postsynaptic_r_traces[lid];
-            de = v.background_excitatory_postsynaptic_d_traces[lid];
-            di = v.background_inhibitory_postsynaptic_d_traces[lid];
         }
 #pragma unroll
         for (int t = 0; t < kMaxTraceSets; ++t) {
@@ -228,34 +229,10 @@ __global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
                 tr_d[t] = T.d[gid - T.pre_offset];
             }
         }
-#pragma unroll
-        for (int q = 0; q < kMaxPlast; ++q) {
-            const PlastTraces& Q = P.pt[q];
-            if ((uint32_t)q >= P.n_pt) continue;
-            if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
-                q_pre[q] = Q.pre[gid - Q.pre_offset];
-                if (Q.kind == kPlastTriplet) q_tpre[q] = Q.trip_pre[gid - Q.pre_offset];
-            }
-            if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
-                q_post[q] = Q.post[gid - Q.post_offset];
-                if (Q.kind == kPlastTriplet) q_tpost[q] = Q.trip_post[gid - Q.post_offset];
-            }
-        }
         if (P.v_ring) vslot = P.vrec_index[gid];
     }
 
-    // Poisson tables to LDS (all waves).
-    for (uint32_t g = 0; g < P.n_groups; ++g) {
-        const NeuronGroupView& v = P.g[g].v;
-        if (v.poisson_table_excitatory_len <= kTabCap)
-            for (uint32_t k = threadIdx.x; k < v.poisson_table_excitatory_len; k += blockDim.x)
-                tab[g][0][k] = v.poisson_table_excitatory[k];
-        if (v.poisson_table_inhibitory_len <= kTabCap)
-            for (uint32_t k = threadIdx.x; k < v.poisson_table_inhibitory_len; k += blockDim.x)
-                tab[g][1][k] = v.poisson_table_inhibitory[k];
-    }
-
-    // Part sums of the propagate partials (all waves).
+    // All waves: part sums of the propagate partials.
     if (live) {
         for (uint32_t g = 0; g < P.n_groups; ++g) {
             const EvolveGroup& G = P.g[g];
@@ -270,10 +247,83 @@ __global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
             }
         }
     }
+
+    // Wave 1: background (kernel_neuron_group_evolve.cu:11-83).
+    if (wave == 1 && live) {
+        for (uint32_t g = 0; g < P.n_groups; ++g) {
+            const NeuronGroupView& v = P.g[g].v;
+            if (gid < v.offset || gid >= v.offset + v.num_neurons) continue;
+            const uint32_t lid = gid - v.offset;
+            float re = v.background_excitatory_postsynaptic_r_traces[lid];
+            float ri = v.background_inhibitory_postsynaptic_r_traces[lid];
+            float de = v.background_excitatory_postsynaptic_d_traces[lid];
+            float di = v.background_inhibitory_postsynaptic_d_traces[lid];
+            uint32_t he[4], hi[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+                he[k] = k < v.poisson_table_excitatory_len ? v.poisson_table_excitatory[k] : 0u;
+                hi[k] = k < v.poisson_table_inhibitory_len ? v.poisson_table_inhibitory[k] : 0u;
+            }
+            re = flush_trace(re * v.background_excitatory_postsynaptic_r_trace_decay);
+            ri = flush_trace(ri * v.background_inhibitory_postsynaptic_r_trace_decay);
+            de = flush_trace(de * v.background_excitatory_postsynaptic_d_trace_decay);
+            di = flush_trace(di * v.background_inhibitory_postsynaptic_d_trace_decay);
+            uint32_t ke = 0, ki = 0;
+            if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
+                const U32x4 r = background_draws(clk, step, gid, v.rng_instance);
+                if (v.mean_background_excitatory_spikes > 0.0f)
+                    ke = poisson_lookup(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len, he);
+                if (v.mean_background_inhibitory_spikes > 0.0f)
+                    ki = poisson_lookup(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len, hi);
+            }
+            if (ke > 0u) {
+                const float inc = (float)ke;
+                re += inc;
+                de += inc;
+            }
+            const float bg_ge = (de - re) * v.background_excitatory_scaling_factor;
+            if (ki > 0u) {
+                const float inc = (float)ki;
+                ri += inc;
+                di += inc;
+            }
+            const float bg_gi = (di - ri) * v.background_inhibitory_scaling_factor;
+            background_store(v, lid, re, ri, de, di, bg_ge, bg_gi);
+            bgl[0][lane] = bg_ge;
+            bgl[1][lane] = bg_gi;
+        }
+    }
+
+    // Wave 2: plasticity traces, +1 of the previous step then this step's decay.
+    if (wave == 2 && live) {
+        uint32_t spike_prev = 0;
+        for (uint32_t g = 0; g < P.n_groups; ++g) {
+            const NeuronGroupView& v = P.g[g].v;
+            if (gid >= v.offset && gid < v.offset + v.num_neurons) spike_prev = v.spikes[gid - v.offset];
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxPlast; ++q) {
+            const PlastTraces& Q = P.pt[q];
+            if ((uint32_t)q >= P.n_pt) continue;
+            if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
+                const uint32_t i = gid - Q.pre_offset;
+                const float a = Q.pre[i];
+                const float b = Q.kind == kPlastTriplet ? Q.trip_pre[i] : 0.0f;
+                Q.pre[i] = trace_step(a, spike_prev, Q.pre_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_pre[i] = trace_step(b, spike_prev, Q.trip_pre_decay);
+            }
+            if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
+                const uint32_t j = gid - Q.post_offset;
+                const float a = Q.post[j];
+                const float b = Q.kind == kPlastTriplet ? Q.trip_post[j] : 0.0f;
+                Q.post[j] = trace_step(a, spike_prev, Q.post_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_post[j] = trace_step(b, spike_prev, Q.trip_post_decay);
+            }
+        }
+    }
     __syncthreads();
     if (wave != 0) return;
 
-    const uint32_t step = clk.clock_step;
     uint32_t spike = 0;
     if (live) {
         for (uint32_t g = 0; g < P.n_groups; ++g) {
@@ -284,16 +334,11 @@ __global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
             for (uint32_t s = 0; s < G.n_fin_e; ++s) ge = combine_parts(ge, G.fin_e[s], fin[s], lane);
             for (uint32_t s = 0; s < G.n_fin_i; ++s) gi_ = combine_parts(gi_, G.fin_i[s], fin[kMaxSources + s], lane);
             eif_step(v, ge, gi_, istim, V, refr, spike);
-            float bg_ge, bg_gi;
-            const uint32_t* te = v.poisson_table_excitatory_len <= kTabCap ? tab[g][0] : v.poisson_table_excitatory;
-            const uint32_t* ti = v.poisson_table_inhibitory_len <= kTabCap ? tab[g][1] : v.poisson_table_inhibitory;
-            background_advance(v, gid, step, clk, te, ti, re, ri, de, di, bg_ge, bg_gi);
             v.membrane_potentials[lid] = V;
             v.refractory_counter[lid] = refr;
             v.spikes[lid] = spike;
-            background_store(v, lid, re, ri, de, di, bg_ge, bg_gi);
-            v.g_excitatory[lid] = bg_ge;
-            v.g_inhibitory[lid] = bg_gi;
+            v.g_excitatory[lid] = bgl[0][lane];
+            v.g_inhibitory[lid] = bgl[1][lane];
         }
 #pragma unroll
         for (int t = 0; t < kMaxTraceSets; ++t) {
@@ -309,21 +354,6 @@ __global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
                 T.d[i] = dd;
                 T.r[i] = rr;
                 T.x[i] = dd - rr;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kMaxPlast; ++q) {
-            const PlastTraces& Q = P.pt[q];
-            if ((uint32_t)q >= P.n_pt) continue;
-            if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
-                const uint32_t i = gid - Q.pre_offset;
-                Q.pre[i] = trace_step(q_pre[q], spike_prev, Q.pre_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_pre[i] = trace_step(q_tpre[q], spike_prev, Q.trip_pre_decay);
-            }
-            if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
-                const uint32_t j = gid - Q.post_offset;
-                Q.post[j] = trace_step(q_post[q], spike_prev, Q.post_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_post[j] = trace_step(q_tpost[q], spike_prev, Q.trip_post_decay);
             }
         }
     }
@@ -656,8 +686,8 @@ __device__ __forceinline__ void plasticity_spiking_row(const PlastDev& D, uint32
     const bool triplet = D.kind == kPlastTriplet;
     const float pre_i = D.pre[i];
     double acc = 0.0;
-    // Batches of 8 columns per thread: all loads first, then updates/stores in column order.
-    constexpr int kB = 8;
+    // Batches of 16 columns per thread: all loads first, then updates/stores in column order.
+    constexpr int kB = 16;
     for (uint32_t j0 = threadIdx.x; j0 < D.n_post; j0 += kB * kNormThreads) {
         float m[kB], post_j[kB], tpost_j[kB];
         uint32_t sp[kB];
@@ -769,16 +799,26 @@ __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
             base += D.n_pre;
         }
     } else if (b < P.row_blocks + P.spiking_blocks) {
+        // Spiking rows of all plasticities in one sequence (plasticity order, increasing row);
+        // block sb takes the entries e = sb, sb + spiking_blocks, ...
         const uint32_t sb = b - P.row_blocks;
+        const uint32_t G = P.spiking_blocks;
+        uint32_t base = 0;
         for (uint32_t k = 0; k < P.pl.n; ++k) {
             const PlastDev& D = P.pl.p[k];
             const uint32_t count = block_compact_bits(D.bits, D.pre_offset, D.pre_offset + D.n_pre, list, kListCap, wsum);
+            const uint32_t first = base + (sb + G - base % G) % G;
             if (count <= kListCap) {
-                for (uint32_t e = sb; e < count; e += P.spiking_blocks) plasticity_spiking_row(D, list[e], lds4);
+                for (uint32_t e = first; e < base + count; e += G) plasticity_spiking_row(D, list[e - base], lds4);
             } else {
-                for (uint32_t i = sb; i < D.n_pre; i += P.spiking_blocks)
-                    if (D.pre_spikes[i] == 1) plasticity_spiking_row(D, i, lds4);
+                uint32_t rank = 0;
+                for (uint32_t i = 0; i < D.n_pre; ++i)
+                    if (D.pre_spikes[i] == 1) {
+                        if ((base + rank) % G == sb) plasticity_spiking_row(D, i, lds4);
+                        ++rank;
+                    }
             }
+            base += count;
             __syncthreads();
         }
     }
@@ -995,6 +1035,15 @@ __global__ void k_stimulus_device_cursor(StimulusView v, bn_device_clock* clk) {
 
 __global__ void k_clock_advance(bn_device_clock* clk) {
     if (blockIdx.x == 0 && threadIdx.x == 0) clk->clock_step = clk->clock_step + 1;
+}
+
+__global__ void k_clock_init(bn_device_clock* clk, uint32_t key_lo, uint32_t key_hi) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        clk->clock_step = 0;
+        clk->next_event_idx = 0;
+        clk->rng_key[0] = key_lo;
+        clk->rng_key[1] = key_hi;
+    }
 }
 
 template <typename T>
@@ -1261,6 +1310,13 @@ extern "C" int bn_historicize_membrane_potentials(const float* d_v, float* hist,
     hipLaunchKernelGGL(k_historicize<float>, dim3(cdiv(n, 256)), dim3(256), 0, s, d_v, hist, n, interval,
                        step);
     BN_HIP_TRY(hipGetLastError(), "kernel_historicize_membrane_potentials launch");
+    return BN_OK;
+}
+
+extern "C" int bn_clock_init(bn_device_clock* clock, uint64_t rng_seed, bn_stream_t s) {
+    if (!clock) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_clock_init: clock is null");
+    hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(64), 0, s, clock, (uint32_t)rng_seed, (uint32_t)(rng_seed >> 32));
+    BN_HIP_TRY(hipGetLastError(), "kernel_init_neuron_rng_states launch");
     return BN_OK;
 }
 

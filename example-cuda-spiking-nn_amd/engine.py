@@ -64,7 +64,7 @@ EXPORTED_SYMBOLS = [
     "bn_last_error", "bn_poisson_table", "bn_neuron_group_evolve", "bn_connection_propagate",
     "bn_stimulus_apply", "bn_stimulus_apply_device_cursor", "bn_triplet_plasticity_update",
     "bn_triplet_plasticity_update_normalize", "bn_istdp_plasticity_update", "bn_stdp_plasticity_update",
-    "bn_historicize_spikes", "bn_historicize_membrane_potentials", "bn_clock_advance", "bn_sim_create",
+    "bn_historicize_spikes", "bn_historicize_membrane_potentials", "bn_clock_init", "bn_clock_advance", "bn_sim_create",
     "bn_sim_create_from_json_text", "bn_sim_destroy", "bn_sim_run", "bn_sim_step_components",
     "bn_sim_synchronize", "bn_sim_get_info", "bn_sim_get_neuron_state", "bn_sim_get_weights",
     "bn_sim_get_connection_traces", "bn_sim_get_plasticity_traces", "bn_sim_run_timed",

@@ -270,6 +270,10 @@ int bn_historicize_membrane_potentials(const float* d_membrane_potentials,
                                        uint32_t recording_interval_steps, uint32_t clock_step,
                                        bn_stream_t stream);
 
+/* Replaces kernel_init_neuron_rng_states_launch (gpu_core/kernel_init_neuron_rng_states.h:7-11):
+ * there is no per-neuron RNG state to initialise; this sets clock_step = 0, next_event_idx = 0
+ * and the Philox key to the two words of rng_seed, asynchronously on `stream`. */
+int bn_clock_init(bn_device_clock* clock, uint64_t rng_seed, bn_stream_t stream);
 /* Advances clock->clock_step by one (the last node of a captured step). */
 int bn_clock_advance(bn_device_clock* clock, bn_stream_t stream);
 
@@ -343,8 +347,11 @@ int bn_sim_get_plasticity_traces(bn_sim* sim, uint32_t plasticity_offset, float*
                                  float* triplet_pre, float* triplet_post);
 /* Times `num_steps` steps between two HIP events on the simulation stream; *ms = elapsed. */
 int bn_sim_run_timed(bn_sim* sim, uint32_t num_steps, float* ms);
-/* Runs `num_steps` steps launched eagerly with HIP events around the propagate (GEMV) kernel
- * of every step; *kernel_ms = summed kernel time, *step_ms = summed wall time of the steps. */
+/* Kernel timing on the simulation stream.  *step_ms = wall time of `num_steps` whole steps
+ * launched one by one (no graph; the simulation advances by num_steps).  *kernel_ms = wall time
+ * of `num_steps` back-to-back launches of the propagate (GEMV) kernel alone over the current
+ * weights; the pending propagate partials are saved before and restored after, so the state of
+ * the simulation is unchanged by this part. */
 int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_ms, float* step_ms);
 /* Writes the JSON of the parsed config (all fields, after defaults/overrides/prefixing). */
 int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, size_t* needed);

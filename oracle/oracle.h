@@ -10,15 +10,18 @@
  *     Poisson by inversion of a floor(2^32 * CDF) table;
  *   - expf: Cody–Waite reduction + degree-7 Taylor (same algorithm as the kernels);
  *   - GEMV: FP64 accumulation of exact FP32 products, rows summed sequentially inside chunks of
- *     gemv_row_block rows, chunk sums added to (double)g in chunk order, one rounding to FP32;
+ *     gemv_row_block rows; the chunk sums cut into 8 contiguous parts of ceil(chunks/8), each
+ *     part summed in chunk order from 0.0, the parts added to (double)g in order, one rounding
+ *     to FP32;
  *   - triplet row normalisation sum: 256 interleaved FP64 partials, xor butterfly inside each
  *     64-lane group (32,16,8,4,2,1), the four group sums added in order.
  *   - lazy normalisation (lazy_normalize = 1): the triplet matrix is kept as W = diag(s) M.  A
  *     touched entry is updated in true units, w' = clamp(s_i*M_ij + delta), M_ij = w'/s_i (rows
  *     with s_i = 0 stay untouched); R_i = sum_j M_ij is kept in FP64: rows whose pre neuron
- *     spiked (LTD) are re-summed in the normalisation order above, other rows add the sum of
- *     their LTP changes (double)M'_ij - (double)M_ij accumulated in increasing post order;
- *     then s_i = fl32(target_i / R_i) when R_i > 0.
+ *     spiked (LTD) are re-summed in the normalisation order above, other rows add their LTP
+ *     changes (double)M'_ij - (double)M_ij: the list of spiking posts (increasing post order)
+ *     is cut into 4 contiguous parts of ceil(count/4), each part's changes summed in list order
+ *     from 0.0, the parts added to R_i in order; then s_i = fl32(target_i / R_i) when R_i > 0.
  * precision 32 reproduces the engine bit for bit; precision 64 runs every quantity in FP64
  * (libm exp, plain sequential sums) as an independent accuracy reference.
  *
