@@ -29,8 +29,11 @@ HBM_PEAK_GBS = 8000.0
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=2000)
+    ap.add_argument("--mode", choices=("plain", "batched-shared"), default="plain",
+                    help="plain: one network; batched-shared: 64 instances sharing frozen weights")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (plain 10000, batched 5001)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (plain 2000, batched 0)")
+    ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--config", default=os.path.join(REPO, "configs", "net3125_stdp.json"))
     ap.add_argument("--normalization", choices=("lazy", "faithful"), default="lazy")
     ap.add_argument("--kernel-steps", type=int, default=100)
@@ -45,6 +48,12 @@ def main() -> int:
         print(f"bench.py: this engine runs one network on one GPU; --gpus {args.gpus} is not supported "
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         return 2
+    if args.mode == "batched-shared":
+        args.steps = 5001 if args.steps is None else args.steps
+        args.warmup = 0 if args.warmup is None else args.warmup
+        return bench_batched(args)
+    args.steps = 10000 if args.steps is None else args.steps
+    args.warmup = 2000 if args.warmup is None else args.warmup
 
     eng = importlib.import_module("example-cuda-spiking-nn_amd")
     cfg = eng.config_dump(args.config)
@@ -107,6 +116,122 @@ def main() -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+MFMA_FP32_PEAK_TFLOPS = 157.3
+
+
+def bench_batched(args) -> int:
+    """The shared-weight batch: `instances` copies of the network with plasticity off and one
+    weight set, noise key = instance index, a 2,000-4,000-step pulse of 200 pA x U(0,1) on 100
+    excitatory neurons per instance (the recorded per-instance draw); `steps` steps timed
+    after `warmup`.  Metric: instance-steps/s.  Beside it: the same network run as one plain
+    instance on the GPU (the sequential alternative), and the GEMM's FP32 matrix-core rate."""
+    import tempfile
+
+    eng = importlib.import_module("example-cuda-spiking-nn_amd")
+    base = json.load(open(args.config))
+    base["e_e_plasticity_enabled"] = False
+    base["i_e_plasticity_enabled"] = False
+    base["simulation_steps"] = max(int(base.get("simulation_steps", 0)), args.warmup + args.steps)
+    n_e = int(base["num_excitatory_neurons"])
+    insts = []
+    for k in range(args.instances):
+        ids, amp = eng.batch_workload_instance(k, n_e, 100, 200e-12)
+        insts.append({"noise_seed": k, "stimulus_ids": ids, "stimulus_amplitude": amp,
+                      "stimulus_start": 2000, "stimulus_end": 4000})
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "batched.json")
+        json.dump(base, open(path, "w"))
+        batch = eng.BatchSimulation(path, insts)
+        info = batch.info
+        if args.warmup:
+            batch.run_timed(args.warmup)
+        ms = batch.run_timed(args.steps)
+        gemm_ms = batch.time_gemm(args.kernel_steps) / args.kernel_steps
+        batch.close()
+        # Sequential alternative: one plain instance (same network, plasticity off, fused step).
+        plain_cfg = dict(base)
+        plain_cfg.update({"background_noise_seed": 0, "stimulus_selected_input_ids": [int(i) for i in insts[0]["stimulus_ids"]],
+                          "stimulus_selected_input_post_group_offsets": [0] * 100,
+                          "stimulus_start_clock_steps": [2000], "stimulus_end_clock_steps": [4000],
+                          "stimulus_input_values": [float(insts[0]["stimulus_amplitude"])]})
+        ppath = os.path.join(d, "plain.json")
+        json.dump(plain_cfg, open(ppath, "w"))
+        sim = eng.Simulation(ppath, flags=("no_monitors", "quiet"))
+        plain_ms = sim.run_timed(args.steps)
+        sim.close()
+    inst_steps_per_s = args.instances * args.steps / (ms / 1000.0)
+    plain_inst_steps_per_s = args.steps / (plain_ms / 1000.0)
+    n_w = sum(int(info.weight_pre[k]) * int(info.weight_post[k]) for k in range(4))
+    B = int(info.padded_instances)
+    flops = 2.0 * B * n_w
+    tflops = flops / (gemm_ms / 1000.0) / 1e12
+    gemm_bytes = 4 * n_w + 4 * B * sum(int(info.weight_pre[k]) for k in range(4))
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_batched(base, args.cpu_seconds)
+    line = {
+        "metric": "batched instance-steps/s (64 x 3,125-nrn nets, shared frozen weights)",
+        "value": round(inst_steps_per_s, 1),
+        "unit": "instance-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (GEMM: FP32 matrix cores, FP32 accumulation)",
+        "data": "synthetic: per-instance Philox noise keys 0..63, recorded per-instance stimulus draws",
+        "config": {"workload": f"batched_shared ({args.instances} instances of {os.path.basename(args.config)} "
+                               "with plasticity off, one weight set)",
+                   "instances": args.instances, "padded_instances": B, "neurons_per_instance": int(info.total_neurons),
+                   "gemm_rows_per_wave": int(info.gemm_rows_per_wave), "kernels_per_step": int(info.kernels_per_step),
+                   "parallelism": "single GPU"},
+        "vs_sequential_plain": round(inst_steps_per_s / plain_inst_steps_per_s, 2),
+        "sequential_plain_instance_steps_per_s": round(plain_inst_steps_per_s, 1),
+        "roofline": {"bound": "mfma", "kernel": "k_batch_gemm", "achieved": round(tflops, 2),
+                     "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / MFMA_FP32_PEAK_TFLOPS, 4),
+                     "traffic": None, "algorithmic_flops_per_launch": flops,
+                     "algorithmic_bytes_per_launch": gemm_bytes,
+                     "bandwidth_GBs": round(gemm_bytes / (gemm_ms / 1000.0) / 1e9, 1),
+                     "bandwidth_frac": round(gemm_bytes / (gemm_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
+                     "avg_kernel_us": round(gemm_ms * 1000.0, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline_batched(base: dict, seconds: float) -> dict:
+    """One instance of the batch on the oracle's FP32 OpenMP loop (instances run one after the
+    other on the CPU, so instance-steps/s = steps/s of one instance)."""
+    import tempfile
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py
+
+    eng = importlib.import_module("example-cuda-spiking-nn_amd")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cpu.json")
+        json.dump(base, open(path, "w"))
+        cfg = eng.config_dump(path)
+        weights = [eng.generate_weight_matrix(path, k) for k in range(4)]
+    p = oracle_py.params_from_config(cfg, gemv_row_block=(0, 0, 0, 0), dense_gemv=True)
+    orc = oracle_py.OracleSim(p, weights, precision=32)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc.step(1)
+    t0 = time.perf_counter()
+    n = 0
+    while time.perf_counter() - t0 < seconds:
+        orc.step(1)
+        n += 1
+    dt = time.perf_counter() - t0
+    orc.close()
+    return {"value": round(n / dt, 3), "unit": "instance-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n} steps of one instance (plasticity off) from its initial state, FP32 oracle, "
+                      f"{threads} OpenMP threads, {dt:.1f} s"}
 
 
 def copy_calibration(mib: int) -> int:

@@ -8,6 +8,8 @@ was not built.
 """
 from .engine import (  # noqa: F401
     BURYN_FLAGS,
+    BatchSimulation,
+    batch_workload_instance,
     LIB_PATH,
     BnError,
     Simulation,

@@ -230,6 +230,7 @@ Config from_value(const Value& j) {
     read_u32_array(j, "gemv_row_blocks", c.gemv_row_blocks);
     if (c.gemv_row_blocks.size() != 4) throw std::runtime_error("gemv_row_blocks must hold 4 values (EE, EI, IE, II)");
     c.gemv_nontemporal_loads = value_or<int>(j, "gemv_nontemporal_loads", c.gemv_nontemporal_loads);
+    c.background_noise_seed = value_or<long long>(j, "background_noise_seed", c.background_noise_seed);
 
     const std::string& od = c.output_directory;
     c.excitatory_locations_file = prefix_output_path(od, c.excitatory_locations_file);
@@ -349,6 +350,7 @@ std::string config_to_json(const Config& c) {
     S(e_e_weight_recording_file)
     d.ua("gemv_row_blocks", c.gemv_row_blocks);
     d.i("gemv_nontemporal_loads", c.gemv_nontemporal_loads);
+    d.i("background_noise_seed", c.background_noise_seed);
 #undef F
 #undef U
 #undef S

@@ -132,6 +132,9 @@ struct Config {
     // extension (engine tuning): weight loads of the propagate kernel bypass the caches (1),
     // use them (0), or the engine decides from the size of the weight set (-1).
     int gemv_nontemporal_loads = -1;
+    // extension: key of the background-noise Philox streams; -1 uses rng_seed (which also seeds
+    // the connectivity), so one weight set can be run with several noise streams.
+    long long background_noise_seed = -1;
 };
 
 Config load_config_json(const std::string& path);

@@ -60,6 +60,20 @@ class NeuronStateHost(C.Structure):
     _fields_ = [(n, C.c_void_p) for n, _ in _STATE_FIELDS]
 
 
+class BatchInstance(C.Structure):
+    _fields_ = [("noise_seed", C.c_uint64), ("stimulus_amplitude", C.c_float), ("stimulus_start", C.c_uint32),
+                ("stimulus_end", C.c_uint32), ("num_stimulus_ids", C.c_uint32),
+                ("stimulus_ids", C.POINTER(C.c_uint32))]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "num_instances", "padded_instances", "num_excitatory_neurons", "num_inhibitory_neurons", "total_neurons",
+        "clock_step", "gemm_rows_per_wave", "gemm_blocks")] + [
+        ("gemm_groups", C.c_uint32 * 4), ("weight_pre", C.c_uint32 * 4), ("weight_post", C.c_uint32 * 4),
+        ("kernels_per_step", C.c_uint32)]
+
+
 EXPORTED_SYMBOLS = [
     "bn_last_error", "bn_poisson_table", "bn_neuron_group_evolve", "bn_connection_propagate",
     "bn_stimulus_apply", "bn_stimulus_apply_device_cursor", "bn_triplet_plasticity_update",
@@ -69,6 +83,9 @@ EXPORTED_SYMBOLS = [
     "bn_sim_synchronize", "bn_sim_get_info", "bn_sim_get_neuron_state", "bn_sim_get_weights",
     "bn_sim_get_connection_traces", "bn_sim_get_plasticity_traces", "bn_sim_run_timed",
     "bn_sim_time_propagate_kernel", "bn_config_dump_json", "bn_generate_weight_matrix",
+    "bn_batch_create", "bn_batch_destroy", "bn_batch_run", "bn_batch_synchronize", "bn_batch_run_timed",
+    "bn_batch_time_gemm", "bn_batch_set_gemm_rows", "bn_batch_get_info", "bn_batch_get_state",
+    "bn_batch_workload_instance",
 ]
 
 _lib = None
@@ -107,8 +124,19 @@ def lib() -> C.CDLL:
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
     L.bn_config_dump_json.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.bn_generate_weight_matrix.argtypes = [C.c_char_p, U, fp, up, up]
+    L.bn_batch_create.argtypes = [C.c_char_p, U, C.POINTER(BatchInstance), C.POINTER(P)]
+    L.bn_batch_destroy.argtypes = [P]
+    L.bn_batch_destroy.restype = None
+    L.bn_batch_run.argtypes = [P, U]
+    L.bn_batch_synchronize.argtypes = [P]
+    L.bn_batch_run_timed.argtypes = [P, U, fp]
+    L.bn_batch_time_gemm.argtypes = [P, U, fp]
+    L.bn_batch_set_gemm_rows.argtypes = [P, U]
+    L.bn_batch_get_info.argtypes = [P, C.POINTER(BatchInfo)]
+    L.bn_batch_get_state.argtypes = [P, U, C.POINTER(NeuronStateHost)]
+    L.bn_batch_workload_instance.argtypes = [U, U, U, C.c_float, up, fp]
     for f in EXPORTED_SYMBOLS:
-        if f not in ("bn_last_error", "bn_sim_destroy"):
+        if f not in ("bn_last_error", "bn_sim_destroy", "bn_batch_destroy"):
             getattr(L, f).restype = I
     _lib = L
     return L
@@ -228,3 +256,77 @@ class Simulation:
         a = [np.zeros(self.N, np.float32) for _ in range(4)]
         _check(lib().bn_sim_get_plasticity_traces(self.h, off, *[_fp(x) for x in a]))
         return a
+
+
+def batch_workload_instance(instance: int, num_excitatory: int, num_ids: int = 100,
+                            max_amplitude: float = 200e-12):
+    """The recorded stimulus draw of one instance of the shared-weight workload: (ids, amplitude)."""
+    ids = np.zeros(num_ids, np.uint32)
+    amp = C.c_float(0)
+    _check(lib().bn_batch_workload_instance(instance, num_excitatory, num_ids, max_amplitude, _up(ids), C.byref(amp)))
+    return ids, amp.value
+
+
+class BatchSimulation:
+    """B instances of one plasticity-free network sharing its weights (bn_batch_* API).
+
+    instances: list of dicts with keys noise_seed, and optionally stimulus_ids (excitatory-local),
+    stimulus_amplitude, stimulus_start, stimulus_end.
+    """
+
+    def __init__(self, config: str, instances):
+        self._keep = []
+        arr = (BatchInstance * len(instances))()
+        for k, inst in enumerate(instances):
+            ids = np.ascontiguousarray(inst.get("stimulus_ids", []), np.uint32)
+            self._keep.append(ids)
+            arr[k].noise_seed = int(inst["noise_seed"])
+            arr[k].stimulus_amplitude = float(inst.get("stimulus_amplitude", 0.0))
+            arr[k].stimulus_start = int(inst.get("stimulus_start", 0))
+            arr[k].stimulus_end = int(inst.get("stimulus_end", 0))
+            arr[k].num_stimulus_ids = ids.size
+            arr[k].stimulus_ids = _up(ids) if ids.size else None
+        h = C.c_void_p()
+        _check(lib().bn_batch_create(config.encode(), len(instances), arr, C.byref(h)))
+        self.h = h
+        self.info = self.get_info()
+        self.N = self.info.total_neurons
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bn_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get_info(self) -> BatchInfo:
+        info = BatchInfo()
+        _check(lib().bn_batch_get_info(self.h, C.byref(info)))
+        return info
+
+    def run(self, n: int) -> None:
+        _check(lib().bn_batch_run(self.h, n))
+
+    def synchronize(self) -> None:
+        _check(lib().bn_batch_synchronize(self.h))
+
+    def run_timed(self, n: int) -> float:
+        ms = C.c_float(0)
+        _check(lib().bn_batch_run_timed(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def time_gemm(self, n: int) -> float:
+        ms = C.c_float(0)
+        _check(lib().bn_batch_time_gemm(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def set_gemm_rows(self, rows: int) -> None:
+        _check(lib().bn_batch_set_gemm_rows(self.h, rows))
+        self.info = self.get_info()
+
+    def state(self, instance: int) -> dict:
+        arrays = {n: np.zeros(self.N, dt) for n, dt in _STATE_FIELDS}
+        st = NeuronStateHost(*[arrays[n].ctypes.data for n, _ in _STATE_FIELDS])
+        _check(lib().bn_batch_get_state(self.h, instance, C.byref(st)))
+        return arrays

@@ -361,6 +361,62 @@ int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, siz
 int bn_generate_weight_matrix(const char* config_path, uint32_t weight_matrix_idx, float* dst,
                               uint32_t* pre, uint32_t* post);
 
+/* -------------------------------------------------------------------------------------
+ * Batched shared-weight form: B instances of one network whose weights are frozen
+ * (plasticity off), each with its own background-noise key and stimulus pulse, advanced
+ * together.  The propagate of all instances is one GEMM per connection on the FP32 matrix
+ * cores (FP32 accumulation), so each instance matches a plain run of the same config with
+ * background_noise_seed = its noise seed and the same pulse, to FP32 summation-order
+ * rounding.  No counterpart in the reference (it runs one network per process).
+ * ------------------------------------------------------------------------------------- */
+typedef struct bn_batch bn_batch;
+
+typedef struct bn_batch_instance {
+    uint64_t noise_seed;             /* Philox key of this instance's background noise */
+    float stimulus_amplitude;        /* square pulse: +amplitude in the stimulus phase of step */
+    uint32_t stimulus_start;         /* stimulus_start, -amplitude in that of stimulus_end + 1 */
+    uint32_t stimulus_end;           /* (events at or after simulation_steps are dropped, as the
+                                        plain schedule does) */
+    uint32_t num_stimulus_ids;       /* excitatory-local ids receiving the pulse (distinct) */
+    const uint32_t* stimulus_ids;
+} bn_batch_instance;
+
+typedef struct bn_batch_info {
+    uint32_t num_instances;
+    uint32_t padded_instances;       /* instances padded to a multiple of 64 (one wave) */
+    uint32_t num_excitatory_neurons;
+    uint32_t num_inhibitory_neurons;
+    uint32_t total_neurons;
+    uint32_t clock_step;
+    uint32_t gemm_rows_per_wave;     /* pre rows per MFMA wave chunk (the tuned tile height) */
+    uint32_t gemm_blocks;
+    uint32_t gemm_groups[4];         /* FP32 partial sums per post neuron, EE, EI, IE, II */
+    uint32_t weight_pre[4];
+    uint32_t weight_post[4];
+    uint32_t kernels_per_step;
+} bn_batch_info;
+
+/* config_path: the plain network's JSON config (plasticity must be off; rng_seed seeds the
+ * shared connectivity). */
+int bn_batch_create(const char* config_path, uint32_t num_instances, const bn_batch_instance* instances,
+                    bn_batch** out);
+void bn_batch_destroy(bn_batch* batch);
+int bn_batch_run(bn_batch* batch, uint32_t num_steps);
+int bn_batch_synchronize(bn_batch* batch);
+int bn_batch_run_timed(bn_batch* batch, uint32_t num_steps, float* ms);
+/* Wall time of `num_launches` back-to-back GEMM launches; the batch's state is unchanged. */
+int bn_batch_time_gemm(bn_batch* batch, uint32_t num_launches, float* ms);
+/* Rows of pre per MFMA wave chunk (multiple of 8; 0 = heuristic); before the first step only. */
+int bn_batch_set_gemm_rows(bn_batch* batch, uint32_t rows_per_wave);
+int bn_batch_get_info(bn_batch* batch, bn_batch_info* info);
+/* State of one instance, indexed by global neuron id (g_* after the last propagate). */
+int bn_batch_get_state(bn_batch* batch, uint32_t instance, const bn_neuron_state_host* dst);
+/* The recorded per-instance stimulus draw of the shared-weight workload: mt19937 seeded with
+ * `instance`, num_ids distinct excitatory ids (partial Fisher-Yates, returned sorted) and
+ * amplitude = max_amplitude * U(0,1). */
+int bn_batch_workload_instance(uint32_t instance, uint32_t num_excitatory, uint32_t num_ids, float max_amplitude,
+                               uint32_t* ids, float* amplitude);
+
 #ifdef __cplusplus
 }
 #endif

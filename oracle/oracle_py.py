@@ -95,7 +95,8 @@ def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=Fa
     p = Params()
     p.num_excitatory_neurons = cfg["num_excitatory_neurons"]
     p.num_inhibitory_neurons = cfg["num_inhibitory_neurons"]
-    p.rng_seed = cfg["rng_seed"]
+    noise_seed = cfg.get("background_noise_seed", -1)
+    p.rng_seed = noise_seed if noise_seed is not None and noise_seed >= 0 else cfg["rng_seed"]
     p.rng_instance = rng_instance
     for k in ("timestep", "g_leak", "membrane_capacitance", "depolarization_slope_factor",
               "v_effective_spike_threshold", "v_excitatory_reversal", "v_inhibitory_reversal",
