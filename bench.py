@@ -29,8 +29,11 @@ HBM_PEAK_GBS = 8000.0
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("plain", "batched-shared"), default="plain",
-                    help="plain: one network; batched-shared: 64 instances sharing frozen weights")
+    ap.add_argument("--mode", choices=("plain", "batched-shared", "batched-uneven"), default="plain",
+                    help="plain: one network; batched-shared: 64 instances sharing frozen weights; "
+                         "batched-uneven: 48 plastic networks of uneven size")
+    ap.add_argument("--networks", type=int, default=48)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (plain 10000, batched 5001)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (plain 2000, batched 0)")
     ap.add_argument("--instances", type=int, default=64)
@@ -52,6 +55,10 @@ def main() -> int:
         args.steps = 5001 if args.steps is None else args.steps
         args.warmup = 0 if args.warmup is None else args.warmup
         return bench_batched(args)
+    if args.mode == "batched-uneven":
+        args.steps = 10001 if args.steps is None else args.steps
+        args.warmup = 0 if args.warmup is None else args.warmup
+        return bench_uneven(args)
     args.steps = 10000 if args.steps is None else args.steps
     args.warmup = 2000 if args.warmup is None else args.warmup
 
@@ -199,6 +206,91 @@ def bench_batched(args) -> int:
                      "bandwidth_frac": round(gemm_bytes / (gemm_ms / 1000.0) / 1e9 / HBM_PEAK_GBS, 4),
                      "avg_kernel_us": round(gemm_ms * 1000.0, 2)},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_uneven(args) -> int:
+    """48 independent plastic networks (N_E drawn from {400, 900, 1600, 2500}, triplet STDP +
+    iSTDP, own seeds), one launch per phase for all of them; the GEMV row block of each size is
+    chosen by the autotuner first.  Metric: network-steps/s.  Per size: the count and a plain
+    single-network run of that size on the GPU (the sequential alternative)."""
+    import tempfile
+
+    eng = importlib.import_module("example-cuda-spiking-nn_amd")
+    wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+    total = args.warmup + args.steps
+    cfgs = wl.uneven_plastic_configs(count=args.networks, steps=total + 4000)
+    sizes = [c["num_excitatory_neurons"] for c in cfgs]
+    with tempfile.TemporaryDirectory() as d:
+        tuned = {}
+        if not args.no_autotune:
+            tuned = wl.autotune_row_blocks(eng, sizes, d)
+            for c in cfgs:
+                c["gemv_row_blocks"] = [tuned[c["num_excitatory_neurons"]][0]] * 4
+        paths = []
+        for k, c in enumerate(cfgs):
+            paths.append(os.path.join(d, f"net{k}.json"))
+            json.dump(c, open(paths[-1], "w"))
+        multi = eng.MultiSimulation(paths, flags=("lazy_normalize",))
+        sched = None
+        if not args.no_autotune:
+            # Tuning steps advance the networks; the timed run then starts from that state.
+            sched = wl.autotune_schedule(multi)
+        minfo = multi.get_info()
+        if args.warmup:
+            multi.run_timed(args.warmup)
+        ms = multi.run_timed(args.steps)
+        k_ms = multi.time_propagate(args.kernel_steps) / args.kernel_steps
+        n_w = 0
+        for k in range(len(cfgs)):
+            inf = multi.networks[k].info
+            n_w += sum(int(inf.weight_pre[c]) * int(inf.weight_post[c]) for c in range(4))
+        n_pre = sum(sum(int(multi.networks[k].info.weight_pre[c]) for c in range(4)) for k in range(len(cfgs)))
+        rates = {}
+        for k, c in enumerate(cfgs):
+            rates[c["num_excitatory_neurons"]] = rates.get(c["num_excitatory_neurons"], [])
+            rates[c["num_excitatory_neurons"]].append(k)
+        per_size = {}
+        for n_e, idx in sorted(rates.items()):
+            sim = eng.Simulation(paths[idx[0]], flags=("no_monitors", "quiet", "lazy_normalize"))
+            plain_steps = min(args.steps, 2000)
+            pms = sim.run_timed(plain_steps)
+            sim.close()
+            per_size[str(n_e)] = {"networks": len(idx), "plain_single_steps_per_s": round(plain_steps / (pms / 1000.0), 1),
+                                  "gemv_row_block": tuned.get(n_e, (None,))[0],
+                                  "autotune_us_per_step": {str(k): round(v, 2) for k, v in tuned.get(n_e, (None, {}))[1].items()}}
+        multi.close()
+    net_steps_per_s = len(cfgs) * args.steps / (ms / 1000.0)
+    seq_s_per_step = sum(len(v) / per_size[str(n_e)]["plain_single_steps_per_s"] for n_e, v in rates.items())
+    seq_rate = len(cfgs) / seq_s_per_step
+    gemv_bytes = 4 * n_w + 4 * n_pre
+    achieved = gemv_bytes / (k_ms / 1000.0) / 1e9
+    line = {
+        "metric": "batched network-steps/s (48 uneven plastic nets)",
+        "value": round(net_steps_per_s, 1),
+        "unit": "network-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state/weights, f64 GEMV accumulation",
+        "data": "synthetic: per-network seeds 0..47, sizes from a recorded Mersenne Twister draw (seed 2024)",
+        "config": {"workload": f"batched_uneven ({len(cfgs)} plastic networks, N_E in {{400,900,1600,2500}}, "
+                               "triplet STDP + iSTDP, lazy normalisation)",
+                   "networks": len(cfgs), "weight_entries": n_w, "kernels_per_step": int(minfo.kernels_per_step),
+                   "per_size": per_size, "schedule": sched, "groups": int(minfo.num_groups),
+                   "parallelism": "single GPU"},
+        "vs_sequential_plain": round(net_steps_per_s / seq_rate, 2),
+        "sequential_plain_network_steps_per_s": round(seq_rate, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_gemv_multi (propagate, all networks)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": gemv_bytes, "avg_kernel_us": round(k_ms * 1000.0, 2)},
+        "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0

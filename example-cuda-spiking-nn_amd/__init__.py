@@ -9,6 +9,7 @@ was not built.
 from .engine import (  # noqa: F401
     BURYN_FLAGS,
     BatchSimulation,
+    MultiSimulation,
     batch_workload_instance,
     LIB_PATH,
     BnError,

@@ -156,6 +156,23 @@ struct TailPlan {
 
 // Launchers used by the host engine (return hipError_t as int).
 int launch_evolve(const EvolvePlan& plan, bn_stream_t s);
+
+// Several networks' fused steps in one launch per phase: per-network plans in device memory
+// and, per phase, a block -> network table and each network's first block.
+struct MultiPhase {
+    const uint32_t* block_net;
+    const uint32_t* net_first_block;
+    uint32_t blocks;
+};
+struct MultiStep {
+    const EvolvePlan* evolve;
+    const GemvPlan* gemv;
+    const PlastStepPlan* plast;
+    MultiPhase ph_evolve, ph_gemv, ph_plast;
+    uint32_t nontemporal;
+};
+int launch_multi_step(const MultiStep& m, bn_stream_t s);
+int launch_multi_propagate(const MultiStep& m, bn_stream_t s);
 int launch_gemv(const GemvPlan& plan, bn_stream_t s);
 int launch_plasticity(const PlastStepPlan& plan, bn_stream_t s);
 int launch_ltp(const PlastPlan& plan, bn_stream_t s);

@@ -14,6 +14,7 @@
 #include "generation.h"
 #include "api_util.h"
 #include "network.h"
+#include "sim_internal.h"
 
 namespace bn {
 
@@ -67,20 +68,6 @@ void validate_plasticity_params(const PlasticityParams& p, bool requires_triplet
 
 using namespace bn;
 
-struct bn_sim {
-    Config cfg;
-    uint32_t flags = 0;
-    std::unique_ptr<GpuSimulationState> state;
-    std::unique_ptr<System> system;
-    float* d_tmp_ge = nullptr;
-    float* d_tmp_gi = nullptr;
-    ~bn_sim() {
-        system.reset();
-        if (d_tmp_ge) (void)hipFree(d_tmp_ge);
-        if (d_tmp_gi) (void)hipFree(d_tmp_gi);
-        state.reset();
-    }
-};
 
 namespace {
 

@@ -399,6 +399,11 @@ public:
     void launch_fused_steps(uint32_t n);           // enqueue n steps (graph or eager)
     void launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end);
     const GemvPlan& gemv_plan() const { return gemv_; }
+    const EvolvePlan& evolve_plan() const { return evolve_; }
+    const PlastStepPlan& plast_step_plan() const { return plast_step_; }
+    bool has_norm_tail() const { return tail_.norm_rows != 0; }
+    // Steps run by an external launcher on this system's plans (several networks in one launch).
+    void note_external_steps(uint32_t n) { clock_step_ += n; }
     uint32_t kernels_per_step() const { return kernels_per_step_; }
     // Conductances after the last propagate, for state downloads (fused path keeps them as
     // base + pending GEMV partials).

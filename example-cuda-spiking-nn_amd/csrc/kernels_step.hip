@@ -193,12 +193,12 @@ __device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* t
     return poisson_from_table(u, table + 4, len - 4) + 4;
 }
 
-__global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
+__device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) {
     __shared__ FinParts fin;
     __shared__ float bgl[2][kEvolveNeurons];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t gid = blockIdx.x * kEvolveNeurons + lane;
+    const uint32_t gid = blk * kEvolveNeurons + lane;
     const bool live = gid < P.total_neurons;
     const bn_device_clock clk = *P.clock;
     const uint32_t step = clk.clock_step;
@@ -370,6 +370,16 @@ __global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
     if (vslot >= 0) P.v_ring[(size_t)(step % P.interval) * P.n_vrec + vslot] = V;
 }
 
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) { evolve_block(P, blockIdx.x); }
+
+// Several networks in one launch: block -> (network, block of that network) from the tables.
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve_multi(const EvolvePlan* __restrict__ plans,
+                                                                 const uint32_t* __restrict__ block_net,
+                                                                 const uint32_t* __restrict__ net_first_block) {
+    const uint32_t net = __builtin_amdgcn_readfirstlane(block_net[blockIdx.x]);
+    evolve_block(plans[net], blockIdx.x - net_first_block[net]);
+}
+
 // =======================================================================================
 // k_gemv: FP64 partial sums, one wave per tile of (row chunk, 256 columns).
 //   partial[chunk][j] = sum_{i in chunk, increasing} (double)x'_i * (double)W[i][j]
@@ -436,8 +446,8 @@ __device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const flo
 }
 
 template <bool kNT>
-__global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+__device__ __forceinline__ void gemv_block(const GemvPlan& P, uint32_t blk) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(blk * 4 + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63;
     if (wave >= P.total_tiles) return;
     uint32_t ci = 0;
@@ -452,6 +462,17 @@ __global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
     if (j0 >= C.n_post) return;
     if (C.row_scale) gemv_tile<true, kNT>(C.W, C.x, C.row_scale, C.partials, C.ld, i0, i1, j0, chunk);
     else gemv_tile<false, kNT>(C.W, C.x, nullptr, C.partials, C.ld, i0, i1, j0, chunk);
+}
+
+template <bool kNT>
+__global__ __launch_bounds__(256) void k_gemv(GemvPlan P) { gemv_block<kNT>(P, blockIdx.x); }
+
+template <bool kNT>
+__global__ __launch_bounds__(256) void k_gemv_multi(const GemvPlan* __restrict__ plans,
+                                                    const uint32_t* __restrict__ block_net,
+                                                    const uint32_t* __restrict__ net_first_block) {
+    const uint32_t net = __builtin_amdgcn_readfirstlane(block_net[blockIdx.x]);
+    gemv_block<kNT>(plans[net], blockIdx.x - net_first_block[net]);
 }
 
 // Scalar-load variant for row strides that are not 16-byte aligned (reference layout with
@@ -753,13 +774,12 @@ __device__ __forceinline__ void stimulus_and_clock(const StimTail& T) {
 // Blocks [0, row_blocks): kQuietRows rows each of the concatenated pre rows of all plasticities
 // (rows whose pre did not spike).  Blocks after: the rows whose pre spiked, one block each.
 // The last block also applies the stimulus and advances the clock (no block reads the clock).
-__global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
+__device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_t b, uint32_t nblocks) {
     __shared__ double lds4[4];
     __shared__ uint32_t wsum[4];
     __shared__ uint32_t list[kListCap];
     __shared__ float facs[kListCap];
     __shared__ double qsum[kQuietParts][64];
-    const uint32_t b = blockIdx.x;
     if (b < P.row_blocks) {
         // kQuietRows rows per block, lane = row; wave q applies part q of the spiking-post list.
         const uint32_t r0 = b * kQuietRows;
@@ -822,7 +842,17 @@ __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) {
             __syncthreads();
         }
     }
-    if (P.tail.active && b == gridDim.x - 1 && threadIdx.x == 0) stimulus_and_clock(P.tail);
+    if (P.tail.active && b == nblocks - 1 && threadIdx.x == 0) stimulus_and_clock(P.tail);
+}
+
+__global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) { plasticity_block(P, blockIdx.x, gridDim.x); }
+
+__global__ __launch_bounds__(256) void k_plasticity_multi(const PlastStepPlan* __restrict__ plans,
+                                                          const uint32_t* __restrict__ block_net,
+                                                          const uint32_t* __restrict__ net_first_block) {
+    const uint32_t net = __builtin_amdgcn_readfirstlane(block_net[blockIdx.x]);
+    const PlastStepPlan& P = plans[net];
+    plasticity_block(P, blockIdx.x - net_first_block[net], P.row_blocks + P.spiking_blocks);
 }
 
 
@@ -1097,6 +1127,35 @@ static uint32_t max_post(const PlastPlan& p) {
     uint32_t m = 0;
     for (uint32_t k = 0; k < p.n; ++k) m = std::max(m, p.p[k].n_post);
     return m;
+}
+
+int launch_multi_propagate(const MultiStep& m, bn_stream_t s) {
+    if (!m.ph_gemv.blocks) return 0;
+    if (m.nontemporal)
+        hipLaunchKernelGGL(k_gemv_multi<true>, dim3(m.ph_gemv.blocks), dim3(256), 0, s, m.gemv, m.ph_gemv.block_net,
+                           m.ph_gemv.net_first_block);
+    else
+        hipLaunchKernelGGL(k_gemv_multi<false>, dim3(m.ph_gemv.blocks), dim3(256), 0, s, m.gemv, m.ph_gemv.block_net,
+                           m.ph_gemv.net_first_block);
+    return (int)hipGetLastError();
+}
+
+int launch_multi_step(const MultiStep& m, bn_stream_t s) {
+    if (m.ph_evolve.blocks)
+        hipLaunchKernelGGL(k_evolve_multi, dim3(m.ph_evolve.blocks), dim3(kFinParts * 64), 0, s, m.evolve,
+                           m.ph_evolve.block_net, m.ph_evolve.net_first_block);
+    if (m.ph_gemv.blocks) {
+        if (m.nontemporal)
+            hipLaunchKernelGGL(k_gemv_multi<true>, dim3(m.ph_gemv.blocks), dim3(256), 0, s, m.gemv,
+                               m.ph_gemv.block_net, m.ph_gemv.net_first_block);
+        else
+            hipLaunchKernelGGL(k_gemv_multi<false>, dim3(m.ph_gemv.blocks), dim3(256), 0, s, m.gemv,
+                               m.ph_gemv.block_net, m.ph_gemv.net_first_block);
+    }
+    if (m.ph_plast.blocks)
+        hipLaunchKernelGGL(k_plasticity_multi, dim3(m.ph_plast.blocks), dim3(256), 0, s, m.plast,
+                           m.ph_plast.block_net, m.ph_plast.net_first_block);
+    return (int)hipGetLastError();
 }
 
 int launch_plasticity(const PlastStepPlan& plan, bn_stream_t s) {

@@ -66,6 +66,12 @@ class BatchInstance(C.Structure):
                 ("stimulus_ids", C.POINTER(C.c_uint32))]
 
 
+class MultiInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("num_networks", "clock_step", "evolve_blocks", "propagate_blocks",
+                                          "plasticity_blocks", "nontemporal", "kernels_per_step", "num_groups",
+                                          "steps_per_graph")] + [("group_weight_bytes", C.c_uint64)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in (
         "num_instances", "padded_instances", "num_excitatory_neurons", "num_inhibitory_neurons", "total_neurons",
@@ -85,7 +91,9 @@ EXPORTED_SYMBOLS = [
     "bn_sim_time_propagate_kernel", "bn_config_dump_json", "bn_generate_weight_matrix",
     "bn_batch_create", "bn_batch_destroy", "bn_batch_run", "bn_batch_synchronize", "bn_batch_run_timed",
     "bn_batch_time_gemm", "bn_batch_set_gemm_rows", "bn_batch_get_info", "bn_batch_get_state",
-    "bn_batch_workload_instance",
+    "bn_batch_workload_instance", "bn_multi_create", "bn_multi_destroy", "bn_multi_run", "bn_multi_run_timed",
+    "bn_multi_synchronize", "bn_multi_network", "bn_multi_get_info", "bn_multi_time_propagate",
+    "bn_multi_set_schedule",
 ]
 
 _lib = None
@@ -135,8 +143,18 @@ def lib() -> C.CDLL:
     L.bn_batch_get_info.argtypes = [P, C.POINTER(BatchInfo)]
     L.bn_batch_get_state.argtypes = [P, U, C.POINTER(NeuronStateHost)]
     L.bn_batch_workload_instance.argtypes = [U, U, U, C.c_float, up, fp]
+    L.bn_multi_create.argtypes = [C.POINTER(C.c_char_p), U, U, C.POINTER(P)]
+    L.bn_multi_destroy.argtypes = [P]
+    L.bn_multi_destroy.restype = None
+    L.bn_multi_run.argtypes = [P, U]
+    L.bn_multi_run_timed.argtypes = [P, U, fp]
+    L.bn_multi_synchronize.argtypes = [P]
+    L.bn_multi_network.argtypes = [P, U, C.POINTER(P)]
+    L.bn_multi_get_info.argtypes = [P, C.POINTER(MultiInfo)]
+    L.bn_multi_time_propagate.argtypes = [P, U, fp]
+    L.bn_multi_set_schedule.argtypes = [P, C.c_uint64, U]
     for f in EXPORTED_SYMBOLS:
-        if f not in ("bn_last_error", "bn_sim_destroy", "bn_batch_destroy"):
+        if f not in ("bn_last_error", "bn_sim_destroy", "bn_batch_destroy", "bn_multi_destroy"):
             getattr(L, f).restype = I
     _lib = L
     return L
@@ -197,7 +215,23 @@ class Simulation:
         self.NE = self.info.num_excitatory_neurons
         self.NI = self.info.num_inhibitory_neurons
 
+    @classmethod
+    def _borrowed(cls, handle, owner):
+        """A view of a bn_sim owned by another object (a network of a MultiSimulation)."""
+        self = cls.__new__(cls)
+        self.flag_bits = 0
+        self.h = handle
+        self._owner = owner
+        self.info = self.get_info()
+        self.N = self.info.total_neurons
+        self.NE = self.info.num_excitatory_neurons
+        self.NI = self.info.num_inhibitory_neurons
+        return self
+
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self.h = None
+            return
         if getattr(self, "h", None):
             lib().bn_sim_destroy(self.h)
             self.h = None
@@ -330,3 +364,61 @@ class BatchSimulation:
         st = NeuronStateHost(*[arrays[n].ctypes.data for n, _ in _STATE_FIELDS])
         _check(lib().bn_batch_get_state(self.h, instance, C.byref(st)))
         return arrays
+
+
+class MultiSimulation:
+    """Several plastic networks advanced together (bn_multi_* API); networks[k] is the
+    network's own Simulation view (state, weights, traces)."""
+
+    def __init__(self, configs, flags=("lazy_normalize",)):
+        bits = 0
+        for f in flags:
+            bits |= BURYN_FLAGS[f]
+        paths = (C.c_char_p * len(configs))(*[c.encode() for c in configs])
+        h = C.c_void_p()
+        _check(lib().bn_multi_create(paths, len(configs), bits, C.byref(h)))
+        self.h = h
+        self.networks = []
+        for k in range(len(configs)):
+            sh = C.c_void_p()
+            _check(lib().bn_multi_network(self.h, k, C.byref(sh)))
+            self.networks.append(Simulation._borrowed(sh, self))
+
+    def close(self):
+        if getattr(self, "h", None):
+            for s in self.networks:
+                s.close()
+            lib().bn_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get_info(self) -> MultiInfo:
+        info = MultiInfo()
+        _check(lib().bn_multi_get_info(self.h, C.byref(info)))
+        return info
+
+    def run(self, n: int) -> None:
+        _check(lib().bn_multi_run(self.h, n))
+
+    def synchronize(self) -> None:
+        _check(lib().bn_multi_synchronize(self.h))
+
+    def run_timed(self, n: int) -> float:
+        ms = C.c_float(0)
+        _check(lib().bn_multi_run_timed(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def time_propagate(self, n: int) -> float:
+        ms = C.c_float(0)
+        _check(lib().bn_multi_time_propagate(self.h, n, C.byref(ms)))
+        return ms.value
+
+    def set_schedule(self, group_weight_bytes: int, steps_per_graph: int = 10) -> None:
+        _check(lib().bn_multi_set_schedule(self.h, group_weight_bytes, steps_per_graph))
+
+    def network(self, k: int) -> "Simulation":
+        sh = C.c_void_p()
+        _check(lib().bn_multi_network(self.h, k, C.byref(sh)))  # synchronises the multi stream
+        return self.networks[k]

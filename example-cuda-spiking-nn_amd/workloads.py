@@ -1,0 +1,109 @@
+"""Workload definitions of the batched forms (configs written from the 3,125-neuron network's
+parameters, configs/net3125_stdp.json).
+
+- ``uneven_plastic_configs``: independent plastic networks of uneven size.  N_E is drawn from
+  {400, 900, 1600, 2500} by Python's Mersenne Twister seeded with ``seed`` (the recorded draw);
+  N_I = N_E / 4; E on a sqrt(N_E) grid at spacing 1, I on a sqrt(N_I) grid at spacing 2, side
+  sqrt(N_E), Gaussian spread S = 25, no distance cut-off; network k uses rng_seed = k (its own
+  connectivity and noise).  Triplet STDP on E->E and iSTDP on I->E as in the 3,125-neuron net.
+- ``shared_batch_instances``: the shared-weight batch's instances (noise key = instance, a
+  200 pA x U(0,1) pulse on 100 E neurons from step 2,000 to 4,000, the engine's recorded
+  per-instance draw).
+"""
+from __future__ import annotations
+
+import copy
+import json
+import math
+import os
+import random
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SIZES = (400, 900, 1600, 2500)
+
+
+def base_config() -> dict:
+    return json.load(open(os.path.join(REPO, "configs", "net3125_stdp.json")))
+
+
+def network_config(n_e: int, seed: int, steps: int = 10001) -> dict:
+    """The 3,125-neuron network's model on a sqrt(n_e) x sqrt(n_e) excitatory grid."""
+    side = int(round(math.sqrt(n_e)))
+    if side * side != n_e or side % 2:
+        raise ValueError("n_e must be the square of an even number")
+    c = copy.deepcopy(base_config())
+    c.update({"num_excitatory_neurons": n_e, "num_inhibitory_neurons": n_e // 4,
+              "excitatory_side_points": side, "inhibitory_side_points": side // 2,
+              "excitatory_location_spacing": 1.0, "inhibitory_location_spacing": 2.0,
+              "side_length": float(side), "rng_seed": seed, "simulation_steps": steps,
+              "output_directory": f"outputs/uneven_{n_e}_{seed}",
+              "voltage_recording_neuron_ids": [0]})
+    for p in ("e_to_e", "e_to_i", "i_to_e", "i_to_i"):
+        c[p + "_x_location_length"] = float(side)
+        c[p + "_y_location_length"] = float(side)
+        c[p + "_max_distance"] = float(side)
+        c[p + "_spread_scale"] = 25.0
+    return c
+
+
+def uneven_sizes(count: int = 48, seed: int = 2024) -> list[int]:
+    rng = random.Random(seed)
+    return [SIZES[rng.randrange(len(SIZES))] for _ in range(count)]
+
+
+def uneven_plastic_configs(count: int = 48, seed: int = 2024, steps: int = 10001) -> list[dict]:
+    return [network_config(n_e, k, steps) for k, n_e in enumerate(uneven_sizes(count, seed))]
+
+
+def shared_batch_instances(engine, n_e: int, count: int = 64, start: int = 2000, end: int = 4000) -> list[dict]:
+    out = []
+    for k in range(count):
+        ids, amp = engine.batch_workload_instance(k, n_e, 100, 200e-12)
+        out.append({"noise_seed": k, "stimulus_ids": ids, "stimulus_amplitude": amp,
+                    "stimulus_start": start, "stimulus_end": end})
+    return out
+
+
+def autotune_row_blocks(engine, sizes, workdir: str, candidates=(32, 64, 128, 256, 512), reps: int = 4,
+                        warmup: int = 100, steps: int = 200) -> dict:
+    """The batched plastic step's tile autotuner: for each network size, time the multi-network
+    step of `reps` networks of that size with every candidate GEMV row block (rows summed per
+    propagate partial; it also fixes the FP64 summation order, so the choice is written into
+    each network's config as gemv_row_blocks and a plain run of that config reproduces the
+    network bit for bit).  Returns {n_e: (best_rows, {rows: us_per_step})}."""
+    out = {}
+    for n_e in sorted(set(sizes)):
+        times = {}
+        for rb in candidates:
+            if rb > n_e // 4 and rb != candidates[0]:
+                continue
+            paths = []
+            for r in range(reps):
+                cfg = network_config(n_e, 1000 + r, steps=warmup + steps + 1)
+                cfg["gemv_row_blocks"] = [rb] * 4
+                path = os.path.join(workdir, f"tune_{n_e}_{rb}_{r}.json")
+                json.dump(cfg, open(path, "w"))
+                paths.append(path)
+            multi = engine.MultiSimulation(paths, flags=("lazy_normalize",))
+            multi.run_timed(warmup)
+            times[rb] = multi.run_timed(steps) * 1000.0 / steps
+            multi.close()
+        best = min(times, key=times.get)
+        out[n_e] = (best, times)
+    return out
+
+
+def autotune_schedule(multi, group_mib=(0, 256, 192, 128, 96, 64), steps_per_graph=(10, 50), steps: int = 200) -> dict:
+    """The batched plastic step's schedule autotuner: time `steps` steps of every network under
+    each (weight MiB per network group, steps per graph) candidate and keep the fastest on
+    `multi`.  Results do not depend on the schedule (the networks are independent)."""
+    times = {}
+    for mib in group_mib:
+        for spg in steps_per_graph:
+            multi.set_schedule(mib << 20, spg)
+            multi.run_timed(spg)
+            times[(mib, spg)] = multi.run_timed(steps) * 1000.0 / steps
+    best = min(times, key=times.get)
+    multi.set_schedule(best[0] << 20, best[1])
+    return {"group_mib": best[0], "steps_per_graph": best[1],
+            "us_per_step": {f"{m}MiB/{k}": round(v, 2) for (m, k), v in times.items()}}

@@ -417,6 +417,49 @@ int bn_batch_get_state(bn_batch* batch, uint32_t instance, const bn_neuron_state
 int bn_batch_workload_instance(uint32_t instance, uint32_t num_excitatory, uint32_t num_ids, float max_amplitude,
                                uint32_t* ids, float* amplitude);
 
+/* -------------------------------------------------------------------------------------
+ * Several independent plastic networks (any sizes, own weights, seeds and plasticity) advanced
+ * together: one launch per phase (evolve, propagate, plasticity) covers every network, ten
+ * steps per hipGraph.  Each network keeps its own bn_sim (state, weights, plans) and computes
+ * bit for bit what it computes alone.  No counterpart in the reference (one network per
+ * process there).
+ * ------------------------------------------------------------------------------------- */
+typedef struct bn_multi bn_multi;
+
+typedef struct bn_multi_info {
+    uint32_t num_networks;
+    uint32_t clock_step;
+    uint32_t evolve_blocks;          /* blocks of one launch of each phase, all networks */
+    uint32_t propagate_blocks;
+    uint32_t plasticity_blocks;
+    uint32_t nontemporal;            /* weight loads past the caches (total weights > 192 MB) */
+    uint32_t kernels_per_step;       /* launches per step of every network (3 per group) */
+    uint32_t num_groups;             /* schedule: network groups stepped one after the other */
+    uint32_t steps_per_graph;
+    uint64_t group_weight_bytes;     /* weight bytes per group (0 = one group) */
+} bn_multi_info;
+
+/* flags: BN_SIM_* of every network (BN_SIM_LAZY_NORMALIZE is required when a network
+ * normalises; monitors, files and the component path are not available here). */
+int bn_multi_create(const char* const* config_paths, uint32_t num_networks, uint32_t flags, bn_multi** out);
+void bn_multi_destroy(bn_multi* multi);
+int bn_multi_run(bn_multi* multi, uint32_t num_steps);
+int bn_multi_run_timed(bn_multi* multi, uint32_t num_steps, float* ms);
+int bn_multi_synchronize(bn_multi* multi);
+/* Schedule: networks are grouped in order so that a group's weights stay under
+ * group_weight_bytes (0 = one group); each group advances steps_per_graph steps per graph
+ * launch, the groups one after the other (networks are independent, so results do not depend
+ * on the schedule).  Sized to the 256 MB MALL, a group's weights stay cache-resident across
+ * its steps. */
+int bn_multi_set_schedule(bn_multi* multi, uint64_t group_weight_bytes, uint32_t steps_per_graph);
+/* Wall time of `num_launches` back-to-back propagate launches over every network; the state of
+ * every network is unchanged (pending partial sums saved and restored). */
+int bn_multi_time_propagate(bn_multi* multi, uint32_t num_launches, float* ms);
+/* The bn_sim of network `index` (owned by the multi object; synchronises first) for state,
+ * weight and trace downloads. */
+int bn_multi_network(bn_multi* multi, uint32_t index, bn_sim** out);
+int bn_multi_get_info(bn_multi* multi, bn_multi_info* info);
+
 #ifdef __cplusplus
 }
 #endif

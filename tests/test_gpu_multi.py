@@ -1,0 +1,99 @@
+"""Several plastic networks of uneven size in one launch per phase (bn_multi_*): every network
+must compute bit for bit what it computes alone (same plans, same kernels), so each is checked
+against a plain run of its own config: spikes every step, and state, traces and weights."""
+import importlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import engine, small_config
+
+pytestmark = pytest.mark.gpu
+
+
+def _write(tmp_path, cfg, name):
+    p = os.path.join(str(tmp_path), name)
+    json.dump(cfg, open(p, "w"))
+    return p
+
+
+def _check_same(a, b, label):
+    sa, sb = a.state(), b.state()
+    for key in sa:
+        assert np.array_equal(sa[key], sb[key]), f"{label}: {key}"
+    for k in range(4):
+        assert np.array_equal(a.weights(k), b.weights(k)), f"{label}: weights {k}"
+        for x, y in zip(a.connection_traces(k), b.connection_traces(k)):
+            assert np.array_equal(x, y), f"{label}: connection traces {k}"
+    for off in range(a.info.num_plasticities):
+        for x, y in zip(a.plasticity_traces(off), b.plasticity_traces(off)):
+            assert np.array_equal(x, y), f"{label}: plasticity traces {off}"
+
+
+def test_multi_networks_match_their_plain_runs(tmp_path):
+    cfgs = [small_config(n_e_side=12, n_i_side=6, steps=300, seed=11),
+            small_config(n_e_side=20, n_i_side=10, steps=300, seed=12),
+            small_config(n_e_side=16, n_i_side=8, steps=300, seed=13, stimulus=False),
+            small_config(n_e_side=24, n_i_side=12, steps=300, seed=14)]
+    paths = [_write(tmp_path, c, f"n{k}.json") for k, c in enumerate(cfgs)]
+    multi = engine().MultiSimulation(paths, flags=("lazy_normalize",))
+    assert multi.get_info().num_networks == 4
+    plains = [engine().Simulation(p, flags=("no_monitors", "quiet", "lazy_normalize")) for p in paths]
+    for step in range(120):
+        multi.run(1)
+        for k, p in enumerate(plains):
+            p.run(1)
+            if step % 10 == 9:
+                assert np.array_equal(multi.network(k).state()["spikes"], p.state()["spikes"]), (k, step)
+    multi.run(30)
+    for p in plains:
+        p.run(30)
+    for k, p in enumerate(plains):
+        _check_same(multi.network(k), p, f"network {k}")
+    for p in plains:
+        p.close()
+    multi.close()
+
+
+def test_multi_uneven_workload_networks(tmp_path):
+    # Eight networks of the uneven plastic workload at their real sizes (every size class), 100 steps.
+    wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+    all48 = wl.uneven_plastic_configs(count=48, steps=300)
+    cfgs = [all48[i] for i in (0, 1, 2, 13, 19, 3, 4, 5)]
+    assert sorted(set(c["num_excitatory_neurons"] for c in cfgs[:4])) == [400, 900, 1600, 2500]
+    paths = [_write(tmp_path, c, f"u{k}.json") for k, c in enumerate(cfgs)]
+    multi = engine().MultiSimulation(paths, flags=("lazy_normalize",))
+    multi.run(100)
+    for k in (0, 1, 2, 3):
+        p = engine().Simulation(paths[k], flags=("no_monitors", "quiet", "lazy_normalize"))
+        p.run(100)
+        _check_same(multi.network(k), p, f"uneven network {k} (N_E {cfgs[k]['num_excitatory_neurons']})")
+        p.close()
+    multi.close()
+
+
+def test_multi_schedule_does_not_change_results(tmp_path):
+    # Networks in separate groups advancing 7 steps per graph (and a remainder) compute what
+    # they compute in one group.
+    cfgs = [small_config(n_e_side=12, n_i_side=6, steps=300, seed=21),
+            small_config(n_e_side=16, n_i_side=8, steps=300, seed=22),
+            small_config(n_e_side=20, n_i_side=10, steps=300, seed=23)]
+    paths = [_write(tmp_path, c, f"s{k}.json") for k, c in enumerate(cfgs)]
+    a = engine().MultiSimulation(paths, flags=("lazy_normalize",))
+    b = engine().MultiSimulation(paths, flags=("lazy_normalize",))
+    b.set_schedule(1, 7)  # every network in its own group
+    assert b.get_info().num_groups == 3
+    a.run(100)
+    b.run(100)
+    for k in range(3):
+        _check_same(a.network(k), b.network(k), f"network {k}")
+    a.close()
+    b.close()
+
+
+def test_multi_requires_lazy_normalisation(tmp_path):
+    path = _write(tmp_path, small_config(steps=300), "n.json")
+    with pytest.raises(engine().BnError, match="lazy"):
+        engine().MultiSimulation([path], flags=())
