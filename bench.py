@@ -29,9 +29,10 @@ HBM_PEAK_GBS = 8000.0
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("plain", "batched-shared", "batched-uneven"), default="plain",
+    ap.add_argument("--mode", choices=("plain", "batched-shared", "batched-uneven", "adjoint"), default="plain",
                     help="plain: one network; batched-shared: 64 instances sharing frozen weights; "
-                         "batched-uneven: 48 plastic networks of uneven size")
+                         "batched-uneven: 48 plastic networks of uneven size; adjoint: gradient of a "
+                         "1,000-step window of the 3,125-neuron network (plasticity off)")
     ap.add_argument("--networks", type=int, default=48)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (plain 10000, batched 5001)")
@@ -59,6 +60,10 @@ def main() -> int:
         args.steps = 10001 if args.steps is None else args.steps
         args.warmup = 0 if args.warmup is None else args.warmup
         return bench_uneven(args)
+    if args.mode == "adjoint":
+        args.steps = 5 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        return bench_adjoint(args)
     args.steps = 10000 if args.steps is None else args.steps
     args.warmup = 2000 if args.warmup is None else args.warmup
 
@@ -294,6 +299,116 @@ def bench_uneven(args) -> int:
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_adjoint(args) -> int:
+    """Differentiable window: the 3,125-neuron network with plasticity off, 1,000 steps from the
+    state after a 1,000-step warm-up, loss against the targets of the E->E x 1.1 network,
+    gradient of all four weight matrices, g_leak and tau_decay_e by the hand-written adjoint
+    (surrogate slope 1/(1 mV), checkpoints every 100 steps).  A "step" of the bench is one
+    forward + backward pass over the window; metric: seconds per 1,000 simulated steps."""
+    eng = importlib.import_module("example-cuda-spiking-nn_amd")
+    wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+    T, S, warm = 1000, 100, 1000
+    sim, win, base = wl.adjoint_workload(eng, steps=T, warmup=warm, checkpoint_interval=S)
+    for _ in range(args.warmup):
+        win.forward(outputs=False)
+        win.backward()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f = win.forward(outputs=False)
+        g = win.backward()
+    dt = time.perf_counter() - t0
+    # Forward alone, for the split.
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        win.forward(outputs=False)
+    dt_fwd = time.perf_counter() - t1
+    s_per_1k = dt / args.steps * 1000.0 / T
+    n_w = sum(w.size for w in base)
+    n = sim.N
+    # Algorithmic bytes per window: the weight set read 3 times per step (forward and recompute
+    # propagate, transposed product of the reverse step) plus 160 B of state per neuron per
+    # propagate pass, and dW read-modify-written once per segment.
+    b_step = 4 * n_w + 160 * n
+    alg_bytes = 3 * T * b_step + (T // S) * 2 * 4 * n_w
+    achieved = alg_bytes / (dt / args.steps) / 1e9
+    cpu = None if args.no_cpu_baseline else cpu_baseline_adjoint(sim, win, base, args.cpu_seconds)
+    gnorm = float(sum((d.astype("float64") ** 2).sum() for d in g["dW"]) ** 0.5)
+    line = {
+        "metric": "adjoint s/1k steps (3,125-nrn net, plasticity off)",
+        "value": round(s_per_1k, 5),
+        "unit": "s/1k steps",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1000.0, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state/weights, f64 adjoint reductions",
+        "data": "synthetic: seeded Philox Poisson background, targets from the E->E x 1.1 network",
+        "config": {"workload": f"adjoint window (net3125 {sim.NE:,} E + {sim.NI:,} I, plasticity off, {T} steps "
+                               f"from the state after {warm} steps, 50 recorded E, checkpoints every {S}, "
+                               "surrogate slope 1/(1 mV))",
+                   "window_steps": T, "checkpoint_interval": S, "neurons": n, "weight_entries": int(n_w),
+                   "forward_s_per_1k": round(dt_fwd / args.steps * 1000.0 / T, 5),
+                   "loss": f["loss"], "grad_norm_w": gnorm, "dg_leak": g["dg_leak"],
+                   "dtau_decay_e": g["dtau_decay_e"], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "whole window (forward + recompute propagate, transposed "
+                                                "products, dW updates)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(alg_bytes)},
+        "cpu_baseline": cpu,
+    }
+    win.close()
+    sim.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_baseline_adjoint(sim, win, base, seconds: float) -> dict:
+    """The oracle's FP64 OpenMP adjoint (forward + reverse, all states kept) on the window's start
+    state and weights, over as many steps of the window as fit the time budget."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+    import oracle_py
+
+    eng = importlib.import_module("example-cuda-spiking-nn_amd")
+    st = sim.state()
+    rE, dE, _ = sim.connection_traces(0)
+    rI, dI, _ = sim.connection_traces(2)
+    state = {"V": st["membrane_potentials"], "refr": st["refractory_counter"],
+             "bg_ge": st["background_g_excitatory_synaptic_conductances"],
+             "bg_gi": st["background_g_inhibitory_synaptic_conductances"],
+             "bre": st["background_excitatory_postsynaptic_r_traces"],
+             "bde": st["background_excitatory_postsynaptic_d_traces"],
+             "bri": st["background_inhibitory_postsynaptic_r_traces"],
+             "bdi": st["background_inhibitory_postsynaptic_d_traces"],
+             "r_e": rE, "d_e": dE, "r_i": rI, "d_i": dI, "clock_step": sim.get_info().clock_step}
+    import tempfile
+
+    wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "adj.json")
+        json.dump(wl.adjoint_config(), open(path, "w"))
+        cfg = eng.config_dump(path)
+    p = oracle_py.params_from_config(cfg)
+    W = [b.astype(np.float64) for b in base]
+    ids = np.array([int(k * sim.NE // 50) for k in range(50)], np.uint32)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    steps = 20
+    while True:
+        t0 = time.perf_counter()
+        oracle_py.window(p, W, float(cfg["g_leak"]), float(cfg["tau_decay_e"]), state, steps, ids, beta=1e-3)
+        dt = time.perf_counter() - t0
+        if dt * 2 > seconds or steps >= 1000:
+            break
+        steps = min(1000, max(steps * 2, int(steps * seconds / max(dt, 1e-3) / 2)))
+    return {"value": round(dt / steps * 1000.0, 3), "unit": "s/1k steps", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the window (forward + reverse) from the same start state, FP64 oracle "
+                      f"adjoint, {threads} OpenMP threads, {dt:.1f} s"}
 
 
 def cpu_baseline_batched(base: dict, seconds: float) -> dict:

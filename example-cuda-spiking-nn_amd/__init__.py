@@ -14,6 +14,7 @@ from .engine import (  # noqa: F401
     LIB_PATH,
     BnError,
     Simulation,
+    Window,
     build_library,
     config_dump,
     generate_weight_matrix,

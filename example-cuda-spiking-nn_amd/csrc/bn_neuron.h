@@ -1,0 +1,104 @@
+// Per-neuron device functions shared by the step kernels and the differentiable window:
+// the EIF update, the background update, and the combination of the FP64 propagate partials.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "bn_device.h"
+#include "bn_step.h"
+
+namespace bn {
+
+// kernel_neuron_group_evolve.cu:85-123 — refractory pin, or forward-Euler EIF step.
+__device__ __forceinline__ void eif_step(const NeuronGroupView& v, float ge, float gi, float istim,
+                                         float& V, uint32_t& refr, uint32_t& spike) {
+    if (refr > 0) {
+        refr = refr - 1;
+        spike = 0;
+        V = v.v_reset;
+        return;
+    }
+    const float vv = V;
+    const float linear_leak_term = -1.0f * v.g_leak * (vv - v.v_rest);
+    const float exponential_depolarization_term =
+        v.g_leak * v.depolarization_slope_factor *
+        bn_expf((vv - v.v_effective_spike_threshold) / v.depolarization_slope_factor);
+    const float excitatory_synaptic_term = -1.0f * ge * (vv - v.v_excitatory_reversal);
+    const float inhibitory_synaptic_term = -1.0f * gi * (vv - v.v_inhibitory_reversal);
+    const float dv = (v.timestep / v.membrane_capacitance) *
+                     (linear_leak_term + exponential_depolarization_term +
+                      excitatory_synaptic_term + inhibitory_synaptic_term + istim);
+    const float v_next = vv + dv;
+    if (v_next >= v.v_threshold) {
+        refr = v.refractory_steps;
+        spike = 1;
+        V = v.v_threshold;
+    } else {
+        spike = 0;
+        V = v_next;
+    }
+}
+
+// kernel_neuron_group_evolve.cu:11-83 on values already in registers: decay (+flush) of the
+// four background traces, the Poisson shot counts, the increments, the background conductances.
+__device__ __forceinline__ void background_advance(const NeuronGroupView& v, uint32_t gid, uint32_t step,
+                                                   const bn_device_clock& clk, const uint32_t* tab_e,
+                                                   const uint32_t* tab_i, float& re, float& ri, float& de,
+                                                   float& di, float& bg_ge, float& bg_gi) {
+    re = flush_trace(re * v.background_excitatory_postsynaptic_r_trace_decay);
+    ri = flush_trace(ri * v.background_inhibitory_postsynaptic_r_trace_decay);
+    de = flush_trace(de * v.background_excitatory_postsynaptic_d_trace_decay);
+    di = flush_trace(di * v.background_inhibitory_postsynaptic_d_trace_decay);
+    uint32_t ke = 0, ki = 0;
+    if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
+        const U32x4 r = background_draws(clk, step, gid, v.rng_instance);
+        if (v.mean_background_excitatory_spikes > 0.0f)
+            ke = poisson_from_table(r.x, tab_e, v.poisson_table_excitatory_len);
+        if (v.mean_background_inhibitory_spikes > 0.0f)
+            ki = poisson_from_table(r.y, tab_i, v.poisson_table_inhibitory_len);
+    }
+    if (ke > 0u) {
+        const float inc = (float)ke;
+        re += inc;
+        de += inc;
+    }
+    bg_ge = (de - re) * v.background_excitatory_scaling_factor;
+    if (ki > 0u) {
+        const float inc = (float)ki;
+        ri += inc;
+        di += inc;
+    }
+    bg_gi = (di - ri) * v.background_inhibitory_scaling_factor;
+}
+
+// Sum of FP64 GEMV partials into an FP32 conductance.  Order of the combination: the chunks
+// of a source are cut into kFinParts contiguous parts of ceil(chunks / kFinParts); each part is
+// summed in increasing chunk order from 0.0, then g' = fl32(g + part_0 + part_1 + ...) over the
+// non-empty parts, in FP64.
+__device__ __forceinline__ uint32_t fin_part_size(uint32_t chunks) { return (chunks + kFinParts - 1) / kFinParts; }
+constexpr uint32_t kFinBatch = 16;  // partial loads issued together
+
+__device__ __forceinline__ double fin_part_sum(const FinalizeSource& s, uint32_t j, uint32_t c0, uint32_t c1) {
+    const double* p = s.partials + j;
+    const size_t ld = s.ld;
+    double acc = 0.0;
+    for (uint32_t c = c0; c < c1; c += kFinBatch) {
+        double v[kFinBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kFinBatch; ++k) v[k] = c + k < c1 ? p[(size_t)(c + k) * ld] : 0.0;
+#pragma unroll
+        for (uint32_t k = 0; k < kFinBatch; ++k)
+            if (c + k < c1) acc += v[k];
+    }
+    return acc;
+}
+
+// One thread, every part in order (component path and g materialisation).
+__device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, uint32_t j) {
+    const uint32_t per = fin_part_size(s.chunks);
+    double acc = (double)g;
+    for (uint32_t c0 = 0; per && c0 < s.chunks; c0 += per) acc += fin_part_sum(s, j, c0, min(c0 + per, s.chunks));
+    return (float)acc;
+}
+
+}  // namespace bn

@@ -80,6 +80,17 @@ class BatchInfo(C.Structure):
         ("kernels_per_step", C.c_uint32)]
 
 
+class WindowSpec(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("checkpoint_interval", C.c_uint32), ("n_vrec", C.c_uint32),
+                ("vrec_ids", C.POINTER(C.c_uint32)), ("rate_tau", C.c_float), ("surrogate_beta", C.c_float)]
+
+
+class WindowInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("steps", "checkpoint_interval", "start_step", "total_neurons",
+                                          "num_excitatory_neurons", "num_inhibitory_neurons", "n_vrec")] + [
+        ("g_leak", C.c_float), ("tau_decay_e", C.c_float)]
+
+
 EXPORTED_SYMBOLS = [
     "bn_last_error", "bn_poisson_table", "bn_neuron_group_evolve", "bn_connection_propagate",
     "bn_stimulus_apply", "bn_stimulus_apply_device_cursor", "bn_triplet_plasticity_update",
@@ -93,7 +104,9 @@ EXPORTED_SYMBOLS = [
     "bn_batch_time_gemm", "bn_batch_set_gemm_rows", "bn_batch_get_info", "bn_batch_get_state",
     "bn_batch_workload_instance", "bn_multi_create", "bn_multi_destroy", "bn_multi_run", "bn_multi_run_timed",
     "bn_multi_synchronize", "bn_multi_network", "bn_multi_get_info", "bn_multi_time_propagate",
-    "bn_multi_set_schedule",
+    "bn_multi_set_schedule", "bn_window_create", "bn_window_destroy", "bn_window_set_params",
+    "bn_window_get_weights", "bn_window_set_targets", "bn_window_forward", "bn_window_backward",
+    "bn_window_get_info",
 ]
 
 _lib = None
@@ -153,8 +166,18 @@ def lib() -> C.CDLL:
     L.bn_multi_get_info.argtypes = [P, C.POINTER(MultiInfo)]
     L.bn_multi_time_propagate.argtypes = [P, U, fp]
     L.bn_multi_set_schedule.argtypes = [P, C.c_uint64, U]
+    dp = C.POINTER(C.c_double)
+    L.bn_window_create.argtypes = [P, C.POINTER(WindowSpec), C.POINTER(P)]
+    L.bn_window_destroy.argtypes = [P]
+    L.bn_window_set_params.argtypes = [P, C.POINTER(P), C.c_float, C.c_float]
+    L.bn_window_get_weights.argtypes = [P, U, fp]
+    L.bn_window_set_targets.argtypes = [P, dp, dp]
+    L.bn_window_forward.argtypes = [P, dp, dp, fp, C.POINTER(C.c_uint8)]
+    L.bn_window_backward.argtypes = [P, C.POINTER(P), dp, dp]
+    L.bn_window_get_info.argtypes = [P, C.POINTER(WindowInfo)]
     for f in EXPORTED_SYMBOLS:
-        if f not in ("bn_last_error", "bn_sim_destroy", "bn_batch_destroy", "bn_multi_destroy"):
+        if f not in ("bn_last_error", "bn_sim_destroy", "bn_batch_destroy", "bn_multi_destroy",
+                     "bn_window_destroy"):
             getattr(L, f).restype = I
     _lib = L
     return L
@@ -422,3 +445,101 @@ class MultiSimulation:
         sh = C.c_void_p()
         _check(lib().bn_multi_network(self.h, k, C.byref(sh)))  # synchronises the multi stream
         return self.networks[k]
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ptr(a):
+    """Address of a host numpy array or a device tensor (anything with data_ptr())."""
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    return a.ctypes.data
+
+
+class Window:
+    """Differentiable window of a plasticity-free network (bn_window_* API): ``steps`` steps
+    from the current state of ``sim``, the rate + voltage loss against targets, and its gradient
+    with respect to the four weight matrices, g_leak and tau_decay_e (hand-written adjoint with
+    checkpoint recompute)."""
+
+    def __init__(self, sim: Simulation, steps: int, vrec_ids, checkpoint_interval: int = 100,
+                 rate_tau: float = 5e-3, surrogate_beta: float = 1e-3):
+        self._ids = np.ascontiguousarray(vrec_ids, dtype=np.uint32)
+        spec = WindowSpec(steps, checkpoint_interval, self._ids.size, _up(self._ids), rate_tau, surrogate_beta)
+        h = C.c_void_p()
+        _check(lib().bn_window_create(sim.h, C.byref(spec), C.byref(h)))
+        self.h = h
+        self.info = self.get_info()
+        self.T = self.info.steps
+        self.N = self.info.total_neurons
+        self.NE = self.info.num_excitatory_neurons
+        self.NI = self.info.num_inhibitory_neurons
+        self.nv = self.info.n_vrec
+        self.shapes = [(self.NE, self.NE), (self.NE, self.NI), (self.NI, self.NE), (self.NI, self.NI)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bn_window_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def get_info(self) -> WindowInfo:
+        info = WindowInfo()
+        _check(lib().bn_window_get_info(self.h, C.byref(info)))
+        return info
+
+    def set_params(self, weights=None, g_leak: float | None = None, tau_decay_e: float | None = None) -> None:
+        """weights: None or 4 entries (None = keep), each a float32 [pre][post] numpy array or a
+        contiguous float32 device tensor."""
+        info = self.get_info()
+        gl = info.g_leak if g_leak is None else g_leak
+        td = info.tau_decay_e if tau_decay_e is None else tau_decay_e
+        ptrs = None
+        if weights is not None:
+            keep = []
+            for k, w in enumerate(weights):
+                if w is not None and not hasattr(w, "data_ptr"):
+                    w = np.ascontiguousarray(w, dtype=np.float32)
+                    assert w.shape == self.shapes[k], (w.shape, self.shapes[k])
+                keep.append(w)
+            self._keep = keep
+            ptrs = (C.c_void_p * 4)(*[_ptr(w) for w in keep])
+        _check(lib().bn_window_set_params(self.h, ptrs, gl, td))
+
+    def weights(self, idx: int) -> np.ndarray:
+        out = np.zeros(self.shapes[idx], np.float32)
+        _check(lib().bn_window_get_weights(self.h, idx, _fp(out)))
+        return out
+
+    def set_targets(self, target_rates=None, target_v=None) -> None:
+        tr = None if target_rates is None else np.ascontiguousarray(target_rates, dtype=np.float64)
+        tv = None if target_v is None else np.ascontiguousarray(target_v, dtype=np.float64)
+        if tr is not None:
+            assert tr.shape == (self.T, 2), tr.shape
+        if tv is not None:
+            assert tv.shape == (self.T, self.nv), tv.shape
+        _check(lib().bn_window_set_targets(self.h, _dp(tr), _dp(tv)))
+
+    def forward(self, outputs: bool = True) -> dict:
+        loss = C.c_double(0)
+        rates = np.zeros((self.T, 2), np.float64) if outputs else None
+        v = np.zeros((self.T, self.nv), np.float32) if outputs else None
+        raster = np.zeros((self.T, self.N), np.uint8) if outputs else None
+        _check(lib().bn_window_forward(self.h, C.byref(loss), _dp(rates), _fp(v),
+                                       None if raster is None else raster.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return {"loss": loss.value, "rates": rates, "v": v, "raster": raster}
+
+    def backward(self, dW=None) -> dict:
+        """Gradient of the last forward's loss.  dW: None (host arrays returned) or 4 contiguous
+        float32 device tensors to fill."""
+        out = dW if dW is not None else [np.zeros(s, np.float32) for s in self.shapes]
+        ptrs = (C.c_void_p * 4)(*[_ptr(w) for w in out])
+        dgl, dtau = C.c_double(0), C.c_double(0)
+        _check(lib().bn_window_backward(self.h, ptrs, C.byref(dgl), C.byref(dtau)))
+        return {"dW": out, "dg_leak": dgl.value, "dtau_decay_e": dtau.value}

@@ -6,6 +6,10 @@ parameters, configs/net3125_stdp.json).
   N_I = N_E / 4; E on a sqrt(N_E) grid at spacing 1, I on a sqrt(N_I) grid at spacing 2, side
   sqrt(N_E), Gaussian spread S = 25, no distance cut-off; network k uses rng_seed = k (its own
   connectivity and noise).  Triplet STDP on E->E and iSTDP on I->E as in the 3,125-neuron net.
+- ``adjoint_workload``: the differentiable window on the 3,125-neuron network with plasticity
+  off: the state after a warm-up run, 50 recorded E neurons (every N_E/50-th id), targets
+  (smoothed rates and recorded voltages) produced by the same window with the E->E weights
+  scaled by 1.1, surrogate slope 1/(1 mV), checkpoints every 100 steps.
 - ``shared_batch_instances``: the shared-weight batch's instances (noise key = instance, a
   200 pA x U(0,1) pulse on 100 E neurons from step 2,000 to 4,000, the engine's recorded
   per-instance draw).
@@ -107,3 +111,30 @@ def autotune_schedule(multi, group_mib=(0, 256, 192, 128, 96, 64), steps_per_gra
     multi.set_schedule(best[0] << 20, best[1])
     return {"group_mib": best[0], "steps_per_graph": best[1],
             "us_per_step": {f"{m}MiB/{k}": round(v, 2) for (m, k), v in times.items()}}
+
+
+def adjoint_config(base: dict | None = None) -> dict:
+    c = copy.deepcopy(base if base is not None else base_config())
+    c["e_e_plasticity_enabled"] = False
+    c["i_e_plasticity_enabled"] = False
+    return c
+
+
+def adjoint_workload(engine, config: dict | None = None, steps: int = 1000, warmup: int = 1000, n_vrec: int = 50,
+                     checkpoint_interval: int = 100, surrogate_beta: float = 1e-3, target_scale: float = 1.1):
+    """Returns (sim, window, base weights): the window starts from the state after `warmup`
+    steps and holds the targets of the E->E x target_scale network; its parameters are back at
+    the base weights."""
+    cfg = adjoint_config(config)
+    sim = engine.Simulation(cfg, flags=("no_monitors", "quiet"))
+    sim.run(warmup)
+    sim.synchronize()
+    ne = sim.NE
+    ids = [int(k * ne // n_vrec) for k in range(n_vrec)]
+    win = engine.Window(sim, steps, ids, checkpoint_interval=checkpoint_interval, surrogate_beta=surrogate_beta)
+    base = [win.weights(k) for k in range(4)]
+    win.set_params([base[0] * target_scale, None, None, None])
+    tgt = win.forward()
+    win.set_params([base[0], None, None, None])
+    win.set_targets(tgt["rates"], tgt["v"].astype("float64"))
+    return sim, win, base

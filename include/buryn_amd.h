@@ -460,6 +460,62 @@ int bn_multi_time_propagate(bn_multi* multi, uint32_t num_launches, float* ms);
 int bn_multi_network(bn_multi* multi, uint32_t index, bn_sim** out);
 int bn_multi_get_info(bn_multi* multi, bn_multi_info* info);
 
+/* -------------------------------------------------------------------------------------
+ * Differentiable window: `steps` plasticity-free, stimulus-free steps of a network from the
+ * state of a bn_sim, taken when the window is created; the loss
+ *   L = (1/T) sum_t [ sum_{p=E,I} (rho_p,t - rho*_p,t)^2 + (1/n_vrec) sum_k ((V_k,t - V*_k,t) / 1 mV)^2 ],
+ *   rho_p,t = a rho_p,t-1 + (1 - a) (spikes of p at t / N_p) / dt,  a = exp(-dt / rate_tau),
+ * and its gradient with respect to the four weight matrices, g_leak and tau_decay_e by the
+ * adjoint (reverse) pass: the forward stores the state every checkpoint_interval steps, the
+ * backward recomputes each segment with per-step records and walks it backwards.  The spike
+ * derivative is the fast sigmoid 1 / (beta (1 + |U - v_threshold| / beta)^2), beta =
+ * surrogate_beta (0: spikes held fixed — the exact derivative wherever no spike flips); the
+ * reset passes gradient with the spike held fixed, refractory steps pass none.  No
+ * counterpart in the reference (no gradient code there); the step it differentiates is
+ * kernel_neuron_group_evolve.cu:11-130 with kernel_connection_propagate.cu:8-94 and the
+ * trace decay exp(-dt / tau_decay_e) of connection.cpp:46-47.
+ * ------------------------------------------------------------------------------------- */
+typedef struct bn_window bn_window;
+
+typedef struct bn_window_spec {
+    uint32_t steps;                  /* T */
+    uint32_t checkpoint_interval;    /* steps per recompute segment (100) */
+    uint32_t n_vrec;                 /* recorded excitatory neurons entering the loss */
+    const uint32_t* vrec_ids;        /* [n_vrec] global ids < N_E */
+    float rate_tau;                  /* s (5e-3) */
+    float surrogate_beta;            /* V (1e-3 = slope 1/(1 mV)); 0 = spikes held fixed */
+} bn_window_spec;
+
+typedef struct bn_window_info {
+    uint32_t steps;
+    uint32_t checkpoint_interval;
+    uint32_t start_step;             /* noise counter of the first step */
+    uint32_t total_neurons;
+    uint32_t num_excitatory_neurons;
+    uint32_t num_inhibitory_neurons;
+    uint32_t n_vrec;
+    float g_leak;                    /* current parameters */
+    float tau_decay_e;
+} bn_window_info;
+
+/* The network must have both plasticities off.  Weights, g_leak and tau_decay_e start at the
+ * simulation's values. */
+int bn_window_create(bn_sim* sim, const bn_window_spec* spec, bn_window** out);
+void bn_window_destroy(bn_window* window);
+/* weights: NULL or 4 pointers (EE, EI, IE, II; any may be NULL = keep), dense [pre][post]
+ * FP32, host or device memory. */
+int bn_window_set_params(bn_window* window, const float* const* weights, float g_leak, float tau_decay_e);
+int bn_window_get_weights(bn_window* window, uint32_t weight_matrix_idx, float* dst);
+/* target_rates [steps][2] Hz (E, I), target_v [steps][n_vrec] V; NULL = zero. */
+int bn_window_set_targets(bn_window* window, const double* target_rates, const double* target_v);
+/* Runs the window from the start state and evaluates L.  Outputs may be NULL: rates
+ * [steps][2] (FP64), v [steps][n_vrec], raster [steps][N] (0/1). */
+int bn_window_forward(bn_window* window, double* loss, double* rates, float* v, uint8_t* raster);
+/* Gradient of the last forward's L.  dW: NULL or 4 pointers (NULL = skip), dense [pre][post]
+ * FP32, host or device memory. */
+int bn_window_backward(bn_window* window, float* const* dW, double* dg_leak, double* dtau_decay_e);
+int bn_window_get_info(bn_window* window, bn_window_info* info);
+
 #ifdef __cplusplus
 }
 #endif

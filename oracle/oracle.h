@@ -105,6 +105,42 @@ void oracle_get_connection_traces(const oracle_sim* s, uint32_t idx, float* r, f
 void oracle_get_plasticity_traces(const oracle_sim* s, uint32_t plasticity_offset, float* pre, float* post,
                                   float* trip_pre, float* trip_post);
 
+/* Differentiable window (adjoint.cpp): FP64 forward of the plasticity-free, stimulus-free step
+ * from a given state, the window loss, and its gradient.  State after the step before the
+ * window, global ids (length N) or per pre group (r_e, d_e: N_E; r_i, d_i: N_I). */
+typedef struct oracle_window_state {
+    const float* V;
+    const uint32_t* refr;
+    const float* bg_ge;
+    const float* bg_gi;
+    const float* bre;
+    const float* bde;
+    const float* bri;
+    const float* bdi;
+    const float* r_e;
+    const float* d_e;
+    const float* r_i;
+    const float* d_i;
+    uint32_t clock_step;          /* step index of the window's first step (noise counter) */
+} oracle_window_state;
+
+typedef struct oracle_window_spec {
+    uint32_t steps;
+    uint32_t n_vrec;
+    const uint32_t* vrec_ids;      /* excitatory ids whose V enters the loss */
+    double rate_tau;               /* s, smoothing of the population rates */
+    double surrogate_beta;         /* V; 0 = spikes held fixed (exact derivative) */
+    const double* target_rates;    /* [steps][2] Hz (E, I), or NULL (zero) */
+    const double* target_v;        /* [steps][n_vrec] V, or NULL (zero) */
+    const uint8_t* forced_spikes;  /* [steps][N] raster to follow instead of thresholding, or NULL */
+} oracle_window_spec;
+
+/* W[c]: dense [pre][post] FP64 (EE, EI, IE, II).  dW = NULL: forward and loss only.  Outputs
+ * may be NULL.  rates_out [steps][2], v_out [steps][n_vrec], spikes_out [steps][N]. */
+int oracle_window(const oracle_params* p, const double* const* W, double g_leak, double tau_decay_e,
+                  const oracle_window_state* s0, const oracle_window_spec* spec, double* loss, double* const* dW,
+                  double* dg_leak, double* dtau_decay_e, double* rates_out, double* v_out, uint8_t* spikes_out);
+
 /* Primitives (for the pinning tests). */
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float oracle_expf(float x);

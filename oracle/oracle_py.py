@@ -50,6 +50,17 @@ class Params(C.Structure):
 _lib = None
 
 
+class WindowState(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("V", "refr", "bg_ge", "bg_gi", "bre", "bde", "bri", "bdi", "r_e", "d_e",
+                                          "r_i", "d_i")] + [("clock_step", C.c_uint32)]
+
+
+class WindowSpec(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("n_vrec", C.c_uint32), ("vrec_ids", C.c_void_p), ("rate_tau", C.c_double),
+                ("surrogate_beta", C.c_double), ("target_rates", C.c_void_p), ("target_v", C.c_void_p),
+                ("forced_spikes", C.c_void_p)]
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
@@ -77,6 +88,10 @@ def lib() -> C.CDLL:
         L.oracle_expf.restype = F
         L.oracle_poisson_table.argtypes = [F, up, U]
         L.oracle_poisson_table.restype = U
+        L.oracle_window.argtypes = [C.POINTER(Params), C.POINTER(dp), C.c_double, C.c_double,
+                                    C.POINTER(WindowState), C.POINTER(WindowSpec), dp, C.POINTER(dp), dp, dp, dp, dp,
+                                    C.POINTER(C.c_uint8)]
+        L.oracle_window.restype = I
         _lib = L
     return _lib
 
@@ -223,3 +238,55 @@ def poisson_table(mean: float, cap=256):
     t = np.zeros(cap, np.uint32)
     n = lib().oracle_poisson_table(mean, _up(t), cap)
     return t[:n]
+
+
+def window(params: Params, W, g_leak: float, tau_decay_e: float, state: dict, steps: int, vrec_ids,
+           rate_tau: float = 5e-3, beta: float = 1e-3, target_rates=None, target_v=None, forced_spikes=None,
+           grad: bool = True) -> dict:
+    """FP64 forward + reverse pass of the differentiable window (oracle/adjoint.cpp).
+
+    W: four float64 arrays [pre][post] (EE, EI, IE, II).  state: dict of float32/uint32 arrays
+    V, refr, bg_ge, bg_gi, bre, bde, bri, bdi (length N), r_e, d_e (N_E), r_i, d_i (N_I) and
+    clock_step.  Returns loss, rates [steps][2], v [steps][n_vrec], spikes [steps][N] and, with
+    grad, dW (list of 4), dg_leak, dtau_decay_e."""
+    NE, NI = params.num_excitatory_neurons, params.num_inhibitory_neurons
+    N = NE + NI
+    Wd = [np.ascontiguousarray(w, np.float64) for w in W]
+    keep = []
+
+    def ptr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    st = WindowState()
+    for name in ("V", "bg_ge", "bg_gi", "bre", "bde", "bri", "bdi", "r_e", "d_e", "r_i", "d_i"):
+        setattr(st, name, ptr(state[name], np.float32))
+    st.refr = ptr(state["refr"], np.uint32)
+    st.clock_step = int(state["clock_step"])
+    vrec = np.ascontiguousarray(vrec_ids, np.uint32)
+    sp = WindowSpec()
+    sp.steps, sp.n_vrec, sp.vrec_ids = steps, vrec.size, vrec.ctypes.data
+    sp.rate_tau, sp.surrogate_beta = rate_tau, beta
+    sp.target_rates = None if target_rates is None else ptr(target_rates, np.float64)
+    sp.target_v = None if target_v is None else ptr(target_v, np.float64)
+    sp.forced_spikes = None if forced_spikes is None else ptr(forced_spikes, np.uint8)
+    dp = C.POINTER(C.c_double)
+    Wp = (dp * 4)(*[w.ctypes.data_as(dp) for w in Wd])
+    loss = C.c_double(0)
+    rates = np.zeros((steps, 2))
+    v = np.zeros((steps, vrec.size))
+    spikes = np.zeros((steps, N), np.uint8)
+    out = {"loss": None, "rates": rates, "v": v, "spikes": spikes}
+    dW = [np.zeros_like(w) for w in Wd] if grad else None
+    dWp = (dp * 4)(*[d.ctypes.data_as(dp) for d in dW]) if grad else None
+    dgl, dtau = C.c_double(0), C.c_double(0)
+    rc = lib().oracle_window(C.byref(params), Wp, g_leak, tau_decay_e, C.byref(st), C.byref(sp), C.byref(loss), dWp,
+                             C.byref(dgl), C.byref(dtau), rates.ctypes.data_as(dp), v.ctypes.data_as(dp),
+                             spikes.ctypes.data_as(C.POINTER(C.c_uint8)))
+    if rc:
+        raise RuntimeError("oracle_window failed")
+    out["loss"] = loss.value
+    if grad:
+        out.update({"dW": dW, "dg_leak": dgl.value, "dtau_decay_e": dtau.value})
+    return out

@@ -6,6 +6,7 @@ import pytest
 TESTS = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, TESTS)
 sys.path.insert(0, os.path.dirname(TESTS))
+sys.path.insert(0, os.path.join(os.path.dirname(TESTS), "oracle"))
 
 
 def pytest_configure(config):
