@@ -7,9 +7,29 @@ recompute, surrogate spike derivative) and returns the gradients of the four wei
 (copied device to device, no host round trip) or on the host.  PyTorch is only the gradient
 interface here: every step, loss and adjoint runs in the HIP library.
 """
+import os
+
 import torch
 
+from . import engine as _engine
 from .engine import Window
+
+
+def _check_one_hip_runtime() -> None:
+    # The PyTorch wheel bundles its own HIP runtime.  Loaded after torch, the engine library
+    # binds to that runtime (same soname); loaded before, it brings a second runtime into the
+    # process and PyTorch can no longer open the GPU.
+    if _engine._lib is None:
+        return
+    torch_lib = os.path.realpath(os.path.join(os.path.dirname(torch.__file__), "lib"))
+    with open("/proc/self/maps") as f:
+        hip = {line.split()[-1] for line in f if "libamdhip64" in line}
+    if any(not os.path.realpath(p).startswith(torch_lib) for p in hip):
+        raise ImportError("the engine library was loaded before torch: import torch (or this module) "
+                          "before the first engine call to share one HIP runtime with PyTorch")
+
+
+_check_one_hip_runtime()
 
 
 class WindowLoss(torch.autograd.Function):

@@ -62,12 +62,146 @@ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
 
 inline double flush(double v) { return v < 1e-30 ? 0.0 : v; }
 
-}  // namespace
 
-extern "C" int oracle_window(const oracle_params* p, const double* const* W, double g_leak, double tau_decay_e,
-                             const oracle_window_state* s0, const oracle_window_spec* spec, double* loss,
-                             double* const* dW, double* dg_leak, double* dtau_decay_e, double* rates_out,
-                             double* v_out, uint8_t* spikes_out) {
+struct WinConst {
+    uint32_t NE, NI, N, T, nv;
+    double dt, kappa, Vr, VT, DT, Ee, Ei, theta, K, aE_d, aE_r, aI_d, aI_r, a_rate, beta, g_leak, tau_decay_e;
+};
+
+WinConst window_constants(const oracle_params* p, double g_leak, double tau_decay_e, const oracle_window_spec* spec) {
+    WinConst c;
+    c.NE = p->num_excitatory_neurons, c.NI = p->num_inhibitory_neurons, c.N = c.NE + c.NI, c.T = spec->steps;
+    c.nv = spec->n_vrec;
+    c.dt = p->timestep;
+    c.kappa = c.dt / (double)p->membrane_capacitance;
+    c.Vr = p->v_rest, c.VT = p->v_effective_spike_threshold, c.DT = p->depolarization_slope_factor;
+    c.Ee = p->v_excitatory_reversal, c.Ei = p->v_inhibitory_reversal, c.theta = p->v_threshold;
+    c.K = p->postsynaptic_kernel_normalization_value;
+    c.aE_d = std::exp(-c.dt / tau_decay_e), c.aE_r = std::exp(-c.dt / (double)p->tau_rise_e);
+    c.aI_d = std::exp(-c.dt / (double)p->tau_decay_i), c.aI_r = std::exp(-c.dt / (double)p->tau_rise_i);
+    c.a_rate = std::exp(-c.dt / spec->rate_tau);
+    c.beta = spec->surrogate_beta;
+    c.g_leak = g_leak;
+    c.tau_decay_e = tau_decay_e;
+    return c;
+}
+
+// Reverse pass over the records of a forward run: dW (dense [pre][post]), dL/dg_leak and
+// dL/dtau_decay_e (through a_d = exp(-dt / tau_decay_e): dL/dtau = dL/da_d a_d dt / tau^2).
+void window_reverse(const WinConst& wc, const double* const* W, const WinRun& R, const oracle_window_spec* spec,
+                    double* const* dW, double* dg_leak, double* dtau_decay_e) {
+    const uint32_t NE = wc.NE, NI = wc.NI, N = wc.N, T = wc.T, nv = wc.nv;
+    const double dt = wc.dt, kappa = wc.kappa, Vr = wc.Vr, VT = wc.VT, DT = wc.DT, Ee = wc.Ee, Ei = wc.Ei;
+    const double theta = wc.theta, K = wc.K, aE_d = wc.aE_d, aE_r = wc.aE_r, aI_d = wc.aI_d, aI_r = wc.aI_r;
+    const double a_rate = wc.a_rate, beta = wc.beta, g_leak = wc.g_leak, tau = wc.tau_decay_e;
+    const uint32_t pre_of[4] = {NE, NE, NI, NI}, post_of[4] = {NE, NI, NE, NI};
+    // Reverse pass.
+    for (int c = 0; c < 4; ++c) std::fill(dW[c], dW[c] + (size_t)pre_of[c] * post_of[c], 0.0);
+    std::vector<double> lamV(N, 0.0), lge(N, 0.0), lgi(N, 0.0), lge_next(N, 0.0), lgi_next(N, 0.0);
+    std::vector<double> cdE(NE, 0.0), crE(NE, 0.0), cdI(NI, 0.0), crI(NI, 0.0), lsTrace(N, 0.0);
+    std::vector<double> lxE(NE), lxI(NI);
+    std::vector<double> vdirect(N, 0.0);
+    std::vector<int> vslot(N, -1);
+    for (uint32_t k = 0; k < nv; ++k) vslot[spec->vrec_ids[k]] = (int)k;
+    double lrho[2] = {0.0, 0.0};
+    double dgl = 0.0, dalpha = 0.0;
+    for (int64_t t = (int64_t)T - 1; t >= 0; --t) {
+        // lambda x_t = W lambda g_{t+1}
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)N; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            const bool e = i < NE;
+            const uint32_t il = e ? i : i - NE;
+            const double* W1 = W[e ? 0 : 2];  // to E posts
+            const double* W2 = W[e ? 1 : 3];  // to I posts
+            const std::vector<double>& l = e ? lge_next : lgi_next;
+            double acc = 0.0;
+            for (uint32_t j = 0; j < NE; ++j) acc += W1[(size_t)il * NE + j] * l[j];
+            for (uint32_t j = 0; j < NI; ++j) acc += W2[(size_t)il * NI + j] * l[NE + j];
+            if (e) lxE[il] = acc;
+            else lxI[il] = acc;
+        }
+        // traces
+        for (uint32_t i = 0; i < NE; ++i) {
+            const double d0 = R.dE_prev[(size_t)t * NE + i], r0 = R.rE_prev[(size_t)t * NE + i];
+            const double ld = lxE[i] + cdE[i], lr = -lxE[i] + crE[i];
+            const double nd = d0 * aE_d < 1e-30 ? 0.0 : 1.0, nr = r0 * aE_r < 1e-30 ? 0.0 : 1.0;
+            dalpha += ld * d0 * nd;
+            cdE[i] = aE_d * nd * ld;
+            crE[i] = aE_r * nr * lr;
+            lsTrace[i] = K * (ld + lr);
+        }
+        for (uint32_t i = 0; i < NI; ++i) {
+            const double d0 = R.dI_prev[(size_t)t * NI + i], r0 = R.rI_prev[(size_t)t * NI + i];
+            const double ld = lxI[i] + cdI[i], lr = -lxI[i] + crI[i];
+            const double nd = d0 * aI_d < 1e-30 ? 0.0 : 1.0, nr = r0 * aI_r < 1e-30 ? 0.0 : 1.0;
+            cdI[i] = aI_d * nd * ld;
+            crI[i] = aI_r * nr * lr;
+            lsTrace[NE + i] = K * (ld + lr);
+        }
+        // rates
+        for (int q = 0; q < 2; ++q) {
+            const double tgt = spec->target_rates ? spec->target_rates[(size_t)t * 2 + q] : 0.0;
+            lrho[q] = 2.0 / T * (R.rates[(size_t)t * 2 + q] - tgt) + a_rate * lrho[q];
+        }
+        // neurons
+        for (uint32_t j = 0; j < N; ++j) {
+            const size_t k = (size_t)t * N + j;
+            const int q = j < NE ? 0 : 1;
+            const double ls = lsTrace[j] + lrho[q] * (1.0 - a_rate) / ((q == 0 ? NE : NI) * dt);
+            double lv = lamV[j];
+            if (vslot[j] >= 0) {
+                const double tgt = spec->target_v ? spec->target_v[(size_t)t * nv + vslot[j]] : 0.0;
+                const double v = R.V_rec[(size_t)t * nv + vslot[j]];
+                lv += 2.0 / ((double)T * nv) * (v - tgt) * 1e6;
+            }
+            if (R.refr_flag[k]) {
+                lamV[j] = 0.0;
+                lge[j] = lgi[j] = 0.0;
+                continue;
+            }
+            const double U = R.U[k], s = R.S[k], Vp = R.Vprev[k];
+            double sg = 0.0;
+            if (beta > 0.0) {
+                const double z = 1.0 + std::fabs(U - theta) / beta;
+                sg = 1.0 / (beta * z * z);
+            }
+            const double lU = lv * (1.0 - s) + ls * sg;
+            const double ex = std::exp((Vp - VT) / DT);
+            lamV[j] = lU * (1.0 + kappa * (-g_leak + g_leak * ex - R.ge[k] - R.gi[k]));
+            lge[j] = -lU * kappa * (Vp - Ee);
+            lgi[j] = -lU * kappa * (Vp - Ei);
+            dgl += lU * kappa * (-(Vp - Vr) + DT * ex);
+        }
+        // dW += x_{t-1} (outer) lambda g_t
+        const double* dEp = &R.dE_prev[(size_t)t * NE];
+        const double* rEp = &R.rE_prev[(size_t)t * NE];
+        const double* dIp = &R.dI_prev[(size_t)t * NI];
+        const double* rIp = &R.rI_prev[(size_t)t * NI];
+#pragma omp parallel for schedule(static)
+        for (int64_t ii = 0; ii < (int64_t)N; ++ii) {
+            const uint32_t i = (uint32_t)ii;
+            const bool e = i < NE;
+            const uint32_t il = e ? i : i - NE;
+            const double x = e ? dEp[il] - rEp[il] : dIp[il] - rIp[il];
+            double* D1 = dW[e ? 0 : 2] + (size_t)il * NE;
+            double* D2 = dW[e ? 1 : 3] + (size_t)il * NI;
+            const std::vector<double>& l = e ? lge : lgi;
+            for (uint32_t j = 0; j < NE; ++j) D1[j] += x * l[j];
+            for (uint32_t j = 0; j < NI; ++j) D2[j] += x * l[NE + j];
+        }
+        lge_next.swap(lge);
+        lgi_next.swap(lgi);
+    }
+    if (dg_leak) *dg_leak = dgl;
+    if (dtau_decay_e) *dtau_decay_e = dalpha * aE_d * dt / (tau * tau);
+}
+
+// FP64 forward from s0 with the loss; with dW, the reverse pass; with capture, the records.
+int window_impl(const oracle_params* p, const double* const* W, double g_leak, double tau_decay_e,
+                const oracle_window_state* s0, const oracle_window_spec* spec, double* loss, double* const* dW,
+                double* dg_leak, double* dtau_decay_e, double* rates_out, double* v_out, uint8_t* spikes_out,
+                WinRun* capture) {
     const uint32_t NE = p->num_excitatory_neurons, NI = p->num_inhibitory_neurons, N = NE + NI, T = spec->steps;
     const double dt = p->timestep;
     const double kappa = dt / (double)p->membrane_capacitance;
@@ -78,9 +212,7 @@ extern "C" int oracle_window(const oracle_params* p, const double* const* W, dou
     const double aE_d = std::exp(-dt / tau_decay_e), aE_r = std::exp(-dt / (double)p->tau_rise_e);
     const double aI_d = std::exp(-dt / (double)p->tau_decay_i), aI_r = std::exp(-dt / (double)p->tau_rise_i);
     const double a_rate = std::exp(-dt / spec->rate_tau);
-    const double beta = spec->surrogate_beta;
     const uint32_t nv = spec->n_vrec;
-    const uint32_t pre_of[4] = {NE, NE, NI, NI}, post_of[4] = {NE, NI, NE, NI};
 
     // Poisson tables and background parameters per group (same tables as the engine).
     std::vector<uint32_t> tab[2][2];
@@ -105,7 +237,7 @@ extern "C" int oracle_window(const oracle_params* p, const double* const* W, dou
 
     WinRun R;
     R.NE = NE, R.NI = NI, R.N = N, R.T = T;
-    const bool grad = dW != nullptr;
+    const bool grad = dW != nullptr || capture != nullptr;
     if (grad) {
         for (auto* v : {&R.Vprev, &R.ge, &R.gi, &R.U, &R.S}) v->assign((size_t)T * N, 0.0);
         R.refr_flag.assign((size_t)T * N, 0);
@@ -214,107 +346,84 @@ extern "C" int oracle_window(const oracle_params* p, const double* const* W, dou
     if (loss) *loss = L;
     if (rates_out) std::memcpy(rates_out, R.rates.data(), R.rates.size() * sizeof(double));
     if (v_out) std::memcpy(v_out, R.V_rec.data(), R.V_rec.size() * sizeof(double));
-    if (!grad) return 0;
+    if (capture) *capture = R;
+    if (!dW) return 0;
 
-    // Reverse pass.
-    for (int c = 0; c < 4; ++c) std::fill(dW[c], dW[c] + (size_t)pre_of[c] * post_of[c], 0.0);
-    std::vector<double> lamV(N, 0.0), lge(N, 0.0), lgi(N, 0.0), lge_next(N, 0.0), lgi_next(N, 0.0);
-    std::vector<double> cdE(NE, 0.0), crE(NE, 0.0), cdI(NI, 0.0), crI(NI, 0.0), lsTrace(N, 0.0);
-    std::vector<double> lxE(NE), lxI(NI);
-    std::vector<double> vdirect(N, 0.0);
-    std::vector<int> vslot(N, -1);
-    for (uint32_t k = 0; k < nv; ++k) vslot[spec->vrec_ids[k]] = (int)k;
-    double lrho[2] = {0.0, 0.0};
-    double dgl = 0.0, dalpha = 0.0;
-    for (int64_t t = (int64_t)T - 1; t >= 0; --t) {
-        // lambda x_t = W lambda g_{t+1}
-#pragma omp parallel for schedule(static)
-        for (int64_t ii = 0; ii < (int64_t)N; ++ii) {
-            const uint32_t i = (uint32_t)ii;
-            const bool e = i < NE;
-            const uint32_t il = e ? i : i - NE;
-            const double* W1 = W[e ? 0 : 2];  // to E posts
-            const double* W2 = W[e ? 1 : 3];  // to I posts
-            const std::vector<double>& l = e ? lge_next : lgi_next;
-            double acc = 0.0;
-            for (uint32_t j = 0; j < NE; ++j) acc += W1[(size_t)il * NE + j] * l[j];
-            for (uint32_t j = 0; j < NI; ++j) acc += W2[(size_t)il * NI + j] * l[NE + j];
-            if (e) lxE[il] = acc;
-            else lxI[il] = acc;
+    window_reverse(window_constants(p, g_leak, tau_decay_e, spec), W, R, spec, dW, dg_leak, dtau_decay_e);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int oracle_window(const oracle_params* p, const double* const* W, double g_leak, double tau_decay_e,
+                             const oracle_window_state* s0, const oracle_window_spec* spec, double* loss,
+                             double* const* dW, double* dg_leak, double* dtau_decay_e, double* rates_out,
+                             double* v_out, uint8_t* spikes_out) {
+    return window_impl(p, W, g_leak, tau_decay_e, s0, spec, loss, dW, dg_leak, dtau_decay_e, rates_out, v_out,
+                       spikes_out, nullptr);
+}
+
+extern "C" int oracle_window_records(const oracle_params* p, const double* const* W, double g_leak,
+                                     double tau_decay_e, const oracle_window_state* s0,
+                                     const oracle_window_spec* spec, float* Vprev, float* ge, float* gi,
+                                     uint8_t* refr, uint8_t* spikes, float* d_e, float* r_e, float* d_i, float* r_i,
+                                     float* v_rec) {
+    WinRun R;
+    const int rc = window_impl(p, W, g_leak, tau_decay_e, s0, spec, nullptr, nullptr, nullptr, nullptr, nullptr,
+                               nullptr, nullptr, &R);
+    if (rc) return rc;
+    auto put = [](float* dst, const std::vector<double>& src) {
+        for (size_t k = 0; k < src.size(); ++k) dst[k] = (float)src[k];
+    };
+    put(Vprev, R.Vprev), put(ge, R.ge), put(gi, R.gi);
+    for (size_t k = 0; k < R.S.size(); ++k) spikes[k] = R.S[k] != 0.0, refr[k] = R.refr_flag[k];
+    put(d_e, R.dE_prev), put(r_e, R.rE_prev), put(d_i, R.dI_prev), put(r_i, R.rI_prev), put(v_rec, R.V_rec);
+    return 0;
+}
+
+extern "C" int oracle_window_along(const oracle_params* p, const double* const* W, double g_leak, double tau_decay_e,
+                                   const oracle_window_trajectory* tr, const oracle_window_spec* spec, double* loss,
+                                   double* const* dW, double* dg_leak, double* dtau_decay_e) {
+    const WinConst wc = window_constants(p, g_leak, tau_decay_e, spec);
+    const uint32_t NE = wc.NE, NI = wc.NI, N = wc.N, T = wc.T, nv = wc.nv;
+    WinRun R;
+    R.NE = NE, R.NI = NI, R.N = N, R.T = T;
+    const size_t TN = (size_t)T * N;
+    R.Vprev.resize(TN), R.ge.resize(TN), R.gi.resize(TN), R.U.assign(TN, 0.0), R.S.resize(TN);
+    R.refr_flag.resize(TN);
+    for (size_t k = 0; k < TN; ++k) {
+        R.Vprev[k] = tr->Vprev[k], R.ge[k] = tr->ge[k], R.gi[k] = tr->gi[k];
+        R.S[k] = tr->spikes[k], R.refr_flag[k] = tr->refr[k];
+        if (!R.refr_flag[k]) {
+            const double Vp = R.Vprev[k];
+            const double F = -g_leak * (Vp - wc.Vr) + g_leak * wc.DT * std::exp((Vp - wc.VT) / wc.DT) -
+                             R.ge[k] * (Vp - wc.Ee) - R.gi[k] * (Vp - wc.Ei);
+            R.U[k] = Vp + wc.kappa * F;
         }
-        // traces
-        for (uint32_t i = 0; i < NE; ++i) {
-            const double d0 = R.dE_prev[(size_t)t * NE + i], r0 = R.rE_prev[(size_t)t * NE + i];
-            const double ld = lxE[i] + cdE[i], lr = -lxE[i] + crE[i];
-            const double nd = d0 * aE_d < 1e-30 ? 0.0 : 1.0, nr = r0 * aE_r < 1e-30 ? 0.0 : 1.0;
-            dalpha += ld * d0 * nd;
-            cdE[i] = aE_d * nd * ld;
-            crE[i] = aE_r * nr * lr;
-            lsTrace[i] = K * (ld + lr);
-        }
-        for (uint32_t i = 0; i < NI; ++i) {
-            const double d0 = R.dI_prev[(size_t)t * NI + i], r0 = R.rI_prev[(size_t)t * NI + i];
-            const double ld = lxI[i] + cdI[i], lr = -lxI[i] + crI[i];
-            const double nd = d0 * aI_d < 1e-30 ? 0.0 : 1.0, nr = r0 * aI_r < 1e-30 ? 0.0 : 1.0;
-            cdI[i] = aI_d * nd * ld;
-            crI[i] = aI_r * nr * lr;
-            lsTrace[NE + i] = K * (ld + lr);
-        }
-        // rates
-        for (int q = 0; q < 2; ++q) {
-            const double tgt = spec->target_rates ? spec->target_rates[(size_t)t * 2 + q] : 0.0;
-            lrho[q] = 2.0 / T * (R.rates[(size_t)t * 2 + q] - tgt) + a_rate * lrho[q];
-        }
-        // neurons
-        for (uint32_t j = 0; j < N; ++j) {
-            const size_t k = (size_t)t * N + j;
-            const int q = j < NE ? 0 : 1;
-            const double ls = lsTrace[j] + lrho[q] * (1.0 - a_rate) / ((q == 0 ? NE : NI) * dt);
-            double lv = lamV[j];
-            if (vslot[j] >= 0) {
-                const double tgt = spec->target_v ? spec->target_v[(size_t)t * nv + vslot[j]] : 0.0;
-                const double v = R.V_rec[(size_t)t * nv + vslot[j]];
-                lv += 2.0 / ((double)T * nv) * (v - tgt) * 1e6;
-            }
-            if (R.refr_flag[k]) {
-                lamV[j] = 0.0;
-                lge[j] = lgi[j] = 0.0;
-                continue;
-            }
-            const double U = R.U[k], s = R.S[k], Vp = R.Vprev[k];
-            double sg = 0.0;
-            if (beta > 0.0) {
-                const double z = 1.0 + std::fabs(U - theta) / beta;
-                sg = 1.0 / (beta * z * z);
-            }
-            const double lU = lv * (1.0 - s) + ls * sg;
-            const double ex = std::exp((Vp - VT) / DT);
-            lamV[j] = lU * (1.0 + kappa * (-g_leak + g_leak * ex - R.ge[k] - R.gi[k]));
-            lge[j] = -lU * kappa * (Vp - Ee);
-            lgi[j] = -lU * kappa * (Vp - Ei);
-            dgl += lU * kappa * (-(Vp - Vr) + DT * ex);
-        }
-        // dW += x_{t-1} (outer) lambda g_t
-        const double* dEp = &R.dE_prev[(size_t)t * NE];
-        const double* rEp = &R.rE_prev[(size_t)t * NE];
-        const double* dIp = &R.dI_prev[(size_t)t * NI];
-        const double* rIp = &R.rI_prev[(size_t)t * NI];
-#pragma omp parallel for schedule(static)
-        for (int64_t ii = 0; ii < (int64_t)N; ++ii) {
-            const uint32_t i = (uint32_t)ii;
-            const bool e = i < NE;
-            const uint32_t il = e ? i : i - NE;
-            const double x = e ? dEp[il] - rEp[il] : dIp[il] - rIp[il];
-            double* D1 = dW[e ? 0 : 2] + (size_t)il * NE;
-            double* D2 = dW[e ? 1 : 3] + (size_t)il * NI;
-            const std::vector<double>& l = e ? lge : lgi;
-            for (uint32_t j = 0; j < NE; ++j) D1[j] += x * l[j];
-            for (uint32_t j = 0; j < NI; ++j) D2[j] += x * l[NE + j];
-        }
-        lge_next.swap(lge);
-        lgi_next.swap(lgi);
     }
-    if (dg_leak) *dg_leak = dgl;
-    if (dtau_decay_e) *dtau_decay_e = dalpha * aE_d * dt / (tau_decay_e * tau_decay_e);
+    R.dE_prev.assign(tr->d_e, tr->d_e + (size_t)T * NE), R.rE_prev.assign(tr->r_e, tr->r_e + (size_t)T * NE);
+    R.dI_prev.assign(tr->d_i, tr->d_i + (size_t)T * NI), R.rI_prev.assign(tr->r_i, tr->r_i + (size_t)T * NI);
+    R.V_rec.assign(tr->v_rec, tr->v_rec + (size_t)T * nv);
+    R.rates.assign((size_t)T * 2, 0.0);
+    double rho[2] = {0.0, 0.0}, L = 0.0;
+    for (uint32_t t = 0; t < T; ++t) {
+        uint32_t count[2] = {0, 0};
+        for (uint32_t j = 0; j < N; ++j) count[j < NE ? 0 : 1] += tr->spikes[(size_t)t * N + j] != 0;
+        double lt = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            rho[q] = wc.a_rate * rho[q] + (1.0 - wc.a_rate) * ((double)count[q] / (q == 0 ? NE : NI)) / wc.dt;
+            R.rates[(size_t)t * 2 + q] = rho[q];
+            const double tgt = spec->target_rates ? spec->target_rates[(size_t)t * 2 + q] : 0.0;
+            lt += (rho[q] - tgt) * (rho[q] - tgt);
+        }
+        for (uint32_t k = 0; k < nv; ++k) {
+            const double v = R.V_rec[(size_t)t * nv + k];
+            const double tgt = spec->target_v ? spec->target_v[(size_t)t * nv + k] : 0.0;
+            lt += ((v - tgt) * 1e3) * ((v - tgt) * 1e3) / nv;
+        }
+        L += lt / T;
+    }
+    if (loss) *loss = L;
+    if (dW) window_reverse(wc, W, R, spec, dW, dg_leak, dtau_decay_e);
     return 0;
 }

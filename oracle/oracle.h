@@ -141,6 +141,34 @@ int oracle_window(const oracle_params* p, const double* const* W, double g_leak,
                   const oracle_window_state* s0, const oracle_window_spec* spec, double* loss, double* const* dW,
                   double* dg_leak, double* dtau_decay_e, double* rates_out, double* v_out, uint8_t* spikes_out);
 
+/* A recorded forward trajectory of the window (for instance the FP32 step's): per step t the
+ * state before the step and what the step produced. */
+typedef struct oracle_window_trajectory {
+    const float* Vprev;            /* [steps][N] V before the step */
+    const float* ge;               /* [steps][N] conductances the step uses */
+    const float* gi;
+    const uint8_t* refr;           /* [steps][N] 1 = refractory at the step */
+    const uint8_t* spikes;         /* [steps][N] */
+    const float* d_e;              /* [steps][N_E] connection traces before the step */
+    const float* r_e;
+    const float* d_i;              /* [steps][N_I] */
+    const float* r_i;
+    const float* v_rec;            /* [steps][n_vrec] V after the step */
+} oracle_window_trajectory;
+
+/* The records of oracle_window's FP64 forward, rounded to FP32 (trajectory layout above;
+ * outputs sized for spec->steps). */
+int oracle_window_records(const oracle_params* p, const double* const* W, double g_leak, double tau_decay_e,
+                          const oracle_window_state* s0, const oracle_window_spec* spec, float* Vprev, float* ge,
+                          float* gi, uint8_t* refr, uint8_t* spikes, float* d_e, float* r_e, float* d_i, float* r_i,
+                          float* v_rec);
+
+/* The loss and FP64 reverse pass of oracle_window along a given trajectory instead of the
+ * oracle's own FP64 forward (spec->forced_spikes unused). */
+int oracle_window_along(const oracle_params* p, const double* const* W, double g_leak, double tau_decay_e,
+                        const oracle_window_trajectory* traj, const oracle_window_spec* spec, double* loss,
+                        double* const* dW, double* dg_leak, double* dtau_decay_e);
+
 /* Primitives (for the pinning tests). */
 void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 float oracle_expf(float x);

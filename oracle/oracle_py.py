@@ -55,6 +55,11 @@ class WindowState(C.Structure):
                                           "r_i", "d_i")] + [("clock_step", C.c_uint32)]
 
 
+class WindowTrajectory(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("Vprev", "ge", "gi", "refr", "spikes", "d_e", "r_e", "d_i", "r_i",
+                                          "v_rec")]
+
+
 class WindowSpec(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("n_vrec", C.c_uint32), ("vrec_ids", C.c_void_p), ("rate_tau", C.c_double),
                 ("surrogate_beta", C.c_double), ("target_rates", C.c_void_p), ("target_v", C.c_void_p),
@@ -92,6 +97,16 @@ def lib() -> C.CDLL:
                                     C.POINTER(WindowState), C.POINTER(WindowSpec), dp, C.POINTER(dp), dp, dp, dp, dp,
                                     C.POINTER(C.c_uint8)]
         L.oracle_window.restype = I
+        L.oracle_window_along.argtypes = [C.POINTER(Params), C.POINTER(dp), C.c_double, C.c_double,
+                                          C.POINTER(WindowTrajectory), C.POINTER(WindowSpec), dp, C.POINTER(dp), dp,
+                                          dp]
+        L.oracle_window_along.restype = I
+        fpp = C.POINTER(C.c_float)
+        u8 = C.POINTER(C.c_uint8)
+        L.oracle_window_records.argtypes = [C.POINTER(Params), C.POINTER(dp), C.c_double, C.c_double,
+                                            C.POINTER(WindowState), C.POINTER(WindowSpec), fpp, fpp, fpp, u8, u8,
+                                            fpp, fpp, fpp, fpp, fpp]
+        L.oracle_window_records.restype = I
         _lib = L
     return _lib
 
@@ -290,3 +305,81 @@ def window(params: Params, W, g_leak: float, tau_decay_e: float, state: dict, st
     if grad:
         out.update({"dW": dW, "dg_leak": dgl.value, "dtau_decay_e": dtau.value})
     return out
+
+
+def _window_state(state: dict, keep: list) -> "WindowState":
+    def ptr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    st = WindowState()
+    for name in ("V", "bg_ge", "bg_gi", "bre", "bde", "bri", "bdi", "r_e", "d_e", "r_i", "d_i"):
+        setattr(st, name, ptr(state[name], np.float32))
+    st.refr = ptr(state["refr"], np.uint32)
+    st.clock_step = int(state["clock_step"])
+    return st
+
+
+def window_records(params: Params, W, g_leak: float, tau_decay_e: float, state: dict, steps: int, vrec_ids) -> dict:
+    """The FP64 forward's per-step records rounded to FP32, in window_along's trajectory layout."""
+    NE, NI = params.num_excitatory_neurons, params.num_inhibitory_neurons
+    N = NE + NI
+    keep = []
+    st = _window_state(state, keep)
+    vrec = np.ascontiguousarray(vrec_ids, np.uint32)
+    sp = WindowSpec()
+    sp.steps, sp.n_vrec, sp.vrec_ids = steps, vrec.size, vrec.ctypes.data
+    sp.rate_tau, sp.surrogate_beta = 5e-3, 0.0
+    dp = C.POINTER(C.c_double)
+    Wd = [np.ascontiguousarray(w, np.float64) for w in W]
+    Wp = (dp * 4)(*[w.ctypes.data_as(dp) for w in Wd])
+    out = {k: np.zeros((steps, N), np.float32) for k in ("Vprev", "ge", "gi")}
+    out.update({k: np.zeros((steps, N), np.uint8) for k in ("refr", "spikes")})
+    out.update({k: np.zeros((steps, NE), np.float32) for k in ("d_e", "r_e")})
+    out.update({k: np.zeros((steps, NI), np.float32) for k in ("d_i", "r_i")})
+    out["v_rec"] = np.zeros((steps, vrec.size), np.float32)
+    f = lambda k: out[k].ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    u = lambda k: out[k].ctypes.data_as(C.POINTER(C.c_uint8))  # noqa: E731
+    if lib().oracle_window_records(C.byref(params), Wp, g_leak, tau_decay_e, C.byref(st), C.byref(sp), f("Vprev"),
+                                   f("ge"), f("gi"), u("refr"), u("spikes"), f("d_e"), f("r_e"), f("d_i"), f("r_i"),
+                                   f("v_rec")):
+        raise RuntimeError("oracle_window_records failed")
+    return out
+
+
+def window_along(params: Params, W, g_leak: float, tau_decay_e: float, traj: dict, vrec_ids,
+                 rate_tau: float = 5e-3, beta: float = 1e-3, target_rates=None, target_v=None) -> dict:
+    """Loss and FP64 reverse pass of the window along a recorded trajectory (oracle/adjoint.cpp
+    oracle_window_along).  traj: Vprev, ge, gi [T][N] float32; refr, spikes [T][N] uint8;
+    d_e, r_e [T][N_E], d_i, r_i [T][N_I] float32 (traces before each step); v_rec [T][n_vrec]
+    float32 (V after each step)."""
+    T = traj["Vprev"].shape[0]
+    keep = []
+
+    def ptr(a, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    tr = WindowTrajectory()
+    for name in ("Vprev", "ge", "gi", "d_e", "r_e", "d_i", "r_i", "v_rec"):
+        setattr(tr, name, ptr(traj[name], np.float32))
+    tr.refr = ptr(traj["refr"], np.uint8)
+    tr.spikes = ptr(traj["spikes"], np.uint8)
+    vrec = np.ascontiguousarray(vrec_ids, np.uint32)
+    sp = WindowSpec()
+    sp.steps, sp.n_vrec, sp.vrec_ids = T, vrec.size, vrec.ctypes.data
+    sp.rate_tau, sp.surrogate_beta = rate_tau, beta
+    sp.target_rates = None if target_rates is None else ptr(target_rates, np.float64)
+    sp.target_v = None if target_v is None else ptr(target_v, np.float64)
+    dp = C.POINTER(C.c_double)
+    Wd = [np.ascontiguousarray(w, np.float64) for w in W]
+    Wp = (dp * 4)(*[w.ctypes.data_as(dp) for w in Wd])
+    dW = [np.zeros_like(w) for w in Wd]
+    dWp = (dp * 4)(*[d.ctypes.data_as(dp) for d in dW])
+    loss, dgl, dtau = C.c_double(0), C.c_double(0), C.c_double(0)
+    if lib().oracle_window_along(C.byref(params), Wp, g_leak, tau_decay_e, C.byref(tr), C.byref(sp), C.byref(loss),
+                                 dWp, C.byref(dgl), C.byref(dtau)):
+        raise RuntimeError("oracle_window_along failed")
+    return {"loss": loss.value, "dW": dW, "dg_leak": dgl.value, "dtau_decay_e": dtau.value}

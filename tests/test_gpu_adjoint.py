@@ -1,24 +1,29 @@
 """The differentiable window on the GPU (bn_window_*): forward against the plain engine and the
-oracle, gradients against the oracle's FP64 adjoint on the same spike raster.
+FP32 oracle, gradients against the oracle's FP64 adjoint.
 
-- Forward: the window's first steps are the plain step without plasticity (same kernels, same
-  propagate partials), so its raster and recorded voltages equal a plain run's bit for bit.
-- Gradients: the oracle's FP64 adjoint (itself pinned by FP64 central differences in
-  test_oracle_adjoint.py) run with the GPU's raster forced, in the surrogate (beta = 1 mV) and
-  the spikes-held-fixed (beta = 0) modes; every weight matrix within 1e-4 of the FP64 gradient
-  in norm-relative error, g_leak and tau_decay_e within 1e-4 relative.
+- Forward: the window's steps are the plain step without plasticity (same kernels, same
+  propagate partials), so its raster and recorded voltages equal a plain run's, and the FP32
+  oracle's, bit for bit.
+- Gradients: the oracle's FP64 reverse pass along the same (FP32) trajectory, in the surrogate
+  (beta = 1 mV) and the spikes-held-fixed (beta = 0) modes: every weight matrix within 1e-4
+  of the FP64 gradient in norm-relative error, g_leak and tau_decay_e within 1e-4 relative.
+  That reverse pass is pinned to the FP64-forward adjoint (itself pinned by FP64 central
+  differences) in test_oracle_adjoint.py.  Against the FP64-forward adjoint directly the
+  bound is the problem's own conditioning: near threshold the exponential term amplifies an
+  FP32-rounding-sized difference of the trajectory (a 1e-7 relative change of the start
+  voltages moves the FP64 surrogate gradient by ~5e-3 and the exact one by ~2e-4), so that
+  comparison is held to 5e-2 (surrogate) and 5e-3 (exact).
 - Checkpointing: the gradient does not depend on the checkpoint interval (ragged last segment
   included) beyond FP32 summation order.
 - torch.autograd: WindowLoss returns the library's gradients, scaled by the incoming gradient.
 """
-import importlib
 import json
 import os
 
 import numpy as np
 import pytest
 
-from helpers import REPO, engine, oracle_py, small_config
+from helpers import REPO, engine, oracle_for, oracle_py, small_config
 
 pytestmark = pytest.mark.gpu
 
@@ -46,6 +51,30 @@ def _oracle_state(sim):
             "r_e": rE, "d_e": dE, "r_i": rI, "d_i": dI, "clock_step": sim.get_info().clock_step}
 
 
+def _fp32_trajectory(path, sim, warm, T, ids):
+    """The window's trajectory from the FP32 oracle stepped like the plain engine (same GEMV
+    chunking): per step the state before it, its spikes and the recorded voltages after it."""
+    orc = oracle_for(path, sim)
+    orc.step(warm)
+    keys = ("Vprev", "ge", "gi", "refr", "spikes", "d_e", "r_e", "d_i", "r_i", "v_rec")
+    tr = {k: [] for k in keys}
+    st = orc.state()
+    rE, dE, _ = orc.connection_traces(0)
+    rI, dI, _ = orc.connection_traces(2)
+    for _ in range(T):
+        tr["Vprev"].append(st["V"]), tr["ge"].append(st["ge"]), tr["gi"].append(st["gi"])
+        tr["refr"].append((st["refr"] > 0).astype(np.uint8))
+        tr["d_e"].append(dE), tr["r_e"].append(rE), tr["d_i"].append(dI), tr["r_i"].append(rI)
+        orc.step(1)
+        st = orc.state()
+        rE, dE, _ = orc.connection_traces(0)
+        rI, dI, _ = orc.connection_traces(2)
+        tr["spikes"].append(st["spikes"].astype(np.uint8))
+        tr["v_rec"].append(st["V"][ids])
+    orc.close()
+    return {k: np.stack(v) for k, v in tr.items()}
+
+
 def _norm_rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
 
@@ -71,7 +100,7 @@ def test_window_forward_equals_plain_steps(tmp_path):
     for t in range(T):
         c = [out["raster"][t, : sim.NE].sum() / sim.NE, out["raster"][t, sim.NE:].sum() / sim.NI]
         rho = a * rho + (1 - a) * np.array(c) / dt
-        np.testing.assert_allclose(out["rates"][t], rho, rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(out["rates"][t], rho, rtol=1e-6, atol=1e-9)  # dt as FP32 in the engine
     # A second forward from the same start reproduces the first.
     again = win.forward()
     assert np.array_equal(again["raster"], out["raster"]) and again["loss"] == out["loss"]
@@ -83,7 +112,8 @@ def test_window_forward_equals_plain_steps(tmp_path):
 def test_window_gradients_match_fp64_adjoint(tmp_path, beta):
     path = _small(tmp_path, seed=7)
     sim = engine().Simulation(path)
-    sim.run(250)
+    warm = 250
+    sim.run(warm)
     state = _oracle_state(sim)
     T, ids = 230, np.arange(0, sim.NE, 3, dtype=np.uint32)
     rng = np.random.default_rng(3)
@@ -96,21 +126,32 @@ def test_window_gradients_match_fp64_adjoint(tmp_path, beta):
     assert out["raster"].sum() > 20
     dump = engine().config_dump(path)
     params = oracle_py.params_from_config(dump)
+    gl, td = float(dump["g_leak"]), float(dump["tau_decay_e"])
     W = [win.weights(k).astype(np.float64) for k in range(4)]
-    ref = oracle_py.window(params, W, float(dump["g_leak"]), float(dump["tau_decay_e"]), state, T, ids,
-                           rate_tau=5e-3, beta=beta, target_rates=tr, target_v=tv, forced_spikes=out["raster"])
-    free = oracle_py.window(params, W, float(dump["g_leak"]), float(dump["tau_decay_e"]), state, T, ids,
-                            rate_tau=5e-3, beta=beta, target_rates=tr, target_v=tv, grad=False)
-    # The FP64 forward thresholds to the same raster (no near-threshold flip in this window).
-    assert np.array_equal(free["spikes"], out["raster"])
-    assert out["loss"] == pytest.approx(ref["loss"], rel=1e-5)
-    np.testing.assert_allclose(out["v"], ref["v"], rtol=0, atol=2e-6)
+    # The FP32 oracle's trajectory is the window's, bit for bit.
+    traj = _fp32_trajectory(path, sim, warm, T, ids)
+    assert np.array_equal(traj["spikes"], out["raster"])
+    assert np.array_equal(traj["v_rec"], out["v"])
+    # FP64 reverse pass along it: the 1e-4 bar.
+    ref = oracle_py.window_along(params, W, gl, td, traj, ids, rate_tau=5e-3, beta=beta, target_rates=tr,
+                                 target_v=tv)
+    assert out["loss"] == pytest.approx(ref["loss"], rel=1e-6)  # rate_tau is FP32 in the C ABI
     for k in range(4):
-        assert np.linalg.norm(ref["dW"][k]) > 0
+        if np.linalg.norm(ref["dW"][k]) == 0:
+            assert not np.any(g["dW"][k]), k
+            continue
         err = _norm_rel(g["dW"][k], ref["dW"][k])
         assert err <= 1e-4, (k, err)
     assert g["dg_leak"] == pytest.approx(ref["dg_leak"], rel=1e-4)
     assert g["dtau_decay_e"] == pytest.approx(ref["dtau_decay_e"], rel=1e-4)
+    # The FP64-forward adjoint on the same raster: held to the problem's conditioning.
+    fwd = oracle_py.window(params, W, gl, td, state, T, ids, rate_tau=5e-3, beta=beta, target_rates=tr,
+                           target_v=tv, forced_spikes=out["raster"])
+    bound = 5e-2 if beta > 0 else 5e-3
+    for k in range(4):
+        if np.linalg.norm(fwd["dW"][k]) > 0:
+            assert _norm_rel(g["dW"][k], fwd["dW"][k]) <= bound, k
+    assert g["dg_leak"] == pytest.approx(fwd["dg_leak"], rel=bound)
     win.close()
     sim.close()
 
@@ -170,29 +211,53 @@ def test_window_params_change_loss_and_reset(tmp_path):
     psim.close()
 
 
-def test_torch_autograd_function(tmp_path):
-    import torch
+_TORCH_SCRIPT = r"""
+import importlib, json, sys
+import numpy as np
+import torch  # first: the engine library then binds to PyTorch's HIP runtime
+adj = importlib.import_module("example-cuda-spiking-nn_amd.adjoint")
+eng = importlib.import_module("example-cuda-spiking-nn_amd")
+sim = eng.Simulation(sys.argv[1])
+sim.run(200)
+ids = np.arange(0, sim.NE, 5, dtype=np.uint32)
+win = eng.Window(sim, 120, ids, checkpoint_interval=50)
+rng = np.random.default_rng(5)
+win.set_targets(rng.uniform(0, 40, (120, 2)), rng.uniform(-0.068, -0.052, (120, ids.size)))
+info = win.get_info()
+ws = [torch.tensor(win.weights(k), device="cuda:0", requires_grad=True) for k in range(4)]
+gl = torch.tensor(float(info.g_leak), dtype=torch.float64, requires_grad=True)
+td = torch.tensor(float(info.tau_decay_e), dtype=torch.float64, requires_grad=True)
+loss = adj.window_loss(win, ws, gl, td)
+(2.0 * loss).backward()
+direct_f = win.forward(outputs=False)
+direct = win.backward()
+assert loss.item() == direct_f["loss"]
+for k in range(4):
+    assert np.array_equal(ws[k].grad.cpu().numpy(), 2.0 * direct["dW"][k]), k
+assert abs(gl.grad.item() - 2.0 * direct["dg_leak"]) <= 1e-12 * abs(direct["dg_leak"])
+assert abs(td.grad.item() - 2.0 * direct["dtau_decay_e"]) <= 1e-12 * abs(direct["dtau_decay_e"])
+# Host tensors take the same path through host pointers.
+wc = [torch.tensor(win.weights(k), requires_grad=True) for k in range(4)]
+adj.window_loss(win, wc, gl.detach(), td.detach()).backward()
+for k in range(4):
+    assert np.array_equal(wc[k].grad.numpy(), direct["dW"][k]), k
+win.close()
+sim.close()
+print("torch autograd ok", flush=True)
+"""
 
-    adj = importlib.import_module("example-cuda-spiking-nn_amd.adjoint")
+
+def test_torch_autograd_function(tmp_path):
+    # In a process of its own: PyTorch must load before the engine library (one HIP runtime),
+    # while this test process has loaded the library already.
+    import subprocess
+    import sys
+
     path = _small(tmp_path, seed=10)
-    sim = engine().Simulation(path)
-    sim.run(200)
-    ids = np.arange(0, sim.NE, 5, dtype=np.uint32)
-    win = engine().Window(sim, 120, ids, checkpoint_interval=50)
-    rng = np.random.default_rng(5)
-    win.set_targets(rng.uniform(0, 40, (120, 2)), rng.uniform(-0.068, -0.052, (120, ids.size)))
-    info = win.get_info()
-    ws = [torch.tensor(win.weights(k), device="cuda:0", requires_grad=True) for k in range(4)]
-    gl = torch.tensor(float(info.g_leak), dtype=torch.float64, requires_grad=True)
-    td = torch.tensor(float(info.tau_decay_e), dtype=torch.float64, requires_grad=True)
-    loss = adj.window_loss(win, ws, gl, td)
-    (2.0 * loss).backward()
-    direct_f = win.forward(outputs=False)
-    direct = win.backward()
-    assert loss.item() == direct_f["loss"]
-    for k in range(4):
-        np.testing.assert_array_equal(ws[k].grad.cpu().numpy(), 2.0 * direct["dW"][k])
-    assert gl.grad.item() == pytest.approx(2.0 * direct["dg_leak"], rel=1e-12)
-    assert td.grad.item() == pytest.approx(2.0 * direct["dtau_decay_e"], rel=1e-12)
-    win.close()
-    sim.close()
+    script = os.path.join(str(tmp_path), "torch_window.py")
+    open(script, "w").write(_TORCH_SCRIPT)
+    env = dict(os.environ)
+    env["PYTHONPATH"] = os.pathsep.join([REPO, env.get("PYTHONPATH", "")])
+    r = subprocess.run([sys.executable, script, path], capture_output=True, text=True, timeout=240, env=env)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "torch autograd ok" in r.stdout

@@ -112,3 +112,25 @@ def test_window_surrogate_changes_only_spike_terms(tmp_path):
     b = oracle_py.window(params, Wz, gl, taud, state, 50, vrec, beta=1e-3, target_v=tv[:50])
     assert a["spikes"].sum() == 0
     assert a["dg_leak"] == pytest.approx(b["dg_leak"], rel=1e-12)
+
+
+@pytest.mark.parametrize("beta", [1e-3, 0.0])
+def test_reverse_along_trajectory_matches_fp64_forward_adjoint(tmp_path, beta):
+    # The reverse pass along a recorded trajectory (the form the GPU's FP32 run is checked
+    # with) must reproduce the FP64 adjoint when the trajectory is the FP64 forward's own,
+    # rounded to FP32: only the rounding of the records separates them (no re-simulation).
+    dump, params, W, state, T, vrec, tr, tv = _setup(tmp_path)
+    gl, taud = float(dump["g_leak"]), float(dump["tau_decay_e"])
+    a = oracle_py.window(params, W, gl, taud, state, T, vrec, beta=beta, target_rates=tr, target_v=tv)
+    rec = oracle_py.window_records(params, W, gl, taud, state, T, vrec)
+    assert np.array_equal(rec["spikes"], a["spikes"])
+    b = oracle_py.window_along(params, W, gl, taud, rec, vrec, beta=beta, target_rates=tr, target_v=tv)
+    assert b["loss"] == pytest.approx(a["loss"], rel=1e-8)
+    for k in range(4):
+        n = np.linalg.norm(a["dW"][k])
+        if n == 0:
+            assert not np.any(b["dW"][k])
+            continue
+        assert np.linalg.norm(a["dW"][k] - b["dW"][k]) <= 2e-5 * n, k
+    assert b["dg_leak"] == pytest.approx(a["dg_leak"], rel=2e-5)
+    assert b["dtau_decay_e"] == pytest.approx(a["dtau_decay_e"], rel=2e-5)
