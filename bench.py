@@ -310,15 +310,21 @@ def bench_adjoint(args) -> int:
     eng = importlib.import_module("example-cuda-spiking-nn_amd")
     wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
     T, S, warm = 1000, 100, 1000
-    sim, win, base = wl.adjoint_workload(eng, steps=T, warmup=warm, checkpoint_interval=S)
+    sim, win, base, _ = wl.adjoint_workload(eng, steps=T, warmup=warm, checkpoint_interval=S)
     for _ in range(args.warmup):
         win.forward(outputs=False)
-        win.backward()
+        win.backward(download=False)
+    # Timed: forward + backward with the gradient left in device memory (its download to the
+    # host is reported separately).
     t0 = time.perf_counter()
     for _ in range(args.steps):
         f = win.forward(outputs=False)
-        g = win.backward()
+        g = win.backward(download=False)
     dt = time.perf_counter() - t0
+    dev = win.get_info()
+    t2 = time.perf_counter()
+    g["dW"] = [win.gradient(k) for k in range(4)]
+    dt_dl = time.perf_counter() - t2
     # Forward alone, for the split.
     t1 = time.perf_counter()
     for _ in range(args.steps):
@@ -353,6 +359,8 @@ def bench_adjoint(args) -> int:
                                "surrogate slope 1/(1 mV))",
                    "window_steps": T, "checkpoint_interval": S, "neurons": n, "weight_entries": int(n_w),
                    "forward_s_per_1k": round(dt_fwd / args.steps * 1000.0 / T, 5),
+                   "device_ms_last_forward": round(dev.forward_ms, 3), "device_ms_last_backward": round(dev.backward_ms, 3),
+                   "gradient_download_ms": round(dt_dl * 1000.0, 2),
                    "loss": f["loss"], "grad_norm_w": gnorm, "dg_leak": g["dg_leak"],
                    "dtau_decay_e": g["dtau_decay_e"], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "whole window (forward + recompute propagate, transposed "

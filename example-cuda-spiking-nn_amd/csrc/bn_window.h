@@ -9,6 +9,13 @@
 
 namespace bn {
 
+// The differentiated scalars, read from device memory so that captured graphs stay valid when
+// they change.
+struct WinParams {
+    float g_leak;
+    float decay_e_d;            // exp(-dt / tau_decay_e)
+};
+
 struct WinGroup {
     NeuronGroupView v;          // parameters (g_leak = the differentiated value) and Poisson tables
     FinalizeSource fin_e, fin_i;
@@ -51,42 +58,46 @@ struct WinForwardPlan {
     WinGroup g[2];
     WinState st;
     WinRecords rec;             // rec.Vprev == nullptr: no records (plain forward)
+    const WinParams* prm;
     uint32_t NE, NI, N, slot, step;   // slot in the segment, absolute step (noise counter)
-    float decay_e_r, decay_e_d, decay_i_r, decay_i_d, K;
+    float decay_e_r, decay_i_r, decay_i_d, K;
     bn_device_clock clock;      // Philox key (clock_step unused: `step` is the counter)
-    uint32_t* counts;           // [2] spike counts of E, I this step (window-global array offset)
-    float* v_rec;               // [n_vrec] V of the recorded neurons this step
+    uint32_t* counts;           // [2] spike counts of E, I this step, or nullptr
+    float* v_rec;               // [n_vrec] V of the recorded neurons this step, or nullptr
     const int32_t* vrec_slot;   // [N] index in v_rec or -1
-    uint8_t* raster;            // [N] spikes this step (window-global array offset) or nullptr
+    uint8_t* raster;            // [N] spikes this step, or nullptr
 };
 
-struct WinBackwardPlan {
+// Copy of the whole window state (checkpoints).
+struct WinCopyPlan {
+    WinState dst, src;
+    uint32_t NE, NI, N;
+};
+
+// One reverse step: lambda x = W lambda g of the next step (row products), then the adjoint of
+// every neuron's connection traces (as a pre neuron) and EIF update (as a post neuron).
+struct WinReversePlan {
     WinGroup g[2];
     WinRecords rec;
-    uint32_t NE, NI, N, slot;
-    float decay_e_r, decay_e_d, decay_i_r, decay_i_d, K;
-    float theta, beta;          // beta = 0: spikes held fixed
-    float rate_coef_e, rate_coef_i;   // (1 - a) / (N_p dt)
-    float lrho_e, lrho_i;       // lambda of the smoothed rates at this step
-    const float* dv_direct;     // [n_vrec] dL/dV of the recorded neurons at this step
-    const int32_t* vrec_slot;
-    const double* lx;           // [N] lambda x of this step (row products with lambda g of the next)
-    float* carry_V;             // [N]
-    float* carry_d;             // [N] (pre neuron = global id)
-    float* carry_r;
-    float* lg_e;                // [N] lambda g of this step (output; read by the next row products)
-    float* lg_i;
-    double* acc_gleak;          // [N]
-    double* acc_alpha;          // [NE]
-};
-
-struct WinRowPlan {
+    const WinParams* prm;
     const float* W[4];          // EE, EI, IE, II, row stride ld[c]
     uint32_t ld[4];
-    uint32_t NE, NI;
-    const float* lg_e;          // [N] lambda g_e of the next step
-    const float* lg_i;
-    double* lx;                 // [N]
+    uint32_t NE, NI, N, slot;
+    float decay_e_r, decay_i_r, decay_i_d, K;
+    float theta, beta;          // beta = 0: spikes held fixed
+    double rate_coef_e, rate_coef_i;  // (1 - a) / (N_p dt)
+    const double* lrho;         // [2] lambda of the smoothed rates (E, I) at this step
+    const double* dv_direct;    // [n_vrec] dL/dV of the recorded neurons at this step
+    const int32_t* vrec_slot;
+    const double* lg_in_e;      // [N] lambda g of the next step (zero after the last step)
+    const double* lg_in_i;
+    double* lg_out_e;           // [N] lambda g of this step (a different buffer)
+    double* lg_out_i;
+    double* carry_V;            // [N]
+    double* carry_d;            // [N] (pre neuron = global id)
+    double* carry_r;
+    double* acc_gleak;          // [N]
+    double* acc_alpha;          // [NE]
 };
 
 struct WinOuterPlan {
@@ -100,8 +111,8 @@ struct WinOuterPlan {
 };
 
 int launch_win_forward(const WinForwardPlan& p, bn_stream_t s);
-int launch_win_rows(const WinRowPlan& p, bn_stream_t s);
-int launch_win_backward(const WinBackwardPlan& p, bn_stream_t s);
+int launch_win_copy(const WinCopyPlan& p, bn_stream_t s);
+int launch_win_reverse(const WinReversePlan& p, bn_stream_t s);
 int launch_win_outer(const WinOuterPlan& p, bn_stream_t s);
 
 }  // namespace bn

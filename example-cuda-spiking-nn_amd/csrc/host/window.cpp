@@ -39,13 +39,20 @@ struct bn_window {
     int32_t* vrec_slot = nullptr;
     std::vector<uint32_t> vrec_ids;
     uint32_t* d_counts = nullptr;      // [T][2]
-    uint32_t* d_counts_scratch = nullptr;
     float* d_vrec = nullptr;           // [T][nv]
-    float* d_vrec_scratch = nullptr;   // [nv]
     uint8_t* d_raster = nullptr;       // [T][N]
-    float* d_dv = nullptr;             // [T][nv]
-    double* lx = nullptr;
-    float *carry_V = nullptr, *carry_d = nullptr, *carry_r = nullptr, *lg_e = nullptr, *lg_i = nullptr;
+    double* d_dv = nullptr;            // [T][nv]
+    double* d_lrho = nullptr;          // [T][2]
+    WinParams* d_params = nullptr;
+    // Forward (T steps with checkpoints) and backward (every segment's recompute and reverse
+    // walk) captured once per window; parameters, targets and loss terms live in device
+    // memory, so the graphs stay valid.
+    hipGraphExec_t fwd_exec = nullptr, bwd_exec = nullptr;
+    float* staging = nullptr;           // pinned host buffer for gradient downloads
+    hipEvent_t ev[4] = {};              // around the last forward and backward graph launches
+    float fwd_ms = 0.0f, bwd_ms = 0.0f;
+    double *carry_V = nullptr, *carry_d = nullptr, *carry_r = nullptr;
+    double *lg_e = nullptr, *lg_i = nullptr;  // [2][N]: lambda g of step t in buffer t % 2
     double *acc_gleak = nullptr, *acc_alpha = nullptr;
     float* dW[4] = {};
     std::vector<double> target_rates, target_v, lrho;  // [T][2], [T][nv], [T][2]
@@ -63,6 +70,11 @@ struct bn_window {
     }
     ~bn_window() {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (staging) (void)hipHostFree(staging);
+        if (fwd_exec) (void)hipGraphExecDestroy(fwd_exec);
+        if (bwd_exec) (void)hipGraphExecDestroy(bwd_exec);
         for (void* p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -86,22 +98,16 @@ struct bn_window {
         return s;
     }
     void copy_state(const WinState& dst, const WinState& src) {
-        auto cp = [&](void* d, const void* s, size_t bytes) {
-            BN_HIP_CHECK(hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, stream), "state copy");
-        };
-        for (auto pr : {std::make_pair(dst.V, src.V), std::make_pair(dst.bg_ge, src.bg_ge), std::make_pair(dst.bg_gi, src.bg_gi),
-                        std::make_pair(dst.bre, src.bre), std::make_pair(dst.bde, src.bde), std::make_pair(dst.bri, src.bri),
-                        std::make_pair(dst.bdi, src.bdi)})
-            cp(pr.first, pr.second, N * sizeof(float));
-        cp(dst.refr, src.refr, N * sizeof(uint32_t));
-        for (auto pr : {std::make_pair(dst.dE, src.dE), std::make_pair(dst.rE, src.rE), std::make_pair(dst.xE, src.xE)})
-            cp(pr.first, pr.second, NE * sizeof(float));
-        for (auto pr : {std::make_pair(dst.dI, src.dI), std::make_pair(dst.rI, src.rI), std::make_pair(dst.xI, src.xI)})
-            cp(pr.first, pr.second, NI * sizeof(float));
+        WinCopyPlan p{dst, src, NE, NI, N};
+        BN_HIP_CHECK((hipError_t)launch_win_copy(p, stream), "window state copy launch");
     }
 
     void apply_params() {
-        for (int g = 0; g < 2; ++g) grp[g].v.g_leak = g_leak;
+        WinParams h{};
+        h.g_leak = g_leak;
+        h.decay_e_d = std::exp(-1.0f * cfg.timestep / tau_decay_e);  // connection.cpp:46-47
+        BN_HIP_CHECK(hipMemcpyAsync(d_params, &h, sizeof h, hipMemcpyHostToDevice, stream), "window params");
+        BN_HIP_CHECK(hipStreamSynchronize(stream), "window params sync");
     }
     WinForwardPlan fwd_plan(uint32_t t, bool record) const {
         WinForwardPlan p{};
@@ -109,20 +115,17 @@ struct bn_window {
         p.g[1] = grp[1];
         p.st = st;
         if (record) p.rec = rec;
+        p.prm = d_params;
         p.NE = NE, p.NI = NI, p.N = N;
         p.slot = t % S;
         p.step = start_step + t;
         p.decay_e_r = std::exp(-1.0f * cfg.timestep / cfg.tau_rise_e);  // connection.cpp:46-47
-        p.decay_e_d = std::exp(-1.0f * cfg.timestep / tau_decay_e);
         p.decay_i_r = std::exp(-1.0f * cfg.timestep / cfg.tau_rise_i);
         p.decay_i_d = std::exp(-1.0f * cfg.timestep / cfg.tau_decay_i);
         p.K = cfg.postsynaptic_kernel_normalization_value;
         p.clock = clock;
         p.vrec_slot = vrec_slot;
-        if (record) {
-            p.counts = d_counts_scratch;
-            p.v_rec = d_vrec_scratch;
-        } else {
+        if (!record) {
             p.counts = d_counts + 2 * (size_t)t;
             p.v_rec = d_vrec + (size_t)nv * t;
             p.raster = d_raster + (size_t)N * t;
@@ -134,19 +137,46 @@ struct bn_window {
         BN_HIP_CHECK((hipError_t)launch_win_forward(fwd_plan(t, record), stream), "window forward launch");
     }
 
+    template <typename F>
+    hipGraphExec_t capture(F&& body) {
+        BN_HIP_CHECK(hipStreamSynchronize(stream), "pre-capture sync");
+        BN_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t graph = nullptr;
+        BN_HIP_CHECK(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        BN_HIP_CHECK(e, "hipGraphInstantiate");
+        return exec;
+    }
+
     void forward() {
         apply_params();
-        copy_state(st, snap);
         BN_HIP_CHECK(hipMemsetAsync(d_counts, 0, 2 * (size_t)T * sizeof(uint32_t), stream), "clear counts");
-        for (uint32_t t = 0; t < T; ++t) {
-            if (t % S == 0) copy_state(ckpt[t / S], st);
-            step_forward(t, false);
-        }
+        if (!fwd_exec)
+            fwd_exec = capture([&] {
+                copy_state(st, snap);
+                for (uint32_t t = 0; t < T; ++t) {
+                    if (t % S == 0) copy_state(ckpt[t / S], st);
+                    step_forward(t, false);
+                }
+            });
+        BN_HIP_CHECK(hipEventRecord(ev[0], stream), "hipEventRecord");
+        BN_HIP_CHECK(hipGraphLaunch(fwd_exec, stream), "window forward graph");
+        BN_HIP_CHECK(hipEventRecord(ev[1], stream), "hipEventRecord");
         std::vector<uint32_t> counts(2 * (size_t)T);
         std::vector<float> vrec(std::max<size_t>(1, (size_t)T * nv));
         BN_HIP_CHECK(hipMemcpyAsync(counts.data(), d_counts, counts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream), "counts");
         if (nv) BN_HIP_CHECK(hipMemcpyAsync(vrec.data(), d_vrec, (size_t)T * nv * sizeof(float), hipMemcpyDeviceToHost, stream), "v");
         BN_HIP_CHECK(hipStreamSynchronize(stream), "forward sync");
+        BN_HIP_CHECK(hipEventElapsedTime(&fwd_ms, ev[0], ev[1]), "hipEventElapsedTime");
         // Loss (FP64): smoothed population rates and recorded voltages against the targets.
         const double dt = cfg.timestep, a = std::exp(-dt / (double)rate_tau);
         rates.assign(2 * (size_t)T, 0.0);
@@ -173,55 +203,43 @@ struct bn_window {
                 carry[q] = 2.0 / T * (rates[2 * t + q] - target_rates[2 * t + q]) + a * carry[q];
                 lrho[2 * t + q] = carry[q];
             }
-        std::vector<float> dvf(dv.begin(), dv.end());
-        if (nv) {
-            BN_HIP_CHECK(hipMemcpyAsync(d_dv, dvf.data(), dvf.size() * sizeof(float), hipMemcpyHostToDevice, stream), "dv");
-            BN_HIP_CHECK(hipStreamSynchronize(stream), "dv sync");
-        }
+        BN_HIP_CHECK(hipMemcpyAsync(d_lrho, lrho.data(), lrho.size() * sizeof(double), hipMemcpyHostToDevice, stream), "lrho");
+        if (nv) BN_HIP_CHECK(hipMemcpyAsync(d_dv, dv.data(), dv.size() * sizeof(double), hipMemcpyHostToDevice, stream), "dv");
+        BN_HIP_CHECK(hipStreamSynchronize(stream), "loss terms sync");
         have_forward = true;
     }
 
-    void backward(double* dgl, double* dtau) {
-        if (!have_forward) throw Error(BN_ERR_RUNTIME, "bn_window_backward needs a forward first");
-        apply_params();
-        for (int c = 0; c < 4; ++c)
-            BN_HIP_CHECK(hipMemsetAsync(dW[c], 0, (size_t)pre[c] * ld[c] * sizeof(float), stream), "clear dW");
-        for (float* p : {carry_V, carry_d, carry_r, lg_e, lg_i})
-            BN_HIP_CHECK(hipMemsetAsync(p, 0, N * sizeof(float), stream), "clear adjoint");
-        BN_HIP_CHECK(hipMemsetAsync(acc_gleak, 0, N * sizeof(double), stream), "clear acc");
-        BN_HIP_CHECK(hipMemsetAsync(acc_alpha, 0, NE * sizeof(double), stream), "clear acc");
+    void enqueue_backward() {
         const double dt = cfg.timestep, a = std::exp(-dt / (double)rate_tau);
-        WinRowPlan rp{};
-        for (int c = 0; c < 4; ++c) rp.W[c] = W[c], rp.ld[c] = ld[c];
-        rp.NE = NE, rp.NI = NI, rp.lg_e = lg_e, rp.lg_i = lg_i, rp.lx = lx;
         for (int64_t seg = (int64_t)n_seg - 1; seg >= 0; --seg) {
             const uint32_t t0 = (uint32_t)seg * S, t1 = std::min(T, t0 + S), slots = t1 - t0;
             copy_state(st, ckpt[seg]);
             for (uint32_t t = t0; t < t1; ++t) step_forward(t, true);
             for (int64_t sl = (int64_t)slots - 1; sl >= 0; --sl) {
                 const uint32_t t = t0 + (uint32_t)sl;
-                BN_HIP_CHECK((hipError_t)launch_win_rows(rp, stream), "window rows launch");
-                WinBackwardPlan bp{};
-                bp.g[0] = grp[0];
-                bp.g[1] = grp[1];
-                bp.rec = rec;
-                bp.NE = NE, bp.NI = NI, bp.N = N, bp.slot = (uint32_t)sl;
+                WinReversePlan rp{};
+                rp.g[0] = grp[0];
+                rp.g[1] = grp[1];
+                rp.rec = rec;
+                rp.prm = d_params;
+                for (int c = 0; c < 4; ++c) rp.W[c] = W[c], rp.ld[c] = ld[c];
+                rp.NE = NE, rp.NI = NI, rp.N = N, rp.slot = (uint32_t)sl;
                 const WinForwardPlan fp = fwd_plan(t, true);
-                bp.decay_e_r = fp.decay_e_r, bp.decay_e_d = fp.decay_e_d, bp.decay_i_r = fp.decay_i_r;
-                bp.decay_i_d = fp.decay_i_d, bp.K = fp.K;
-                bp.theta = cfg.v_threshold;
-                bp.beta = beta;
-                bp.rate_coef_e = (float)((1.0 - a) / (NE * dt));
-                bp.rate_coef_i = (float)((1.0 - a) / (NI * dt));
-                bp.lrho_e = (float)lrho[2 * (size_t)t];
-                bp.lrho_i = (float)lrho[2 * (size_t)t + 1];
-                bp.dv_direct = d_dv + (size_t)nv * t;
-                bp.vrec_slot = vrec_slot;
-                bp.lx = lx;
-                bp.carry_V = carry_V, bp.carry_d = carry_d, bp.carry_r = carry_r;
-                bp.lg_e = lg_e, bp.lg_i = lg_i;
-                bp.acc_gleak = acc_gleak, bp.acc_alpha = acc_alpha;
-                BN_HIP_CHECK((hipError_t)launch_win_backward(bp, stream), "window backward launch");
+                rp.decay_e_r = fp.decay_e_r, rp.decay_i_r = fp.decay_i_r, rp.decay_i_d = fp.decay_i_d;
+                rp.K = fp.K;
+                rp.theta = cfg.v_threshold;
+                rp.beta = beta;
+                rp.rate_coef_e = (1.0 - a) / (NE * dt);
+                rp.rate_coef_i = (1.0 - a) / (NI * dt);
+                rp.lrho = d_lrho + 2 * (size_t)t;
+                rp.dv_direct = d_dv + (size_t)nv * t;
+                rp.vrec_slot = vrec_slot;
+                const size_t in = (size_t)((t + 1) & 1) * N, out = (size_t)(t & 1) * N;
+                rp.lg_in_e = lg_e + in, rp.lg_in_i = lg_i + in;
+                rp.lg_out_e = lg_e + out, rp.lg_out_i = lg_i + out;
+                rp.carry_V = carry_V, rp.carry_d = carry_d, rp.carry_r = carry_r;
+                rp.acc_gleak = acc_gleak, rp.acc_alpha = acc_alpha;
+                BN_HIP_CHECK((hipError_t)launch_win_reverse(rp, stream), "window reverse launch");
             }
             // dW_c += sum over the segment of x_{t-1} (outer) lambda g_t.
             for (int c = 0; c < 4; ++c) {
@@ -242,10 +260,27 @@ struct bn_window {
                 BN_HIP_CHECK((hipError_t)launch_win_outer(op, stream), "window outer launch");
             }
         }
+    }
+
+    void backward(double* dgl, double* dtau) {
+        if (!have_forward) throw Error(BN_ERR_RUNTIME, "bn_window_backward needs a forward first");
+        for (int c = 0; c < 4; ++c)
+            BN_HIP_CHECK(hipMemsetAsync(dW[c], 0, (size_t)pre[c] * ld[c] * sizeof(float), stream), "clear dW");
+        for (double* p : {carry_V, carry_d, carry_r})
+            BN_HIP_CHECK(hipMemsetAsync(p, 0, N * sizeof(double), stream), "clear adjoint");
+        for (double* p : {lg_e, lg_i}) BN_HIP_CHECK(hipMemsetAsync(p, 0, 2 * N * sizeof(double), stream), "clear adjoint");
+        BN_HIP_CHECK(hipMemsetAsync(acc_gleak, 0, N * sizeof(double), stream), "clear acc");
+        BN_HIP_CHECK(hipMemsetAsync(acc_alpha, 0, NE * sizeof(double), stream), "clear acc");
+        if (!bwd_exec) bwd_exec = capture([&] { enqueue_backward(); });
+        BN_HIP_CHECK(hipEventRecord(ev[2], stream), "hipEventRecord");
+        BN_HIP_CHECK(hipGraphLaunch(bwd_exec, stream), "window backward graph");
+        BN_HIP_CHECK(hipEventRecord(ev[3], stream), "hipEventRecord");
+        const double dt = cfg.timestep;
         std::vector<double> ag(N), aa(NE);
         BN_HIP_CHECK(hipMemcpyAsync(ag.data(), acc_gleak, N * sizeof(double), hipMemcpyDeviceToHost, stream), "acc");
         BN_HIP_CHECK(hipMemcpyAsync(aa.data(), acc_alpha, NE * sizeof(double), hipMemcpyDeviceToHost, stream), "acc");
         BN_HIP_CHECK(hipStreamSynchronize(stream), "backward sync");
+        BN_HIP_CHECK(hipEventElapsedTime(&bwd_ms, ev[2], ev[3]), "hipEventElapsedTime");
         double sg = 0.0, sa = 0.0;
         for (double v : ag) sg += v;
         for (double v : aa) sa += v;
@@ -270,6 +305,7 @@ void build_window(bn_window* w, bn_sim* sim, const bn_window_spec* spec) {
     w->g_leak = c.g_leak, w->tau_decay_e = c.tau_decay_e;
     const uint32_t N = w->N, NE = w->NE, NI = w->NI, T = w->T, S = w->S;
     BN_HIP_CHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (hipEvent_t& e : w->ev) BN_HIP_CHECK(hipEventCreate(&e), "hipEventCreate");
     // Weights: the simulation's current ones, rows padded to 64 floats.
     const uint32_t pres[4] = {NE, NE, NI, NI}, posts[4] = {NE, NI, NE, NI};
     for (int k = 0; k < 4; ++k) {
@@ -408,17 +444,16 @@ void build_window(bn_window* w, bn_sim* sim, const bn_window_spec* spec) {
     w->vrec_slot = w->alloc<int32_t>(N, "vrec slots");
     BN_HIP_CHECK(hipMemcpyAsync(w->vrec_slot, slot.data(), N * sizeof(int32_t), hipMemcpyHostToDevice, w->stream), "slots");
     w->d_counts = w->alloc<uint32_t>(2 * (size_t)T, "counts");
-    w->d_counts_scratch = w->alloc<uint32_t>(2, "counts");
     w->d_vrec = w->alloc<float>((size_t)T * std::max(1u, w->nv), "v");
-    w->d_vrec_scratch = w->alloc<float>(std::max(1u, w->nv), "v");
+    w->d_lrho = w->alloc<double>(2 * (size_t)T, "rate adjoint");
+    w->d_params = w->alloc<WinParams>(1, "window params");
     w->d_raster = w->alloc<uint8_t>((size_t)T * N, "raster");
-    w->d_dv = w->alloc<float>((size_t)T * std::max(1u, w->nv), "dv");
-    w->lx = w->alloc<double>(N, "lambda x");
-    w->carry_V = w->alloc<float>(N, "adjoint");
-    w->carry_d = w->alloc<float>(N, "adjoint");
-    w->carry_r = w->alloc<float>(N, "adjoint");
-    w->lg_e = w->alloc<float>(N, "adjoint");
-    w->lg_i = w->alloc<float>(N, "adjoint");
+    w->d_dv = w->alloc<double>((size_t)T * std::max(1u, w->nv), "dv");
+    w->carry_V = w->alloc<double>(N, "adjoint");
+    w->carry_d = w->alloc<double>(N, "adjoint");
+    w->carry_r = w->alloc<double>(N, "adjoint");
+    w->lg_e = w->alloc<double>(2 * (size_t)N, "adjoint");
+    w->lg_i = w->alloc<double>(2 * (size_t)N, "adjoint");
     w->acc_gleak = w->alloc<double>(N, "adjoint");
     w->acc_alpha = w->alloc<double>(NE, "adjoint");
     w->target_rates.assign(2 * (size_t)T, 0.0);
@@ -506,6 +541,24 @@ int bn_window_backward(bn_window* w, float* const* dW, double* dg_leak, double* 
     });
 }
 
+int bn_window_get_gradient(bn_window* w, uint32_t idx, float* dst) {
+    if (!w || !dst || idx >= 4) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_window_get_gradient: bad argument");
+    return guarded([&] {
+        const size_t n = (size_t)w->pre[idx] * w->post[idx];
+        if (!w->staging) {
+            size_t most = 0;
+            for (int k = 0; k < 4; ++k) most = std::max(most, (size_t)w->pre[k] * w->post[k]);
+            BN_HIP_CHECK(hipHostMalloc((void**)&w->staging, std::max<size_t>(most, 1) * sizeof(float), hipHostMallocDefault),
+                         "hipHostMalloc");
+        }
+        BN_HIP_CHECK(hipMemcpy2DAsync(w->staging, w->post[idx] * sizeof(float), w->dW[idx], w->ld[idx] * sizeof(float),
+                                      w->post[idx] * sizeof(float), w->pre[idx], hipMemcpyDeviceToHost, w->stream),
+                     "gradient download");
+        BN_HIP_CHECK(hipStreamSynchronize(w->stream), "gradient download sync");
+        std::memcpy(dst, w->staging, n * sizeof(float));
+    });
+}
+
 int bn_window_get_info(bn_window* w, bn_window_info* info) {
     if (!w || !info) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_window_get_info: null argument");
     std::memset(info, 0, sizeof *info);
@@ -518,6 +571,8 @@ int bn_window_get_info(bn_window* w, bn_window_info* info) {
     info->n_vrec = w->nv;
     info->g_leak = w->g_leak;
     info->tau_decay_e = w->tau_decay_e;
+    info->forward_ms = w->fwd_ms;
+    info->backward_ms = w->bwd_ms;
     return BN_OK;
 }
 

@@ -1,7 +1,8 @@
 // Differentiable window for CDNA4 (gfx950): forward step with per-step records, and the
 // reverse pass of the window loss (adjoint).  Forward per step: the propagate (k_gemv, shared
-// with the plain step) then k_win_forward.  Reverse per step: k_win_rows (lambda x = W lambda g
-// of the next step, one wave per pre row) then k_win_backward (traces, rates, EIF adjoint per
+// with the plain step) then k_win_forward (the plain evolve's layout: 64 neurons per block,
+// the partial sums split over 8 waves).  Reverse per step: k_win_reverse (lambda x = W lambda g
+// of the next step, one wave per pre row, then the traces, rates and EIF adjoint of that
 // neuron); per checkpoint segment, k_win_outer adds sum_t x_{t-1} (outer) lambda g_t to dW.
 //
 // Forward semantics: the plain step without plasticity or stimulus
@@ -17,89 +18,124 @@
 
 namespace bn {
 
-__global__ __launch_bounds__(256) void k_win_forward(WinForwardPlan P) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= P.N) return;
+// A block owns 64 neurons and has kFinParts waves: every wave sums one part of the propagate
+// partials of g_e and g_i for the 64 neurons, wave 1 also runs the background update; after the
+// barrier wave 0 combines the parts in order and runs the EIF step, the traces, the records.
+__global__ __launch_bounds__(kEvolveNeurons * kFinParts) void k_win_forward(WinForwardPlan P) {
+    __shared__ double fin[2][kFinParts][kEvolveNeurons];
+    __shared__ float bgl[2][kEvolveNeurons];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t j = blockIdx.x * kEvolveNeurons + lane;
+    const bool live = j < P.N;
     const bool exc = j < P.NE;
     const WinGroup& G = P.g[exc ? 0 : 1];
-    const NeuronGroupView& v = G.v;
     const uint32_t lid = exc ? j : j - P.NE;
     const WinState& S = P.st;
-    // g of this step: background of the last step + the propagate partials of the last step.
-    const float ge = finalize_g(S.bg_ge[j], G.fin_e, lid);
-    const float gi = finalize_g(S.bg_gi[j], G.fin_i, lid);
-    float V = S.V[j];
-    uint32_t refr = S.refr[j];
-    float* dT = exc ? S.dE : S.dI;
-    float* rT = exc ? S.rE : S.rI;
-    const float d0 = dT[lid], r0 = rT[lid];
-    if (P.rec.Vprev) {
-        const size_t k = (size_t)P.slot * P.N + j;
-        P.rec.Vprev[k] = V;
-        P.rec.ge[k] = ge;
-        P.rec.gi[k] = gi;
-        P.rec.refr[k] = refr > 0 ? 1 : 0;
-        float* rd = exc ? P.rec.dE : P.rec.dI;
-        float* rr = exc ? P.rec.rE : P.rec.rI;
-        const uint32_t n = exc ? P.NE : P.NI;
-        rd[(size_t)P.slot * n + lid] = d0;
-        rr[(size_t)P.slot * n + lid] = r0;
+    if (live) {
+        for (int q = 0; q < 2; ++q) {
+            const FinalizeSource& F = q == 0 ? G.fin_e : G.fin_i;
+            const uint32_t per = fin_part_size(F.chunks);
+            const uint32_t c0 = wave * per;
+            if (c0 < F.chunks) fin[q][wave][lane] = fin_part_sum(F, lid, c0, min(c0 + per, F.chunks));
+        }
+        if (wave == 1) {
+            const NeuronGroupView& v = G.v;
+            float re = S.bre[j], ri = S.bri[j], de = S.bde[j], di = S.bdi[j], bg_ge, bg_gi;
+            background_advance(v, j, P.step, P.clock, v.poisson_table_excitatory, v.poisson_table_inhibitory, re, ri,
+                               de, di, bg_ge, bg_gi);
+            S.bre[j] = re, S.bri[j] = ri, S.bde[j] = de, S.bdi[j] = di;
+            bgl[0][lane] = bg_ge;
+            bgl[1][lane] = bg_gi;
+        }
     }
+    __syncthreads();
+    if (wave != 0) return;
     uint32_t spike = 0;
-    eif_step(v, ge, gi, 0.0f, V, refr, spike);
-    float re = S.bre[j], ri = S.bri[j], de = S.bde[j], di = S.bdi[j], bg_ge, bg_gi;
-    background_advance(v, j, P.step, P.clock, v.poisson_table_excitatory, v.poisson_table_inhibitory, re, ri, de, di,
-                       bg_ge, bg_gi);
-    S.V[j] = V;
-    S.refr[j] = refr;
-    S.bre[j] = re, S.bri[j] = ri, S.bde[j] = de, S.bdi[j] = di;
-    S.bg_ge[j] = bg_ge;
-    S.bg_gi[j] = bg_gi;
-    // Connection traces (kernel_connection_propagate.cu:8-46).
-    const float ad = exc ? P.decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
-    float dd = flush_trace(d0 * ad), rr = flush_trace(r0 * ar);
-    if (spike == 1) {
-        dd += P.K;
-        rr += P.K;
+    if (live) {
+        // g of this step: background of the last step + the propagate partials (finalize_g's
+        // order: fl32(g + part_0 + part_1 + ...) in FP64).
+        float g2[2];
+        for (int q = 0; q < 2; ++q) {
+            const FinalizeSource& F = q == 0 ? G.fin_e : G.fin_i;
+            const uint32_t per = fin_part_size(F.chunks);
+            double acc = (double)(q == 0 ? S.bg_ge[j] : S.bg_gi[j]);
+            for (uint32_t w = 0; per && w * per < F.chunks; ++w) acc += fin[q][w][lane];
+            g2[q] = (float)acc;
+        }
+        const float ge = g2[0], gi = g2[1];
+        float V = S.V[j];
+        uint32_t refr = S.refr[j];
+        float* dT = exc ? S.dE : S.dI;
+        float* rT = exc ? S.rE : S.rI;
+        const float d0 = dT[lid], r0 = rT[lid];
+        if (P.rec.Vprev) {
+            const size_t k = (size_t)P.slot * P.N + j;
+            P.rec.Vprev[k] = V;
+            P.rec.ge[k] = ge;
+            P.rec.gi[k] = gi;
+            P.rec.refr[k] = refr > 0 ? 1 : 0;
+            const uint32_t n = exc ? P.NE : P.NI;
+            (exc ? P.rec.dE : P.rec.dI)[(size_t)P.slot * n + lid] = d0;
+            (exc ? P.rec.rE : P.rec.rI)[(size_t)P.slot * n + lid] = r0;
+        }
+        NeuronGroupView v = G.v;
+        v.g_leak = P.prm->g_leak;
+        eif_step(v, ge, gi, 0.0f, V, refr, spike);
+        S.V[j] = V;
+        S.refr[j] = refr;
+        S.bg_ge[j] = bgl[0][lane];
+        S.bg_gi[j] = bgl[1][lane];
+        // Connection traces (kernel_connection_propagate.cu:8-46).
+        const float ad = exc ? P.prm->decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
+        float dd = flush_trace(d0 * ad), rr = flush_trace(r0 * ar);
+        if (spike == 1) {
+            dd += P.K;
+            rr += P.K;
+        }
+        dT[lid] = dd;
+        rT[lid] = rr;
+        (exc ? S.xE : S.xI)[lid] = dd - rr;
+        if (P.rec.Vprev) P.rec.s[(size_t)P.slot * P.N + j] = (float)spike;
+        if (P.raster) P.raster[j] = (uint8_t)spike;
+        const int32_t vs = P.vrec_slot[j];
+        if (vs >= 0 && P.v_rec) P.v_rec[vs] = V;
     }
-    dT[lid] = dd;
-    rT[lid] = rr;
-    (exc ? S.xE : S.xI)[lid] = dd - rr;
-    if (P.rec.Vprev) P.rec.s[(size_t)P.slot * P.N + j] = (float)spike;
-    if (spike) atomicAdd(&P.counts[exc ? 0 : 1], 1u);
-    if (P.raster) P.raster[j] = (uint8_t)spike;
-    const int32_t vs = P.vrec_slot[j];
-    if (vs >= 0) P.v_rec[vs] = V;
+    if (P.counts) {
+        const uint64_t be = __ballot(live && spike && exc), bi = __ballot(live && spike && !exc);
+        if (lane == 0) {
+            if (be) atomicAdd(&P.counts[0], (uint32_t)__popcll(be));
+            if (bi) atomicAdd(&P.counts[1], (uint32_t)__popcll(bi));
+        }
+    }
 }
 
-// lambda x_i = sum_j W[i][j] lambda g_j over the two matrices leaving pre neuron i (E rows: EE
-// and EI with lambda g_e; I rows: IE and II with lambda g_i); one wave per row, FP64 sums.
-__global__ __launch_bounds__(256) void k_win_rows(WinRowPlan P) {
-    const uint32_t row = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t N = P.NE + P.NI;
-    if (row >= N) return;
-    const bool exc = row < P.NE;
-    const uint32_t il = exc ? row : row - P.NE;
-    const float* lg = exc ? P.lg_e : P.lg_i;
-    double acc = 0.0;
-    for (int h = 0; h < 2; ++h) {
-        const int c = (exc ? 0 : 2) + h;   // to E posts, then to I posts
-        const uint32_t n_post = h == 0 ? P.NE : P.NI;
-        const float* w = P.W[c] + (size_t)il * P.ld[c];
-        const float* l = lg + (h == 0 ? 0 : P.NE);
-        for (uint32_t j = lane; j < n_post; j += 64) acc += (double)w[j] * (double)l[j];
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) P.lx[row] = acc;
-}
-
-// Per neuron j at one step: the adjoint of its connection traces (as a pre neuron), then of its
-// EIF update (as a post neuron).
-__global__ __launch_bounds__(256) void k_win_backward(WinBackwardPlan P) {
+__global__ __launch_bounds__(256) void k_win_copy(WinCopyPlan P) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= P.N) return;
+    if (j < P.N) {
+        P.dst.V[j] = P.src.V[j];
+        P.dst.refr[j] = P.src.refr[j];
+        P.dst.bg_ge[j] = P.src.bg_ge[j];
+        P.dst.bg_gi[j] = P.src.bg_gi[j];
+        P.dst.bre[j] = P.src.bre[j];
+        P.dst.bde[j] = P.src.bde[j];
+        P.dst.bri[j] = P.src.bri[j];
+        P.dst.bdi[j] = P.src.bdi[j];
+    }
+    if (j < P.NE) {
+        P.dst.dE[j] = P.src.dE[j];
+        P.dst.rE[j] = P.src.rE[j];
+        P.dst.xE[j] = P.src.xE[j];
+    }
+    if (j < P.NI) {
+        P.dst.dI[j] = P.src.dI[j];
+        P.dst.rI[j] = P.src.rI[j];
+        P.dst.xI[j] = P.src.xI[j];
+    }
+}
+
+// Per neuron j at one step, given lambda x_j: the adjoint of its connection traces (as a pre
+// neuron), then of its EIF update (as a post neuron).
+__device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t j, double lx) {
     const bool exc = j < P.NE;
     const uint32_t lid = exc ? j : j - P.NE;
     const size_t k = (size_t)P.slot * P.N + j;
@@ -107,50 +143,83 @@ __global__ __launch_bounds__(256) void k_win_backward(WinBackwardPlan P) {
     const uint32_t n = exc ? P.NE : P.NI;
     const float d0 = (exc ? P.rec.dE : P.rec.dI)[(size_t)P.slot * n + lid];
     const float r0 = (exc ? P.rec.rE : P.rec.rI)[(size_t)P.slot * n + lid];
-    const float ad = exc ? P.decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
-    const double lx = P.lx[j];
-    const double ld = lx + (double)P.carry_d[j];
-    const double lr = -lx + (double)P.carry_r[j];
+    const float ad = exc ? P.prm->decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
+    const double ld = lx + P.carry_d[j];
+    const double lr = -lx + P.carry_r[j];
     const bool nd = !(d0 * ad < kTraceFlush), nr = !(r0 * ar < kTraceFlush);
     if (exc && nd) P.acc_alpha[j] += ld * (double)d0;
-    P.carry_d[j] = nd ? (float)((double)ad * ld) : 0.0f;
-    P.carry_r[j] = nr ? (float)((double)ar * lr) : 0.0f;
+    P.carry_d[j] = nd ? (double)ad * ld : 0.0;
+    P.carry_r[j] = nr ? (double)ar * lr : 0.0;
     // Spike: traces and the smoothed population rate.
-    const double ls = (double)P.K * (ld + lr) + (exc ? (double)P.lrho_e * P.rate_coef_e : (double)P.lrho_i * P.rate_coef_i);
-    double lv = (double)P.carry_V[j];
+    const double ls = (double)P.K * (ld + lr) + (exc ? P.lrho[0] * P.rate_coef_e : P.lrho[1] * P.rate_coef_i);
+    double lv = P.carry_V[j];
     const int32_t vs = P.vrec_slot[j];
-    if (vs >= 0) lv += (double)P.dv_direct[vs];
+    if (vs >= 0) lv += P.dv_direct[vs];
     if (P.rec.refr[k]) {
-        P.carry_V[j] = 0.0f;
-        P.lg_e[j] = 0.0f;
-        P.lg_i[j] = 0.0f;
+        P.carry_V[j] = 0.0;
+        P.lg_out_e[j] = 0.0;
+        P.lg_out_i[j] = 0.0;
         P.rec.le[k] = 0.0f;
         P.rec.li[k] = 0.0f;
         return;
     }
     const NeuronGroupView& v = P.g[exc ? 0 : 1].v;
-    const float Vp = P.rec.Vprev[k], ge = P.rec.ge[k], gi = P.rec.gi[k], s = P.rec.s[k];
-    // U of the forward step, recomputed with the forward's arithmetic.
-    const float ex = bn_expf((Vp - v.v_effective_spike_threshold) / v.depolarization_slope_factor);
-    const float F = -1.0f * v.g_leak * (Vp - v.v_rest) + v.g_leak * v.depolarization_slope_factor * ex -
-                    1.0f * ge * (Vp - v.v_excitatory_reversal) - 1.0f * gi * (Vp - v.v_inhibitory_reversal);
-    const float kappa = v.timestep / v.membrane_capacitance;
-    const float U = Vp + kappa * F;
+    const double Vp = P.rec.Vprev[k], ge = P.rec.ge[k], gi = P.rec.gi[k], s = P.rec.s[k];
+    // U of the step in FP64 from its recorded FP32 inputs (the spike itself is the record's).
+    const double DT = v.depolarization_slope_factor, gl = P.prm->g_leak;
+    const double ex = exp((Vp - (double)v.v_effective_spike_threshold) / DT);
+    const double F = -gl * (Vp - (double)v.v_rest) + gl * DT * ex - ge * (Vp - (double)v.v_excitatory_reversal) -
+                     gi * (Vp - (double)v.v_inhibitory_reversal);
+    const double kd = (double)v.timestep / (double)v.membrane_capacitance;
+    const double U = Vp + kd * F;
     double sg = 0.0;
     if (P.beta > 0.0f) {
-        const double z = 1.0 + fabs((double)U - (double)P.theta) / (double)P.beta;
+        const double z = 1.0 + fabs(U - (double)P.theta) / (double)P.beta;
         sg = 1.0 / ((double)P.beta * z * z);
     }
-    const double lU = lv * (1.0 - (double)s) + ls * sg;
-    const double kd = (double)kappa, exd = (double)ex, gl = (double)v.g_leak;
-    P.carry_V[j] = (float)(lU * (1.0 + kd * (-gl + gl * exd - (double)ge - (double)gi)));
-    const float le = (float)(-lU * kd * ((double)Vp - (double)v.v_excitatory_reversal));
-    const float li = (float)(-lU * kd * ((double)Vp - (double)v.v_inhibitory_reversal));
-    P.lg_e[j] = le;
-    P.lg_i[j] = li;
-    P.rec.le[k] = le;
-    P.rec.li[k] = li;
-    P.acc_gleak[j] += lU * kd * (-((double)Vp - (double)v.v_rest) + (double)v.depolarization_slope_factor * exd);
+    const double lU = lv * (1.0 - s) + ls * sg;
+    P.carry_V[j] = lU * (1.0 + kd * (-gl + gl * ex - ge - gi));
+    const double le = -lU * kd * (Vp - (double)v.v_excitatory_reversal);
+    const double li = -lU * kd * (Vp - (double)v.v_inhibitory_reversal);
+    P.lg_out_e[j] = le;
+    P.lg_out_i[j] = li;
+    P.rec.le[k] = (float)le;
+    P.rec.li[k] = (float)li;
+    P.acc_gleak[j] += lU * kd * (-(Vp - (double)v.v_rest) + DT * ex);
+}
+
+// One wave per neuron j: lambda x_j = sum_k W[j][k] lambda g_k over the two matrices leaving j
+// (E rows: EE and EI with lambda g_e; I rows: IE and II with lambda g_i), float4 weight loads
+// along the row, FP64 sums; then lane 0 runs the neuron's reverse step.
+__global__ __launch_bounds__(256) void k_win_reverse(WinReversePlan P) {
+    const uint32_t row = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const uint32_t lane = threadIdx.x & 63;
+    if (row >= P.N) return;
+    const bool exc = row < P.NE;
+    const uint32_t il = exc ? row : row - P.NE;
+    const double* lg = exc ? P.lg_in_e : P.lg_in_i;
+    double acc = 0.0;
+    for (int h = 0; h < 2; ++h) {
+        const int c = (exc ? 0 : 2) + h;   // to E posts, then to I posts
+        const uint32_t n_post = h == 0 ? P.NE : P.NI;
+        const float* w = P.W[c] + (size_t)il * P.ld[c];
+        const double* l = lg + (h == 0 ? 0 : P.NE);
+        const uint32_t n4 = n_post >> 2;
+        const float4* w4 = reinterpret_cast<const float4*>(w);
+#pragma unroll 4
+        for (uint32_t q = lane; q < n4; q += 64) {
+            const float4 wv = w4[q];
+            const double* lq = l + 4 * q;
+            acc += (double)wv.x * lq[0];
+            acc += (double)wv.y * lq[1];
+            acc += (double)wv.z * lq[2];
+            acc += (double)wv.w * lq[3];
+        }
+        for (uint32_t jj = 4 * n4 + lane; jj < n_post; jj += 64) acc += (double)w[jj] * l[jj];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) neuron_reverse(P, row, acc);
 }
 
 // dW[i][j] += sum_slots x_slot[i] * lambda_slot[j], x = d - r at the start of the step.  A block
@@ -205,17 +274,18 @@ __global__ __launch_bounds__(256) void k_win_outer(WinOuterPlan P) {
 }
 
 int launch_win_forward(const WinForwardPlan& p, bn_stream_t s) {
-    hipLaunchKernelGGL(k_win_forward, dim3((p.N + 255) / 256), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_win_forward, dim3((p.N + kEvolveNeurons - 1) / kEvolveNeurons), dim3(kEvolveNeurons * kFinParts),
+                       0, s, p);
     return (int)hipGetLastError();
 }
 
-int launch_win_rows(const WinRowPlan& p, bn_stream_t s) {
-    hipLaunchKernelGGL(k_win_rows, dim3((p.NE + p.NI + 3) / 4), dim3(256), 0, s, p);
+int launch_win_copy(const WinCopyPlan& p, bn_stream_t s) {
+    hipLaunchKernelGGL(k_win_copy, dim3((p.N + 255) / 256), dim3(256), 0, s, p);
     return (int)hipGetLastError();
 }
 
-int launch_win_backward(const WinBackwardPlan& p, bn_stream_t s) {
-    hipLaunchKernelGGL(k_win_backward, dim3((p.N + 255) / 256), dim3(256), 0, s, p);
+int launch_win_reverse(const WinReversePlan& p, bn_stream_t s) {
+    hipLaunchKernelGGL(k_win_reverse, dim3((p.N + 3) / 4), dim3(256), 0, s, p);
     return (int)hipGetLastError();
 }
 

@@ -88,7 +88,7 @@ class WindowSpec(C.Structure):
 class WindowInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("steps", "checkpoint_interval", "start_step", "total_neurons",
                                           "num_excitatory_neurons", "num_inhibitory_neurons", "n_vrec")] + [
-        ("g_leak", C.c_float), ("tau_decay_e", C.c_float)]
+        ("g_leak", C.c_float), ("tau_decay_e", C.c_float), ("forward_ms", C.c_float), ("backward_ms", C.c_float)]
 
 
 EXPORTED_SYMBOLS = [
@@ -106,7 +106,7 @@ EXPORTED_SYMBOLS = [
     "bn_multi_synchronize", "bn_multi_network", "bn_multi_get_info", "bn_multi_time_propagate",
     "bn_multi_set_schedule", "bn_window_create", "bn_window_destroy", "bn_window_set_params",
     "bn_window_get_weights", "bn_window_set_targets", "bn_window_forward", "bn_window_backward",
-    "bn_window_get_info",
+    "bn_window_get_info", "bn_window_get_gradient",
 ]
 
 _lib = None
@@ -175,6 +175,7 @@ def lib() -> C.CDLL:
     L.bn_window_forward.argtypes = [P, dp, dp, fp, C.POINTER(C.c_uint8)]
     L.bn_window_backward.argtypes = [P, C.POINTER(P), dp, dp]
     L.bn_window_get_info.argtypes = [P, C.POINTER(WindowInfo)]
+    L.bn_window_get_gradient.argtypes = [P, U, fp]
     for f in EXPORTED_SYMBOLS:
         if f not in ("bn_last_error", "bn_sim_destroy", "bn_batch_destroy", "bn_multi_destroy",
                      "bn_window_destroy"):
@@ -535,11 +536,19 @@ class Window:
                                        None if raster is None else raster.ctypes.data_as(C.POINTER(C.c_uint8))))
         return {"loss": loss.value, "rates": rates, "v": v, "raster": raster}
 
-    def backward(self, dW=None) -> dict:
-        """Gradient of the last forward's loss.  dW: None (host arrays returned) or 4 contiguous
-        float32 device tensors to fill."""
-        out = dW if dW is not None else [np.zeros(s, np.float32) for s in self.shapes]
-        ptrs = (C.c_void_p * 4)(*[_ptr(w) for w in out])
+    def backward(self, dW=None, download: bool = True) -> dict:
+        """Gradient of the last forward's loss.  dW: None or 4 contiguous float32 device tensors
+        to fill.  With dW None: host arrays (download) or nothing (the gradient stays on the
+        device; gradient(k) fetches it)."""
+        ptrs = None if dW is None else (C.c_void_p * 4)(*[_ptr(w) for w in dW])
         dgl, dtau = C.c_double(0), C.c_double(0)
         _check(lib().bn_window_backward(self.h, ptrs, C.byref(dgl), C.byref(dtau)))
+        out = dW
+        if dW is None and download:
+            out = [self.gradient(k) for k in range(4)]
         return {"dW": out, "dg_leak": dgl.value, "dtau_decay_e": dtau.value}
+
+    def gradient(self, idx: int) -> np.ndarray:
+        out = np.zeros(self.shapes[idx], np.float32)
+        _check(lib().bn_window_get_gradient(self.h, idx, _fp(out)))
+        return out

@@ -122,9 +122,9 @@ def adjoint_config(base: dict | None = None) -> dict:
 
 def adjoint_workload(engine, config: dict | None = None, steps: int = 1000, warmup: int = 1000, n_vrec: int = 50,
                      checkpoint_interval: int = 100, surrogate_beta: float = 1e-3, target_scale: float = 1.1):
-    """Returns (sim, window, base weights): the window starts from the state after `warmup`
-    steps and holds the targets of the E->E x target_scale network; its parameters are back at
-    the base weights."""
+    """Returns (sim, window, base weights, targets): the window starts from the state after
+    `warmup` steps and holds the targets (rates [steps][2], voltages [steps][n_vrec]) of the
+    E->E x target_scale network; its parameters are back at the base weights."""
     cfg = adjoint_config(config)
     sim = engine.Simulation(cfg, flags=("no_monitors", "quiet"))
     sim.run(warmup)
@@ -136,5 +136,6 @@ def adjoint_workload(engine, config: dict | None = None, steps: int = 1000, warm
     win.set_params([base[0] * target_scale, None, None, None])
     tgt = win.forward()
     win.set_params([base[0], None, None, None])
-    win.set_targets(tgt["rates"], tgt["v"].astype("float64"))
-    return sim, win, base
+    targets = (tgt["rates"], tgt["v"].astype("float64"))
+    win.set_targets(*targets)
+    return sim, win, base, targets

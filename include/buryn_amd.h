@@ -496,6 +496,8 @@ typedef struct bn_window_info {
     uint32_t n_vrec;
     float g_leak;                    /* current parameters */
     float tau_decay_e;
+    float forward_ms;                /* device time of the last forward's steps (HIP events) */
+    float backward_ms;               /* device time of the last backward's recompute + reverse */
 } bn_window_info;
 
 /* The network must have both plasticities off.  Weights, g_leak and tau_decay_e start at the
@@ -514,6 +516,9 @@ int bn_window_forward(bn_window* window, double* loss, double* rates, float* v, 
 /* Gradient of the last forward's L.  dW: NULL or 4 pointers (NULL = skip), dense [pre][post]
  * FP32, host or device memory. */
 int bn_window_backward(bn_window* window, float* const* dW, double* dg_leak, double* dtau_decay_e);
+/* dL/dW of weight matrix idx from the last backward into host memory [pre][post] (through a
+ * pinned staging buffer: the fast path to the host). */
+int bn_window_get_gradient(bn_window* window, uint32_t weight_matrix_idx, float* dst);
 int bn_window_get_info(bn_window* window, bn_window_info* info);
 
 #ifdef __cplusplus
