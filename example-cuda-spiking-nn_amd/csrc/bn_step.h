@@ -15,6 +15,17 @@
 //                  last block applies this step's stimulus events and advances the clock
 //   k_norm_tail    (reference-exact normalisation only) triplet row normalisation, then the
 //                  stimulus and clock advance instead of k_plasticity
+//
+// Overlapped form of the same steps (captured graphs with lazy or no normalisation): the
+// plasticity of step t-1 touches only weights, row sums and scales and reads the spike bits and
+// plasticity traces of step t-1, so it runs in the launch of the evolve of step t
+// (k_evolve_plast) while that evolve writes its spike bits and traces to the other buffer of a
+// pair (even steps of the graph: current -> staging, odd steps: staging -> current; graphs have
+// an even number of steps, so they end on the current buffers).  The stimulus events of step
+// t-1 are applied per neuron by the evolve of step t, and the clock advances once per graph,
+// each launch of the graph knowing its step offset.  Per graph of n steps:
+//   k_evolve(t0) k_gemv(t0) [k_evolve_plast(t, t-1) k_gemv(t)] x (n-1) k_plasticity(t0+n-1, tail)
+// The arithmetic of every neuron and weight is the sequential step's.
 #pragma once
 
 #include <stdint.h>
@@ -69,6 +80,10 @@ struct PlastTraces {
     float* post;       // indexed by post-local id
     float* trip_pre;
     float* trip_post;
+    float* pre_out;    // where the step writes them (the same arrays, or staging copies)
+    float* post_out;
+    float* trip_pre_out;
+    float* trip_post_out;
 };
 
 struct EvolvePlan {
@@ -77,11 +92,20 @@ struct EvolvePlan {
     PlastTraces pt[kMaxPlast];
     uint32_t n_groups, n_ts, n_pt, total_neurons;
     const bn_device_clock* clock;
-    uint32_t* spike_bits;        // [words] bits of this step (global ids)
+    uint32_t* spike_bits;        // [words] bits of this step (global ids), or their staging copy
     uint32_t* spike_bits_ring;   // [interval][words] or nullptr
     float* v_ring;               // [interval][n_vrec] or nullptr
     const int32_t* vrec_index;   // [total_neurons] slot in v_ring or -1
     uint32_t n_vrec, interval, words;
+    uint32_t step_offset;        // the step is clock_step + step_offset
+    // Overlapped graphs: the stimulus events of the previous step, applied per neuron (in list
+    // order) before its EIF step.
+    uint32_t stim_prev;
+    const uint32_t* ev_start;    // [ev_steps + 1] first event of each step
+    const uint32_t* ev_ids;
+    const uint32_t* ev_offsets;
+    const float* ev_amps;
+    uint32_t ev_steps, ev_count;
 };
 
 struct GemvConn {
@@ -137,6 +161,11 @@ struct StimTail {
     bn_device_clock* clock;
     uint32_t* status;            // device error word (1 = stimulus cursor overflow)
     uint32_t active;             // 0: nothing to do in this kernel
+    // End of an overlapped graph: the events of step clock_step + step_offset, found through
+    // ev_start (the cursor then points past them), and a clock advance of `advance` steps.
+    // Sequential steps: ev_start nullptr (the cursor's events), offset 0, advance 0 (= one step).
+    const uint32_t* ev_start;
+    uint32_t step_offset, advance;
 };
 
 struct PlastStepPlan {
@@ -174,6 +203,9 @@ struct MultiStep {
 int launch_multi_step(const MultiStep& m, bn_stream_t s);
 int launch_multi_propagate(const MultiStep& m, bn_stream_t s);
 int launch_gemv(const GemvPlan& plan, bn_stream_t s);
+// Overlapped step: evolve of step t with the plasticity of step t-1 (plast = nullptr: evolve
+// only).
+int launch_evolve_plast(const EvolvePlan& ev, const PlastStepPlan* plast, bn_stream_t s);
 int launch_plasticity(const PlastStepPlan& plan, bn_stream_t s);
 int launch_ltp(const PlastPlan& plan, bn_stream_t s);
 int launch_ltd(const PlastPlan& plan, bn_stream_t s);

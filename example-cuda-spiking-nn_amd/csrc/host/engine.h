@@ -36,6 +36,7 @@ struct StateOptions {
     int64_t rng_seed = 314;  // Philox key (buryn::rng_seed, buryn_definitions.h:11)
     std::array<uint32_t, 4> gemv_row_block = {0, 0, 0, 0};  // 0 = heuristic
     int gemv_nontemporal = -1;  // weight loads past the caches: 1 yes, 0 no, -1 when W >> MALL
+    bool overlap_steps = true;  // captured graphs overlap plasticity(t-1) with evolve(t)
 };
 
 class GpuSimulationState {
@@ -402,6 +403,8 @@ public:
     const EvolvePlan& evolve_plan() const { return evolve_; }
     const PlastStepPlan& plast_step_plan() const { return plast_step_; }
     bool has_norm_tail() const { return tail_.norm_rows != 0; }
+    // Captured graphs overlap the plasticity of step t-1 with the evolve of step t.
+    bool overlapped() const { return overlap_; }
     // Steps run by an external launcher on this system's plans (several networks in one launch).
     void note_external_steps(uint32_t n) { clock_step_ += n; }
     uint32_t kernels_per_step() const { return kernels_per_step_; }
@@ -420,7 +423,9 @@ public:
 
 private:
     void build_plans();
+    void build_overlap_plans();
     void enqueue_fused_step(hipEvent_t gemv_begin, hipEvent_t gemv_end);
+    void enqueue_overlapped_steps(uint32_t n);
 
     GpuSimulationState& state_;
     std::vector<std::unique_ptr<NeuronGroup>> neuron_groups_;
@@ -442,6 +447,15 @@ private:
     TailPlan tail_{};
     std::vector<int> conn_trace_set_;
     uint32_t kernels_per_step_ = 0;
+    // Overlapped graphs: even steps of a graph evolve from the current spike bits and
+    // plasticity traces into the staging ones, odd steps back; the plasticity of a step reads
+    // the buffers its evolve wrote; the last plasticity of a graph carries the stimulus/clock
+    // tail.
+    bool overlap_ = false;
+    EvolvePlan evolve_even_{}, evolve_odd_{};
+    PlastStepPlan plast_staging_{}, plast_current_{}, plast_final_{};
+    uint32_t* d_ev_start_ = nullptr;
+    std::vector<void*> staging_;
 };
 
 }  // namespace bn

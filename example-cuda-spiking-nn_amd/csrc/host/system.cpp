@@ -12,6 +12,7 @@ namespace bn {
 System::~System() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
+    for (void* p : staging_) (void)hipFree(p);
 }
 
 void System::add_neuron_group(std::unique_ptr<NeuronGroup> g) {
@@ -148,10 +149,10 @@ void System::build_plans() {
         Q.post_decay = pp.post_trace_decay;
         Q.trip_pre_decay = pp.triplet_pre_trace_decay;
         Q.trip_post_decay = pp.triplet_post_trace_decay;
-        Q.pre = v.pre_traces;
-        Q.post = v.post_traces;
-        Q.trip_pre = v.triplet_pre_traces;
-        Q.trip_post = v.triplet_post_traces;
+        Q.pre = Q.pre_out = v.pre_traces;
+        Q.post = Q.post_out = v.post_traces;
+        Q.trip_pre = Q.trip_pre_out = v.triplet_pre_traces;
+        Q.trip_post = Q.trip_post_out = v.triplet_post_traces;
         PlastDev& D = plast_.p[k];
         D.kind = P.kind();
         D.normalize = P.normalizes() ? 1u : 0u;
@@ -261,6 +262,84 @@ void System::build_plans() {
     if (lazy_idx >= 0) BN_HIP_CHECK((hipError_t)launch_lazy_init(plast_.p[lazy_idx], S.stream()), "lazy normalisation init");
 }
 
+void System::build_overlap_plans() {
+    hipStream_t s = state_.stream();
+    evolve_even_ = evolve_odd_ = evolve_;
+    plast_staging_ = plast_current_ = plast_step_;
+    auto alloc = [&](size_t bytes) {
+        void* p = nullptr;
+        BN_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 4)), "hipMalloc staging");
+        staging_.push_back(p);
+        return p;
+    };
+    // Staging twins of the plasticity traces: even steps write them, odd steps read them.
+    for (uint32_t k = 0; k < evolve_.n_pt; ++k) {
+        const PlastTraces& Q = evolve_.pt[k];
+        PlastTraces& E = evolve_even_.pt[k];
+        PlastTraces& O = evolve_odd_.pt[k];
+        PlastDev& D = plast_staging_.pl.p[k];
+        auto twin = [&](float* cur, uint32_t n, float*& even_out, float*& odd_in, float*& plast_in) {
+            if (!cur || n == 0) return;
+            float* st = static_cast<float*>(alloc((size_t)n * sizeof(float)));
+            even_out = st;
+            odd_in = st;
+            plast_in = st;
+        };
+        twin(Q.pre, Q.n_pre, E.pre_out, O.pre, D.pre);
+        twin(Q.post, Q.n_post, E.post_out, O.post, D.post);
+        if (Q.kind == kPlastTriplet) {
+            twin(Q.trip_pre, Q.n_pre, E.trip_pre_out, O.trip_pre, D.trip_pre);
+            twin(Q.trip_post, Q.n_post, E.trip_post_out, O.trip_post, D.trip_post);
+        }
+    }
+    uint32_t* bits = static_cast<uint32_t*>(alloc((size_t)std::max<uint32_t>(1, evolve_.words) * sizeof(uint32_t)));
+    evolve_even_.spike_bits = bits;
+    for (uint32_t k = 0; k < plast_staging_.pl.n; ++k) plast_staging_.pl.p[k].bits = bits;
+    plast_staging_.tail.active = plast_current_.tail.active = 0;
+    // The stimulus: event starts per step for the per-neuron application and the final tail.
+    plast_final_ = plast_current_;
+    plast_final_.tail = plast_step_.tail;
+    plast_final_.tail.active = 1;
+    plast_final_.tail.step_offset = steps_per_graph_ - 1;
+    plast_final_.tail.advance = steps_per_graph_;
+    const StimTail& st = plast_step_.tail;
+    if (st.timestep_event_count && st.num_timesteps) {
+        std::vector<uint32_t> counts(st.num_timesteps), start(st.num_timesteps + 1, 0);
+        BN_HIP_CHECK(hipMemcpy(counts.data(), st.timestep_event_count, counts.size() * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost), "event counts");
+        for (uint32_t t = 0; t < st.num_timesteps; ++t) start[t + 1] = start[t] + counts[t];
+        d_ev_start_ = static_cast<uint32_t*>(alloc(start.size() * sizeof(uint32_t)));
+        BN_HIP_CHECK(hipMemcpy(d_ev_start_, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+                     "event starts");
+        for (EvolvePlan* e : {&evolve_even_, &evolve_odd_}) {
+            e->ev_start = d_ev_start_;
+            e->ev_ids = st.event_ids;
+            e->ev_offsets = st.event_offsets;
+            e->ev_amps = st.event_amplitudes;
+            e->ev_steps = st.num_timesteps;
+            e->ev_count = st.num_events;
+        }
+        plast_final_.tail.ev_start = d_ev_start_;
+    }
+    BN_HIP_CHECK(hipStreamSynchronize(s), "staging sync");
+}
+
+// A graph of n (even) steps: k_evolve, k_gemv for the first, then (evolve of t + plasticity of
+// t-1, propagate of t) for the others, and the plasticity of the last step with the stimulus
+// events of that step and the clock advance of the whole graph.
+void System::enqueue_overlapped_steps(uint32_t n) {
+    hipStream_t s = state_.stream();
+    for (uint32_t k = 0; k < n; ++k) {
+        EvolvePlan e = k % 2 == 0 ? evolve_even_ : evolve_odd_;
+        e.step_offset = k;
+        e.stim_prev = k > 0 && e.ev_start ? 1u : 0u;
+        const PlastStepPlan* prev = k == 0 ? nullptr : (k - 1) % 2 == 0 ? &plast_staging_ : &plast_current_;
+        BN_HIP_CHECK((hipError_t)launch_evolve_plast(e, prev, s), "overlapped evolve launch");
+        BN_HIP_CHECK((hipError_t)launch_gemv(gemv_, s), "overlapped propagate launch");
+    }
+    BN_HIP_CHECK((hipError_t)launch_plasticity(plast_final_, s), "overlapped plasticity launch");
+}
+
 void System::enqueue_fused_step(hipEvent_t gemv_begin, hipEvent_t gemv_end) {
     hipStream_t s = state_.stream();
     BN_HIP_CHECK((hipError_t)launch_evolve(evolve_, s), "fused evolve launch");
@@ -278,11 +357,18 @@ void System::enable_fused(bool use_graph, uint32_t steps_per_graph) {
     for (auto& m : monitors_) m->fused = true;
     use_graph_ = use_graph;
     steps_per_graph_ = std::max<uint32_t>(1, steps_per_graph);
+    overlap_ = use_graph_ && state_.options().overlap_steps && tail_.norm_rows == 0 && steps_per_graph_ % 2 == 0;
+    if (overlap_) {
+        build_overlap_plans();
+        kernels_per_step_ = 2;
+    }
     if (use_graph_) {
         hipStream_t s = state_.stream();
         BN_HIP_CHECK(hipStreamSynchronize(s), "pre-capture sync");
         BN_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        for (uint32_t k = 0; k < steps_per_graph_; ++k) enqueue_fused_step(nullptr, nullptr);
+        if (overlap_) enqueue_overlapped_steps(steps_per_graph_);
+        else
+            for (uint32_t k = 0; k < steps_per_graph_; ++k) enqueue_fused_step(nullptr, nullptr);
         BN_HIP_CHECK(hipStreamEndCapture(s, &graph_), "hipStreamEndCapture");
         BN_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
     }

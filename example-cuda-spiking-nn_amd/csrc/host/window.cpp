@@ -47,6 +47,7 @@ struct bn_window {
     // Forward (T steps with checkpoints) and backward (every segment's recompute and reverse
     // walk) captured once per window; parameters, targets and loss terms live in device
     // memory, so the graphs stay valid.
+    hipGraph_t fwd_graph = nullptr, bwd_graph = nullptr;  // kept for the executables' lifetime
     hipGraphExec_t fwd_exec = nullptr, bwd_exec = nullptr;
     float* staging = nullptr;           // pinned host buffer for gradient downloads
     hipEvent_t ev[4] = {};              // around the last forward and backward graph launches
@@ -75,6 +76,8 @@ struct bn_window {
         if (staging) (void)hipHostFree(staging);
         if (fwd_exec) (void)hipGraphExecDestroy(fwd_exec);
         if (bwd_exec) (void)hipGraphExecDestroy(bwd_exec);
+        if (fwd_graph) (void)hipGraphDestroy(fwd_graph);
+        if (bwd_graph) (void)hipGraphDestroy(bwd_graph);
         for (void* p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -138,7 +141,7 @@ struct bn_window {
     }
 
     template <typename F>
-    hipGraphExec_t capture(F&& body) {
+    hipGraphExec_t capture(hipGraph_t& graph, F&& body) {
         BN_HIP_CHECK(hipStreamSynchronize(stream), "pre-capture sync");
         BN_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         try {
@@ -148,12 +151,9 @@ struct bn_window {
             if (hipStreamEndCapture(stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
             throw;
         }
-        hipGraph_t graph = nullptr;
         BN_HIP_CHECK(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
         hipGraphExec_t exec = nullptr;
-        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        BN_HIP_CHECK(e, "hipGraphInstantiate");
+        BN_HIP_CHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
         return exec;
     }
 
@@ -161,7 +161,7 @@ struct bn_window {
         apply_params();
         BN_HIP_CHECK(hipMemsetAsync(d_counts, 0, 2 * (size_t)T * sizeof(uint32_t), stream), "clear counts");
         if (!fwd_exec)
-            fwd_exec = capture([&] {
+            fwd_exec = capture(fwd_graph, [&] {
                 copy_state(st, snap);
                 for (uint32_t t = 0; t < T; ++t) {
                     if (t % S == 0) copy_state(ckpt[t / S], st);
@@ -271,7 +271,7 @@ struct bn_window {
         for (double* p : {lg_e, lg_i}) BN_HIP_CHECK(hipMemsetAsync(p, 0, 2 * N * sizeof(double), stream), "clear adjoint");
         BN_HIP_CHECK(hipMemsetAsync(acc_gleak, 0, N * sizeof(double), stream), "clear acc");
         BN_HIP_CHECK(hipMemsetAsync(acc_alpha, 0, NE * sizeof(double), stream), "clear acc");
-        if (!bwd_exec) bwd_exec = capture([&] { enqueue_backward(); });
+        if (!bwd_exec) bwd_exec = capture(bwd_graph, [&] { enqueue_backward(); });
         BN_HIP_CHECK(hipEventRecord(ev[2], stream), "hipEventRecord");
         BN_HIP_CHECK(hipGraphLaunch(bwd_exec, stream), "window backward graph");
         BN_HIP_CHECK(hipEventRecord(ev[3], stream), "hipEventRecord");

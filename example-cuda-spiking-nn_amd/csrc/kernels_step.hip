@@ -111,7 +111,7 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) 
     const uint32_t gid = blk * kEvolveNeurons + lane;
     const bool live = gid < P.total_neurons;
     const bn_device_clock clk = *P.clock;
-    const uint32_t step = clk.clock_step;
+    const uint32_t step = clk.clock_step + P.step_offset;
 
     // Wave 0: per-neuron inputs of the EIF step and the connection traces.
     float V = 0.0f, ge = 0.0f, gi_ = 0.0f, istim = 0.0f;
@@ -130,6 +130,17 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) 
             ge = v.g_excitatory[lid];
             gi_ = v.g_inhibitory[lid];
             istim = v.i_stimulus[lid];
+            if (P.stim_prev && step - 1 < P.ev_steps) {
+                // stimulus.cpp:46-60 for this neuron: the previous step's events in list order.
+                const uint32_t e1 = min(P.ev_start[step], P.ev_count);
+                bool hit = false;
+                for (uint32_t e = P.ev_start[step - 1]; e < e1; ++e)
+                    if (P.ev_ids[e] + P.ev_offsets[e] == gid) {
+                        istim += P.ev_amps[e];
+                        hit = true;
+                    }
+                if (hit) v.i_stimulus[lid] = istim;
+            }
         }
 #pragma unroll
         for (int t = 0; t < kMaxTraceSets; ++t) {
@@ -219,15 +230,15 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) 
                 const uint32_t i = gid - Q.pre_offset;
                 const float a = Q.pre[i];
                 const float b = Q.kind == kPlastTriplet ? Q.trip_pre[i] : 0.0f;
-                Q.pre[i] = trace_step(a, spike_prev, Q.pre_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_pre[i] = trace_step(b, spike_prev, Q.trip_pre_decay);
+                Q.pre_out[i] = trace_step(a, spike_prev, Q.pre_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_pre_out[i] = trace_step(b, spike_prev, Q.trip_pre_decay);
             }
             if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
                 const uint32_t j = gid - Q.post_offset;
                 const float a = Q.post[j];
                 const float b = Q.kind == kPlastTriplet ? Q.trip_post[j] : 0.0f;
-                Q.post[j] = trace_step(a, spike_prev, Q.post_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_post[j] = trace_step(b, spike_prev, Q.trip_post_decay);
+                Q.post_out[j] = trace_step(a, spike_prev, Q.post_decay);
+                if (Q.kind == kPlastTriplet) Q.trip_post_out[j] = trace_step(b, spike_prev, Q.trip_post_decay);
             }
         }
     }
@@ -470,7 +481,7 @@ __device__ __forceinline__ double block_reduce_sum(double acc, double* lds4) {
     const uint32_t t = threadIdx.x;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if ((t & 63) == 0) lds4[t >> 6] = acc;
+    if ((t & 63) == 0 && t < kNormThreads) lds4[t >> 6] = acc;
     __syncthreads();
     const double total = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
     __syncthreads();
@@ -491,6 +502,9 @@ __device__ __forceinline__ double block_row_sum(const float* vals, uint32_t n, d
 // offsets from g0 in increasing order, compacted by the whole block (popcount + block scan).
 // Returns the count (the same in every thread); entries past `cap` are not stored.
 constexpr uint32_t kListCap = 4096;
+constexpr uint32_t kMaxBlockWaves = 8;  // blocks of 256 or 512 threads (the latter: overlapped step)
+__device__ __forceinline__ bool spike_bit(const uint32_t* bits, uint32_t gid) { return (bits[gid >> 5] >> (gid & 31)) & 1u; }
+
 __device__ __forceinline__ uint32_t block_compact_bits(const uint32_t* bits, uint32_t g0, uint32_t g1,
                                                        uint32_t* list, uint32_t cap, uint32_t* wsum) {
     if (g1 <= g0) return 0;
@@ -516,8 +530,10 @@ __device__ __forceinline__ uint32_t block_compact_bits(const uint32_t* bits, uin
         if (lane == 63) wsum[wv] = v;
         __syncthreads();
         uint32_t wbase = 0, btot = 0;
+        const uint32_t nwaves = blockDim.x >> 6;
 #pragma unroll
-        for (uint32_t k = 0; k < 4; ++k) {
+        for (uint32_t k = 0; k < kMaxBlockWaves; ++k) {
+            if (k >= nwaves) break;
             if (k < wv) wbase += wsum[k];
             btot += wsum[k];
         }
@@ -619,15 +635,16 @@ __device__ __forceinline__ void plasticity_spiking_row(const PlastDev& D, uint32
     double acc = 0.0;
     // Batches of 16 columns per thread: all loads first, then updates/stores in column order.
     constexpr int kB = 16;
-    for (uint32_t j0 = threadIdx.x; j0 < D.n_post; j0 += kB * kNormThreads) {
+    const uint32_t j_first = threadIdx.x < kNormThreads ? threadIdx.x : D.n_post;  // threads past 256 idle
+    for (uint32_t j0 = j_first; j0 < D.n_post; j0 += kB * kNormThreads) {
         float m[kB], post_j[kB], tpost_j[kB];
-        uint32_t sp[kB];
+        uint32_t spw[kB];
 #pragma unroll
         for (int k = 0; k < kB; ++k) {
             const uint32_t j = j0 + (uint32_t)k * kNormThreads;
             if (j < D.n_post) {
                 m[k] = row[j];
-                sp[k] = D.post_spikes[j];
+                spw[k] = D.bits[(D.post_offset + j) >> 5];
                 post_j[k] = D.post[j];
                 tpost_j[k] = triplet ? D.trip_post[j] : 0.0f;
             }
@@ -638,7 +655,7 @@ __device__ __forceinline__ void plasticity_spiking_row(const PlastDev& D, uint32
             if (j >= D.n_post) continue;
             float v = m[k];
             if (update && j != i) {
-                if (sp[k] == 1) {
+                if ((spw[k] >> ((D.post_offset + j) & 31)) & 1u) {
                     const float fac = triplet ? D.p.ltp_A + (D.p.ltp_B * tpost_j[k]) : D.p.ltp_A;
                     v = apply_delta(D, v, si, triplet ? pre_i * fac : D.p.ltp_A * pre_i);
                 }
@@ -666,10 +683,11 @@ __device__ __forceinline__ void plasticity_spiking_row(const PlastDev& D, uint32
 // independent of scheduling (the reference uses atomicAdd).
 __device__ __forceinline__ void stimulus_and_clock(const StimTail& T) {
     bn_device_clock* clk = T.clock;
-    const uint32_t s = clk->clock_step;
+    const uint32_t base = clk->clock_step;
+    const uint32_t s = base + T.step_offset;
     if (T.timestep_event_count && s < T.num_timesteps) {
         const uint32_t n = T.timestep_event_count[s];
-        const uint32_t cur = clk->next_event_idx;
+        const uint32_t cur = T.ev_start ? T.ev_start[s] : clk->next_event_idx;
         if ((uint64_t)cur + n > T.num_events) {
             *T.status = 1u;
         } else {
@@ -678,7 +696,7 @@ __device__ __forceinline__ void stimulus_and_clock(const StimTail& T) {
             clk->next_event_idx = cur + n;
         }
     }
-    clk->clock_step = s + 1;
+    clk->clock_step = base + (T.advance ? T.advance : 1u);
 }
 
 // Blocks [0, row_blocks): kQuietRows rows each of the concatenated pre rows of all plasticities
@@ -686,7 +704,7 @@ __device__ __forceinline__ void stimulus_and_clock(const StimTail& T) {
 // The last block also applies the stimulus and advances the clock (no block reads the clock).
 __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_t b, uint32_t nblocks) {
     __shared__ double lds4[4];
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[kMaxBlockWaves];
     __shared__ uint32_t list[kListCap];
     __shared__ float facs[kListCap];
     __shared__ double qsum[kQuietParts][64];
@@ -703,7 +721,7 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
                 for (uint32_t e = threadIdx.x; e < min(count, kListCap); e += blockDim.x) facs[e] = ltp_factor(D, list[e]);
                 __syncthreads();
                 const uint32_t r = r0 + lane;
-                const bool mine = r >= base && r < base + D.n_pre && D.pre_spikes[r - base] != 1;
+                const bool mine = r >= base && r < base + D.n_pre && !spike_bit(D.bits, D.pre_offset + (r - base));
                 double R0 = 0.0;
                 float target = 0.0f;
                 if (D.lazy && q == 0 && mine) {
@@ -712,7 +730,7 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
                 }
                 const double dq = mine && count ? plasticity_quiet_part(D, r - base, list, facs, count, q) : 0.0;
                 if (D.lazy) {
-                    qsum[q][lane] = dq;
+                    if (q < kQuietParts) qsum[q][lane] = dq;
                     __syncthreads();
                     if (q == 0 && mine) {
                         // R_i += the part sums in part order (non-empty parts only).
@@ -743,7 +761,7 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
             } else {
                 uint32_t rank = 0;
                 for (uint32_t i = 0; i < D.n_pre; ++i)
-                    if (D.pre_spikes[i] == 1) {
+                    if (spike_bit(D.bits, D.pre_offset + i)) {
                         if ((base + rank) % G == sb) plasticity_spiking_row(D, i, lds4);
                         ++rank;
                     }
@@ -756,6 +774,13 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
 }
 
 __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) { plasticity_block(P, blockIdx.x, gridDim.x); }
+
+// Overlapped step: blocks [0, n_evolve) evolve step t, the rest run the plasticity of step t-1
+// (its 256-thread body; the upper four waves take part in its barriers only).
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve_plast(EvolvePlan E, PlastStepPlan P, uint32_t n_evolve) {
+    if (blockIdx.x < n_evolve) evolve_block(E, blockIdx.x);
+    else plasticity_block(P, blockIdx.x - n_evolve, gridDim.x - n_evolve);
+}
 
 __global__ __launch_bounds__(256) void k_plasticity_multi(const PlastStepPlan* __restrict__ plans,
                                                           const uint32_t* __restrict__ block_net,
@@ -1018,6 +1043,17 @@ static bool gemv_vector_ok(const GemvPlan& plan) {
         if (((uintptr_t)plan.c[c].W) % 16 != 0) return false;
     }
     return true;
+}
+
+int launch_evolve_plast(const EvolvePlan& ev, const PlastStepPlan* plast, bn_stream_t s) {
+    static_assert(sizeof(EvolvePlan) + sizeof(PlastStepPlan) + 4 < 3800, "kernel argument too large");
+    const uint32_t ne = cdiv(ev.total_neurons, kEvolveNeurons);
+    const uint32_t np = plast ? plast->row_blocks + plast->spiking_blocks : 0;
+    if (np == 0)
+        hipLaunchKernelGGL(k_evolve, dim3(ne), dim3(kFinParts * 64), 0, s, ev);
+    else
+        hipLaunchKernelGGL(k_evolve_plast, dim3(ne + np), dim3(kFinParts * 64), 0, s, ev, *plast, ne);
+    return (int)hipGetLastError();
 }
 
 int launch_gemv(const GemvPlan& plan, bn_stream_t s) {

@@ -23,6 +23,7 @@ BURYN_FLAGS = {
     "component_path": 1 << 3,
     "no_graph": 1 << 4,
     "quiet": 1 << 5,
+    "sequential_graph": 1 << 6,
 }
 
 

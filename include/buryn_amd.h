@@ -290,7 +290,10 @@ enum {
                                            (W = s_i * M) instead of rewriting W_EE every step */
     BN_SIM_COMPONENT_PATH = 1u << 3,    /* step through the component launchers (drop-in path) */
     BN_SIM_NO_GRAPH = 1u << 4,          /* launch the fused step eagerly instead of a hipGraph */
-    BN_SIM_QUIET = 1u << 5              /* no progress prints */
+    BN_SIM_QUIET = 1u << 5,             /* no progress prints */
+    BN_SIM_SEQUENTIAL_GRAPH = 1u << 6   /* graphs run each step's launches one after the other
+                                         * instead of overlapping the plasticity of step t-1
+                                         * with the evolve of step t (same results) */
 };
 
 typedef struct bn_sim_info {

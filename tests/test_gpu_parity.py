@@ -58,6 +58,42 @@ def test_fused_step_bit_exact(tmp_path, graph):
     sim.close()
 
 
+@pytest.mark.parametrize("mode", ["lazy", "lazy-sequential", "faithful", "burst"])
+def test_graph_steps_bit_exact(tmp_path, mode):
+    # Whole graphs of 10 steps: the overlapped form (plasticity of step t-1 in the launch of the
+    # evolve of step t, staging copies, stimulus in the propagate launch) and the sequential
+    # one must both leave exactly the oracle's state after every graph; faithful normalisation
+    # always runs sequentially.
+    if mode == "burst":
+        cfg = small_config(n_e_side=24, n_i_side=12, steps=400)
+        cfg["stimulus_selected_input_ids"] = list(range(300))
+        cfg["stimulus_selected_input_post_group_offsets"] = [0] * 300
+        cfg["stimulus_start_clock_steps"] = [30, 95]
+        cfg["stimulus_end_clock_steps"] = [40, 101]
+        cfg["stimulus_input_values"] = [4e-9, 3e-9]
+    else:
+        cfg = small_config(steps=600)
+    path = _write(tmp_path, cfg)
+    lazy = mode != "faithful"
+    flags = ("no_monitors", "quiet") + (("lazy_normalize",) if lazy else ()) + \
+        (("sequential_graph",) if mode == "lazy-sequential" else ())
+    sim = engine().Simulation(path, flags=flags)
+    orc = oracle_for(path, sim, lazy=lazy)
+    for chunk in range(25):
+        sim.run(10)
+        orc.step(10)
+        assert_state_equal(sim.state(), orc.state(), 10 * chunk + 9)
+        if chunk % 8 == 7:
+            _check_traces_and_weights(sim, orc)
+    sim.run(3)  # a remainder of eager steps after the graphs, then graphs again
+    orc.step(3)
+    sim.run(20)
+    orc.step(20)
+    assert_state_equal(sim.state(), orc.state(), 272)
+    _check_traces_and_weights(sim, orc)
+    sim.close()
+
+
 @pytest.mark.parametrize("row_blocks,nontemporal", [([8, 8, 16, 8], 1), ([3, 5, 7, 1], 0), ([512, 512, 1, 2], 1)],
                          ids=["rb8-nt", "rb-odd", "rb-mixed-nt"])
 def test_gemv_tilings_bit_exact(tmp_path, row_blocks, nontemporal):
