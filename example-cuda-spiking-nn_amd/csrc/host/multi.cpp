@@ -1,7 +1,10 @@
 // Several independent plastic networks of any sizes advanced together: every network keeps its
 // own state, weights, plasticity and fused-step plans (exactly as a single bn_sim), and one
 // launch per phase (evolve, propagate, plasticity) runs that phase for all of them, ten steps
-// per hipGraph.  Each network therefore computes bit for bit what it computes alone.
+// per hipGraph.  Each network therefore computes bit for bit what it computes alone.  With
+// several streams, the networks are split into sets of about equal weight bytes stepped on
+// their own streams, so one set's latency-bound evolve and plasticity launches overlap another
+// set's bandwidth-bound propagate.
 // extern "C" bn_multi_* API of include/buryn_amd.h.
 #include <algorithm>
 #include <cstring>
@@ -21,6 +24,7 @@ struct bn_multi {
         hipGraphExec_t exec = nullptr;
         uint64_t bytes = 0;
         uint32_t networks = 0;
+        hipStream_t stream = nullptr;    // the main stream, or a lane of its own
     };
     std::vector<bn_sim*> sims;
     hipStream_t stream = nullptr;
@@ -30,6 +34,10 @@ struct bn_multi {
     std::vector<Group> groups;       // the schedule: network groups, each stepped by its own graph
     uint32_t steps_per_graph = 10;
     uint64_t group_bytes = 0;        // weight bytes per group (0 = one group)
+    uint32_t num_streams = 1;        // > 1: concurrent network sets, one stream each
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> lane_events;
+    hipEvent_t main_event = nullptr;
     uint32_t clock = 0;
     const EvolvePlan* d_ev = nullptr;
     const GemvPlan* d_gm = nullptr;
@@ -40,6 +48,9 @@ struct bn_multi {
     ~bn_multi() {
         clear_groups();
         if (stream) (void)hipStreamSynchronize(stream);
+        for (hipStream_t l : lanes) (void)hipStreamDestroy(l);
+        for (hipEvent_t e : lane_events) (void)hipEventDestroy(e);
+        if (main_event) (void)hipEventDestroy(main_event);
         for (void* p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
         for (bn_sim* s : sims) bn_sim_destroy(s);
@@ -114,10 +125,11 @@ struct bn_multi {
         std::vector<uint32_t> all(sims.size());
         for (uint32_t n = 0; n < all.size(); ++n) all[n] = n;
         step = make_step(all, allocs);
-        schedule(0, 10);
+        schedule(0, 10, 1);
     }
 
     void clear_groups() {
+        join();
         for (Group& g : groups) {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             if (g.graph) (void)hipGraphDestroy(g.graph);
@@ -132,42 +144,87 @@ struct bn_multi {
     // groups of networks whose weights fit `bytes` (0 = all in one group) each advance
     // `spg` steps per graph launch, one group after the other, so a group's weights can stay
     // in the MALL across its steps.
-    void schedule(uint64_t bytes, uint32_t spg) {
+    void schedule(uint64_t bytes, uint32_t spg, uint32_t nstreams) {
         if (spg == 0) throw Error(BN_ERR_INVALID_ARGUMENT, "steps per graph must be positive");
+        if (nstreams == 0 || nstreams > 4) throw Error(BN_ERR_INVALID_ARGUMENT, "streams must be 1 to 4");
         clear_groups();
-        group_bytes = bytes;
+        group_bytes = nstreams > 1 ? 0 : bytes;
         steps_per_graph = spg;
+        num_streams = std::min<uint32_t>(nstreams, (uint32_t)sims.size());
         std::vector<std::vector<uint32_t>> sets(1);
-        uint64_t acc = 0;
-        for (uint32_t n = 0; n < sims.size(); ++n) {
-            if (bytes && !sets.back().empty() && acc + w_bytes[n] > bytes) {
-                sets.emplace_back();
-                acc = 0;
+        if (num_streams > 1) {
+            // Heaviest networks first, each to the lightest set; each set keeps network order.
+            sets.assign(num_streams, {});
+            std::vector<uint64_t> load(num_streams, 0);
+            std::vector<uint32_t> order(sims.size());
+            for (uint32_t n = 0; n < order.size(); ++n) order[n] = n;
+            std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return w_bytes[a] > w_bytes[b]; });
+            for (uint32_t n : order) {
+                const size_t k = std::min_element(load.begin(), load.end()) - load.begin();
+                sets[k].push_back(n);
+                load[k] += w_bytes[n];
             }
-            sets.back().push_back(n);
-            acc += w_bytes[n];
+            for (auto& set : sets) std::sort(set.begin(), set.end());
+            while (lanes.size() < num_streams) {
+                hipStream_t l = nullptr;
+                BN_HIP_CHECK(hipStreamCreateWithFlags(&l, hipStreamNonBlocking), "hipStreamCreate");
+                lanes.push_back(l);
+                hipEvent_t e = nullptr;
+                BN_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+                lane_events.push_back(e);
+            }
+        } else {
+            uint64_t acc = 0;
+            for (uint32_t n = 0; n < sims.size(); ++n) {
+                if (bytes && !sets.back().empty() && acc + w_bytes[n] > bytes) {
+                    sets.emplace_back();
+                    acc = 0;
+                }
+                sets.back().push_back(n);
+                acc += w_bytes[n];
+            }
         }
-        for (const auto& set : sets) {
+        for (size_t i = 0; i < sets.size(); ++i) {
+            const auto& set = sets[i];
             Group g;
             g.step = make_step(set, group_allocs);
             g.networks = (uint32_t)set.size();
+            g.stream = num_streams > 1 ? lanes[i] : stream;
             for (uint32_t n : set) g.bytes += w_bytes[n];
-            BN_HIP_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            BN_HIP_CHECK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             for (uint32_t k = 0; k < steps_per_graph; ++k)
-                BN_HIP_CHECK((hipError_t)launch_multi_step(g.step, stream), "multi-network step launch");
-            BN_HIP_CHECK(hipStreamEndCapture(stream, &g.graph), "hipStreamEndCapture");
+                BN_HIP_CHECK((hipError_t)launch_multi_step(g.step, g.stream), "multi-network step launch");
+            BN_HIP_CHECK(hipStreamEndCapture(g.stream, &g.graph), "hipStreamEndCapture");
             BN_HIP_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
             groups.push_back(g);
         }
     }
 
+    // Lanes start after the work already on the main stream, and the main stream continues
+    // after the lanes' work, so everything else (timing events, downloads) orders as before.
+    void fork() {
+        if (num_streams <= 1) return;
+        if (!main_event) BN_HIP_CHECK(hipEventCreateWithFlags(&main_event, hipEventDisableTiming), "hipEventCreate");
+        BN_HIP_CHECK(hipEventRecord(main_event, stream), "hipEventRecord");
+        for (size_t i = 0; i < num_streams; ++i) BN_HIP_CHECK(hipStreamWaitEvent(lanes[i], main_event, 0), "hipStreamWaitEvent");
+    }
+    void join() {
+        if (num_streams <= 1 || lanes.empty()) return;
+        for (size_t i = 0; i < num_streams; ++i) {
+            BN_HIP_CHECK(hipEventRecord(lane_events[i], lanes[i]), "hipEventRecord");
+            BN_HIP_CHECK(hipStreamWaitEvent(stream, lane_events[i], 0), "hipStreamWaitEvent");
+        }
+    }
+
     void run(uint32_t n) {
+        fork();
         uint32_t k = 0;
         for (; k + steps_per_graph <= n; k += steps_per_graph)
-            for (Group& g : groups) BN_HIP_CHECK(hipGraphLaunch(g.exec, stream), "hipGraphLaunch");
+            for (Group& g : groups) BN_HIP_CHECK(hipGraphLaunch(g.exec, g.stream), "hipGraphLaunch");
         for (Group& g : groups)
             for (uint32_t r = k; r < n; ++r)
-                BN_HIP_CHECK((hipError_t)launch_multi_step(g.step, stream), "multi-network step launch");
+                BN_HIP_CHECK((hipError_t)launch_multi_step(g.step, g.stream), "multi-network step launch");
+        join();
         for (bn_sim* s : sims) s->system->note_external_steps(n);
         clock += n;
     }
@@ -265,7 +322,12 @@ int bn_multi_time_propagate(bn_multi* m, uint32_t num_launches, float* ms) {
 
 int bn_multi_set_schedule(bn_multi* m, uint64_t group_weight_bytes, uint32_t steps_per_graph) {
     if (!m) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_multi_set_schedule: null handle");
-    return guarded([&] { m->schedule(group_weight_bytes, steps_per_graph); });
+    return guarded([&] { m->schedule(group_weight_bytes, steps_per_graph, 1); });
+}
+
+int bn_multi_set_streams(bn_multi* m, uint32_t streams, uint32_t steps_per_graph) {
+    if (!m) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_multi_set_streams: null handle");
+    return guarded([&] { m->schedule(0, steps_per_graph, streams); });
 }
 
 int bn_multi_synchronize(bn_multi* m) {
@@ -295,6 +357,7 @@ int bn_multi_get_info(bn_multi* m, bn_multi_info* info) {
     info->nontemporal = m->step.nontemporal;
     info->kernels_per_step = 3 * (uint32_t)m->groups.size();
     info->num_groups = (uint32_t)m->groups.size();
+    info->num_streams = m->num_streams;
     info->steps_per_graph = m->steps_per_graph;
     info->group_weight_bytes = m->group_bytes;
     return BN_OK;

@@ -70,7 +70,8 @@ class BatchInstance(C.Structure):
 class MultiInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("num_networks", "clock_step", "evolve_blocks", "propagate_blocks",
                                           "plasticity_blocks", "nontemporal", "kernels_per_step", "num_groups",
-                                          "steps_per_graph")] + [("group_weight_bytes", C.c_uint64)]
+                                          "steps_per_graph")] + [("group_weight_bytes", C.c_uint64),
+                                                                 ("num_streams", C.c_uint32)]
 
 
 class BatchInfo(C.Structure):
@@ -105,7 +106,7 @@ EXPORTED_SYMBOLS = [
     "bn_batch_time_gemm", "bn_batch_set_gemm_rows", "bn_batch_get_info", "bn_batch_get_state",
     "bn_batch_workload_instance", "bn_multi_create", "bn_multi_destroy", "bn_multi_run", "bn_multi_run_timed",
     "bn_multi_synchronize", "bn_multi_network", "bn_multi_get_info", "bn_multi_time_propagate",
-    "bn_multi_set_schedule", "bn_window_create", "bn_window_destroy", "bn_window_set_params",
+    "bn_multi_set_schedule", "bn_multi_set_streams", "bn_window_create", "bn_window_destroy", "bn_window_set_params",
     "bn_window_get_weights", "bn_window_set_targets", "bn_window_forward", "bn_window_backward",
     "bn_window_get_info", "bn_window_get_gradient",
 ]
@@ -167,6 +168,7 @@ def lib() -> C.CDLL:
     L.bn_multi_get_info.argtypes = [P, C.POINTER(MultiInfo)]
     L.bn_multi_time_propagate.argtypes = [P, U, fp]
     L.bn_multi_set_schedule.argtypes = [P, C.c_uint64, U]
+    L.bn_multi_set_streams.argtypes = [P, U, U]
     dp = C.POINTER(C.c_double)
     L.bn_window_create.argtypes = [P, C.POINTER(WindowSpec), C.POINTER(P)]
     L.bn_window_destroy.argtypes = [P]
@@ -442,6 +444,9 @@ class MultiSimulation:
 
     def set_schedule(self, group_weight_bytes: int, steps_per_graph: int = 10) -> None:
         _check(lib().bn_multi_set_schedule(self.h, group_weight_bytes, steps_per_graph))
+
+    def set_streams(self, streams: int, steps_per_graph: int = 10) -> None:
+        _check(lib().bn_multi_set_streams(self.h, streams, steps_per_graph))
 
     def network(self, k: int) -> "Simulation":
         sh = C.c_void_p()

@@ -97,20 +97,31 @@ def autotune_row_blocks(engine, sizes, workdir: str, candidates=(32, 64, 128, 25
     return out
 
 
-def autotune_schedule(multi, group_mib=(0, 256, 192, 128, 96, 64), steps_per_graph=(10, 50), steps: int = 200) -> dict:
+def autotune_schedule(multi, group_mib=(0, 256, 192, 128, 96, 64), steps_per_graph=(10, 50), streams=(2, 3, 4),
+                      steps: int = 200) -> dict:
     """The batched plastic step's schedule autotuner: time `steps` steps of every network under
-    each (weight MiB per network group, steps per graph) candidate and keep the fastest on
-    `multi`.  Results do not depend on the schedule (the networks are independent)."""
+    each candidate — (weight MiB per network group stepped one after the other, steps per
+    graph), or (concurrent network sets on their own streams, steps per graph) — and keep the
+    fastest on `multi`.  Results do not depend on the schedule (the networks are independent)."""
     times = {}
-    for mib in group_mib:
-        for spg in steps_per_graph:
+    for spg in steps_per_graph:
+        for mib in group_mib:
             multi.set_schedule(mib << 20, spg)
             multi.run_timed(spg)
-            times[(mib, spg)] = multi.run_timed(steps) * 1000.0 / steps
+            times[("groups", mib, spg)] = multi.run_timed(steps) * 1000.0 / steps
+        for ns in streams:
+            multi.set_streams(ns, spg)
+            multi.run_timed(spg)
+            times[("streams", ns, spg)] = multi.run_timed(steps) * 1000.0 / steps
     best = min(times, key=times.get)
-    multi.set_schedule(best[0] << 20, best[1])
-    return {"group_mib": best[0], "steps_per_graph": best[1],
-            "us_per_step": {f"{m}MiB/{k}": round(v, 2) for (m, k), v in times.items()}}
+    if best[0] == "groups":
+        multi.set_schedule(best[1] << 20, best[2])
+    else:
+        multi.set_streams(best[1], best[2])
+    label = lambda k: f"{k[1]}MiB/{k[2]}" if k[0] == "groups" else f"{k[1]}streams/{k[2]}"  # noqa: E731
+    return {"kind": best[0], "group_mib": best[1] if best[0] == "groups" else 0,
+            "streams": best[1] if best[0] == "streams" else 1, "steps_per_graph": best[2],
+            "us_per_step": {label(k): round(v, 2) for k, v in times.items()}}
 
 
 def adjoint_config(base: dict | None = None) -> dict:
