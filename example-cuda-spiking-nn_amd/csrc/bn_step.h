@@ -193,14 +193,26 @@ struct MultiPhase {
     const uint32_t* net_first_block;
     uint32_t blocks;
 };
+// Per-network plans of the overlapped graph form (device memory; null: not prepared).
+struct MultiOverlap {
+    const EvolvePlan* evolve_even;     // even steps of a graph: current -> staging buffers
+    const EvolvePlan* evolve_odd;      // odd steps: staging -> current
+    const PlastStepPlan* plast_staging;  // plasticity of an even step
+    const PlastStepPlan* plast_current;  // of an odd step
+    const PlastStepPlan* plast_final;    // of the last step, with the stimulus/clock tail
+};
 struct MultiStep {
     const EvolvePlan* evolve;
     const GemvPlan* gemv;
     const PlastStepPlan* plast;
     MultiPhase ph_evolve, ph_gemv, ph_plast;
     uint32_t nontemporal;
+    MultiOverlap ov;
 };
 int launch_multi_step(const MultiStep& m, bn_stream_t s);
+// n (even) overlapped steps of every network of m (the graph form of a single network's
+// overlapped steps, with the networks' launches merged per phase).
+int launch_multi_overlapped(const MultiStep& m, uint32_t n, bn_stream_t s);
 int launch_multi_propagate(const MultiStep& m, bn_stream_t s);
 int launch_gemv(const GemvPlan& plan, bn_stream_t s);
 // Overlapped step: evolve of step t with the plasticity of step t-1 (plast = nullptr: evolve

@@ -405,6 +405,14 @@ public:
     bool has_norm_tail() const { return tail_.norm_rows != 0; }
     // Captured graphs overlap the plasticity of step t-1 with the evolve of step t.
     bool overlapped() const { return overlap_; }
+    // The plans of the overlapped form for graphs of `steps_per_graph` (even) steps, built on
+    // first use (also without a graph of this system's own, for the multi-network step).
+    void prepare_overlap(uint32_t steps_per_graph);
+    const EvolvePlan& evolve_even() const { return evolve_even_; }
+    const EvolvePlan& evolve_odd() const { return evolve_odd_; }
+    const PlastStepPlan& plast_staging() const { return plast_staging_; }
+    const PlastStepPlan& plast_current() const { return plast_current_; }
+    const PlastStepPlan& plast_final() const { return plast_final_; }
     // Steps run by an external launcher on this system's plans (several networks in one launch).
     void note_external_steps(uint32_t n) { clock_step_ += n; }
     uint32_t kernels_per_step() const { return kernels_per_step_; }
@@ -455,6 +463,7 @@ private:
     EvolvePlan evolve_even_{}, evolve_odd_{};
     PlastStepPlan plast_staging_{}, plast_current_{}, plast_final_{};
     uint32_t* d_ev_start_ = nullptr;
+    bool overlap_built_ = false;
     std::vector<void*> staging_;
 };
 

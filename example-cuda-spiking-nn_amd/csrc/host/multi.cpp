@@ -17,6 +17,10 @@
 
 using namespace bn;
 
+namespace {
+constexpr uint32_t kMultiSpikingBlocks = 16;  // blocks per network for its spiking rows
+}  // namespace
+
 struct bn_multi {
     struct Group {
         MultiStep step{};
@@ -35,6 +39,8 @@ struct bn_multi {
     uint32_t steps_per_graph = 10;
     uint64_t group_bytes = 0;        // weight bytes per group (0 = one group)
     uint32_t num_streams = 1;        // > 1: concurrent network sets, one stream each
+    bool overlap_enabled = false;    // graphs overlap plasticity(t-1) with evolve(t)
+    bool overlapped = false;         // the current graphs do
     std::vector<hipStream_t> lanes;
     std::vector<hipEvent_t> lane_events;
     hipEvent_t main_event = nullptr;
@@ -42,6 +48,9 @@ struct bn_multi {
     const EvolvePlan* d_ev = nullptr;
     const GemvPlan* d_gm = nullptr;
     const PlastStepPlan* d_pl = nullptr;
+    MultiOverlap ov{};                // overlapped form (graphs of an even number of steps)
+    std::vector<PlastStepPlan> pl_final;
+    PlastStepPlan* d_pl_final = nullptr;
     std::vector<uint32_t> nb_ev, nb_gm, nb_pl;
     std::vector<uint64_t> w_bytes;
 
@@ -93,14 +102,15 @@ struct bn_multi {
         for (uint32_t n : nets) bytes += w_bytes[n];
         // Weight sets well above the MALL stream past the caches; smaller ones stay resident.
         st.nontemporal = bytes > (192ull << 20) ? 1u : 0u;
+        st.ov = ov;
         return st;
     }
 
     void build() {
         BN_HIP_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-        std::vector<EvolvePlan> ev;
+        std::vector<EvolvePlan> ev, ev_even, ev_odd;
         std::vector<GemvPlan> gm;
-        std::vector<PlastStepPlan> pl;
+        std::vector<PlastStepPlan> pl, pl_stage, pl_cur;
         for (bn_sim* s : sims) {
             System& sys = *s->system;
             if (!sys.fused()) throw Error(BN_ERR_RUNTIME, "multi-network step needs the fused step of every network");
@@ -110,10 +120,23 @@ struct bn_multi {
             BN_HIP_CHECK(hipStreamSynchronize(s->state->stream()), "network setup sync");
             ev.push_back(sys.evolve_plan());
             gm.push_back(sys.gemv_plan());
-            pl.push_back(sys.plast_step_plan());
+            // Fewer blocks share a network's spiking rows than when it runs alone (a network has
+            // a few spiking rows per step and the launch holds every network's blocks); which
+            // block takes a row does not change the row's result.
+            auto fewer = [](PlastStepPlan p) {
+                p.spiking_blocks = std::min<uint32_t>(p.spiking_blocks, kMultiSpikingBlocks);
+                return p;
+            };
+            pl.push_back(fewer(sys.plast_step_plan()));
+            sys.prepare_overlap(2);
+            ev_even.push_back(sys.evolve_even());
+            ev_odd.push_back(sys.evolve_odd());
+            pl_stage.push_back(fewer(sys.plast_staging()));
+            pl_cur.push_back(fewer(sys.plast_current()));
+            pl_final.push_back(fewer(sys.plast_final()));
             nb_ev.push_back((sys.evolve_plan().total_neurons + kEvolveNeurons - 1) / kEvolveNeurons);
             nb_gm.push_back((sys.gemv_plan().total_tiles + 3) / 4);
-            nb_pl.push_back(sys.plast_step_plan().row_blocks + sys.plast_step_plan().spiking_blocks);
+            nb_pl.push_back(pl.back().row_blocks + pl.back().spiking_blocks);
             uint64_t wb = 0;
             for (uint32_t c = 0; c < sys.gemv_plan().n_conn; ++c)
                 wb += 4ull * sys.gemv_plan().c[c].n_pre * sys.gemv_plan().c[c].ld;
@@ -122,6 +145,12 @@ struct bn_multi {
         d_ev = upload(ev, "evolve plans", allocs);
         d_gm = upload(gm, "propagate plans", allocs);
         d_pl = upload(pl, "plasticity plans", allocs);
+        ov.evolve_even = upload(ev_even, "evolve plans", allocs);
+        ov.evolve_odd = upload(ev_odd, "evolve plans", allocs);
+        ov.plast_staging = upload(pl_stage, "plasticity plans", allocs);
+        ov.plast_current = upload(pl_cur, "plasticity plans", allocs);
+        d_pl_final = upload(pl_final, "plasticity plans", allocs);
+        ov.plast_final = d_pl_final;
         std::vector<uint32_t> all(sims.size());
         for (uint32_t n = 0; n < all.size(); ++n) all[n] = n;
         step = make_step(all, allocs);
@@ -184,6 +213,16 @@ struct bn_multi {
                 acc += w_bytes[n];
             }
         }
+        // The last plasticity of an overlapped graph advances the clock by the graph's steps.
+        overlapped = overlap_enabled && spg % 2 == 0;
+        if (overlapped) {
+            for (PlastStepPlan& p : pl_final) {
+                p.tail.step_offset = spg - 1;
+                p.tail.advance = spg;
+            }
+            BN_HIP_CHECK(hipMemcpy(d_pl_final, pl_final.data(), pl_final.size() * sizeof(PlastStepPlan),
+                                   hipMemcpyHostToDevice), "final plasticity plans");
+        }
         for (size_t i = 0; i < sets.size(); ++i) {
             const auto& set = sets[i];
             Group g;
@@ -192,8 +231,12 @@ struct bn_multi {
             g.stream = num_streams > 1 ? lanes[i] : stream;
             for (uint32_t n : set) g.bytes += w_bytes[n];
             BN_HIP_CHECK(hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            for (uint32_t k = 0; k < steps_per_graph; ++k)
-                BN_HIP_CHECK((hipError_t)launch_multi_step(g.step, g.stream), "multi-network step launch");
+            if (overlapped)
+                BN_HIP_CHECK((hipError_t)launch_multi_overlapped(g.step, steps_per_graph, g.stream),
+                             "multi-network overlapped steps launch");
+            else
+                for (uint32_t k = 0; k < steps_per_graph; ++k)
+                    BN_HIP_CHECK((hipError_t)launch_multi_step(g.step, g.stream), "multi-network step launch");
             BN_HIP_CHECK(hipStreamEndCapture(g.stream, &g.graph), "hipStreamEndCapture");
             BN_HIP_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
             groups.push_back(g);
@@ -322,12 +365,20 @@ int bn_multi_time_propagate(bn_multi* m, uint32_t num_launches, float* ms) {
 
 int bn_multi_set_schedule(bn_multi* m, uint64_t group_weight_bytes, uint32_t steps_per_graph) {
     if (!m) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_multi_set_schedule: null handle");
-    return guarded([&] { m->schedule(group_weight_bytes, steps_per_graph, 1); });
+    return guarded([&] {
+        m->overlap_enabled = false;
+        m->schedule(group_weight_bytes, steps_per_graph, 1);
+    });
 }
 
-int bn_multi_set_streams(bn_multi* m, uint32_t streams, uint32_t steps_per_graph) {
+int bn_multi_set_streams(bn_multi* m, uint32_t streams, uint32_t steps_per_graph, uint32_t overlap) {
     if (!m) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_multi_set_streams: null handle");
-    return guarded([&] { m->schedule(0, steps_per_graph, streams); });
+    if (overlap && steps_per_graph % 2)
+        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_multi_set_streams: overlapped graphs need an even step count");
+    return guarded([&] {
+        m->overlap_enabled = overlap != 0;
+        m->schedule(0, steps_per_graph, streams);
+    });
 }
 
 int bn_multi_synchronize(bn_multi* m) {
@@ -355,9 +406,10 @@ int bn_multi_get_info(bn_multi* m, bn_multi_info* info) {
     info->propagate_blocks = m->step.ph_gemv.blocks;
     info->plasticity_blocks = m->step.ph_plast.blocks;
     info->nontemporal = m->step.nontemporal;
-    info->kernels_per_step = 3 * (uint32_t)m->groups.size();
+    info->kernels_per_step = (m->overlapped ? 2u : 3u) * (uint32_t)m->groups.size();
     info->num_groups = (uint32_t)m->groups.size();
     info->num_streams = m->num_streams;
+    info->overlapped = m->overlapped ? 1u : 0u;
     info->steps_per_graph = m->steps_per_graph;
     info->group_weight_bytes = m->group_bytes;
     return BN_OK;

@@ -262,8 +262,18 @@ void System::build_plans() {
     if (lazy_idx >= 0) BN_HIP_CHECK((hipError_t)launch_lazy_init(plast_.p[lazy_idx], S.stream()), "lazy normalisation init");
 }
 
+void System::prepare_overlap(uint32_t steps_per_graph) {
+    if (!fused_ || tail_.norm_rows || steps_per_graph == 0 || steps_per_graph % 2)
+        throw Error(BN_ERR_INVALID_ARGUMENT, "the overlapped step needs the fused step, lazy or no normalisation "
+                                             "and an even number of steps per graph");
+    if (!overlap_built_) build_overlap_plans();
+    plast_final_.tail.step_offset = steps_per_graph - 1;
+    plast_final_.tail.advance = steps_per_graph;
+}
+
 void System::build_overlap_plans() {
     hipStream_t s = state_.stream();
+    overlap_built_ = true;
     evolve_even_ = evolve_odd_ = evolve_;
     plast_staging_ = plast_current_ = plast_step_;
     auto alloc = [&](size_t bytes) {
@@ -300,8 +310,6 @@ void System::build_overlap_plans() {
     plast_final_ = plast_current_;
     plast_final_.tail = plast_step_.tail;
     plast_final_.tail.active = 1;
-    plast_final_.tail.step_offset = steps_per_graph_ - 1;
-    plast_final_.tail.advance = steps_per_graph_;
     const StimTail& st = plast_step_.tail;
     if (st.timestep_event_count && st.num_timesteps) {
         std::vector<uint32_t> counts(st.num_timesteps), start(st.num_timesteps + 1, 0);
@@ -359,7 +367,7 @@ void System::enable_fused(bool use_graph, uint32_t steps_per_graph) {
     steps_per_graph_ = std::max<uint32_t>(1, steps_per_graph);
     overlap_ = use_graph_ && state_.options().overlap_steps && tail_.norm_rows == 0 && steps_per_graph_ % 2 == 0;
     if (overlap_) {
-        build_overlap_plans();
+        prepare_overlap(steps_per_graph_);
         kernels_per_step_ = 2;
     }
     if (use_graph_) {

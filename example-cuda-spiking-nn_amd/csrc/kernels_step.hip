@@ -103,7 +103,8 @@ __device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* t
     return poisson_from_table(u, table + 4, len - 4) + 4;
 }
 
-__device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) {
+__device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk, uint32_t step_offset,
+                                             uint32_t stim_prev) {
     __shared__ FinParts fin;
     __shared__ float bgl[2][kEvolveNeurons];
     const uint32_t wave = threadIdx.x >> 6;
@@ -111,7 +112,7 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) 
     const uint32_t gid = blk * kEvolveNeurons + lane;
     const bool live = gid < P.total_neurons;
     const bn_device_clock clk = *P.clock;
-    const uint32_t step = clk.clock_step + P.step_offset;
+    const uint32_t step = clk.clock_step + step_offset;
 
     // Wave 0: per-neuron inputs of the EIF step and the connection traces.
     float V = 0.0f, ge = 0.0f, gi_ = 0.0f, istim = 0.0f;
@@ -130,7 +131,7 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) 
             ge = v.g_excitatory[lid];
             gi_ = v.g_inhibitory[lid];
             istim = v.i_stimulus[lid];
-            if (P.stim_prev && step - 1 < P.ev_steps) {
+            if (stim_prev && step - 1 < P.ev_steps) {
                 // stimulus.cpp:46-60 for this neuron: the previous step's events in list order.
                 const uint32_t e1 = min(P.ev_start[step], P.ev_count);
                 bool hit = false;
@@ -291,14 +292,16 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk) 
     if (vslot >= 0) P.v_ring[(size_t)(step % P.interval) * P.n_vrec + vslot] = V;
 }
 
-__global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) { evolve_block(P, blockIdx.x); }
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
+    evolve_block(P, blockIdx.x, P.step_offset, P.stim_prev);
+}
 
 // Several networks in one launch: block -> (network, block of that network) from the tables.
 __global__ __launch_bounds__(kFinParts * 64) void k_evolve_multi(const EvolvePlan* __restrict__ plans,
                                                                  const uint32_t* __restrict__ block_net,
                                                                  const uint32_t* __restrict__ net_first_block) {
     const uint32_t net = __builtin_amdgcn_readfirstlane(block_net[blockIdx.x]);
-    evolve_block(plans[net], blockIdx.x - net_first_block[net]);
+    evolve_block(plans[net], blockIdx.x - net_first_block[net], 0, 0);
 }
 
 // =======================================================================================
@@ -778,8 +781,24 @@ __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) { plasticit
 // Overlapped step: blocks [0, n_evolve) evolve step t, the rest run the plasticity of step t-1
 // (its 256-thread body; the upper four waves take part in its barriers only).
 __global__ __launch_bounds__(kFinParts * 64) void k_evolve_plast(EvolvePlan E, PlastStepPlan P, uint32_t n_evolve) {
-    if (blockIdx.x < n_evolve) evolve_block(E, blockIdx.x);
+    if (blockIdx.x < n_evolve) evolve_block(E, blockIdx.x, E.step_offset, E.stim_prev);
     else plasticity_block(P, blockIdx.x - n_evolve, gridDim.x - n_evolve);
+}
+
+// The same for several networks: evolve blocks of every network (step offset and previous-step
+// stimulus of the launch), then the plasticity blocks of every network (plast = nullptr: none).
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve_plast_multi(const EvolvePlan* __restrict__ ev, MultiPhase pe,
+                                                                       const PlastStepPlan* __restrict__ pl, MultiPhase pp,
+                                                                       uint32_t step_offset, uint32_t stim_prev) {
+    if (blockIdx.x < pe.blocks) {
+        const uint32_t net = __builtin_amdgcn_readfirstlane(pe.block_net[blockIdx.x]);
+        evolve_block(ev[net], blockIdx.x - pe.net_first_block[net], step_offset, stim_prev);
+    } else {
+        const uint32_t b = blockIdx.x - pe.blocks;
+        const uint32_t net = __builtin_amdgcn_readfirstlane(pp.block_net[b]);
+        const PlastStepPlan& P = pl[net];
+        plasticity_block(P, b - pp.net_first_block[net], P.row_blocks + P.spiking_blocks);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_plasticity_multi(const PlastStepPlan* __restrict__ plans,
@@ -1100,6 +1119,30 @@ int launch_multi_step(const MultiStep& m, bn_stream_t s) {
     }
     if (m.ph_plast.blocks)
         hipLaunchKernelGGL(k_plasticity_multi, dim3(m.ph_plast.blocks), dim3(256), 0, s, m.plast,
+                           m.ph_plast.block_net, m.ph_plast.net_first_block);
+    return (int)hipGetLastError();
+}
+
+int launch_multi_overlapped(const MultiStep& m, uint32_t n, bn_stream_t s) {
+    if (!m.ov.evolve_even || n % 2) return (int)hipErrorInvalidValue;
+    for (uint32_t k = 0; k < n; ++k) {
+        const EvolvePlan* ev = k % 2 == 0 ? m.ov.evolve_even : m.ov.evolve_odd;
+        const PlastStepPlan* pl = k == 0 ? nullptr : (k - 1) % 2 == 0 ? m.ov.plast_staging : m.ov.plast_current;
+        const uint32_t blocks = m.ph_evolve.blocks + (pl ? m.ph_plast.blocks : 0);
+        if (blocks)
+            hipLaunchKernelGGL(k_evolve_plast_multi, dim3(blocks), dim3(kFinParts * 64), 0, s, ev, m.ph_evolve, pl,
+                               m.ph_plast, k, k > 0 ? 1u : 0u);
+        if (m.ph_gemv.blocks) {
+            if (m.nontemporal)
+                hipLaunchKernelGGL(k_gemv_multi<true>, dim3(m.ph_gemv.blocks), dim3(256), 0, s, m.gemv,
+                                   m.ph_gemv.block_net, m.ph_gemv.net_first_block);
+            else
+                hipLaunchKernelGGL(k_gemv_multi<false>, dim3(m.ph_gemv.blocks), dim3(256), 0, s, m.gemv,
+                                   m.ph_gemv.block_net, m.ph_gemv.net_first_block);
+        }
+    }
+    if (m.ph_plast.blocks)
+        hipLaunchKernelGGL(k_plasticity_multi, dim3(m.ph_plast.blocks), dim3(256), 0, s, m.ov.plast_final,
                            m.ph_plast.block_net, m.ph_plast.net_first_block);
     return (int)hipGetLastError();
 }

@@ -71,7 +71,8 @@ class MultiInfo(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("num_networks", "clock_step", "evolve_blocks", "propagate_blocks",
                                           "plasticity_blocks", "nontemporal", "kernels_per_step", "num_groups",
                                           "steps_per_graph")] + [("group_weight_bytes", C.c_uint64),
-                                                                 ("num_streams", C.c_uint32)]
+                                                                 ("num_streams", C.c_uint32),
+                                                                 ("overlapped", C.c_uint32)]
 
 
 class BatchInfo(C.Structure):
@@ -168,7 +169,7 @@ def lib() -> C.CDLL:
     L.bn_multi_get_info.argtypes = [P, C.POINTER(MultiInfo)]
     L.bn_multi_time_propagate.argtypes = [P, U, fp]
     L.bn_multi_set_schedule.argtypes = [P, C.c_uint64, U]
-    L.bn_multi_set_streams.argtypes = [P, U, U]
+    L.bn_multi_set_streams.argtypes = [P, U, U, U]
     dp = C.POINTER(C.c_double)
     L.bn_window_create.argtypes = [P, C.POINTER(WindowSpec), C.POINTER(P)]
     L.bn_window_destroy.argtypes = [P]
@@ -445,8 +446,8 @@ class MultiSimulation:
     def set_schedule(self, group_weight_bytes: int, steps_per_graph: int = 10) -> None:
         _check(lib().bn_multi_set_schedule(self.h, group_weight_bytes, steps_per_graph))
 
-    def set_streams(self, streams: int, steps_per_graph: int = 10) -> None:
-        _check(lib().bn_multi_set_streams(self.h, streams, steps_per_graph))
+    def set_streams(self, streams: int, steps_per_graph: int = 10, overlap: bool = False) -> None:
+        _check(lib().bn_multi_set_streams(self.h, streams, steps_per_graph, 1 if overlap else 0))
 
     def network(self, k: int) -> "Simulation":
         sh = C.c_void_p()

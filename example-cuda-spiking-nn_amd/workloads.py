@@ -97,30 +97,39 @@ def autotune_row_blocks(engine, sizes, workdir: str, candidates=(32, 64, 128, 25
     return out
 
 
-def autotune_schedule(multi, group_mib=(0, 256, 192, 128, 96, 64), steps_per_graph=(10, 50), streams=(2, 3, 4),
-                      steps: int = 200) -> dict:
+def autotune_schedule(multi, group_mib=(0, 256, 128), steps_per_graph=(10, 50), streams=(1, 2, 3, 4),
+                      overlap=(False, True), steps: int = 200) -> dict:
     """The batched plastic step's schedule autotuner: time `steps` steps of every network under
     each candidate — (weight MiB per network group stepped one after the other, steps per
-    graph), or (concurrent network sets on their own streams, steps per graph) — and keep the
-    fastest on `multi`.  Results do not depend on the schedule (the networks are independent)."""
+    graph), or (concurrent network sets on their own streams, steps per graph, plasticity of
+    step t-1 overlapped with the evolve of step t or not) — and keep the fastest on `multi`.
+    Results do not depend on the schedule (the networks are independent)."""
     times = {}
     for spg in steps_per_graph:
         for mib in group_mib:
+            if mib == 0:
+                continue  # the one-group schedule is streams = 1 below
             multi.set_schedule(mib << 20, spg)
             multi.run_timed(spg)
-            times[("groups", mib, spg)] = multi.run_timed(steps) * 1000.0 / steps
+            times[("groups", mib, spg, False)] = multi.run_timed(steps) * 1000.0 / steps
         for ns in streams:
-            multi.set_streams(ns, spg)
-            multi.run_timed(spg)
-            times[("streams", ns, spg)] = multi.run_timed(steps) * 1000.0 / steps
+            for ov in overlap:
+                multi.set_streams(ns, spg, ov)
+                multi.run_timed(spg)
+                times[("streams", ns, spg, ov)] = multi.run_timed(steps) * 1000.0 / steps
     best = min(times, key=times.get)
     if best[0] == "groups":
         multi.set_schedule(best[1] << 20, best[2])
     else:
-        multi.set_streams(best[1], best[2])
-    label = lambda k: f"{k[1]}MiB/{k[2]}" if k[0] == "groups" else f"{k[1]}streams/{k[2]}"  # noqa: E731
+        multi.set_streams(best[1], best[2], best[3])
+
+    def label(k):
+        if k[0] == "groups":
+            return f"{k[1]}MiB/{k[2]}"
+        return f"{k[1]}streams/{k[2]}" + ("/overlap" if k[3] else "")
+
     return {"kind": best[0], "group_mib": best[1] if best[0] == "groups" else 0,
-            "streams": best[1] if best[0] == "streams" else 1, "steps_per_graph": best[2],
+            "streams": best[1] if best[0] == "streams" else 1, "steps_per_graph": best[2], "overlap": best[3],
             "us_per_step": {label(k): round(v, 2) for k, v in times.items()}}
 
 

@@ -441,6 +441,7 @@ typedef struct bn_multi_info {
     uint32_t steps_per_graph;
     uint64_t group_weight_bytes;     /* weight bytes per group (0 = one group) */
     uint32_t num_streams;            /* concurrent network sets (bn_multi_set_streams) */
+    uint32_t overlapped;             /* graphs run the plasticity of step t-1 beside the evolve of t */
 } bn_multi_info;
 
 /* flags: BN_SIM_* of every network (BN_SIM_LAZY_NORMALIZE is required when a network
@@ -458,8 +459,10 @@ int bn_multi_synchronize(bn_multi* multi);
 int bn_multi_set_schedule(bn_multi* multi, uint64_t group_weight_bytes, uint32_t steps_per_graph);
 /* Concurrent schedule: the networks split into `streams` (1-4) sets of about equal weight bytes,
  * each set stepped by its own graphs on its own stream, so that one set's latency-bound evolve
- * and plasticity launches run beside another set's propagate.  Results do not change. */
-int bn_multi_set_streams(bn_multi* multi, uint32_t streams, uint32_t steps_per_graph);
+ * and plasticity launches run beside another set's propagate.  overlap = 1 (even
+ * steps_per_graph): each graph runs the plasticity of step t-1 in the launch of the evolve of
+ * step t, as a single network's graphs do.  Results do not change. */
+int bn_multi_set_streams(bn_multi* multi, uint32_t streams, uint32_t steps_per_graph, uint32_t overlap);
 /* Wall time of `num_launches` back-to-back propagate launches over every network; the state of
  * every network is unchanged (pending partial sums saved and restored). */
 int bn_multi_time_propagate(bn_multi* multi, uint32_t num_launches, float* ms);

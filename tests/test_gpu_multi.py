@@ -89,9 +89,10 @@ def test_multi_schedule_does_not_change_results(tmp_path):
     b.run(100)
     for k in range(3):
         _check_same(a.network(k), b.network(k), f"network {k}")
-    # Concurrent sets on two streams, then back to one group: still the same networks.
-    b.set_streams(2, 10)
-    assert b.get_info().num_streams == 2
+    # Concurrent sets on two streams (overlapped graphs), then back to one group: still the same
+    # networks.
+    b.set_streams(2, 10, overlap=True)
+    assert b.get_info().num_streams == 2 and b.get_info().overlapped == 1
     a.run(53)
     b.run(53)
     for k in range(3):
