@@ -156,7 +156,16 @@ def bench_batched(args) -> int:
         path = os.path.join(d, "batched.json")
         json.dump(base, open(path, "w"))
         batch = eng.BatchSimulation(path, insts)
-        info = batch.info
+        tuned_rows, times = None, {}
+        if not args.no_autotune:
+            # GEMM autotuner: rows per wave chunk (split-K depth) by back-to-back GEMM time.
+            for rows in (64, 96, 128, 152, 192, 256, 320):
+                batch.set_gemm_rows(rows)
+                batch.time_gemm(5)
+                times[rows] = batch.time_gemm(50) / 50 * 1000.0
+            tuned_rows = min(times, key=times.get)
+            batch.set_gemm_rows(tuned_rows)
+        info = batch.get_info()
         if args.warmup:
             batch.run_timed(args.warmup)
         ms = batch.run_timed(args.steps)
@@ -200,6 +209,7 @@ def bench_batched(args) -> int:
                                "with plasticity off, one weight set)",
                    "instances": args.instances, "padded_instances": B, "neurons_per_instance": int(info.total_neurons),
                    "gemm_rows_per_wave": int(info.gemm_rows_per_wave), "kernels_per_step": int(info.kernels_per_step),
+                   "gemm_autotune_us": {str(k): round(v, 2) for k, v in times.items()},
                    "parallelism": "single GPU"},
         "vs_sequential_plain": round(inst_steps_per_s / plain_inst_steps_per_s, 2),
         "sequential_plain_instance_steps_per_s": round(plain_inst_steps_per_s, 1),

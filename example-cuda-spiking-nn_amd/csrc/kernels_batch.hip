@@ -174,30 +174,26 @@ __global__ __launch_bounds__(kBatchGemmWaves * 64) void k_batch_gemm(BatchGemmPl
         const uint32_t i1 = min(i0 + C.kc, C.n_pre);
         const float* wp = C.W + ml;
         const float* xp = C.X + n0 + (lane & 31);
-        uint32_t i = i0;
-        for (; i + 8 <= i1; i += 8) {
-            float a[4], b0[4], b1[4];
+        // Batches of kGemmPairs row pairs with every load of the batch issued before its
+        // matrix-core operations: a wave waits one memory latency per batch, so the batch is
+        // long enough (2 x kGemmPairs operations of 64 cycles) to cover it with the few waves
+        // a SIMD holds.  Rows past the chunk load as zeros.
+        constexpr int kGemmPairs = 16;
+        for (uint32_t i = i0; i < i1; i += 2 * kGemmPairs) {
+            float a[kGemmPairs], b0[kGemmPairs], b1[kGemmPairs];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kGemmPairs; ++u) {
                 const uint32_t row = i + 2 * u + kl;
-                a[u] = m_ok ? wp[(size_t)row * C.ld] : 0.0f;
-                b0[u] = xp[(size_t)row * B];
-                b1[u] = xp[(size_t)row * B + 32];
+                const bool r_ok = row < i1;
+                a[u] = (m_ok && r_ok) ? wp[(size_t)row * C.ld] : 0.0f;
+                b0[u] = r_ok ? xp[(size_t)row * B] : 0.0f;
+                b1[u] = r_ok ? xp[(size_t)row * B + 32] : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kGemmPairs; ++u) {
                 acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b0[u], acc0, 0, 0, 0);
                 acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b1[u], acc1, 0, 0, 0);
             }
-        }
-        for (; i < i1; i += 2) {
-            const uint32_t row = i + kl;
-            const bool r_ok = row < i1;
-            const float a = (m_ok && r_ok) ? wp[(size_t)row * C.ld] : 0.0f;
-            const float b0 = r_ok ? xp[(size_t)row * B] : 0.0f;
-            const float b1 = r_ok ? xp[(size_t)row * B + 32] : 0.0f;
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b1, acc1, 0, 0, 0);
         }
     }
 #pragma unroll
