@@ -133,29 +133,53 @@ __global__ __launch_bounds__(256) void k_win_copy(WinCopyPlan P) {
     }
 }
 
-// Per neuron j at one step, given lambda x_j: the adjoint of its connection traces (as a pre
-// neuron), then of its EIF update (as a post neuron).
-__device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t j, double lx) {
+// What the reverse step of neuron j reads besides lambda x_j (loaded before the row product so
+// that their latency overlaps it).
+struct ReverseInputs {
+    float d0, r0, Vp, ge, gi, s;
+    uint32_t refr;
+    double carry_d, carry_r, carry_V, dv;
+};
+
+__device__ __forceinline__ ReverseInputs load_reverse_inputs(const WinReversePlan& P, uint32_t j) {
+    ReverseInputs in;
     const bool exc = j < P.NE;
     const uint32_t lid = exc ? j : j - P.NE;
     const size_t k = (size_t)P.slot * P.N + j;
-    // Traces: d_t = flush(d_{t-1} a_d) + K s_t, r likewise, x_t = d_t - r_t.
     const uint32_t n = exc ? P.NE : P.NI;
-    const float d0 = (exc ? P.rec.dE : P.rec.dI)[(size_t)P.slot * n + lid];
-    const float r0 = (exc ? P.rec.rE : P.rec.rI)[(size_t)P.slot * n + lid];
+    in.d0 = (exc ? P.rec.dE : P.rec.dI)[(size_t)P.slot * n + lid];
+    in.r0 = (exc ? P.rec.rE : P.rec.rI)[(size_t)P.slot * n + lid];
+    in.carry_d = P.carry_d[j];
+    in.carry_r = P.carry_r[j];
+    in.carry_V = P.carry_V[j];
+    const int32_t vs = P.vrec_slot[j];
+    in.dv = vs >= 0 ? P.dv_direct[vs] : 0.0;
+    in.refr = P.rec.refr[k];
+    in.Vp = P.rec.Vprev[k];
+    in.ge = P.rec.ge[k];
+    in.gi = P.rec.gi[k];
+    in.s = P.rec.s[k];
+    return in;
+}
+
+// Per neuron j at one step, given lambda x_j: the adjoint of its connection traces (as a pre
+// neuron), then of its EIF update (as a post neuron).
+__device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t j, double lx, const ReverseInputs& in) {
+    const bool exc = j < P.NE;
+    const size_t k = (size_t)P.slot * P.N + j;
+    // Traces: d_t = flush(d_{t-1} a_d) + K s_t, r likewise, x_t = d_t - r_t.
+    const float d0 = in.d0, r0 = in.r0;
     const float ad = exc ? P.prm->decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
-    const double ld = lx + P.carry_d[j];
-    const double lr = -lx + P.carry_r[j];
+    const double ld = lx + in.carry_d;
+    const double lr = -lx + in.carry_r;
     const bool nd = !(d0 * ad < kTraceFlush), nr = !(r0 * ar < kTraceFlush);
     if (exc && nd) P.acc_alpha[j] += ld * (double)d0;
     P.carry_d[j] = nd ? (double)ad * ld : 0.0;
     P.carry_r[j] = nr ? (double)ar * lr : 0.0;
     // Spike: traces and the smoothed population rate.
     const double ls = (double)P.K * (ld + lr) + (exc ? P.lrho[0] * P.rate_coef_e : P.lrho[1] * P.rate_coef_i);
-    double lv = P.carry_V[j];
-    const int32_t vs = P.vrec_slot[j];
-    if (vs >= 0) lv += P.dv_direct[vs];
-    if (P.rec.refr[k]) {
+    const double lv = in.carry_V + in.dv;
+    if (in.refr) {
         P.carry_V[j] = 0.0;
         P.lg_out_e[j] = 0.0;
         P.lg_out_i[j] = 0.0;
@@ -164,7 +188,7 @@ __device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t
         return;
     }
     const NeuronGroupView& v = P.g[exc ? 0 : 1].v;
-    const double Vp = P.rec.Vprev[k], ge = P.rec.ge[k], gi = P.rec.gi[k], s = P.rec.s[k];
+    const double Vp = in.Vp, ge = in.ge, gi = in.gi, s = in.s;
     // U of the step in FP64 from its recorded FP32 inputs (the spike itself is the record's).
     const double DT = v.depolarization_slope_factor, gl = P.prm->g_leak;
     const double ex = exp((Vp - (double)v.v_effective_spike_threshold) / DT);
@@ -188,38 +212,57 @@ __device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t
     P.acc_gleak[j] += lU * kd * (-(Vp - (double)v.v_rest) + DT * ex);
 }
 
-// One wave per neuron j: lambda x_j = sum_k W[j][k] lambda g_k over the two matrices leaving j
-// (E rows: EE and EI with lambda g_e; I rows: IE and II with lambda g_i), float4 weight loads
-// along the row, FP64 sums; then lane 0 runs the neuron's reverse step.
+// One wave per kRevRows neurons of one group: lambda x_j = sum_k W[j][k] lambda g_k over the
+// two matrices leaving j (E rows: EE and EI with lambda g_e; I rows: IE and II with lambda
+// g_i), float4 weight loads along the rows, each lambda g value loaded once for the wave's rows,
+// FP64 sums; then lane r runs the reverse step of the wave's row r.
+constexpr uint32_t kRevRows = 4;
 __global__ __launch_bounds__(256) void k_win_reverse(WinReversePlan P) {
-    const uint32_t row = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
     const uint32_t lane = threadIdx.x & 63;
-    if (row >= P.N) return;
-    const bool exc = row < P.NE;
-    const uint32_t il = exc ? row : row - P.NE;
+    const uint32_t wave_e = (P.NE + kRevRows - 1) / kRevRows, wave_i = (P.NI + kRevRows - 1) / kRevRows;
+    if (w >= wave_e + wave_i) return;
+    const bool exc = w < wave_e;
+    const uint32_t l0 = (exc ? w : w - wave_e) * kRevRows;          // first group-local row
+    const uint32_t nrows = min(kRevRows, (exc ? P.NE : P.NI) - l0);
+    const uint32_t g0 = exc ? l0 : P.NE + l0;                       // first global id
     const double* lg = exc ? P.lg_in_e : P.lg_in_i;
-    double acc = 0.0;
+    ReverseInputs in{};
+    if (lane < nrows) in = load_reverse_inputs(P, g0 + lane);
+    double acc[kRevRows] = {};
     for (int h = 0; h < 2; ++h) {
         const int c = (exc ? 0 : 2) + h;   // to E posts, then to I posts
         const uint32_t n_post = h == 0 ? P.NE : P.NI;
-        const float* w = P.W[c] + (size_t)il * P.ld[c];
         const double* l = lg + (h == 0 ? 0 : P.NE);
         const uint32_t n4 = n_post >> 2;
-        const float4* w4 = reinterpret_cast<const float4*>(w);
-#pragma unroll 4
+        const float* wr[kRevRows];
+#pragma unroll
+        for (uint32_t r = 0; r < kRevRows; ++r) wr[r] = P.W[c] + (size_t)(l0 + min(r, nrows - 1)) * P.ld[c];
+#pragma unroll 2
         for (uint32_t q = lane; q < n4; q += 64) {
-            const float4 wv = w4[q];
-            const double* lq = l + 4 * q;
-            acc += (double)wv.x * lq[0];
-            acc += (double)wv.y * lq[1];
-            acc += (double)wv.z * lq[2];
-            acc += (double)wv.w * lq[3];
+            const double l_0 = l[4 * q], l_1 = l[4 * q + 1], l_2 = l[4 * q + 2], l_3 = l[4 * q + 3];
+#pragma unroll
+            for (uint32_t r = 0; r < kRevRows; ++r) {
+                const float4 wv = reinterpret_cast<const float4*>(wr[r])[q];
+                acc[r] += (double)wv.x * l_0;
+                acc[r] += (double)wv.y * l_1;
+                acc[r] += (double)wv.z * l_2;
+                acc[r] += (double)wv.w * l_3;
+            }
         }
-        for (uint32_t jj = 4 * n4 + lane; jj < n_post; jj += 64) acc += (double)w[jj] * l[jj];
+        for (uint32_t jj = 4 * n4 + lane; jj < n_post; jj += 64)
+#pragma unroll
+            for (uint32_t r = 0; r < kRevRows; ++r) acc[r] += (double)wr[r][jj] * l[jj];
     }
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) neuron_reverse(P, row, acc);
+    for (uint32_t r = 0; r < kRevRows; ++r)
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) acc[r] += __shfl_xor(acc[r], off, 64);
+    double mine = acc[0];
+#pragma unroll
+    for (uint32_t r = 1; r < kRevRows; ++r)
+        if (lane == r) mine = acc[r];
+    if (lane < nrows) neuron_reverse(P, g0 + lane, mine, in);
 }
 
 // dW[i][j] += sum_slots x_slot[i] * lambda_slot[j], x = d - r at the start of the step.  A block
@@ -285,7 +328,8 @@ int launch_win_copy(const WinCopyPlan& p, bn_stream_t s) {
 }
 
 int launch_win_reverse(const WinReversePlan& p, bn_stream_t s) {
-    hipLaunchKernelGGL(k_win_reverse, dim3((p.N + 3) / 4), dim3(256), 0, s, p);
+    const uint32_t waves = (p.NE + kRevRows - 1) / kRevRows + (p.NI + kRevRows - 1) / kRevRows;
+    hipLaunchKernelGGL(k_win_reverse, dim3((waves + 3) / 4), dim3(256), 0, s, p);
     return (int)hipGetLastError();
 }
 
