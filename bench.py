@@ -50,7 +50,7 @@ def main() -> int:
         return copy_calibration(args.copy_calibration_mib)
     if args.gpus != 1:
         print(f"bench.py: this engine runs one network on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(a larger network would shard by post neuron, each GPU holding W[:, its posts])", file=sys.stderr)
         return 2
     if args.mode == "batched-shared":
         args.steps = 5001 if args.steps is None else args.steps

@@ -1,6 +1,6 @@
 /*
  * ORACLE — test infrastructure only.  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load this library, and only as the checker / CPU baseline; the product
+ * cpu_baseline leg may load this library, and only as the reference for checks / CPU baseline; the product
  * never links or calls it.
  *
  * A CPU (C++/OpenMP) restatement of the reference's per-step semantics

@@ -2,7 +2,7 @@
 
 ctypes wrapper of oracle/liboracle.so (the CPU restatement of the reference step, see
 oracle/oracle.h).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
-and only as the checker / CPU baseline.
+and only as the reference for checks / CPU baseline.
 """
 from __future__ import annotations
 

@@ -15,7 +15,7 @@ for p in (REPO, os.path.join(REPO, "oracle")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-import oracle_py  # noqa: E402  (the checker)
+import oracle_py  # noqa: E402  (the oracle: test infrastructure)
 
 PKG_NAME = "example-cuda-spiking-nn_amd"
 CONFIGS = os.path.join(REPO, "configs")
