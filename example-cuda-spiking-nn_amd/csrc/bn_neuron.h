@@ -93,6 +93,31 @@ __device__ __forceinline__ double fin_part_sum(const FinalizeSource& s, uint32_t
     return acc;
 }
 
+// Part [a0, a1) of source A and part [b0, b1) of source B with their loads issued together;
+// each is summed exactly as fin_part_sum sums it.
+__device__ __forceinline__ void fin_part_sum_pair(const FinalizeSource& A, uint32_t ja, uint32_t a0, uint32_t a1,
+                                                  const FinalizeSource& Bs, uint32_t jb, uint32_t b0, uint32_t b1,
+                                                  double& sa, double& sb) {
+    const double* pa = A.partials + ja;
+    const double* pb = Bs.partials + jb;
+    double acc_a = 0.0, acc_b = 0.0;
+    for (uint32_t ca = a0, cb = b0; ca < a1 || cb < b1; ca += kFinBatch, cb += kFinBatch) {
+        double va[kFinBatch], vb[kFinBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kFinBatch; ++k) {
+            va[k] = ca + k < a1 ? pa[(size_t)(ca + k) * A.ld] : 0.0;
+            vb[k] = cb + k < b1 ? pb[(size_t)(cb + k) * Bs.ld] : 0.0;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kFinBatch; ++k) {
+            if (ca + k < a1) acc_a += va[k];
+            if (cb + k < b1) acc_b += vb[k];
+        }
+    }
+    sa = acc_a;
+    sb = acc_b;
+}
+
 // One thread, every part in order (component path and g materialisation).
 __device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, uint32_t j) {
     const uint32_t per = fin_part_size(s.chunks);

@@ -160,12 +160,25 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk, 
             const EvolveGroup& G = P.g[g];
             if (gid < G.v.offset || gid >= G.v.offset + G.v.num_neurons) continue;
             const uint32_t lid = gid - G.v.offset;
-            for (uint32_t s = 0; s < G.n_fin_e + G.n_fin_i; ++s) {
-                const FinalizeSource& F = s < G.n_fin_e ? G.fin_e[s] : G.fin_i[s - G.n_fin_e];
-                const uint32_t slot = s < G.n_fin_e ? s : kMaxSources + (s - G.n_fin_e);
-                const uint32_t per = fin_part_size(F.chunks);
-                const uint32_t c0 = wave * per;
-                if (c0 < F.chunks) fin[slot][wave][lane] = fin_part_sum(F, lid, c0, min(c0 + per, F.chunks));
+            if (G.n_fin_e == 1 && G.n_fin_i == 1) {
+                // The common case, one excitatory and one inhibitory source: both parts' loads
+                // in flight together.
+                const FinalizeSource& Fe = G.fin_e[0];
+                const FinalizeSource& Fi = G.fin_i[0];
+                const uint32_t pe = fin_part_size(Fe.chunks), pi = fin_part_size(Fi.chunks);
+                const uint32_t e0 = min(wave * pe, Fe.chunks), i0 = min(wave * pi, Fi.chunks);
+                double se, si;
+                fin_part_sum_pair(Fe, lid, e0, min(e0 + pe, Fe.chunks), Fi, lid, i0, min(i0 + pi, Fi.chunks), se, si);
+                if (wave * pe < Fe.chunks) fin[0][wave][lane] = se;
+                if (wave * pi < Fi.chunks) fin[kMaxSources][wave][lane] = si;
+            } else {
+                for (uint32_t s = 0; s < G.n_fin_e + G.n_fin_i; ++s) {
+                    const FinalizeSource& F = s < G.n_fin_e ? G.fin_e[s] : G.fin_i[s - G.n_fin_e];
+                    const uint32_t slot = s < G.n_fin_e ? s : kMaxSources + (s - G.n_fin_e);
+                    const uint32_t per = fin_part_size(F.chunks);
+                    const uint32_t c0 = wave * per;
+                    if (c0 < F.chunks) fin[slot][wave][lane] = fin_part_sum(F, lid, c0, min(c0 + per, F.chunks));
+                }
             }
         }
     }
