@@ -26,7 +26,19 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 
 
+def host_cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main() -> int:
+    # The oracle's OpenMP threads stay close to their parent (set before its library loads).
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--mode", choices=("plain", "batched-shared", "batched-uneven", "adjoint"), default="plain",
@@ -87,6 +99,9 @@ def main() -> int:
     state = sim.state()
     rate_hz = float(state["spikes"].mean()) / cfg["timestep"]
     sim.close()
+    # The applicable read bandwidth for this weight set: a plain streaming read kernel over a
+    # working set of the same size (MALL-resident below 256 MB, HBM-bound above).
+    read_gbs = eng.measure_read_bandwidth(4 * n_w, 50)
 
     n_total = int(info.total_neurons)
     name = os.path.splitext(os.path.basename(args.config))[0]
@@ -123,7 +138,12 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": "k_gemv (propagate)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": gemv_bytes,
-                     "avg_kernel_us": round(k_avg_s * 1e6, 2)},
+                     "avg_kernel_us": round(k_avg_s * 1e6, 2),
+                     "measured_read_GBs_same_working_set": round(read_gbs, 1),
+                     "frac_of_measured_read": round(achieved / read_gbs, 4),
+                     "step_bytes_B_step": 4 * n_w + 160 * n_total,
+                     "step_GBs": round((4 * n_w + 160 * n_total) * steps_per_s / 1e9, 1),
+                     "step_frac_of_peak": round((4 * n_w + 160 * n_total) * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -483,19 +503,28 @@ def cpu_baseline(path: str, cfg: dict, seconds: float, info) -> dict:
     eng = importlib.import_module("example-cuda-spiking-nn_amd")
     weights = [eng.generate_weight_matrix(path, k) for k in range(4)]
     p = oracle_py.params_from_config(cfg, gemv_row_block=tuple(info.gemv_row_block), dense_gemv=True)
-    orc = oracle_py.OracleSim(p, weights, precision=32)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    orc.step(1)
-    t0 = time.perf_counter()
-    n = 0
-    while time.perf_counter() - t0 < seconds:
+
+    def timed(precision, budget):
+        orc = oracle_py.OracleSim(p, weights, precision=precision)
         orc.step(1)
-        n += 1
-    dt = time.perf_counter() - t0
-    orc.close()
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < budget:
+            orc.step(1)
+            n += 1
+        dt = time.perf_counter() - t0
+        orc.close()
+        return n, dt
+
+    n, dt = timed(32, seconds)
+    n64, dt64 = timed(64, seconds / 3)
     return {"value": round(n / dt, 3), "unit": "steps/s", "cores": threads, "kind": "port",
+            "fp64_value": round(n64 / dt64, 3),
+            "host": f"{host_cpu_model()}, nproc {os.cpu_count()}, OMP_NUM_THREADS {threads}, "
+                    f"OMP_PROC_BIND {os.environ.get('OMP_PROC_BIND')}",
             "sample": f"{n} steps of the same network from its initial state, FP32 oracle, "
-                      f"{threads} OpenMP threads, {dt:.1f} s"}
+                      f"{threads} OpenMP threads, {dt:.1f} s (FP64 oracle: {n64} steps, {dt64:.1f} s)"}
 
 
 if __name__ == "__main__":

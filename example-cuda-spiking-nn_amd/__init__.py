@@ -19,5 +19,6 @@ from .engine import (  # noqa: F401
     config_dump,
     generate_weight_matrix,
     lib,
+    measure_read_bandwidth,
     poisson_table,
 )

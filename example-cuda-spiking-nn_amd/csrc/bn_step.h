@@ -230,5 +230,6 @@ int launch_materialize_g(float* g_dst, const float* g_base, const FinalizeSource
 // dst[i] = src[i] + (spikes[i] == 1 ? 1 : 0): plasticity traces with this step's increments.
 int launch_add_increments(float* dst, const float* src, const uint32_t* spikes, uint32_t n, bn_stream_t s);
 int launch_fill_u32(uint32_t* p, uint32_t value, size_t n, bn_stream_t s);
+int launch_read_probe(const void* buf, uint64_t bytes, float* sink, uint32_t blocks, bn_stream_t s);
 
 }  // namespace bn

@@ -1,6 +1,7 @@
 // extern "C" whole-simulation API (include/buryn_amd.h) and the network wiring of the
 // reference driver (cpp_src/main_buryn.cpp:31-343).
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -406,6 +407,76 @@ int bn_sim_run_timed(bn_sim* sim, uint32_t num_steps, float* ms) {
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         sim->state->check_status();
+    });
+}
+
+int bn_sim_save_state(bn_sim* sim, const char* path) {
+    if (!sim || !path) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_save_state: null argument");
+    return guarded([&] {
+        const std::vector<uint8_t> bytes = sim->state->snapshot();
+        std::FILE* f = std::fopen(path, "wb");
+        if (!f) throw Error(BN_ERR_RUNTIME, std::string("cannot open ") + path);
+        const size_t w = std::fwrite(bytes.data(), 1, bytes.size(), f);
+        const int rc = std::fclose(f);
+        if (w != bytes.size() || rc != 0) throw Error(BN_ERR_RUNTIME, std::string("cannot write ") + path);
+    });
+}
+
+int bn_sim_load_state(bn_sim* sim, const char* path) {
+    if (!sim || !path) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_load_state: null argument");
+    return guarded([&] {
+        std::FILE* f = std::fopen(path, "rb");
+        if (!f) throw Error(BN_ERR_RUNTIME, std::string("cannot open ") + path);
+        std::vector<uint8_t> bytes;
+        uint8_t buf[1 << 16];
+        size_t n;
+        while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) bytes.insert(bytes.end(), buf, buf + n);
+        std::fclose(f);
+        sim->state->restore(bytes);
+        bn_device_clock clk{};
+        BN_HIP_CHECK(hipMemcpy(&clk, sim->state->d_clock(), sizeof clk, hipMemcpyDeviceToHost), "clock download");
+        sim->system->set_clock_step(clk.clock_step);
+    });
+}
+
+int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s) {
+    if (!gb_per_s || bytes < 16 || repeats == 0)
+        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_measure_read_bandwidth: bad argument");
+    return guarded([&] {
+        bytes &= ~(uint64_t)15;
+        void* buf = nullptr;
+        float* sink = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        const uint32_t blocks = 4096;
+        auto cleanup = [&] {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+            if (s) (void)hipStreamDestroy(s);
+            if (sink) (void)hipFree(sink);
+            if (buf) (void)hipFree(buf);
+        };
+        try {
+            BN_HIP_CHECK(hipMalloc(&buf, bytes), "hipMalloc(probe buffer)");
+            BN_HIP_CHECK(hipMalloc(&sink, blocks * sizeof(float)), "hipMalloc(probe sink)");
+            BN_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+            BN_HIP_CHECK(hipMemsetAsync(buf, 0, bytes, s), "hipMemsetAsync");
+            BN_HIP_CHECK(hipEventCreate(&a), "hipEventCreate");
+            BN_HIP_CHECK(hipEventCreate(&b), "hipEventCreate");
+            BN_HIP_CHECK((hipError_t)launch_read_probe(buf, bytes, sink, blocks, s), "probe launch");
+            BN_HIP_CHECK(hipEventRecord(a, s), "hipEventRecord");
+            for (uint32_t k = 0; k < repeats; ++k)
+                BN_HIP_CHECK((hipError_t)launch_read_probe(buf, bytes, sink, blocks, s), "probe launch");
+            BN_HIP_CHECK(hipEventRecord(b, s), "hipEventRecord");
+            BN_HIP_CHECK(hipEventSynchronize(b), "hipEventSynchronize");
+            float ms = 0.0f;
+            BN_HIP_CHECK(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+            *gb_per_s = (float)((double)bytes * repeats / (ms * 1e-3) / 1e9);
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
     });
 }
 

@@ -41,8 +41,60 @@ T* GpuSimulationState::alloc(size_t n, const char* what) {
     BN_HIP_CHECK(hipMalloc(&p, bytes), std::string("hipMalloc ") + what);
     BN_HIP_CHECK(hipMemsetAsync(p, 0, bytes, stream_), std::string("hipMemset ") + what);
     allocations_.push_back(p);
+    allocation_bytes_.push_back(bytes);
     allocated_ += bytes;
     return static_cast<T*>(p);
+}
+
+// Snapshot layout: "BNSTATE1", uint64 buffer count, the byte size of each buffer, then the
+// buffers in allocation order (the same config and flags allocate the same buffers).
+static const char kSnapshotMagic[8] = {'B', 'N', 'S', 'T', 'A', 'T', 'E', '1'};
+
+std::vector<uint8_t> GpuSimulationState::snapshot() const {
+    size_t total = 16 + 8 * allocations_.size();
+    for (size_t b : allocation_bytes_) total += b;
+    std::vector<uint8_t> out(total);
+    std::memcpy(out.data(), kSnapshotMagic, 8);
+    const uint64_t count = allocations_.size();
+    std::memcpy(out.data() + 8, &count, 8);
+    size_t off = 16;
+    for (size_t b : allocation_bytes_) {
+        const uint64_t v = b;
+        std::memcpy(out.data() + off, &v, 8);
+        off += 8;
+    }
+    BN_HIP_CHECK(hipStreamSynchronize(stream_), "snapshot sync");
+    for (size_t k = 0; k < allocations_.size(); ++k) {
+        BN_HIP_CHECK(hipMemcpy(out.data() + off, allocations_[k], allocation_bytes_[k], hipMemcpyDeviceToHost),
+                     "snapshot download");
+        off += allocation_bytes_[k];
+    }
+    return out;
+}
+
+void GpuSimulationState::restore(const std::vector<uint8_t>& in) {
+    if (in.size() < 16 || std::memcmp(in.data(), kSnapshotMagic, 8) != 0)
+        throw Error(BN_ERR_INVALID_ARGUMENT, "not a state snapshot");
+    uint64_t count = 0;
+    std::memcpy(&count, in.data() + 8, 8);
+    if (count != allocations_.size() || in.size() < 16 + 8 * count)
+        throw Error(BN_ERR_INVALID_ARGUMENT, "the snapshot was taken from a different network or flags");
+    size_t off = 16 + 8 * count, total = off;
+    for (size_t k = 0; k < count; ++k) {
+        uint64_t b = 0;
+        std::memcpy(&b, in.data() + 16 + 8 * k, 8);
+        if (b != allocation_bytes_[k])
+            throw Error(BN_ERR_INVALID_ARGUMENT, "the snapshot was taken from a different network or flags");
+        total += b;
+    }
+    if (in.size() != total) throw Error(BN_ERR_INVALID_ARGUMENT, "truncated state snapshot");
+    BN_HIP_CHECK(hipStreamSynchronize(stream_), "restore sync");
+    for (size_t k = 0; k < count; ++k) {
+        BN_HIP_CHECK(hipMemcpy(allocations_[k], in.data() + off, allocation_bytes_[k], hipMemcpyHostToDevice),
+                     "snapshot upload");
+        off += allocation_bytes_[k];
+    }
+    ring_snapshot_step_ = -1;
 }
 
 GpuSimulationState::GpuSimulationState(uint32_t total_simulation_length, uint32_t num_excitatory_neurons,

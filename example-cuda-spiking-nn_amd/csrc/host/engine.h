@@ -128,6 +128,9 @@ public:
     std::string lookup_weight_matrix_name(uint32_t weight_matrix_idx) const;
     void setup_lazy_normalization(uint32_t weight_matrix_idx);  // after W_EE upload
     bool lazy_active() const { return lazy_active_; }
+    // Every device buffer of the state, in allocation order (snapshot / resume).
+    std::vector<uint8_t> snapshot() const;
+    void restore(const std::vector<uint8_t>& bytes);
 
     uint32_t total_simulation_length() const { return total_simulation_length_; }
     uint32_t num_excitatory_neurons() const { return ne_; }
@@ -146,6 +149,7 @@ private:
     uint32_t total_simulation_length_, ne_, ni_, n_, num_stimulus_events_, num_plasticities_;
     hipStream_t stream_ = nullptr;
     std::vector<void*> allocations_;
+    std::vector<size_t> allocation_bytes_;
     uint64_t allocated_ = 0;
 
     float *v_, *ge_, *gi_, *istim_, *bg_ge_, *bg_gi_, *bre_, *bde_, *bri_, *bdi_;
@@ -415,6 +419,8 @@ public:
     const PlastStepPlan& plast_final() const { return plast_final_; }
     // Steps run by an external launcher on this system's plans (several networks in one launch).
     void note_external_steps(uint32_t n) { clock_step_ += n; }
+    // After a state restore: the host's step counter follows the restored device clock.
+    void set_clock_step(uint32_t step) { clock_step_ = step; }
     uint32_t kernels_per_step() const { return kernels_per_step_; }
     // Conductances after the last propagate, for state downloads (fused path keeps them as
     // base + pending GEMV partials).

@@ -1060,6 +1060,22 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t value, s
 // =======================================================================================
 // Host-side launch helpers
 // =======================================================================================
+// Read-bandwidth probe: every float4 of the buffer read once per pass (grid-stride), folded into
+// one value per thread so the loads are kept.
+__global__ __launch_bounds__(256) void k_read_probe(const float4* __restrict__ p, size_t n4, float* sink) {
+    float acc = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+        const float4 v = p[i];
+        acc += (v.x + v.y) + (v.z + v.w);
+    }
+    if (acc == 12345.678f) sink[blockIdx.x] = acc;  // practically never; the sum must not be dead
+}
+
+int launch_read_probe(const void* buf, uint64_t bytes, float* sink, uint32_t blocks, bn_stream_t s) {
+    hipLaunchKernelGGL(k_read_probe, dim3(blocks), dim3(256), 0, s, static_cast<const float4*>(buf), (size_t)(bytes / 16), sink);
+    return (int)hipGetLastError();
+}
+
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 int launch_evolve(const EvolvePlan& plan, bn_stream_t s) {

@@ -109,7 +109,8 @@ EXPORTED_SYMBOLS = [
     "bn_multi_synchronize", "bn_multi_network", "bn_multi_get_info", "bn_multi_time_propagate",
     "bn_multi_set_schedule", "bn_multi_set_streams", "bn_window_create", "bn_window_destroy", "bn_window_set_params",
     "bn_window_get_weights", "bn_window_set_targets", "bn_window_forward", "bn_window_backward",
-    "bn_window_get_info", "bn_window_get_gradient",
+    "bn_window_get_info", "bn_window_get_gradient", "bn_measure_read_bandwidth", "bn_sim_save_state",
+    "bn_sim_load_state",
 ]
 
 _lib = None
@@ -146,6 +147,9 @@ def lib() -> C.CDLL:
     L.bn_sim_get_plasticity_traces.argtypes = [P, U, fp, fp, fp, fp]
     L.bn_sim_run_timed.argtypes = [P, U, fp]
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
+    L.bn_measure_read_bandwidth.argtypes = [C.c_uint64, U, fp]
+    L.bn_sim_save_state.argtypes = [P, C.c_char_p]
+    L.bn_sim_load_state.argtypes = [P, C.c_char_p]
     L.bn_config_dump_json.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.bn_generate_weight_matrix.argtypes = [C.c_char_p, U, fp, up, up]
     L.bn_batch_create.argtypes = [C.c_char_p, U, C.POINTER(BatchInstance), C.POINTER(P)]
@@ -206,6 +210,13 @@ def poisson_table(mean: float, capacity: int = 256) -> np.ndarray:
     n = C.c_uint32(0)
     _check(lib().bn_poisson_table(mean, _up(t), capacity, C.byref(n)))
     return t[: n.value]
+
+
+def measure_read_bandwidth(nbytes: int, repeats: int = 50) -> float:
+    """GB/s of a plain streaming read over an `nbytes` device buffer, warm (see the C ABI)."""
+    gbs = C.c_float(0)
+    _check(lib().bn_measure_read_bandwidth(nbytes, repeats, C.byref(gbs)))
+    return gbs.value
 
 
 def config_dump(path: str) -> dict:
@@ -296,6 +307,15 @@ class Simulation:
         k, s = C.c_float(0), C.c_float(0)
         _check(lib().bn_sim_time_propagate_kernel(self.h, n, C.byref(k), C.byref(s)))
         return k.value, s.value
+
+    def save_state(self, path: str) -> None:
+        """Every device buffer of the simulation to `path` (see bn_sim_save_state)."""
+        _check(lib().bn_sim_save_state(self.h, path.encode()))
+
+    def load_state(self, path: str) -> None:
+        """Resume from a snapshot taken from a simulation of the same config and flags."""
+        _check(lib().bn_sim_load_state(self.h, path.encode()))
+        self.info = self.get_info()
 
     def state(self) -> dict:
         arrays = {n: np.zeros(self.N, dt) for n, dt in _STATE_FIELDS}

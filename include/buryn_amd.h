@@ -356,6 +356,19 @@ int bn_sim_run_timed(bn_sim* sim, uint32_t num_steps, float* ms);
  * weights; the pending propagate partials are saved before and restored after, so the state of
  * the simulation is unchanged by this part. */
 int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_ms, float* step_ms);
+/* State snapshot / resume (no counterpart in the reference): every device buffer of the
+ * simulation (neuron and background state, weights, connection and plasticity traces, the
+ * pending propagate partials, lazy row sums and scales, monitor rings, the device clock and
+ * stimulus cursor) to a file, and back into a simulation created from the same config and
+ * flags (checked: a different network or flag set is refused).  A restored simulation continues
+ * exactly as the saved one would have. */
+int bn_sim_save_state(bn_sim* sim, const char* path);
+int bn_sim_load_state(bn_sim* sim, const char* path);
+/* Measurement aid (no counterpart in the reference): the read bandwidth of a plain streaming
+ * kernel over a device buffer of `bytes` bytes read `repeats` times back to back after one
+ * warm-up pass (a working set under the 256 MB MALL stays cache-resident between passes).  GB/s
+ * from HIP events; allocates and frees its buffer. */
+int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s);
 /* Writes the JSON of the parsed config (all fields, after defaults/overrides/prefixing). */
 int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, size_t* needed);
 
