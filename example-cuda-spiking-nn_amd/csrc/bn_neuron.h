@@ -9,34 +9,45 @@
 
 namespace bn {
 
-// kernel_neuron_group_evolve.cu:85-123 — refractory pin, or forward-Euler EIF step.
-__device__ __forceinline__ void eif_step(const NeuronGroupView& v, float ge, float gi, float istim,
-                                         float& V, uint32_t& refr, uint32_t& spike) {
+// kernel_neuron_group_evolve.cu:85-123 — refractory pin, or forward-Euler EIF step, on the neuron
+// parameters as values (NeuronGroupView field names).
+__device__ __forceinline__ void eif_core(float timestep, float g_leak, float membrane_capacitance,
+                                         float depolarization_slope_factor, float v_effective_spike_threshold,
+                                         float v_excitatory_reversal, float v_inhibitory_reversal, float v_threshold,
+                                         float v_reset, float v_rest, uint32_t refractory_steps, float ge, float gi,
+                                         float istim, float& V, uint32_t& refr, uint32_t& spike) {
     if (refr > 0) {
         refr = refr - 1;
         spike = 0;
-        V = v.v_reset;
+        V = v_reset;
         return;
     }
     const float vv = V;
-    const float linear_leak_term = -1.0f * v.g_leak * (vv - v.v_rest);
+    const float linear_leak_term = -1.0f * g_leak * (vv - v_rest);
     const float exponential_depolarization_term =
-        v.g_leak * v.depolarization_slope_factor *
-        bn_expf((vv - v.v_effective_spike_threshold) / v.depolarization_slope_factor);
-    const float excitatory_synaptic_term = -1.0f * ge * (vv - v.v_excitatory_reversal);
-    const float inhibitory_synaptic_term = -1.0f * gi * (vv - v.v_inhibitory_reversal);
-    const float dv = (v.timestep / v.membrane_capacitance) *
+        g_leak * depolarization_slope_factor *
+        bn_expf((vv - v_effective_spike_threshold) / depolarization_slope_factor);
+    const float excitatory_synaptic_term = -1.0f * ge * (vv - v_excitatory_reversal);
+    const float inhibitory_synaptic_term = -1.0f * gi * (vv - v_inhibitory_reversal);
+    const float dv = (timestep / membrane_capacitance) *
                      (linear_leak_term + exponential_depolarization_term +
                       excitatory_synaptic_term + inhibitory_synaptic_term + istim);
     const float v_next = vv + dv;
-    if (v_next >= v.v_threshold) {
-        refr = v.refractory_steps;
+    if (v_next >= v_threshold) {
+        refr = refractory_steps;
         spike = 1;
-        V = v.v_threshold;
+        V = v_threshold;
     } else {
         spike = 0;
         V = v_next;
     }
+}
+
+__device__ __forceinline__ void eif_step(const NeuronGroupView& v, float ge, float gi, float istim,
+                                         float& V, uint32_t& refr, uint32_t& spike) {
+    eif_core(v.timestep, v.g_leak, v.membrane_capacitance, v.depolarization_slope_factor,
+             v.v_effective_spike_threshold, v.v_excitatory_reversal, v.v_inhibitory_reversal, v.v_threshold,
+             v.v_reset, v.v_rest, v.refractory_steps, ge, gi, istim, V, refr, spike);
 }
 
 // kernel_neuron_group_evolve.cu:11-83 on values already in registers: decay (+flush) of the

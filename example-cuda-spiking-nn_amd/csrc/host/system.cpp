@@ -253,7 +253,9 @@ void System::build_plans() {
     tail_.increments = 0;  // folded into the next step's trace decay (k_evolve)
     tail_.tail = st;
     plast_step_.pl = plast_;
-    plast_step_.row_blocks = (plast_.items_pre + kPlastQuietRows - 1) / kPlastQuietRows;
+    plast_step_.row_blocks = 0;  // each plasticity's rows padded to whole blocks
+    for (uint32_t k = 0; k < plast_.n; ++k)
+        plast_step_.row_blocks += (plast_.p[k].n_pre + kPlastQuietRows - 1) / kPlastQuietRows;
     plast_step_.spiking_blocks = std::max<uint32_t>(1, std::min<uint32_t>(256, plast_.items_pre));
     plast_step_.tail = st;
     plast_step_.tail.active = tail_.norm_rows == 0 ? 1 : 0;

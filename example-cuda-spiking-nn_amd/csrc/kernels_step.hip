@@ -68,13 +68,13 @@ __device__ __forceinline__ void connection_trace_step(float* r, float* d, float*
 // fin[slot][w][lane], slots 0..kMaxSources-1 for g_e sources, kMaxSources.. for g_i.
 typedef double FinParts[2 * kMaxSources][kFinParts][kEvolveNeurons];
 
-__device__ __forceinline__ float combine_parts(float g, const FinalizeSource& s, const double (*part)[kEvolveNeurons],
+__device__ __forceinline__ float combine_parts(float g, uint32_t chunks, const double (*part)[kEvolveNeurons],
                                                uint32_t lane) {
-    const uint32_t per = fin_part_size(s.chunks);
+    const uint32_t per = fin_part_size(chunks);
     double acc = (double)g;
 #pragma unroll
     for (uint32_t w = 0; w < (uint32_t)kFinParts; ++w)
-        if (w * per < s.chunks) acc += part[w][lane];
+        if (w * per < chunks) acc += part[w][lane];
     return (float)acc;
 }
 
@@ -88,12 +88,13 @@ __device__ __forceinline__ float trace_step(float tr, uint32_t spiked_prev, floa
 // =======================================================================================
 // k_evolve: every neuron of every group in one launch
 // =======================================================================================
-// A block owns 64 neurons (lane = neuron) and has kFinParts waves.  Before the barrier every
-// wave sums one part of the propagate partials of the 64 neurons (the partial reads are spread
-// over many waves), and in the same phase wave 0 loads the neuron state, wave 1 runs the
-// background update (Philox draws, Poisson counts, the four background traces) and wave 2 the
-// plasticity trace step (it needs only the previous step's spikes).  After the barrier wave 0
-// combines the parts and runs the EIF step, the connection traces and the monitors.
+// A block owns 64 neurons (lane = neuron) and has kFinParts waves, each with one role whose
+// loads are all issued at the start (one memory round trip per wave): wave 0 loads the neuron
+// state and the connection traces, wave 1 runs the background update (Philox draws, Poisson
+// counts, the four background traces), wave 2 the plasticity trace step (it needs only the
+// previous step's spikes), and waves 3.. sum the parts of the propagate partials.  After the
+// barrier wave 0 combines the parts and runs the EIF step, the connection traces and the
+// monitors.
 __device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* table, uint32_t len,
                                                    const uint32_t (&head)[4]) {
     // min{k : u < table[k]}; the first four entries come from registers.
@@ -103,35 +104,352 @@ __device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* t
     return poisson_from_table(u, table + 4, len - 4) + 4;
 }
 
+constexpr uint32_t kPartWave0 = 3;                          // first wave that sums partials
+constexpr uint32_t kPartWaves = kFinParts - kPartWave0;     // parts p, p + kPartWaves, ... per wave
+constexpr uint32_t kRangeBatch = 8;
+
+// Sums of R chunk ranges: range r adds p[r][c * ld[r]] for c in [c0[r], c1[r]) in increasing c,
+// from 0.0 (the order fin_part_sum uses); the loads of a batch of every range are in flight
+// together.
+template <int R>
+__device__ __forceinline__ void range_sums(const double* const (&p)[R], const uint32_t (&ld)[R],
+                                           const uint32_t (&c0)[R], const uint32_t (&c1)[R], double (&out)[R]) {
+    uint32_t n = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        out[r] = 0.0;
+        n = max(n, c1[r] > c0[r] ? c1[r] - c0[r] : 0u);
+    }
+    for (uint32_t b = 0; b < n; b += kRangeBatch) {
+        double v[R][kRangeBatch];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (uint32_t k = 0; k < kRangeBatch; ++k) {
+                const uint32_t c = c0[r] + b + k;
+                v[r][k] = c < c1[r] ? p[r][(size_t)c * ld[r]] : 0.0;
+            }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (uint32_t k = 0; k < kRangeBatch; ++k)
+                if (c0[r] + b + k < c1[r]) out[r] += v[r][k];
+    }
+}
+
+// The parts of this wave (pw, pw + kPartWaves, ... < kFinParts) of every source of the group,
+// stored in fin[slot][part][lane] (slots 0.. for g_e sources, kMaxSources.. for g_i).
+// One excitatory and one inhibitory source, given per lane (a block may straddle two groups):
+// parts pw and pw + kPartWaves of both, four ranges with their loads in flight together.
+__device__ __forceinline__ void evolve_parts_pair(const double* pe, uint32_t lde, uint32_t ce, const double* pi,
+                                                  uint32_t ldi, uint32_t ci, uint32_t pw, uint32_t lane,
+                                                  FinParts& fin) {
+    const uint32_t se = fin_part_size(ce), si = fin_part_size(ci);
+    const uint32_t pb = pw + kPartWaves;
+    const bool has_b = pb < (uint32_t)kFinParts;
+    const double* const ptr[4] = {pe, pi, pe, pi};
+    const uint32_t ld[4] = {lde, ldi, lde, ldi};
+    const uint32_t c0[4] = {min(pw * se, ce), min(pw * si, ci), has_b ? min(pb * se, ce) : 0u,
+                            has_b ? min(pb * si, ci) : 0u};
+    const uint32_t c1[4] = {min(c0[0] + se, ce), min(c0[1] + si, ci), has_b ? min(c0[2] + se, ce) : 0u,
+                            has_b ? min(c0[3] + si, ci) : 0u};
+    double out[4];
+    range_sums<4>(ptr, ld, c0, c1, out);
+    if (pw * se < ce) fin[0][pw][lane] = out[0];
+    if (pw * si < ci) fin[kMaxSources][pw][lane] = out[1];
+    if (has_b && pb * se < ce) fin[0][pb][lane] = out[2];
+    if (has_b && pb * si < ci) fin[kMaxSources][pb][lane] = out[3];
+}
+
+// Any number of sources, one group.
+__device__ __forceinline__ void evolve_parts_source(const FinalizeSource& F, uint32_t slot, uint32_t lid, uint32_t pw,
+                                                    uint32_t lane, FinParts& fin) {
+    const uint32_t per = fin_part_size(F.chunks);
+    for (uint32_t q = pw; q < (uint32_t)kFinParts; q += kPartWaves) {
+        const uint32_t c0 = q * per;
+        if (c0 < F.chunks) fin[slot][q][lane] = fin_part_sum(F, lid, c0, min(c0 + per, F.chunks));
+    }
+}
+__device__ __forceinline__ void evolve_parts(const EvolveGroup& G, uint32_t lid, uint32_t pw, uint32_t lane,
+                                             FinParts& fin) {
+    for (uint32_t s = 0; s < G.n_fin_e; ++s) evolve_parts_source(G.fin_e[s], s, lid, pw, lane, fin);
+    for (uint32_t s = 0; s < G.n_fin_i; ++s) evolve_parts_source(G.fin_i[s], kMaxSources + s, lid, pw, lane, fin);
+}
+
+// A plan field as a wave-uniform value.  Used where a lane selects among the fields of several
+// groups: the selected loads then stay scalar loads of the kernel arguments (a select between
+// two loads would otherwise become one load through a per-lane address into them).
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(x))); }
+template <typename T>
+__device__ __forceinline__ T* uni(T* p) {
+    // readfirstlane returns int: each half goes through uint32_t so the low half is not sign-extended.
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return reinterpret_cast<T*>((hi << 32) | lo);
+}
+
+// The groups of a block's first and last neuron (wave-uniform).  Groups are contiguous global id
+// ranges, so a block whose every live lane lies in one of the two straddles at most these two.
+__device__ __forceinline__ void block_groups(const EvolvePlan& P, uint32_t blk, uint32_t& g_lo, uint32_t& g_hi) {
+    const uint32_t first = blk * kEvolveNeurons;
+    const uint32_t last = min(first + (uint32_t)kEvolveNeurons, P.total_neurons) - 1;
+    g_lo = g_hi = 0;
+    for (uint32_t g = 0; g < P.n_groups; ++g) {
+        const uint32_t o = P.g[g].v.offset, n = P.g[g].v.num_neurons;
+        if (first >= o && first - o < n) g_lo = g;
+        if (last >= o && last - o < n) g_hi = g;
+    }
+}
+
 __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk, uint32_t step_offset,
                                              uint32_t stim_prev) {
     __shared__ FinParts fin;
     __shared__ float bgl[2][kEvolveNeurons];
-    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t gid = blk * kEvolveNeurons + lane;
     const bool live = gid < P.total_neurons;
-    const bn_device_clock clk = *P.clock;
-    const uint32_t step = clk.clock_step + step_offset;
 
     // Wave 0: per-neuron inputs of the EIF step and the connection traces.
     float V = 0.0f, ge = 0.0f, gi_ = 0.0f, istim = 0.0f;
+    // The lane's group fields used after the barrier (wave 0; see below).
+    bool two_groups = false, in_group = false;
+    uint32_t lid = 0, n_fe = 0, n_fi = 0, chunks_e[kMaxSources] = {}, chunks_i[kMaxSources] = {};
+    float e_dt = 0.0f, e_gl = 0.0f, e_c = 0.0f, e_slope = 0.0f, e_vt = 0.0f, e_ee = 0.0f, e_ei = 0.0f,
+          e_vthr = 0.0f, e_vreset = 0.0f, e_vrest = 0.0f;
+    uint32_t e_refr = 0;
+    float *pV = nullptr, *pGe = nullptr, *pGi = nullptr, *pIs = nullptr;
+    uint32_t *pR = nullptr, *pS = nullptr;
     uint32_t refr = 0;
     int32_t vslot = -1;
     float tr_r[kMaxTraceSets], tr_d[kMaxTraceSets];
 #pragma unroll
     for (int t = 0; t < kMaxTraceSets; ++t) tr_r[t] = tr_d[t] = 0.0f;
-    if (wave == 0 && live) {
+    if (wave == 0) {
+        if (live) {
+#pragma unroll
+            for (int g = 0; g < kMaxGroups; ++g) {
+                const NeuronGroupView& v = P.g[g].v;
+                if ((uint32_t)g < P.n_groups && gid >= v.offset && gid < v.offset + v.num_neurons) {
+                    const uint32_t lid = gid - v.offset;
+                    V = v.membrane_potentials[lid];
+                    refr = v.refractory_counter[lid];
+                    ge = v.g_excitatory[lid];
+                    gi_ = v.g_inhibitory[lid];
+                    istim = v.i_stimulus[lid];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kMaxTraceSets; ++t) {
+                const TraceSet& T = P.ts[t];
+                if ((uint32_t)t < P.n_ts && gid >= T.pre_offset && gid < T.pre_offset + T.n_pre) {
+                    tr_r[t] = T.r[gid - T.pre_offset];
+                    tr_d[t] = T.d[gid - T.pre_offset];
+                }
+            }
+            if (P.v_ring) vslot = P.vrec_index[gid];
+        }
+        // The lane's EIF parameters, state pointers and source chunk counts from the block's (at
+        // most two) groups, loaded before the barrier.
+        uint32_t g_lo, g_hi;
+        block_groups(P, blk, g_lo, g_hi);
+        const EvolveGroup& GA = P.g[g_lo];
+        const EvolveGroup& GB = P.g[g_hi];
+        const uint32_t oa = uni(GA.v.offset), ob = uni(GB.v.offset);
+        const bool inA = live && gid >= oa && gid - oa < uni(GA.v.num_neurons);
+        const bool inB = !inA && live && gid >= ob && gid - ob < uni(GB.v.num_neurons);
+        two_groups = __ballot(live && !inA && !inB) == 0;
+        in_group = inA || inB;
+        const bool b = inB;
+        lid = b ? gid - ob : gid - oa;
+        e_dt = b ? uni(GB.v.timestep) : uni(GA.v.timestep);
+        e_gl = b ? uni(GB.v.g_leak) : uni(GA.v.g_leak);
+        e_c = b ? uni(GB.v.membrane_capacitance) : uni(GA.v.membrane_capacitance);
+        e_slope = b ? uni(GB.v.depolarization_slope_factor) : uni(GA.v.depolarization_slope_factor);
+        e_vt = b ? uni(GB.v.v_effective_spike_threshold) : uni(GA.v.v_effective_spike_threshold);
+        e_ee = b ? uni(GB.v.v_excitatory_reversal) : uni(GA.v.v_excitatory_reversal);
+        e_ei = b ? uni(GB.v.v_inhibitory_reversal) : uni(GA.v.v_inhibitory_reversal);
+        e_vthr = b ? uni(GB.v.v_threshold) : uni(GA.v.v_threshold);
+        e_vreset = b ? uni(GB.v.v_reset) : uni(GA.v.v_reset);
+        e_vrest = b ? uni(GB.v.v_rest) : uni(GA.v.v_rest);
+        e_refr = b ? uni(GB.v.refractory_steps) : uni(GA.v.refractory_steps);
+        pV = b ? uni(GB.v.membrane_potentials) : uni(GA.v.membrane_potentials);
+        pGe = b ? uni(GB.v.g_excitatory) : uni(GA.v.g_excitatory);
+        pGi = b ? uni(GB.v.g_inhibitory) : uni(GA.v.g_inhibitory);
+        pIs = b ? uni(GB.v.i_stimulus) : uni(GA.v.i_stimulus);
+        pR = b ? uni(GB.v.refractory_counter) : uni(GA.v.refractory_counter);
+        pS = b ? uni(GB.v.spikes) : uni(GA.v.spikes);
+        n_fe = b ? uni(GB.n_fin_e) : uni(GA.n_fin_e);
+        n_fi = b ? uni(GB.n_fin_i) : uni(GA.n_fin_i);
+#pragma unroll
+        for (int q = 0; q < kMaxSources; ++q) {
+            chunks_e[q] = b ? uni(GB.fin_e[q].chunks) : uni(GA.fin_e[q].chunks);
+            chunks_i[q] = b ? uni(GB.fin_i[q].chunks) : uni(GA.fin_i[q].chunks);
+        }
+    } else if (wave >= kPartWave0) {
+        // Waves 3..: part sums of the propagate partials.  The common layout (one excitatory and
+        // one inhibitory source per group) with the lane's sources selected per lane from the
+        // block's (at most two) groups.
+        uint32_t g_lo, g_hi;
+        block_groups(P, blk, g_lo, g_hi);
+        const EvolveGroup& GA = P.g[g_lo];
+        const EvolveGroup& GB = P.g[g_hi];
+        const uint32_t oa = uni(GA.v.offset), ob = uni(GB.v.offset);
+        const bool inA = live && gid >= oa && gid - oa < uni(GA.v.num_neurons);
+        const bool inB = !inA && live && gid >= ob && gid - ob < uni(GB.v.num_neurons);
+        const bool pair = uni(GA.n_fin_e) == 1 && uni(GA.n_fin_i) == 1 && uni(GB.n_fin_e) == 1 &&
+                          uni(GB.n_fin_i) == 1 && __ballot(live && !inA && !inB) == 0;
+        if (pair) {
+            if (inA || inB) {
+                const double* pe = inB ? uni(GB.fin_e[0].partials) + (gid - ob) : uni(GA.fin_e[0].partials) + (gid - oa);
+                const double* pi = inB ? uni(GB.fin_i[0].partials) + (gid - ob) : uni(GA.fin_i[0].partials) + (gid - oa);
+                const uint32_t lde = inB ? uni(GB.fin_e[0].ld) : uni(GA.fin_e[0].ld);
+                const uint32_t ldi = inB ? uni(GB.fin_i[0].ld) : uni(GA.fin_i[0].ld);
+                const uint32_t ce = inB ? uni(GB.fin_e[0].chunks) : uni(GA.fin_e[0].chunks);
+                const uint32_t ci = inB ? uni(GB.fin_i[0].chunks) : uni(GA.fin_i[0].chunks);
+                evolve_parts_pair(pe, lde, ce, pi, ldi, ci, wave - kPartWave0, lane, fin);
+            }
+        } else if (live) {
+            for (uint32_t g = 0; g < P.n_groups; ++g) {
+                const EvolveGroup& G = P.g[g];
+                if (gid < G.v.offset || gid >= G.v.offset + G.v.num_neurons) continue;
+                evolve_parts(G, gid - G.v.offset, wave - kPartWave0, lane, fin);
+            }
+        }
+    } else if (wave == 1) {
+        // Wave 1: background (kernel_neuron_group_evolve.cu:11-83).
+        if (live) {
+            for (uint32_t g = 0; g < P.n_groups; ++g) {
+                const NeuronGroupView& v = P.g[g].v;
+                if (gid < v.offset || gid >= v.offset + v.num_neurons) continue;
+                const uint32_t lid = gid - v.offset;
+                float re = v.background_excitatory_postsynaptic_r_traces[lid];
+                float ri = v.background_inhibitory_postsynaptic_r_traces[lid];
+                float de = v.background_excitatory_postsynaptic_d_traces[lid];
+                float di = v.background_inhibitory_postsynaptic_d_traces[lid];
+                uint32_t he[4], hi[4];
+#pragma unroll
+                for (uint32_t k = 0; k < 4; ++k) {
+                    he[k] = k < v.poisson_table_excitatory_len ? v.poisson_table_excitatory[k] : 0u;
+                    hi[k] = k < v.poisson_table_inhibitory_len ? v.poisson_table_inhibitory[k] : 0u;
+                }
+                const bn_device_clock clk = *P.clock;
+                const uint32_t step = clk.clock_step + step_offset;
+                re = flush_trace(re * v.background_excitatory_postsynaptic_r_trace_decay);
+                ri = flush_trace(ri * v.background_inhibitory_postsynaptic_r_trace_decay);
+                de = flush_trace(de * v.background_excitatory_postsynaptic_d_trace_decay);
+                di = flush_trace(di * v.background_inhibitory_postsynaptic_d_trace_decay);
+                uint32_t ke = 0, ki = 0;
+                if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
+                    const U32x4 r = background_draws(clk, step, gid, v.rng_instance);
+                    if (v.mean_background_excitatory_spikes > 0.0f)
+                        ke = poisson_lookup(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len, he);
+                    if (v.mean_background_inhibitory_spikes > 0.0f)
+                        ki = poisson_lookup(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len, hi);
+                }
+                if (ke > 0u) {
+                    const float inc = (float)ke;
+                    re += inc;
+                    de += inc;
+                }
+                const float bg_ge = (de - re) * v.background_excitatory_scaling_factor;
+                if (ki > 0u) {
+                    const float inc = (float)ki;
+                    ri += inc;
+                    di += inc;
+                }
+                const float bg_gi = (di - ri) * v.background_inhibitory_scaling_factor;
+                background_store(v, lid, re, ri, de, di, bg_ge, bg_gi);
+                bgl[0][lane] = bg_ge;
+                bgl[1][lane] = bg_gi;
+            }
+        }
+    } else {
+        // Wave 2: plasticity traces, +1 of the previous step then this step's decay.
+        if (live) {
+            uint32_t spike_prev = 0;
+#pragma unroll
+            for (int g = 0; g < kMaxGroups; ++g) {
+                const NeuronGroupView& v = P.g[g].v;
+                if ((uint32_t)g < P.n_groups && gid >= v.offset && gid < v.offset + v.num_neurons)
+                    spike_prev = v.spikes[gid - v.offset];
+            }
+            float a[kMaxPlast][2], b[kMaxPlast][2];
+#pragma unroll
+            for (int q = 0; q < kMaxPlast; ++q) {
+                const PlastTraces& Q = P.pt[q];
+                a[q][0] = b[q][0] = a[q][1] = b[q][1] = 0.0f;
+                if ((uint32_t)q >= P.n_pt) continue;
+                const bool trip = Q.kind == kPlastTriplet;
+                if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
+                    a[q][0] = Q.pre[gid - Q.pre_offset];
+                    if (trip) b[q][0] = Q.trip_pre[gid - Q.pre_offset];
+                }
+                if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
+                    a[q][1] = Q.post[gid - Q.post_offset];
+                    if (trip) b[q][1] = Q.trip_post[gid - Q.post_offset];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kMaxPlast; ++q) {
+                const PlastTraces& Q = P.pt[q];
+                if ((uint32_t)q >= P.n_pt) continue;
+                const bool trip = Q.kind == kPlastTriplet;
+                if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
+                    const uint32_t i = gid - Q.pre_offset;
+                    Q.pre_out[i] = trace_step(a[q][0], spike_prev, Q.pre_decay);
+                    if (trip) Q.trip_pre_out[i] = trace_step(b[q][0], spike_prev, Q.trip_pre_decay);
+                }
+                if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
+                    const uint32_t j = gid - Q.post_offset;
+                    Q.post_out[j] = trace_step(a[q][1], spike_prev, Q.post_decay);
+                    if (trip) Q.trip_post_out[j] = trace_step(b[q][1], spike_prev, Q.trip_post_decay);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+
+    const bn_device_clock clk = *P.clock;
+    const uint32_t step = clk.clock_step + step_offset;
+    uint32_t spike = 0;
+    if (two_groups) {
+        if (in_group) {
+            if (stim_prev && P.ev_steps && step - 1 < P.ev_steps) {
+                // stimulus.cpp:46-60 for this neuron: the previous step's events in list order.
+                const uint32_t e1 = min(P.ev_start[step], P.ev_count);
+                bool hit = false;
+                for (uint32_t e = P.ev_start[step - 1]; e < e1; ++e)
+                    if (P.ev_ids[e] + P.ev_offsets[e] == gid) {
+                        istim += P.ev_amps[e];
+                        hit = true;
+                    }
+                if (hit) pIs[lid] = istim;
+            }
+#pragma unroll
+            for (int q = 0; q < kMaxSources; ++q)
+                if ((uint32_t)q < n_fe) ge = combine_parts(ge, chunks_e[q], fin[q], lane);
+#pragma unroll
+            for (int q = 0; q < kMaxSources; ++q)
+                if ((uint32_t)q < n_fi) gi_ = combine_parts(gi_, chunks_i[q], fin[kMaxSources + q], lane);
+            eif_core(e_dt, e_gl, e_c, e_slope, e_vt, e_ee, e_ei, e_vthr, e_vreset, e_vrest, e_refr, ge, gi_, istim,
+                     V, refr, spike);
+            pV[lid] = V;
+            pR[lid] = refr;
+            pS[lid] = spike;
+            pGe[lid] = bgl[0][lane];
+            pGi[lid] = bgl[1][lane];
+        }
+    } else if (live) {
         for (uint32_t g = 0; g < P.n_groups; ++g) {
-            const NeuronGroupView& v = P.g[g].v;
+            const EvolveGroup& G = P.g[g];
+            const NeuronGroupView& v = G.v;
             if (gid < v.offset || gid >= v.offset + v.num_neurons) continue;
             const uint32_t lid = gid - v.offset;
-            V = v.membrane_potentials[lid];
-            refr = v.refractory_counter[lid];
-            ge = v.g_excitatory[lid];
-            gi_ = v.g_inhibitory[lid];
-            istim = v.i_stimulus[lid];
-            if (stim_prev && step - 1 < P.ev_steps) {
+            if (stim_prev && P.ev_steps && step - 1 < P.ev_steps) {
                 // stimulus.cpp:46-60 for this neuron: the previous step's events in list order.
                 const uint32_t e1 = min(P.ev_start[step], P.ev_count);
                 bool hit = false;
@@ -142,132 +460,8 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk, 
                     }
                 if (hit) v.i_stimulus[lid] = istim;
             }
-        }
-#pragma unroll
-        for (int t = 0; t < kMaxTraceSets; ++t) {
-            const TraceSet& T = P.ts[t];
-            if ((uint32_t)t < P.n_ts && gid >= T.pre_offset && gid < T.pre_offset + T.n_pre) {
-                tr_r[t] = T.r[gid - T.pre_offset];
-                tr_d[t] = T.d[gid - T.pre_offset];
-            }
-        }
-        if (P.v_ring) vslot = P.vrec_index[gid];
-    }
-
-    // All waves: part sums of the propagate partials.
-    if (live) {
-        for (uint32_t g = 0; g < P.n_groups; ++g) {
-            const EvolveGroup& G = P.g[g];
-            if (gid < G.v.offset || gid >= G.v.offset + G.v.num_neurons) continue;
-            const uint32_t lid = gid - G.v.offset;
-            if (G.n_fin_e == 1 && G.n_fin_i == 1) {
-                // The common case, one excitatory and one inhibitory source: both parts' loads
-                // in flight together.
-                const FinalizeSource& Fe = G.fin_e[0];
-                const FinalizeSource& Fi = G.fin_i[0];
-                const uint32_t pe = fin_part_size(Fe.chunks), pi = fin_part_size(Fi.chunks);
-                const uint32_t e0 = min(wave * pe, Fe.chunks), i0 = min(wave * pi, Fi.chunks);
-                double se, si;
-                fin_part_sum_pair(Fe, lid, e0, min(e0 + pe, Fe.chunks), Fi, lid, i0, min(i0 + pi, Fi.chunks), se, si);
-                if (wave * pe < Fe.chunks) fin[0][wave][lane] = se;
-                if (wave * pi < Fi.chunks) fin[kMaxSources][wave][lane] = si;
-            } else {
-                for (uint32_t s = 0; s < G.n_fin_e + G.n_fin_i; ++s) {
-                    const FinalizeSource& F = s < G.n_fin_e ? G.fin_e[s] : G.fin_i[s - G.n_fin_e];
-                    const uint32_t slot = s < G.n_fin_e ? s : kMaxSources + (s - G.n_fin_e);
-                    const uint32_t per = fin_part_size(F.chunks);
-                    const uint32_t c0 = wave * per;
-                    if (c0 < F.chunks) fin[slot][wave][lane] = fin_part_sum(F, lid, c0, min(c0 + per, F.chunks));
-                }
-            }
-        }
-    }
-
-    // Wave 1: background (kernel_neuron_group_evolve.cu:11-83).
-    if (wave == 1 && live) {
-        for (uint32_t g = 0; g < P.n_groups; ++g) {
-            const NeuronGroupView& v = P.g[g].v;
-            if (gid < v.offset || gid >= v.offset + v.num_neurons) continue;
-            const uint32_t lid = gid - v.offset;
-            float re = v.background_excitatory_postsynaptic_r_traces[lid];
-            float ri = v.background_inhibitory_postsynaptic_r_traces[lid];
-            float de = v.background_excitatory_postsynaptic_d_traces[lid];
-            float di = v.background_inhibitory_postsynaptic_d_traces[lid];
-            uint32_t he[4], hi[4];
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) {
-                he[k] = k < v.poisson_table_excitatory_len ? v.poisson_table_excitatory[k] : 0u;
-                hi[k] = k < v.poisson_table_inhibitory_len ? v.poisson_table_inhibitory[k] : 0u;
-            }
-            re = flush_trace(re * v.background_excitatory_postsynaptic_r_trace_decay);
-            ri = flush_trace(ri * v.background_inhibitory_postsynaptic_r_trace_decay);
-            de = flush_trace(de * v.background_excitatory_postsynaptic_d_trace_decay);
-            di = flush_trace(di * v.background_inhibitory_postsynaptic_d_trace_decay);
-            uint32_t ke = 0, ki = 0;
-            if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
-                const U32x4 r = background_draws(clk, step, gid, v.rng_instance);
-                if (v.mean_background_excitatory_spikes > 0.0f)
-                    ke = poisson_lookup(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len, he);
-                if (v.mean_background_inhibitory_spikes > 0.0f)
-                    ki = poisson_lookup(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len, hi);
-            }
-            if (ke > 0u) {
-                const float inc = (float)ke;
-                re += inc;
-                de += inc;
-            }
-            const float bg_ge = (de - re) * v.background_excitatory_scaling_factor;
-            if (ki > 0u) {
-                const float inc = (float)ki;
-                ri += inc;
-                di += inc;
-            }
-            const float bg_gi = (di - ri) * v.background_inhibitory_scaling_factor;
-            background_store(v, lid, re, ri, de, di, bg_ge, bg_gi);
-            bgl[0][lane] = bg_ge;
-            bgl[1][lane] = bg_gi;
-        }
-    }
-
-    // Wave 2: plasticity traces, +1 of the previous step then this step's decay.
-    if (wave == 2 && live) {
-        uint32_t spike_prev = 0;
-        for (uint32_t g = 0; g < P.n_groups; ++g) {
-            const NeuronGroupView& v = P.g[g].v;
-            if (gid >= v.offset && gid < v.offset + v.num_neurons) spike_prev = v.spikes[gid - v.offset];
-        }
-#pragma unroll
-        for (int q = 0; q < kMaxPlast; ++q) {
-            const PlastTraces& Q = P.pt[q];
-            if ((uint32_t)q >= P.n_pt) continue;
-            if (gid >= Q.pre_offset && gid < Q.pre_offset + Q.n_pre) {
-                const uint32_t i = gid - Q.pre_offset;
-                const float a = Q.pre[i];
-                const float b = Q.kind == kPlastTriplet ? Q.trip_pre[i] : 0.0f;
-                Q.pre_out[i] = trace_step(a, spike_prev, Q.pre_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_pre_out[i] = trace_step(b, spike_prev, Q.trip_pre_decay);
-            }
-            if (gid >= Q.post_offset && gid < Q.post_offset + Q.n_post) {
-                const uint32_t j = gid - Q.post_offset;
-                const float a = Q.post[j];
-                const float b = Q.kind == kPlastTriplet ? Q.trip_post[j] : 0.0f;
-                Q.post_out[j] = trace_step(a, spike_prev, Q.post_decay);
-                if (Q.kind == kPlastTriplet) Q.trip_post_out[j] = trace_step(b, spike_prev, Q.trip_post_decay);
-            }
-        }
-    }
-    __syncthreads();
-    if (wave != 0) return;
-
-    uint32_t spike = 0;
-    if (live) {
-        for (uint32_t g = 0; g < P.n_groups; ++g) {
-            const EvolveGroup& G = P.g[g];
-            const NeuronGroupView& v = G.v;
-            if (gid < v.offset || gid >= v.offset + v.num_neurons) continue;
-            const uint32_t lid = gid - v.offset;
-            for (uint32_t s = 0; s < G.n_fin_e; ++s) ge = combine_parts(ge, G.fin_e[s], fin[s], lane);
-            for (uint32_t s = 0; s < G.n_fin_i; ++s) gi_ = combine_parts(gi_, G.fin_i[s], fin[kMaxSources + s], lane);
+            for (uint32_t q = 0; q < G.n_fin_e; ++q) ge = combine_parts(ge, G.fin_e[q].chunks, fin[q], lane);
+            for (uint32_t q = 0; q < G.n_fin_i; ++q) gi_ = combine_parts(gi_, G.fin_i[q].chunks, fin[kMaxSources + q], lane);
             eif_step(v, ge, gi_, istim, V, refr, spike);
             v.membrane_potentials[lid] = V;
             v.refractory_counter[lid] = refr;
@@ -275,6 +469,8 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk, 
             v.g_excitatory[lid] = bgl[0][lane];
             v.g_inhibitory[lid] = bgl[1][lane];
         }
+    }
+    if (live) {
 #pragma unroll
         for (int t = 0; t < kMaxTraceSets; ++t) {
             const TraceSet& T = P.ts[t];
@@ -305,7 +501,15 @@ __device__ __forceinline__ void evolve_block(const EvolvePlan& P, uint32_t blk, 
     if (vslot >= 0) P.v_ring[(size_t)(step % P.interval) * P.n_vrec + vslot] = V;
 }
 
-__global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan P) {
+// The plan is read in place from the kernel-argument segment, where the first explicit argument
+// starts at offset 0: reading the by-value parameter itself lets the compiler copy the whole
+// plan into private memory once the kernel reads enough of its fields.
+__device__ __forceinline__ const EvolvePlan& evolve_plan_arg() {
+    return *(const EvolvePlan*)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve(EvolvePlan) {
+    const EvolvePlan& P = evolve_plan_arg();
     evolve_block(P, blockIdx.x, P.step_offset, P.stim_prev);
 }
 
@@ -566,19 +770,102 @@ __device__ __forceinline__ uint32_t block_compact_bits(const uint32_t* bits, uin
     return total;
 }
 
-// LTP entries (i, js[0..n)) of a quiet row, in list order, with the post factors fs[0..n)
-// (ltp_factor of each j); the loads are issued together.
+// Spiking pre neurons of every plasticity in one pass: the set bits of each plasticity's pre
+// range [pre_offset, pre_offset + n_pre) of its spike bits, as pre-local ids, in one list in
+// plasticity order (each range in increasing order); counts[k] (LDS) receives the count of
+// plasticity k.  Returns the total (the same in every thread); entries past `cap` are not stored.
+__device__ __forceinline__ uint32_t block_compact_pre(const PlastPlan& pl, uint32_t* list, uint32_t cap,
+                                                      uint32_t* wsum, uint32_t* counts) {
+    uint32_t vend[kMaxPlast];  // end of plasticity k's words in the concatenated word sequence
+    uint32_t nv = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxPlast; ++k) {
+        if ((uint32_t)k < pl.n && pl.p[k].n_pre > 0)
+            nv += ((pl.p[k].pre_offset + pl.p[k].n_pre - 1) >> 5) - (pl.p[k].pre_offset >> 5) + 1;
+        vend[k] = nv;
+    }
+    const uint32_t t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (t < (uint32_t)kMaxPlast) counts[t] = 0;
+    __syncthreads();
+    uint32_t total = 0;
+    for (uint32_t base = 0; base < nv; base += blockDim.x) {
+        const uint32_t vi = base + t;
+        uint32_t word = 0, w = 0, k = 0, g0 = 0;
+        if (vi < nv) {
+            uint32_t vstart = 0;
+#pragma unroll
+            for (int kk = 0; kk + 1 < kMaxPlast; ++kk)
+                if (vi >= vend[kk]) {
+                    k = kk + 1;
+                    vstart = vend[kk];
+                }
+            const uint32_t* bits = nullptr;
+            uint32_t g1 = 0;
+#pragma unroll
+            for (int kk = 0; kk < kMaxPlast; ++kk)
+                if ((uint32_t)kk == k) {
+                    bits = uni(pl.p[kk].bits);
+                    g0 = uni(pl.p[kk].pre_offset);
+                    g1 = g0 + uni(pl.p[kk].n_pre);
+                }
+            w = (g0 >> 5) + (vi - vstart);
+            word = bits[w];
+            if (w == (g0 >> 5)) word &= 0xFFFFFFFFu << (g0 & 31);
+            if (w == ((g1 - 1) >> 5)) word &= 0xFFFFFFFFu >> (31 - ((g1 - 1) & 31));
+        }
+        const uint32_t c = (uint32_t)__popc(word);
+        uint32_t v = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t m = __shfl_up(v, off, 64);
+            if (lane >= (uint32_t)off) v += m;
+        }
+        if (lane == 63) wsum[wv] = v;
+        if (c) atomicAdd(&counts[k], c);
+        __syncthreads();
+        uint32_t wbase = 0, btot = 0;
+        const uint32_t nwaves = blockDim.x >> 6;
+#pragma unroll
+        for (uint32_t q = 0; q < kMaxBlockWaves; ++q) {
+            if (q >= nwaves) break;
+            if (q < wv) wbase += wsum[q];
+            btot += wsum[q];
+        }
+        uint32_t pos = total + wbase + v - c;
+        while (word) {
+            const uint32_t b = (uint32_t)__ffs(word) - 1;
+            word &= word - 1;
+            if (pos < cap) list[pos] = w * 32 + b - g0;
+            ++pos;
+        }
+        total += btot;
+        __syncthreads();
+    }
+    return total;
+}
+
+// LTP entries (i, js[0..n)) of a quiet row, in list order: quiet_row_load loads the entries and
+// the post traces of their factors (ltp_factor of each j) together, quiet_row_apply updates and
+// stores them.
 constexpr int kLtpBatch = 16;
-__device__ __forceinline__ void quiet_row_batch(const PlastDev& D, uint32_t i, float* row, float si, float pre_i,
-                                                const uint32_t* js, const float* fs, int n, double& dsum) {
-    float m[kLtpBatch];
+__device__ __forceinline__ void quiet_row_load(const PlastDev& D, const float* row, const uint32_t* js, int n,
+                                               float (&m)[kLtpBatch], float (&fs)[kLtpBatch]) {
+    const bool triplet = D.kind == kPlastTriplet;
 #pragma unroll
     for (int k = 0; k < kLtpBatch; ++k)
-        if (k < n) m[k] = row[js[k]];
+        if (k < n) {
+            m[k] = row[js[k]];
+            fs[k] = triplet ? D.trip_post[js[k]] : 0.0f;
+        }
+}
+__device__ __forceinline__ void quiet_row_apply(const PlastDev& D, uint32_t i, float* row, float si, float pre_i,
+                                                const uint32_t* js, int n, const float (&m)[kLtpBatch],
+                                                const float (&fs)[kLtpBatch], double& dsum) {
+    const bool triplet = D.kind == kPlastTriplet;
 #pragma unroll
     for (int k = 0; k < kLtpBatch; ++k) {
         if (k >= n || js[k] == i) continue;
-        const float delta = D.kind == kPlastTriplet ? pre_i * fs[k] : D.p.ltp_A * pre_i;
+        const float delta = triplet ? pre_i * (D.p.ltp_A + (D.p.ltp_B * fs[k])) : D.p.ltp_A * pre_i;
         const float mn = apply_delta(D, m[k], si, delta);
         row[js[k]] = mn;
         if (D.lazy) dsum += (double)mn - (double)m[k];
@@ -586,35 +873,36 @@ __device__ __forceinline__ void quiet_row_batch(const PlastDev& D, uint32_t i, f
 }
 
 // Row i whose pre neuron did NOT spike: the LTP entries of this step's spiking posts.  The list
-// of spiking posts (increasing post order, LDS, with their factors) is cut into kQuietParts
-// contiguous parts of ceil(count / kQuietParts); this call does part q in list order (straight
-// from the spike bits when the list overflowed) and returns the part's sum of the changes of M
-// (lazy), summed from 0.0.
+// of spiking posts (increasing post order, LDS) is cut into kQuietParts contiguous parts of
+// ceil(count / kQuietParts); this call does part q in list order (straight from the spike bits
+// when the list overflowed) and returns the part's sum of the changes of M (lazy), summed from
+// 0.0.  The row's scale, its pre trace and the first batch of entries are loaded together; a
+// row with s_i = 0 is left as it is.
 constexpr uint32_t kQuietParts = 4;
 constexpr uint32_t kQuietRows = kPlastQuietRows;
 __device__ __forceinline__ double plasticity_quiet_part(const PlastDev& D, uint32_t i, const uint32_t* list,
-                                                        const float* facs, uint32_t count, uint32_t q) {
+                                                        uint32_t count, uint32_t q) {
     const bool lazy = D.lazy != 0;
-    const float si = lazy ? D.s[i] : 1.0f;
     double dsum = 0.0;
     const uint32_t per = (count + kQuietParts - 1) / kQuietParts;
     const uint32_t e0 = q * per, e1 = min(count, e0 + per);
-    if (e0 >= e1 || (lazy && si == 0.0f)) return 0.0;
+    if (e0 >= e1) return 0.0;
+    const float si = lazy ? D.s[i] : 1.0f;
     float* row = D.W + (size_t)i * D.ld;
     const float pre_i = D.pre[i];
     uint32_t js[kLtpBatch];
-    float fs[kLtpBatch];
     if (count <= kListCap) {
         for (uint32_t k0 = e0; k0 < e1; k0 += kLtpBatch) {
             const int n = (int)min((uint32_t)kLtpBatch, e1 - k0);
 #pragma unroll
-            for (int k = 0; k < kLtpBatch; ++k) {
-                js[k] = k < n ? list[k0 + k] : 0u;
-                fs[k] = k < n ? facs[k0 + k] : 0.0f;
-            }
-            quiet_row_batch(D, i, row, si, pre_i, js, fs, n, dsum);
+            for (int k = 0; k < kLtpBatch; ++k) js[k] = k < n ? list[k0 + k] : 0u;
+            float m[kLtpBatch], fs[kLtpBatch];
+            quiet_row_load(D, row, js, n, m, fs);
+            if (lazy && si == 0.0f) return 0.0;
+            quiet_row_apply(D, i, row, si, pre_i, js, n, m, fs, dsum);
         }
     } else {
+        if (lazy && si == 0.0f) return 0.0;
         const uint32_t g0 = D.post_offset, g1 = D.post_offset + D.n_post;
         uint32_t rank = 0;
         for (uint32_t w = g0 >> 5; w <= (g1 - 1) >> 5 && rank < e1; ++w) {
@@ -626,8 +914,9 @@ __device__ __forceinline__ double plasticity_quiet_part(const PlastDev& D, uint3
                 bits &= bits - 1;
                 if (rank >= e0 && rank < e1) {
                     js[0] = w * 32 + b - g0;
-                    fs[0] = ltp_factor(D, js[0]);
-                    quiet_row_batch(D, i, row, si, pre_i, js, fs, 1, dsum);
+                    float m[kLtpBatch], fs[kLtpBatch];
+                    quiet_row_load(D, row, js, 1, m, fs);
+                    quiet_row_apply(D, i, row, si, pre_i, js, 1, m, fs, dsum);
                 }
                 ++rank;
             }
@@ -715,15 +1004,16 @@ __device__ __forceinline__ void stimulus_and_clock(const StimTail& T) {
     clk->clock_step = base + (T.advance ? T.advance : 1u);
 }
 
-// Blocks [0, row_blocks): kQuietRows rows each of the concatenated pre rows of all plasticities
+// Blocks [0, row_blocks): kQuietRows rows each of the pre rows of all plasticities, each
+// plasticity's rows padded to a multiple of kQuietRows so that a block serves one plasticity
 // (rows whose pre did not spike).  Blocks after: the rows whose pre spiked, one block each.
 // The last block also applies the stimulus and advances the clock (no block reads the clock).
 __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_t b, uint32_t nblocks) {
     __shared__ double lds4[4];
     __shared__ uint32_t wsum[kMaxBlockWaves];
     __shared__ uint32_t list[kListCap];
-    __shared__ float facs[kListCap];
     __shared__ double qsum[kQuietParts][64];
+    __shared__ uint32_t rcount[kMaxPlast];
     if (b < P.row_blocks) {
         // kQuietRows rows per block, lane = row; wave q applies part q of the spiking-post list.
         const uint32_t r0 = b * kQuietRows;
@@ -732,19 +1022,21 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
         for (uint32_t k = 0; k < P.pl.n; ++k) {
             const PlastDev& D = P.pl.p[k];
             if (r0 < base + D.n_pre && r0 + kQuietRows > base) {
+                // The row's own spike bit is loaded with the words the list is compacted from.
+                const uint32_t r = r0 + lane;
+                const bool in_rows = r >= base && r < base + D.n_pre;
+                const uint32_t own = in_rows ? D.pre_offset + (r - base) : 0u;
+                const uint32_t own_word = in_rows ? D.bits[own >> 5] : 0u;
                 const uint32_t count =
                     block_compact_bits(D.bits, D.post_offset, D.post_offset + D.n_post, list, kListCap, wsum);
-                for (uint32_t e = threadIdx.x; e < min(count, kListCap); e += blockDim.x) facs[e] = ltp_factor(D, list[e]);
-                __syncthreads();
-                const uint32_t r = r0 + lane;
-                const bool mine = r >= base && r < base + D.n_pre && !spike_bit(D.bits, D.pre_offset + (r - base));
+                const bool mine = in_rows && !((own_word >> (own & 31)) & 1u);
                 double R0 = 0.0;
                 float target = 0.0f;
                 if (D.lazy && q == 0 && mine) {
                     R0 = D.R[r - base];
                     target = D.target[r - base];
                 }
-                const double dq = mine && count ? plasticity_quiet_part(D, r - base, list, facs, count, q) : 0.0;
+                const double dq = mine && count ? plasticity_quiet_part(D, r - base, list, count, q) : 0.0;
                 if (D.lazy) {
                     if (q < kQuietParts) qsum[q][lane] = dq;
                     __syncthreads();
@@ -760,20 +1052,22 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
                 }
                 __syncthreads();
             }
-            base += D.n_pre;
+            base += (D.n_pre + kQuietRows - 1) / kQuietRows * kQuietRows;
         }
     } else if (b < P.row_blocks + P.spiking_blocks) {
         // Spiking rows of all plasticities in one sequence (plasticity order, increasing row);
-        // block sb takes the entries e = sb, sb + spiking_blocks, ...
+        // block sb takes the entries e = sb, sb + spiking_blocks, ...  The spiking pre neurons of
+        // every plasticity are compacted in one pass.
         const uint32_t sb = b - P.row_blocks;
         const uint32_t G = P.spiking_blocks;
+        block_compact_pre(P.pl, list, kListCap, wsum, rcount);
         uint32_t base = 0;
         for (uint32_t k = 0; k < P.pl.n; ++k) {
             const PlastDev& D = P.pl.p[k];
-            const uint32_t count = block_compact_bits(D.bits, D.pre_offset, D.pre_offset + D.n_pre, list, kListCap, wsum);
+            const uint32_t count = rcount[k];
             const uint32_t first = base + (sb + G - base % G) % G;
-            if (count <= kListCap) {
-                for (uint32_t e = first; e < base + count; e += G) plasticity_spiking_row(D, list[e - base], lds4);
+            if (base + count <= kListCap) {
+                for (uint32_t e = first; e < base + count; e += G) plasticity_spiking_row(D, list[e], lds4);
             } else {
                 uint32_t rank = 0;
                 for (uint32_t i = 0; i < D.n_pre; ++i)
@@ -783,7 +1077,6 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
                     }
             }
             base += count;
-            __syncthreads();
         }
     }
     if (P.tail.active && b == nblocks - 1 && threadIdx.x == 0) stimulus_and_clock(P.tail);
@@ -793,7 +1086,8 @@ __global__ __launch_bounds__(256) void k_plasticity(PlastStepPlan P) { plasticit
 
 // Overlapped step: blocks [0, n_evolve) evolve step t, the rest run the plasticity of step t-1
 // (its 256-thread body; the upper four waves take part in its barriers only).
-__global__ __launch_bounds__(kFinParts * 64) void k_evolve_plast(EvolvePlan E, PlastStepPlan P, uint32_t n_evolve) {
+__global__ __launch_bounds__(kFinParts * 64) void k_evolve_plast(EvolvePlan, PlastStepPlan P, uint32_t n_evolve) {
+    const EvolvePlan& E = evolve_plan_arg();  // the first argument
     if (blockIdx.x < n_evolve) evolve_block(E, blockIdx.x, E.step_offset, E.stim_prev);
     else plasticity_block(P, blockIdx.x - n_evolve, gridDim.x - n_evolve);
 }
