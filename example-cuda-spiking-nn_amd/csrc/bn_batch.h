@@ -14,6 +14,7 @@ namespace bn {
 constexpr int kBatchMaxGroups = 2;
 constexpr int kBatchMaxConns = 4;
 constexpr int kBatchGemmWaves = 4;   // row chunks (waves) per GEMM block, summed in LDS
+constexpr int kBatchGemmTileM = 64;  // post neurons per GEMM wave tile (ld is a multiple of 64)
 constexpr int kBatchAlign = 64;      // instance count padding (one wave = 64 instances)
 
 struct BatchPartials {

@@ -70,7 +70,7 @@ struct bn_batch {
     void plan_gemm(uint32_t rows) {
         if (rows == 0) {
             uint64_t work = 0;
-            for (int c = 0; c < 4; ++c) work += (uint64_t)cdiv_u(post[c], 32) * (B / 64) * pre[c];
+            for (int c = 0; c < 4; ++c) work += (uint64_t)cdiv_u(post[c], kBatchGemmTileM) * (B / 64) * pre[c];
             rows = (uint32_t)std::min<uint64_t>(512, std::max<uint64_t>(8, round_up_u((uint32_t)(work / 2048), 8)));
         }
         if (rows % 8 != 0 || rows == 0) throw Error(BN_ERR_INVALID_ARGUMENT, "GEMM rows per wave must be a positive multiple of 8");
@@ -91,7 +91,7 @@ struct bn_batch {
             C.kc = rows;
             C.n_chunks = cdiv_u(pre[c], rows);
             C.groups = cdiv_u(C.n_chunks, kBatchGemmWaves);
-            C.m_tiles = cdiv_u(post[c], 32);
+            C.m_tiles = cdiv_u(post[c], kBatchGemmTileM);
             C.n_pairs = B / 64;
             const size_t need = (size_t)C.groups * post[c] * B;
             if (need > part_cap[c]) throw Error(BN_ERR_INVALID_ARGUMENT, "GEMM rows per wave below the allocated minimum");

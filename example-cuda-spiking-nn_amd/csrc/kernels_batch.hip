@@ -143,14 +143,18 @@ __global__ __launch_bounds__(256) void k_batch_evolve(BatchEvolvePlan P) {
     }
 }
 
-// GEMM tile of one wave: 32 post neurons x 64 instances (two 32x32 accumulators) over a chunk
-// of kc pre rows, two rows per v_mfma_f32_32x32x2f32.  Operand lanes: A[m][k] = W[k][m] with
-// m = lane % 32 and k = lane / 32; B[k][n] = X[k][n] with n = lane % 32.  Result lane layout:
-// acc[i] is row m = 8*(i/4) + 4*(lane/32) + i%4, column lane % 32.
+// GEMM tile of one wave: 64 post neurons x 64 instances (four 32x32 accumulators: two post
+// halves x two instance halves) over a chunk of kc pre rows, two rows per
+// v_mfma_f32_32x32x2f32; each W load feeds two operations and so does each X load.  Operand
+// lanes: A[m][k] = W[k][m] with m = lane % 32 and k = lane / 32; B[k][n] = X[k][n] with
+// n = lane % 32.  Result lane layout: acc[i] is row m = 8*(i/4) + 4*(lane/32) + i%4, column
+// lane % 32.
 typedef float v16f __attribute__((ext_vector_type(16)));
+// Word 3 of a raw buffer resource on CDNA3/CDNA4 (32-bit data format, no swizzle, no stride).
+constexpr int kBufferRsrcWord3 = 0x00020000;
 
 __global__ __launch_bounds__(kBatchGemmWaves * 64) void k_batch_gemm(BatchGemmPlan P) {
-    __shared__ float red[kBatchGemmWaves][2][16][64];
+    __shared__ float red[kBatchGemmWaves][4][16][64];
     const uint32_t blk = blockIdx.x;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = threadIdx.x & 63;
@@ -163,58 +167,94 @@ __global__ __launch_bounds__(kBatchGemmWaves * 64) void k_batch_gemm(BatchGemmPl
     const uint32_t mt = (t / C.n_pairs) % C.m_tiles;
     const uint32_t grp = t / (C.n_pairs * C.m_tiles);
     const uint32_t chunk = grp * kBatchGemmWaves + wave;
-    const uint32_t m0 = mt * 32, n0 = np * 64;
+    const uint32_t m0 = mt * kBatchGemmTileM, n0 = np * 64;
     const uint32_t B = P.B;
-    const uint32_t ml = m0 + (lane & 31);
-    const bool m_ok = ml < C.n_post;
+    const uint32_t ml = m0 + (lane & 31);  // ml + 32 < ld: the padded tail of a row holds zeros
     const uint32_t kl = lane >> 5;
-    v16f acc0 = {}, acc1 = {};
     if (chunk < C.n_chunks) {
+        v16f acc00 = {}, acc01 = {}, acc10 = {}, acc11 = {};  // acc[post half][instance half]
         const uint32_t i0 = chunk * C.kc;
         const uint32_t i1 = min(i0 + C.kc, C.n_pre);
-        const float* wp = C.W + ml;
-        const float* xp = C.X + n0 + (lane & 31);
-        // Batches of kGemmPairs row pairs with every load of the batch issued before its
-        // matrix-core operations: a wave waits one memory latency per batch, so the batch is
-        // long enough (2 x kGemmPairs operations of 64 cycles) to cover it with the few waves
-        // a SIMD holds.  Rows past the chunk load as zeros.
-        constexpr int kGemmPairs = 16;
-        for (uint32_t i = i0; i < i1; i += 2 * kGemmPairs) {
-            float a[kGemmPairs], b0[kGemmPairs], b1[kGemmPairs];
+        const uint32_t last = i1 - 1;
+        // Buffer loads: a scalar resource per matrix (base and byte size, both wave-uniform) and
+        // one 32-bit byte offset per load, the +32 column in the instruction's immediate offset
+        // (weights 25 MB, traces 0.8 MB at 3,125 neurons: far below 4 GiB).
+        const __amdgpu_buffer_rsrc_t wr =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(C.W), 0, (int)(C.n_pre * C.ld * 4u), kBufferRsrcWord3);
+        const __amdgpu_buffer_rsrc_t xr =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(C.X), 0, (int)(C.n_pre * B * 4u), kBufferRsrcWord3);
+        const uint32_t ldw = C.ld, xcol = n0 + (lane & 31);
+        // Batches of kGemmPairs row pairs, double-buffered: the loads of the next batch are in
+        // flight while the matrix cores work on this one.  Every load is issued (rows past the
+        // chunk are clamped to its last row and their W operands are zeroed), so the in-order
+        // load counter lets each batch wait only for its own loads.
+        constexpr int kGemmPairs = 8;
+        constexpr uint32_t kRows = 2 * kGemmPairs;
+        typedef float Buf[4][kGemmPairs];  // W low half, W high half, X low half, X high half
+        Buf ba, bb;
+        auto load = [&](uint32_t base, Buf& b) {
 #pragma unroll
             for (int u = 0; u < kGemmPairs; ++u) {
-                const uint32_t row = i + 2 * u + kl;
-                const bool r_ok = row < i1;
-                a[u] = (m_ok && r_ok) ? wp[(size_t)row * C.ld] : 0.0f;
-                b0[u] = r_ok ? xp[(size_t)row * B] : 0.0f;
-                b1[u] = r_ok ? xp[(size_t)row * B + 32] : 0.0f;
+                const uint32_t row = min(base + 2 * u + kl, last);
+                const uint32_t wo = (row * ldw + ml) * 4u, xo = (row * B + xcol) * 4u;
+                b[0][u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(wr, wo, 0, 0));
+                b[1][u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(wr, wo + 128u, 0, 0));
+                b[2][u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(xr, xo, 0, 0));
+                b[3][u] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(xr, xo + 128u, 0, 0));
             }
+        };
+        auto mma = [&](uint32_t base, const Buf& b) {
 #pragma unroll
             for (int u = 0; u < kGemmPairs; ++u) {
-                acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b0[u], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a[u], b1[u], acc1, 0, 0, 0);
+                const bool ok = base + 2 * u + kl <= last;
+                const float w0 = ok ? b[0][u] : 0.0f, w1 = ok ? b[1][u] : 0.0f;
+                acc00 = __builtin_amdgcn_mfma_f32_32x32x2f32(w0, b[2][u], acc00, 0, 0, 0);
+                acc01 = __builtin_amdgcn_mfma_f32_32x32x2f32(w0, b[3][u], acc01, 0, 0, 0);
+                acc10 = __builtin_amdgcn_mfma_f32_32x32x2f32(w1, b[2][u], acc10, 0, 0, 0);
+                acc11 = __builtin_amdgcn_mfma_f32_32x32x2f32(w1, b[3][u], acc11, 0, 0, 0);
             }
+        };
+        // Pairs of batches and one loop exit (an odd count gets a trailing batch whose W
+        // operands are all zero: it adds exact zeros).  The scheduling barriers keep the
+        // compiler from sinking a prefetch below the MFMAs.
+        const uint32_t npairs = ((i1 - i0 + kRows - 1) / kRows + 1) / 2;
+        uint32_t i = i0;
+        load(i, ba);
+        for (uint32_t p = 0; p < npairs; ++p) {
+            load(min(i + kRows, last), bb);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(i, ba);
+            i += kRows;
+            load(min(i + kRows, last), ba);
+            __builtin_amdgcn_sched_barrier(0);
+            mma(i, bb);
+            i += kRows;
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            red[wave][0][r][lane] = acc00[r];
+            red[wave][1][r][lane] = acc01[r];
+            red[wave][2][r][lane] = acc10[r];
+            red[wave][3][r][lane] = acc11[r];
         }
     }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        red[wave][0][r][lane] = acc0[r];
-        red[wave][1][r][lane] = acc1[r];
-    }
+    // (A wave past the last chunk stores nothing: its slots are past nw below.)
     __syncthreads();
-    // Wave 0 adds the chunks of the block in chunk order and writes one partial per group.
-    if (wave == 0) {
+    // Wave q adds the chunks of the block for accumulator q, in chunk order, and writes one
+    // partial per group.
+    static_assert(kBatchGemmWaves == 4, "one wave per accumulator in the block reduction");
+    {
+        const uint32_t q = wave;
         const uint32_t nw = min((uint32_t)kBatchGemmWaves, C.n_chunks - grp * kBatchGemmWaves);
         float* out = C.part + (size_t)grp * C.n_post * B;
+        const uint32_t mh = q >> 1, h = q & 1;
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                float s = red[0][h][r][lane];
-                for (uint32_t w = 1; w < nw; ++w) s += red[w][h][r][lane];
-                const uint32_t m = m0 + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
-                if (m < C.n_post) out[(size_t)m * B + n0 + 32 * h + (lane & 31)] = s;
-            }
+        for (int r = 0; r < 16; ++r) {
+            float s = red[0][q][r][lane];
+            for (uint32_t w = 1; w < nw; ++w) s += red[w][q][r][lane];
+            const uint32_t m = m0 + 32 * mh + 8 * (r / 4) + 4 * (lane >> 5) + (r % 4);
+            if (m < C.n_post) out[(size_t)m * B + n0 + 32 * h + (lane & 31)] = s;
+        }
     }
     // The last block advances the step counter (no block of this launch reads it).
     if (blk == gridDim.x - 1 && threadIdx.x == 0) *P.step = *P.step + 1;
