@@ -106,37 +106,6 @@ __device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* t
 
 constexpr uint32_t kPartWave0 = 3;                          // first wave that sums partials
 constexpr uint32_t kPartWaves = kFinParts - kPartWave0;     // parts p, p + kPartWaves, ... per wave
-constexpr uint32_t kRangeBatch = 8;
-
-// Sums of R chunk ranges: range r adds p[r][c * ld[r]] for c in [c0[r], c1[r]) in increasing c,
-// from 0.0 (the order fin_part_sum uses); the loads of a batch of every range are in flight
-// together.
-template <int R>
-__device__ __forceinline__ void range_sums(const double* const (&p)[R], const uint32_t (&ld)[R],
-                                           const uint32_t (&c0)[R], const uint32_t (&c1)[R], double (&out)[R]) {
-    uint32_t n = 0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        out[r] = 0.0;
-        n = max(n, c1[r] > c0[r] ? c1[r] - c0[r] : 0u);
-    }
-    for (uint32_t b = 0; b < n; b += kRangeBatch) {
-        double v[R][kRangeBatch];
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (uint32_t k = 0; k < kRangeBatch; ++k) {
-                const uint32_t c = c0[r] + b + k;
-                v[r][k] = c < c1[r] ? p[r][(size_t)c * ld[r]] : 0.0;
-            }
-#pragma unroll
-        for (int r = 0; r < R; ++r)
-#pragma unroll
-            for (uint32_t k = 0; k < kRangeBatch; ++k)
-                if (c0[r] + b + k < c1[r]) out[r] += v[r][k];
-    }
-}
-
 // The parts of this wave (pw, pw + kPartWaves, ... < kFinParts) of every source of the group,
 // stored in fin[slot][part][lane] (slots 0.. for g_e sources, kMaxSources.. for g_i).
 // One excitatory and one inhibitory source, given per lane (a block may straddle two groups):
@@ -174,20 +143,6 @@ __device__ __forceinline__ void evolve_parts(const EvolveGroup& G, uint32_t lid,
                                              FinParts& fin) {
     for (uint32_t s = 0; s < G.n_fin_e; ++s) evolve_parts_source(G.fin_e[s], s, lid, pw, lane, fin);
     for (uint32_t s = 0; s < G.n_fin_i; ++s) evolve_parts_source(G.fin_i[s], kMaxSources + s, lid, pw, lane, fin);
-}
-
-// A plan field as a wave-uniform value.  Used where a lane selects among the fields of several
-// groups: the selected loads then stay scalar loads of the kernel arguments (a select between
-// two loads would otherwise become one load through a per-lane address into them).
-__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-__device__ __forceinline__ float uni(float x) { return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(x))); }
-template <typename T>
-__device__ __forceinline__ T* uni(T* p) {
-    // readfirstlane returns int: each half goes through uint32_t so the low half is not sign-extended.
-    const uint64_t a = reinterpret_cast<uint64_t>(p);
-    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)a);
-    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
-    return reinterpret_cast<T*>((hi << 32) | lo);
 }
 
 // The groups of a block's first and last neuron (wave-uniform).  Groups are contiguous global id

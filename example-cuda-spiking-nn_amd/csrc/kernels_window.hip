@@ -18,28 +18,59 @@
 
 namespace bn {
 
-// A block owns 64 neurons and has kFinParts waves: every wave sums one part of the propagate
-// partials of g_e and g_i for the 64 neurons, wave 1 also runs the background update; after the
+// A block owns 64 neurons and has kFinParts waves, each with one role whose loads all go out at
+// the start: wave 0 loads the neuron's state, traces and background g and the EIF parameters of
+// its group, wave 1 runs the background update, waves 2.. sum the parts of the propagate partials
+// of g_e and g_i (a block straddling E|I selects each lane's sources per lane); after the
 // barrier wave 0 combines the parts in order and runs the EIF step, the traces, the records.
+constexpr uint32_t kWinPartWave0 = 2;
+constexpr uint32_t kWinPartWaves = kFinParts - kWinPartWave0;
+
 __global__ __launch_bounds__(kEvolveNeurons * kFinParts) void k_win_forward(WinForwardPlan P) {
     __shared__ double fin[2][kFinParts][kEvolveNeurons];
     __shared__ float bgl[2][kEvolveNeurons];
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const uint32_t j = blockIdx.x * kEvolveNeurons + lane;
     const bool live = j < P.N;
     const bool exc = j < P.NE;
-    const WinGroup& G = P.g[exc ? 0 : 1];
+    const WinGroup& G0 = P.g[0];
+    const WinGroup& G1 = P.g[1];
     const uint32_t lid = exc ? j : j - P.NE;
     const WinState& S = P.st;
-    if (live) {
-        for (int q = 0; q < 2; ++q) {
-            const FinalizeSource& F = q == 0 ? G.fin_e : G.fin_i;
-            const uint32_t per = fin_part_size(F.chunks);
-            const uint32_t c0 = wave * per;
-            if (c0 < F.chunks) fin[q][wave][lane] = fin_part_sum(F, lid, c0, min(c0 + per, F.chunks));
+    float V = 0.0f, d0 = 0.0f, r0 = 0.0f, bg0 = 0.0f, bg1 = 0.0f;
+    uint32_t refr = 0;
+    float e_dt = 0.0f, e_c = 0.0f, e_slope = 0.0f, e_vt = 0.0f, e_ee = 0.0f, e_ei = 0.0f, e_vthr = 0.0f,
+          e_vreset = 0.0f, e_vrest = 0.0f;
+    uint32_t e_refr = 0;
+    float g_leak = 0.0f, decay_e_d = 0.0f;
+    int32_t vs = -1;
+    if (wave == 0) {
+        g_leak = P.prm->g_leak;
+        decay_e_d = P.prm->decay_e_d;
+        if (live) {
+            vs = P.vrec_slot[j];
+            V = S.V[j];
+            refr = S.refr[j];
+            d0 = (exc ? S.dE : S.dI)[lid];
+            r0 = (exc ? S.rE : S.rI)[lid];
+            bg0 = S.bg_ge[j];
+            bg1 = S.bg_gi[j];
         }
-        if (wave == 1) {
-            const NeuronGroupView& v = G.v;
+        const NeuronGroupView& A = G0.v;
+        const NeuronGroupView& B = G1.v;
+        e_dt = exc ? uni(A.timestep) : uni(B.timestep);
+        e_c = exc ? uni(A.membrane_capacitance) : uni(B.membrane_capacitance);
+        e_slope = exc ? uni(A.depolarization_slope_factor) : uni(B.depolarization_slope_factor);
+        e_vt = exc ? uni(A.v_effective_spike_threshold) : uni(B.v_effective_spike_threshold);
+        e_ee = exc ? uni(A.v_excitatory_reversal) : uni(B.v_excitatory_reversal);
+        e_ei = exc ? uni(A.v_inhibitory_reversal) : uni(B.v_inhibitory_reversal);
+        e_vthr = exc ? uni(A.v_threshold) : uni(B.v_threshold);
+        e_vreset = exc ? uni(A.v_reset) : uni(B.v_reset);
+        e_vrest = exc ? uni(A.v_rest) : uni(B.v_rest);
+        e_refr = exc ? uni(A.refractory_steps) : uni(B.refractory_steps);
+    } else if (wave == 1) {
+        if (live) {
+            const NeuronGroupView& v = exc ? G0.v : G1.v;
             float re = S.bre[j], ri = S.bri[j], de = S.bde[j], di = S.bdi[j], bg_ge, bg_gi;
             background_advance(v, j, P.step, P.clock, v.poisson_table_excitatory, v.poisson_table_inhibitory, re, ri,
                                de, di, bg_ge, bg_gi);
@@ -47,6 +78,28 @@ __global__ __launch_bounds__(kEvolveNeurons * kFinParts) void k_win_forward(WinF
             bgl[0][lane] = bg_ge;
             bgl[1][lane] = bg_gi;
         }
+    } else if (live) {
+        // Parts pw and pw + kWinPartWaves of both sources, their loads in flight together.
+        const uint32_t pw = wave - kWinPartWave0, pb = pw + kWinPartWaves;
+        const bool has_b = pb < (uint32_t)kFinParts;
+        const double* pe = (exc ? uni(G0.fin_e.partials) : uni(G1.fin_e.partials)) + lid;
+        const double* pi = (exc ? uni(G0.fin_i.partials) : uni(G1.fin_i.partials)) + lid;
+        const uint32_t lde = exc ? uni(G0.fin_e.ld) : uni(G1.fin_e.ld), ldi = exc ? uni(G0.fin_i.ld) : uni(G1.fin_i.ld);
+        const uint32_t ce = exc ? uni(G0.fin_e.chunks) : uni(G1.fin_e.chunks);
+        const uint32_t ci = exc ? uni(G0.fin_i.chunks) : uni(G1.fin_i.chunks);
+        const uint32_t se = fin_part_size(ce), si = fin_part_size(ci);
+        const double* const ptr[4] = {pe, pi, pe, pi};
+        const uint32_t ld[4] = {lde, ldi, lde, ldi};
+        const uint32_t c0[4] = {min(pw * se, ce), min(pw * si, ci), has_b ? min(pb * se, ce) : 0u,
+                                has_b ? min(pb * si, ci) : 0u};
+        const uint32_t c1[4] = {min(c0[0] + se, ce), min(c0[1] + si, ci), has_b ? min(c0[2] + se, ce) : 0u,
+                                has_b ? min(c0[3] + si, ci) : 0u};
+        double out[4];
+        range_sums<4>(ptr, ld, c0, c1, out);
+        if (pw * se < ce) fin[0][pw][lane] = out[0];
+        if (pw * si < ci) fin[1][pw][lane] = out[1];
+        if (has_b && pb * se < ce) fin[0][pb][lane] = out[2];
+        if (has_b && pb * si < ci) fin[1][pb][lane] = out[3];
     }
     __syncthreads();
     if (wave != 0) return;
@@ -56,18 +109,16 @@ __global__ __launch_bounds__(kEvolveNeurons * kFinParts) void k_win_forward(WinF
         // order: fl32(g + part_0 + part_1 + ...) in FP64).
         float g2[2];
         for (int q = 0; q < 2; ++q) {
-            const FinalizeSource& F = q == 0 ? G.fin_e : G.fin_i;
-            const uint32_t per = fin_part_size(F.chunks);
-            double acc = (double)(q == 0 ? S.bg_ge[j] : S.bg_gi[j]);
-            for (uint32_t w = 0; per && w * per < F.chunks; ++w) acc += fin[q][w][lane];
+            const uint32_t chunks = q == 0 ? (exc ? uni(G0.fin_e.chunks) : uni(G1.fin_e.chunks))
+                                           : (exc ? uni(G0.fin_i.chunks) : uni(G1.fin_i.chunks));
+            const uint32_t per = fin_part_size(chunks);
+            double acc = (double)(q == 0 ? bg0 : bg1);
+            for (uint32_t w = 0; per && w * per < chunks; ++w) acc += fin[q][w][lane];
             g2[q] = (float)acc;
         }
         const float ge = g2[0], gi = g2[1];
-        float V = S.V[j];
-        uint32_t refr = S.refr[j];
         float* dT = exc ? S.dE : S.dI;
         float* rT = exc ? S.rE : S.rI;
-        const float d0 = dT[lid], r0 = rT[lid];
         if (P.rec.Vprev) {
             const size_t k = (size_t)P.slot * P.N + j;
             P.rec.Vprev[k] = V;
@@ -78,15 +129,14 @@ __global__ __launch_bounds__(kEvolveNeurons * kFinParts) void k_win_forward(WinF
             (exc ? P.rec.dE : P.rec.dI)[(size_t)P.slot * n + lid] = d0;
             (exc ? P.rec.rE : P.rec.rI)[(size_t)P.slot * n + lid] = r0;
         }
-        NeuronGroupView v = G.v;
-        v.g_leak = P.prm->g_leak;
-        eif_step(v, ge, gi, 0.0f, V, refr, spike);
+        eif_core(e_dt, g_leak, e_c, e_slope, e_vt, e_ee, e_ei, e_vthr, e_vreset, e_vrest, e_refr, ge, gi, 0.0f,
+                 V, refr, spike);
         S.V[j] = V;
         S.refr[j] = refr;
         S.bg_ge[j] = bgl[0][lane];
         S.bg_gi[j] = bgl[1][lane];
         // Connection traces (kernel_connection_propagate.cu:8-46).
-        const float ad = exc ? P.prm->decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
+        const float ad = exc ? decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
         float dd = flush_trace(d0 * ad), rr = flush_trace(r0 * ar);
         if (spike == 1) {
             dd += P.K;
@@ -97,7 +147,6 @@ __global__ __launch_bounds__(kEvolveNeurons * kFinParts) void k_win_forward(WinF
         (exc ? S.xE : S.xI)[lid] = dd - rr;
         if (P.rec.Vprev) P.rec.s[(size_t)P.slot * P.N + j] = (float)spike;
         if (P.raster) P.raster[j] = (uint8_t)spike;
-        const int32_t vs = P.vrec_slot[j];
         if (vs >= 0 && P.v_rec) P.v_rec[vs] = V;
     }
     if (P.counts) {
