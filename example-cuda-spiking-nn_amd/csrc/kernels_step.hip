@@ -151,10 +151,12 @@ __device__ __forceinline__ void block_groups(const EvolvePlan& P, uint32_t blk, 
     const uint32_t first = blk * kEvolveNeurons;
     const uint32_t last = min(first + (uint32_t)kEvolveNeurons, P.total_neurons) - 1;
     g_lo = g_hi = 0;
-    for (uint32_t g = 0; g < P.n_groups; ++g) {
+#pragma unroll
+    for (int g = 0; g < kMaxGroups; ++g) {  // unrolled: the offset and size loads go out together
         const uint32_t o = P.g[g].v.offset, n = P.g[g].v.num_neurons;
-        if (first >= o && first - o < n) g_lo = g;
-        if (last >= o && last - o < n) g_hi = g;
+        const bool on = (uint32_t)g < P.n_groups;
+        if (on && first >= o && first - o < n) g_lo = g;
+        if (on && last >= o && last - o < n) g_hi = g;
     }
 }
 
