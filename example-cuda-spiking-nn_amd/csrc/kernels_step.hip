@@ -1313,17 +1313,39 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, uint32_t value, s
 // =======================================================================================
 // Read-bandwidth probe: every float4 of the buffer read once per pass (grid-stride), folded into
 // one value per thread so the loads are kept.
+// Streaming read of a buffer: kProbeLoads independent float4 loads per thread and pass, all in
+// flight together (the grid is sized so that one pass covers the buffer when it can); with the
+// propagate's cache policy (non-temporal loads above 192 MB).
+constexpr uint32_t kProbeLoads = 8;
+template <bool kNT>
 __global__ __launch_bounds__(256) void k_read_probe(const float4* __restrict__ p, size_t n4, float* sink) {
-    float acc = 0.0f;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
-        const float4 v = p[i];
-        acc += (v.x + v.y) + (v.z + v.w);
+    float acc[kProbeLoads] = {};
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + (kProbeLoads - 1) * stride < n4; i += kProbeLoads * stride) {
+        float4 v[kProbeLoads];
+#pragma unroll
+        for (uint32_t u = 0; u < kProbeLoads; ++u) v[u] = load_w4<kNT>(reinterpret_cast<const float*>(p + i + u * stride));
+#pragma unroll
+        for (uint32_t u = 0; u < kProbeLoads; ++u) acc[u] += (v[u].x + v[u].y) + (v[u].z + v[u].w);
     }
-    if (acc == 12345.678f) sink[blockIdx.x] = acc;  // practically never; the sum must not be dead
+    for (; i < n4; i += stride) acc[0] += (p[i].x + p[i].y) + (p[i].z + p[i].w);
+    float a = 0.0f;
+#pragma unroll
+    for (uint32_t u = 0; u < kProbeLoads; ++u) a += acc[u];
+    if (a == 12345.678f) sink[blockIdx.x] = a;  // practically never; the sum must not be dead
 }
 
+// `blocks` bounds the grid (and the sink's length); fewer are launched when kProbeLoads loads
+// per thread cover the buffer in one pass.
 int launch_read_probe(const void* buf, uint64_t bytes, float* sink, uint32_t blocks, bn_stream_t s) {
-    hipLaunchKernelGGL(k_read_probe, dim3(blocks), dim3(256), 0, s, static_cast<const float4*>(buf), (size_t)(bytes / 16), sink);
+    const uint64_t n4 = bytes / 16;
+    const uint64_t one_pass = (n4 + 256ull * kProbeLoads - 1) / (256ull * kProbeLoads);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, one_pass));
+    if (bytes > (192ull << 20))
+        hipLaunchKernelGGL(k_read_probe<true>, dim3(grid), dim3(256), 0, s, static_cast<const float4*>(buf), (size_t)n4, sink);
+    else
+        hipLaunchKernelGGL(k_read_probe<false>, dim3(grid), dim3(256), 0, s, static_cast<const float4*>(buf), (size_t)n4, sink);
     return (int)hipGetLastError();
 }
 
