@@ -260,10 +260,14 @@ def bench_uneven(args) -> int:
     sizes = [c["num_excitatory_neurons"] for c in cfgs]
     with tempfile.TemporaryDirectory() as d:
         tuned = {}
+        cap_times = {}
         if not args.no_autotune:
-            tuned = wl.autotune_row_blocks(eng, sizes, d)
+            # Row blocks tuned on the whole multi-network step: one cap for all sizes.
+            cap, cap_times = wl.autotune_row_block_cap(eng, cfgs, d)
             for c in cfgs:
-                c["gemv_row_blocks"] = [tuned[c["num_excitatory_neurons"]][0]] * 4
+                n_e = c["num_excitatory_neurons"]
+                tuned[n_e] = (wl.row_block_for(n_e, cap), {})
+                c["gemv_row_blocks"] = [tuned[n_e][0]] * 4
         paths = []
         for k, c in enumerate(cfgs):
             paths.append(os.path.join(d, f"net{k}.json"))
@@ -289,13 +293,18 @@ def bench_uneven(args) -> int:
             rates[c["num_excitatory_neurons"]].append(k)
         per_size = {}
         for n_e, idx in sorted(rates.items()):
-            sim = eng.Simulation(paths[idx[0]], flags=("no_monitors", "quiet", "lazy_normalize"))
+            # The sequential alternative runs each network as a plain run would: the engine's
+            # default row blocks for a single network, not the batch's.
+            plain_cfg = dict(cfgs[idx[0]])
+            plain_cfg.pop("gemv_row_blocks", None)
+            plain_path = os.path.join(d, f"plain_{n_e}.json")
+            json.dump(plain_cfg, open(plain_path, "w"))
+            sim = eng.Simulation(plain_path, flags=("no_monitors", "quiet", "lazy_normalize"))
             plain_steps = min(args.steps, 2000)
             pms = sim.run_timed(plain_steps)
             sim.close()
             per_size[str(n_e)] = {"networks": len(idx), "plain_single_steps_per_s": round(plain_steps / (pms / 1000.0), 1),
-                                  "gemv_row_block": tuned.get(n_e, (None,))[0],
-                                  "autotune_us_per_step": {str(k): round(v, 2) for k, v in tuned.get(n_e, (None, {}))[1].items()}}
+                                  "gemv_row_block": tuned.get(n_e, (None,))[0]}
         multi.close()
     net_steps_per_s = len(cfgs) * args.steps / (ms / 1000.0)
     seq_s_per_step = sum(len(v) / per_size[str(n_e)]["plain_single_steps_per_s"] for n_e, v in rates.items())
@@ -318,7 +327,8 @@ def bench_uneven(args) -> int:
         "config": {"workload": f"batched_uneven ({len(cfgs)} plastic networks, N_E in {{400,900,1600,2500}}, "
                                "triplet STDP + iSTDP, lazy normalisation)",
                    "networks": len(cfgs), "weight_entries": n_w, "kernels_per_step": int(minfo.kernels_per_step),
-                   "per_size": per_size, "schedule": sched, "groups": int(minfo.num_groups),
+                   "per_size": per_size, "row_block_cap_us_per_step": {str(k): round(v, 2) for k, v in cap_times.items()},
+                   "schedule": sched, "groups": int(minfo.num_groups),
                    "parallelism": "single GPU"},
         "vs_sequential_plain": round(net_steps_per_s / seq_rate, 2),
         "sequential_plain_network_steps_per_s": round(seq_rate, 1),

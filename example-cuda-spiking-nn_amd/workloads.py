@@ -97,6 +97,41 @@ def autotune_row_blocks(engine, sizes, workdir: str, candidates=(32, 64, 128, 25
     return out
 
 
+def row_block_for(n_e: int, cap: int) -> int:
+    """The GEMV row block of a network with n_e excitatory neurons under a cap: the largest power
+    of two <= min(cap, n_e / 4) (at least 8)."""
+    limit = max(8, min(cap, n_e // 4))
+    rb = 8
+    while rb * 2 <= limit:
+        rb *= 2
+    return rb
+
+
+def autotune_row_block_cap(engine, cfgs: list, workdir: str, caps=(32, 64, 128, 256), warmup: int = 100,
+                           steps: int = 300, streams: int = 4, steps_per_graph: int = 50) -> tuple:
+    """The batched plastic step's tile autotuner in its own context: for each cap, every network
+    gets row_block_for(n_e, cap) and the whole multi-network step of all networks is timed (the
+    propagate of many networks is bandwidth-bound, where single-size tuning favours small
+    blocks).  The row block fixes the FP64 summation order, so it is written into each config
+    (gemv_row_blocks) and a plain run of that config reproduces the network bit for bit.
+    Returns (best_cap, {cap: us_per_step})."""
+    times = {}
+    for cap in caps:
+        paths = []
+        for k, c in enumerate(cfgs):
+            c = dict(c)
+            c["gemv_row_blocks"] = [row_block_for(c["num_excitatory_neurons"], cap)] * 4
+            paths.append(os.path.join(workdir, f"cap{cap}_{k}.json"))
+            json.dump(c, open(paths[-1], "w"))
+        multi = engine.MultiSimulation(paths, flags=("lazy_normalize",))
+        multi.set_streams(streams, steps_per_graph, False)
+        multi.run_timed(warmup)
+        times[cap] = multi.run_timed(steps) * 1000.0 / steps
+        multi.close()
+    best = min(times, key=times.get)
+    return best, times
+
+
 def autotune_schedule(multi, group_mib=(0, 256, 128), steps_per_graph=(10, 50), streams=(1, 2, 3, 4),
                       overlap=(False, True), steps: int = 200) -> dict:
     """The batched plastic step's schedule autotuner: time `steps` steps of every network under

@@ -136,3 +136,13 @@ def test_output_path_prefixing_quirk(tmp_path):
     assert c["e_spike_recording_file"] == "../out/run1/rec/spikes.txt"
     assert c["i_spike_recording_file"] == "../elsewhere/i.txt"
     assert c["voltage_recording_file"] == "/abs/v.txt"
+
+
+def test_row_block_cap_rule():
+    """The uneven batch's row-block rule: the largest power of two <= min(cap, N_E / 4), at least 8."""
+    import importlib
+    wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+    assert [wl.row_block_for(n, 128) for n in (400, 900, 1600, 2500)] == [64, 128, 128, 128]
+    assert [wl.row_block_for(n, 32) for n in (400, 900, 1600, 2500)] == [32, 32, 32, 32]
+    assert [wl.row_block_for(n, 256) for n in (100, 400, 1600, 2500)] == [16, 64, 256, 256]
+    assert wl.row_block_for(8, 256) == 8
