@@ -400,6 +400,10 @@ public:
 
     // Fused step: build the plans, optionally capture `steps_per_graph` steps in a hipGraph.
     void enable_fused(bool use_graph, uint32_t steps_per_graph);
+    // Steps per captured graph of the fused step (even: the overlapped form); re-captures the
+    // graph only (plans and state are untouched).
+    void set_steps_per_graph(uint32_t steps_per_graph);
+    uint32_t steps_per_graph() const { return steps_per_graph_; }
     bool fused() const { return fused_; }
     void launch_fused_steps(uint32_t n);           // enqueue n steps (graph or eager)
     void launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end);
@@ -461,6 +465,7 @@ private:
     TailPlan tail_{};
     std::vector<int> conn_trace_set_;
     uint32_t kernels_per_step_ = 0;
+    void capture_graph();  // the graph of steps_per_graph_ steps in the current form
     // Overlapped graphs: even steps of a graph evolve from the current spike bits and
     // plasticity traces into the staging ones, odd steps back; the plasticity of a step reads
     // the buffers its evolve wrote; the last plasticity of a graph carries the stimulus/clock

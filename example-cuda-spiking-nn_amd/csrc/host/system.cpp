@@ -367,6 +367,15 @@ void System::enable_fused(bool use_graph, uint32_t steps_per_graph) {
     for (auto& m : monitors_) m->fused = true;
     use_graph_ = use_graph;
     steps_per_graph_ = std::max<uint32_t>(1, steps_per_graph);
+    capture_graph();
+}
+
+void System::capture_graph() {
+    if (graph_exec_) BN_HIP_CHECK(hipGraphExecDestroy(graph_exec_), "hipGraphExecDestroy");
+    if (graph_) BN_HIP_CHECK(hipGraphDestroy(graph_), "hipGraphDestroy");
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+    kernels_per_step_ = 1 + (gemv_.total_tiles ? 1 : 0) + 1 + (tail_.norm_rows ? 1 : 0);
     overlap_ = use_graph_ && state_.options().overlap_steps && tail_.norm_rows == 0 && steps_per_graph_ % 2 == 0;
     if (overlap_) {
         prepare_overlap(steps_per_graph_);
@@ -382,6 +391,14 @@ void System::enable_fused(bool use_graph, uint32_t steps_per_graph) {
         BN_HIP_CHECK(hipStreamEndCapture(s, &graph_), "hipStreamEndCapture");
         BN_HIP_CHECK(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
     }
+}
+
+void System::set_steps_per_graph(uint32_t steps_per_graph) {
+    if (!fused_) throw Error(BN_ERR_RUNTIME, "steps per graph: the component path has no graph");
+    if (steps_per_graph == 0) throw Error(BN_ERR_INVALID_ARGUMENT, "steps per graph must be at least 1");
+    BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "pre-capture sync");
+    steps_per_graph_ = steps_per_graph;
+    capture_graph();
 }
 
 void System::launch_fused_steps(uint32_t n) {

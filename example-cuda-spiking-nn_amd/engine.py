@@ -102,7 +102,7 @@ EXPORTED_SYMBOLS = [
     "bn_sim_create_from_json_text", "bn_sim_destroy", "bn_sim_run", "bn_sim_step_components",
     "bn_sim_synchronize", "bn_sim_get_info", "bn_sim_get_neuron_state", "bn_sim_get_weights",
     "bn_sim_get_connection_traces", "bn_sim_get_plasticity_traces", "bn_sim_run_timed",
-    "bn_sim_time_propagate_kernel", "bn_config_dump_json", "bn_generate_weight_matrix",
+    "bn_sim_time_propagate_kernel", "bn_sim_set_steps_per_graph", "bn_config_dump_json", "bn_generate_weight_matrix",
     "bn_batch_create", "bn_batch_destroy", "bn_batch_run", "bn_batch_synchronize", "bn_batch_run_timed",
     "bn_batch_time_gemm", "bn_batch_set_gemm_rows", "bn_batch_get_info", "bn_batch_get_state",
     "bn_batch_workload_instance", "bn_multi_create", "bn_multi_destroy", "bn_multi_run", "bn_multi_run_timed",
@@ -147,6 +147,7 @@ def lib() -> C.CDLL:
     L.bn_sim_get_plasticity_traces.argtypes = [P, U, fp, fp, fp, fp]
     L.bn_sim_run_timed.argtypes = [P, U, fp]
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
+    L.bn_sim_set_steps_per_graph.argtypes = [P, U]
     L.bn_measure_read_bandwidth.argtypes = [C.c_uint64, U, fp]
     L.bn_sim_save_state.argtypes = [P, C.c_char_p]
     L.bn_sim_load_state.argtypes = [P, C.c_char_p]
@@ -302,6 +303,10 @@ class Simulation:
         ms = C.c_float(0)
         _check(lib().bn_sim_run_timed(self.h, n, C.byref(ms)))
         return ms.value
+
+    def set_steps_per_graph(self, n: int) -> None:
+        """Steps per captured graph of the fused step (default 100; even: overlapped form)."""
+        _check(lib().bn_sim_set_steps_per_graph(self.h, n))
 
     def time_propagate_kernel(self, n: int):
         k, s = C.c_float(0), C.c_float(0)

@@ -350,6 +350,11 @@ int bn_sim_get_plasticity_traces(bn_sim* sim, uint32_t plasticity_offset, float*
                                  float* triplet_pre, float* triplet_post);
 /* Times `num_steps` steps between two HIP events on the simulation stream; *ms = elapsed. */
 int bn_sim_run_timed(bn_sim* sim, uint32_t num_steps, float* ms);
+/* Steps per captured graph of the fused step (no counterpart in the reference, whose host loop
+ * launches every kernel): default 100; an even count runs the overlapped form (the plasticity of
+ * step t-1 in the launch of the evolve of step t), an odd one the sequential form.  Re-captures
+ * the graph only; the state is untouched.  Runs shorter than a graph are stepped eagerly. */
+int bn_sim_set_steps_per_graph(bn_sim* sim, uint32_t steps_per_graph);
 /* Kernel timing on the simulation stream.  *step_ms = wall time of `num_steps` whole steps
  * launched one by one (no graph; the simulation advances by num_steps).  *kernel_ms = wall time
  * of `num_steps` back-to-back launches of the propagate (GEMV) kernel alone over the current

@@ -78,6 +78,7 @@ def test_graph_steps_bit_exact(tmp_path, mode):
     flags = ("no_monitors", "quiet") + (("lazy_normalize",) if lazy else ()) + \
         (("sequential_graph",) if mode == "lazy-sequential" else ())
     sim = engine().Simulation(path, flags=flags)
+    sim.set_steps_per_graph(10)
     orc = oracle_for(path, sim, lazy=lazy)
     for chunk in range(25):
         sim.run(10)
@@ -91,6 +92,29 @@ def test_graph_steps_bit_exact(tmp_path, mode):
     orc.step(20)
     assert_state_equal(sim.state(), orc.state(), 272)
     _check_traces_and_weights(sim, orc)
+    sim.close()
+
+
+@pytest.mark.parametrize("spg", [100, 7, 2], ids=["default-100", "odd-7-sequential", "2"])
+def test_steps_per_graph_bit_exact(tmp_path, spg):
+    # The default graphs of 100 steps (overlapped form), an odd count (sequential form) and the
+    # shortest overlapped graph, each with an eager remainder, against the oracle; then the
+    # length changed mid-run.
+    path = _write(tmp_path, small_config(steps=600))
+    sim = engine().Simulation(path, flags=("no_monitors", "quiet", "lazy_normalize"))
+    if spg != 100:
+        sim.set_steps_per_graph(spg)
+    orc = oracle_for(path, sim, lazy=True)
+    sim.run(2 * spg + 3)
+    orc.step(2 * spg + 3)
+    assert_state_equal(sim.state(), orc.state(), 2 * spg + 2)
+    sim.set_steps_per_graph(10)
+    sim.run(40)
+    orc.step(40)
+    assert_state_equal(sim.state(), orc.state(), 2 * spg + 42)
+    _check_traces_and_weights(sim, orc)
+    with pytest.raises(engine().BnError):
+        sim.set_steps_per_graph(0)
     sim.close()
 
 
