@@ -192,3 +192,16 @@ def test_workload_3125_raster_1000_steps(tmp_path):
     w_gpu, w_orc = sim.weights(0), orc.weights(0)
     assert np.array_equal(w_gpu, w_orc)
     sim.close()
+
+
+def test_read_bandwidth_probe():
+    # The streaming-read probe the bench reports beside the propagate: one-pass and multi-pass
+    # grids, the non-temporal form above 192 MB, and the argument checks.
+    eng = engine()
+    for nbytes in (1 << 20, 39_062_500 // 16 * 16, 256 << 20):
+        gbs = eng.measure_read_bandwidth(nbytes, 5)
+        assert 10.0 < gbs < 20000.0, (nbytes, gbs)
+    with pytest.raises(eng.BnError):
+        eng.measure_read_bandwidth(8, 1)
+    with pytest.raises(eng.BnError):
+        eng.measure_read_bandwidth(1 << 20, 0)
