@@ -65,6 +65,16 @@ __device__ __forceinline__ uint32_t poisson_from_table(uint32_t u, const uint32_
     return k;
 }
 
+// poisson_from_table with the first four entries already in registers (loaded with the other
+// inputs of the step): min{k : u < table[k]}, the same count.
+__device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* table, uint32_t len,
+                                                   const uint32_t (&head)[4]) {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k)
+        if (k >= len || u < head[k]) return k;
+    return poisson_from_table(u, table + 4, len - 4) + 4;
+}
+
 // ---------------------------------------------------------------------------------------
 // expf of the engine: Cody–Waite argument reduction x = k*ln2 + r with ln2 split into
 // 0.693359375 (exact, 9 significant bits) and -2.12194440e-4, exp(r) by the degree-7

@@ -54,6 +54,19 @@ __global__ __launch_bounds__(256) void k_batch_evolve(BatchEvolvePlan P) {
     const uint32_t step = *P.step;
     const size_t k = (size_t)n * P.B + b;
     uint32_t spike = 0;
+    // The connection traces of this neuron in every set, loaded with the state below (the
+    // stores of the state would otherwise hold these loads back).
+    float tr_d[kBatchMaxGroups], tr_r[kBatchMaxGroups];
+#pragma unroll
+    for (int s = 0; s < kBatchMaxGroups; ++s) {
+        const BatchTraceSet& T = P.ts[s];
+        tr_d[s] = tr_r[s] = 0.0f;
+        if ((uint32_t)s < P.n_ts && n >= T.pre_offset && n < T.pre_offset + T.n_pre) {
+            const size_t kk = (size_t)(n - T.pre_offset) * P.B + b;
+            tr_d[s] = T.d[kk];
+            tr_r[s] = T.r[kk];
+        }
+    }
     for (uint32_t g = 0; g < P.n_groups; ++g) {
         const BatchGroup& G = P.g[g];
         const NeuronGroupView& v = G.v;
@@ -64,6 +77,12 @@ __global__ __launch_bounds__(256) void k_batch_evolve(BatchEvolvePlan P) {
         uint32_t refr = P.refr[k];
         float re = P.bre[k], de = P.bde[k], ri = P.bri[k], di = P.bdi[k];
         const float amp = P.stim_amp ? P.stim_amp[k] : 0.0f;
+        uint32_t he[4], hi[4];  // the first Poisson table entries (wave-uniform)
+#pragma unroll
+        for (uint32_t q = 0; q < 4; ++q) {
+            he[q] = q < v.poisson_table_excitatory_len ? v.poisson_table_excitatory[q] : 0u;
+            hi[q] = q < v.poisson_table_inhibitory_len ? v.poisson_table_inhibitory[q] : 0u;
+        }
         float pe[8], pi[8];
         const uint32_t ne = G.has_fin_e ? min(G.fin_e.groups, 8u) : 0u;
         const uint32_t ni = G.has_fin_i ? min(G.fin_i.groups, 8u) : 0u;
@@ -99,9 +118,9 @@ __global__ __launch_bounds__(256) void k_batch_evolve(BatchEvolvePlan P) {
             const U32x4 r = philox4x32_10(U32x4{step, (uint32_t)BN_RNG_STREAM_BACKGROUND, n, v.rng_instance},
                                           P.keys[2 * b], P.keys[2 * b + 1]);
             if (v.mean_background_excitatory_spikes > 0.0f)
-                ke = poisson_from_table(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len);
+                ke = poisson_lookup(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len, he);
             if (v.mean_background_inhibitory_spikes > 0.0f)
-                ki = poisson_from_table(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len);
+                ki = poisson_lookup(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len, hi);
         }
         if (ke > 0u) {
             const float inc = (float)ke;
@@ -127,12 +146,13 @@ __global__ __launch_bounds__(256) void k_batch_evolve(BatchEvolvePlan P) {
         P.gi[k] = bg_gi;
     }
     // Connection traces (kernel_connection_propagate.cu:8-46) of this neuron in every set.
-    for (uint32_t s = 0; s < P.n_ts; ++s) {
+#pragma unroll
+    for (int s = 0; s < kBatchMaxGroups; ++s) {
         const BatchTraceSet& T = P.ts[s];
-        if (n < T.pre_offset || n >= T.pre_offset + T.n_pre) continue;
+        if ((uint32_t)s >= P.n_ts || n < T.pre_offset || n >= T.pre_offset + T.n_pre) continue;
         const size_t kk = (size_t)(n - T.pre_offset) * P.B + b;
-        float dd = flush_trace(T.d[kk] * T.decay_d);
-        float rr = flush_trace(T.r[kk] * T.decay_r);
+        float dd = flush_trace(tr_d[s] * T.decay_d);
+        float rr = flush_trace(tr_r[s] * T.decay_r);
         if (spike == 1) {
             rr += T.K;
             dd += T.K;

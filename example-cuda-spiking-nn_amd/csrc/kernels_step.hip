@@ -95,14 +95,6 @@ __device__ __forceinline__ float trace_step(float tr, uint32_t spiked_prev, floa
 // previous step's spikes), and waves 3.. sum the parts of the propagate partials.  After the
 // barrier wave 0 combines the parts and runs the EIF step, the connection traces and the
 // monitors.
-__device__ __forceinline__ uint32_t poisson_lookup(uint32_t u, const uint32_t* table, uint32_t len,
-                                                   const uint32_t (&head)[4]) {
-    // min{k : u < table[k]}; the first four entries come from registers.
-#pragma unroll
-    for (uint32_t k = 0; k < 4; ++k)
-        if (k >= len || u < head[k]) return k;
-    return poisson_from_table(u, table + 4, len - 4) + 4;
-}
 
 constexpr uint32_t kPartWave0 = 3;                          // first wave that sums partials
 constexpr uint32_t kPartWaves = kFinParts - kPartWave0;     // parts p, p + kPartWaves, ... per wave
