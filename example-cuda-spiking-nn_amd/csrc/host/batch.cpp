@@ -26,7 +26,7 @@ struct bn_batch {
     Config cfg;
     uint32_t n_inst = 0, B = 0, N = 0, NE = 0, NI = 0;
     uint32_t step = 0;
-    uint32_t steps_per_graph = 10;
+    uint32_t steps_per_graph = 100;  // graph boundaries cost about as much as a step; long graphs amortise them
     uint32_t gemm_rows = 0;  // rows per wave chunk (GEMM tile height along pre)
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;

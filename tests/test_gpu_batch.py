@@ -90,19 +90,37 @@ def test_batch_instances_match_plain_runs(tmp_path):
 def test_batch_instances_are_independent(tmp_path):
     # The same instance in a batch of 1 and at another column of a batch of 70 (two n-pairs):
     # identical state, since every column of the GEMM is summed in the same order.
-    base = small_config(steps=200, stimulus=False, plastic=False)
+    base = small_config(steps=400, stimulus=False, plastic=False)
     path = _write(tmp_path, base, "base.json")
     insts = _instances(base["num_excitatory_neurons"], 70, 10, start=5, end=40)
     big = engine().BatchSimulation(path, insts)
     one = engine().BatchSimulation(path, [insts[66]])
     assert big.info.padded_instances == 128 and one.info.padded_instances == 64
-    big.run(80)
-    one.run(80)
+    big.run(230)  # two graphs of 100 steps and an eager remainder
+    one.run(230)
     a, b = big.state(66), one.state(0)
     for key in ("membrane_potentials", "g_excitatory", "g_inhibitory", "spikes", "i_stimulus"):
         assert np.array_equal(a[key], b[key]), key
     big.close()
     one.close()
+
+
+def test_batch_graphs_equal_eager_steps(tmp_path):
+    # Graph-captured steps (graphs of 100) and the same steps launched one by one.
+    base = small_config(steps=400, stimulus=False, plastic=False)
+    path = _write(tmp_path, base, "base.json")
+    insts = _instances(base["num_excitatory_neurons"], 6, 20, start=30, end=150)
+    a = engine().BatchSimulation(path, insts)
+    b = engine().BatchSimulation(path, insts)
+    a.run(205)
+    for _ in range(205):
+        b.run(1)
+    for k in (0, 5):
+        sa, sb = a.state(k), b.state(k)
+        for key in ("membrane_potentials", "g_excitatory", "g_inhibitory", "spikes", "i_stimulus"):
+            assert np.array_equal(sa[key], sb[key]), (k, key)
+    a.close()
+    b.close()
 
 
 @pytest.mark.parametrize("rows", [8, 24, 512])
