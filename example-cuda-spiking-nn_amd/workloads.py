@@ -132,7 +132,7 @@ def autotune_row_block_cap(engine, cfgs: list, workdir: str, caps=(32, 64, 128, 
     return best, times
 
 
-def autotune_schedule(multi, group_mib=(0, 256, 128), steps_per_graph=(10, 50), streams=(1, 2, 3, 4),
+def autotune_schedule(multi, group_mib=(0, 256, 128), steps_per_graph=(10, 50, 100), streams=(1, 2, 3, 4),
                       overlap=(False, True), steps: int = 200) -> dict:
     """The batched plastic step's schedule autotuner: time `steps` steps of every network under
     each candidate — (weight MiB per network group stepped one after the other, steps per
