@@ -116,11 +116,26 @@ struct GemvConn {
     uint32_t ld, n_pre, n_post, row_block, n_chunks, n_coltiles, tile_begin, tile_end;
 };
 
+// The spiking pre neurons of every plasticity, compacted by one extra block of the propagate
+// launch of the same step (list == nullptr: none), for the plasticity of that step to read
+// instead of compacting the spike bits again: list = pre-local ids in plasticity order, each
+// plasticity's in increasing order (entries past kPreListCap are not stored), counts[k] = the
+// count of plasticity k.
+constexpr uint32_t kPreListCap = 4096;
+struct PreCompact {
+    const uint32_t* bits;    // spike bits of the step (global ids)
+    uint32_t n;              // plasticities
+    uint32_t pre_offset[kMaxPlast], n_pre[kMaxPlast];
+    uint32_t* list;          // [kPreListCap]
+    uint32_t* counts;        // [kMaxPlast]
+};
+
 struct GemvPlan {
     GemvConn c[kMaxConns];
     uint32_t n_conn;
     uint32_t total_tiles;    // one wave per tile
     uint32_t nontemporal;    // stream the weights past the caches (weight set >> MALL)
+    PreCompact pc;           // pc.list == nullptr: no compaction block
 };
 
 struct PlastDev {
@@ -173,6 +188,10 @@ struct PlastStepPlan {
     uint32_t row_blocks;         // blocks of kPlastQuietRows rows (rows whose pre did not spike)
     uint32_t spiking_blocks;     // blocks sharing the rows whose pre neuron spiked
     StimTail tail;
+    // The spiking pre neurons as compacted by this step's propagate launch (PreCompact), or
+    // nullptr: the spiking-row blocks compact the spike bits themselves.
+    const uint32_t* pre_list;
+    const uint32_t* pre_counts;
 };
 
 struct TailPlan {

@@ -505,7 +505,8 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
         BN_HIP_CHECK(hipEventRecord(ev[1], s), "hipEventRecord");
         // (2) num_steps back-to-back propagate launches over the current weights.  The pending
         // partial sums belong to the next step: saved before and restored after.
-        const GemvPlan& G = sys.gemv_plan();
+        GemvPlan G = sys.gemv_plan();
+        G.pc.list = nullptr;  // the propagate alone
         size_t total = 0;
         for (uint32_t c = 0; c < G.n_conn; ++c) total += (size_t)G.c[c].n_chunks * G.c[c].ld;
         double* save = nullptr;

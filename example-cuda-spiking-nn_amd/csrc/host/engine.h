@@ -473,6 +473,8 @@ private:
     bool overlap_ = false;
     EvolvePlan evolve_even_{}, evolve_odd_{};
     PlastStepPlan plast_staging_{}, plast_current_{}, plast_final_{};
+    GemvPlan gemv_even_{}, gemv_odd_{};   // overlapped graphs: the compaction reads the step's bits
+    uint32_t* pre_list_buf_ = nullptr;    // [kPreListCap + kMaxPlast]: PreCompact list and counts
     uint32_t* d_ev_start_ = nullptr;
     bool overlap_built_ = false;
     std::vector<void*> staging_;

@@ -125,6 +125,7 @@ struct bn_multi {
             // block takes a row does not change the row's result.
             auto fewer = [](PlastStepPlan p) {
                 p.spiking_blocks = std::min<uint32_t>(p.spiking_blocks, kMultiSpikingBlocks);
+                p.pre_list = p.pre_counts = nullptr;  // the multi-network propagate does not compact
                 return p;
             };
             pl.push_back(fewer(sys.plast_step_plan()));

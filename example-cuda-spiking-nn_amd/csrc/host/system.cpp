@@ -13,6 +13,7 @@ System::~System() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : staging_) (void)hipFree(p);
+    if (pre_list_buf_) (void)hipFree(pre_list_buf_);
 }
 
 void System::add_neuron_group(std::unique_ptr<NeuronGroup> g) {
@@ -259,6 +260,23 @@ void System::build_plans() {
     plast_step_.spiking_blocks = std::max<uint32_t>(1, std::min<uint32_t>(256, plast_.items_pre));
     plast_step_.tail = st;
     plast_step_.tail.active = tail_.norm_rows == 0 ? 1 : 0;
+    // The propagate launch compacts the step's spiking pre neurons for the plasticity launch.
+    gemv_.pc = PreCompact{};
+    plast_step_.pre_list = plast_step_.pre_counts = nullptr;
+    if (plast_.n > 0) {
+        if (!pre_list_buf_)
+            BN_HIP_CHECK(hipMalloc(&pre_list_buf_, (kPreListCap + kMaxPlast) * sizeof(uint32_t)), "hipMalloc(pre list)");
+        gemv_.pc.bits = S.d_spike_bits();
+        gemv_.pc.n = plast_.n;
+        for (uint32_t k = 0; k < plast_.n; ++k) {
+            gemv_.pc.pre_offset[k] = plast_.p[k].pre_offset;
+            gemv_.pc.n_pre[k] = plast_.p[k].n_pre;
+        }
+        gemv_.pc.list = pre_list_buf_;
+        gemv_.pc.counts = pre_list_buf_ + kPreListCap;
+        plast_step_.pre_list = gemv_.pc.list;
+        plast_step_.pre_counts = gemv_.pc.counts;
+    }
 
     kernels_per_step_ = 1 + (gemv_.total_tiles ? 1 : 0) + 1 + (tail_.norm_rows ? 1 : 0);
     if (lazy_idx >= 0) BN_HIP_CHECK((hipError_t)launch_lazy_init(plast_.p[lazy_idx], S.stream()), "lazy normalisation init");
@@ -307,6 +325,9 @@ void System::build_overlap_plans() {
     uint32_t* bits = static_cast<uint32_t*>(alloc((size_t)std::max<uint32_t>(1, evolve_.words) * sizeof(uint32_t)));
     evolve_even_.spike_bits = bits;
     for (uint32_t k = 0; k < plast_staging_.pl.n; ++k) plast_staging_.pl.p[k].bits = bits;
+    // Even steps' spikes are in the staging bits: their propagate compacts those.
+    gemv_even_ = gemv_odd_ = gemv_;
+    if (gemv_even_.pc.list) gemv_even_.pc.bits = bits;
     plast_staging_.tail.active = plast_current_.tail.active = 0;
     // The stimulus: event starts per step for the per-neuron application and the final tail.
     plast_final_ = plast_current_;
@@ -345,7 +366,7 @@ void System::enqueue_overlapped_steps(uint32_t n) {
         e.stim_prev = k > 0 && e.ev_start ? 1u : 0u;
         const PlastStepPlan* prev = k == 0 ? nullptr : (k - 1) % 2 == 0 ? &plast_staging_ : &plast_current_;
         BN_HIP_CHECK((hipError_t)launch_evolve_plast(e, prev, s), "overlapped evolve launch");
-        BN_HIP_CHECK((hipError_t)launch_gemv(gemv_, s), "overlapped propagate launch");
+        BN_HIP_CHECK((hipError_t)launch_gemv(k % 2 == 0 ? gemv_even_ : gemv_odd_, s), "overlapped propagate launch");
     }
     BN_HIP_CHECK((hipError_t)launch_plasticity(plast_final_, s), "overlapped plasticity launch");
 }

@@ -477,9 +477,15 @@ __global__ __launch_bounds__(kFinParts * 64) void k_evolve_multi(const EvolvePla
 // accumulation in this fixed order, the part sums of the combination, then one rounding to
 // FP32 when g is finalised.
 // =======================================================================================
+// x and the row scales are read-only during a propagate launch (the previous launch writes
+// them): read through the constant address space, so that their wave-uniform loads stay scalar
+// loads however the rest of the launch writes memory.
+typedef const __attribute__((address_space(4))) float ConstFloat;
 template <bool kScaled>
 __device__ __forceinline__ float row_x(const float* __restrict__ x, const float* __restrict__ s, uint32_t i) {
-    return kScaled ? x[i] * s[i] : x[i];
+    const ConstFloat* xc = (const ConstFloat*)x;
+    const ConstFloat* sc = (const ConstFloat*)s;
+    return kScaled ? xc[i] * sc[i] : xc[i];
 }
 
 // One tile: every index is wave-uniform, so x (and s) come through the scalar cache and the
@@ -554,8 +560,17 @@ __device__ __forceinline__ void gemv_block(const GemvPlan& P, uint32_t blk) {
     else gemv_tile<false, kNT>(C.W, C.x, nullptr, C.partials, C.ld, i0, i1, j0, chunk);
 }
 
+// The propagate launch's extra last block (GemvPlan::pc), defined with the compaction helpers.
+__device__ __forceinline__ void compact_pre_to_global(const PreCompact& pc);
+
 template <bool kNT>
-__global__ __launch_bounds__(256) void k_gemv(GemvPlan P) { gemv_block<kNT>(P, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
+    if (P.pc.list && blockIdx.x == gridDim.x - 1) {
+        compact_pre_to_global(P.pc);
+        return;
+    }
+    gemv_block<kNT>(P, blockIdx.x);
+}
 
 template <bool kNT>
 __global__ __launch_bounds__(256) void k_gemv_multi(const GemvPlan* __restrict__ plans,
@@ -568,6 +583,10 @@ __global__ __launch_bounds__(256) void k_gemv_multi(const GemvPlan* __restrict__
 // Scalar-load variant for row strides that are not 16-byte aligned (reference layout with
 // post % 4 != 0).  Same tiles, same summation order.
 __global__ __launch_bounds__(256) void k_gemv_scalar(GemvPlan P) {
+    if (P.pc.list && blockIdx.x == gridDim.x - 1) {
+        compact_pre_to_global(P.pc);
+        return;
+    }
     const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
     if (wave >= P.total_tiles) return;
@@ -717,6 +736,19 @@ __device__ __forceinline__ uint32_t block_compact_bits(const uint32_t* bits, uin
         __syncthreads();
     }
     return total;
+}
+
+// PreCompact: the spiking pre neurons of every plasticity into the global list, in order.
+__device__ __forceinline__ void compact_pre_to_global(const PreCompact& pc) {
+    __shared__ uint32_t wsum[kMaxBlockWaves];
+    uint32_t base = 0;
+    for (uint32_t k = 0; k < pc.n; ++k) {
+        const uint32_t g0 = pc.pre_offset[k], g1 = g0 + pc.n_pre[k];
+        const uint32_t at = min(base, kPreListCap);
+        const uint32_t count = block_compact_bits(pc.bits, g0, g1, pc.list + at, kPreListCap - at, wsum);
+        if (threadIdx.x == 0) pc.counts[k] = count;
+        base += count;
+    }
 }
 
 // Spiking pre neurons of every plasticity in one pass: the set bits of each plasticity's pre
@@ -1009,14 +1041,33 @@ __device__ __forceinline__ void plasticity_block(const PlastStepPlan& P, uint32_
         // every plasticity are compacted in one pass.
         const uint32_t sb = b - P.row_blocks;
         const uint32_t G = P.spiking_blocks;
-        block_compact_pre(P.pl, list, kListCap, wsum, rcount);
+        // The list compacted by this step's propagate launch (the counts and this block's first
+        // entry in one round trip; spiking_blocks <= kPreListCap), or a compaction here.
+        const bool global_list = P.pre_list != nullptr;
+        uint32_t cnt[kMaxPlast];
+        uint32_t first_entry = 0;
+        if (global_list) {
+#pragma unroll
+            for (int k = 0; k < kMaxPlast; ++k) cnt[k] = P.pre_counts[k];
+            first_entry = P.pre_list[sb];
+        } else {
+            block_compact_pre(P.pl, list, kListCap, wsum, rcount);
+#pragma unroll
+            for (int k = 0; k < kMaxPlast; ++k) cnt[k] = rcount[k];
+        }
+        const uint32_t cap = global_list ? kPreListCap : kListCap;
         uint32_t base = 0;
-        for (uint32_t k = 0; k < P.pl.n; ++k) {
+#pragma unroll
+        for (int k = 0; k < kMaxPlast; ++k) {
+            if ((uint32_t)k >= P.pl.n) break;
             const PlastDev& D = P.pl.p[k];
-            const uint32_t count = rcount[k];
+            const uint32_t count = cnt[k];
             const uint32_t first = base + (sb + G - base % G) % G;
-            if (base + count <= kListCap) {
-                for (uint32_t e = first; e < base + count; e += G) plasticity_spiking_row(D, list[e], lds4);
+            if (base + count <= cap) {
+                for (uint32_t e = first; e < base + count; e += G) {
+                    const uint32_t i = !global_list ? list[e] : e == sb ? first_entry : P.pre_list[e];
+                    plasticity_spiking_row(D, i, lds4);
+                }
             } else {
                 uint32_t rank = 0;
                 for (uint32_t i = 0; i < D.n_pre; ++i)
@@ -1370,13 +1421,15 @@ int launch_evolve_plast(const EvolvePlan& ev, const PlastStepPlan* plast, bn_str
 }
 
 int launch_gemv(const GemvPlan& plan, bn_stream_t s) {
-    if (plan.total_tiles == 0) return 0;
+    if (plan.total_tiles == 0 && !plan.pc.list) return 0;
+    // Four tiles per block, and the compaction block last when the plan has one.
+    const uint32_t blocks = cdiv(plan.total_tiles, 4) + (plan.pc.list ? 1u : 0u);
     if (gemv_vector_ok(plan) && plan.nontemporal)
-        hipLaunchKernelGGL(k_gemv<true>, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
+        hipLaunchKernelGGL(k_gemv<true>, dim3(blocks), dim3(256), 0, s, plan);
     else if (gemv_vector_ok(plan))
-        hipLaunchKernelGGL(k_gemv<false>, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
+        hipLaunchKernelGGL(k_gemv<false>, dim3(blocks), dim3(256), 0, s, plan);
     else
-        hipLaunchKernelGGL(k_gemv_scalar, dim3(cdiv(plan.total_tiles, 4)), dim3(256), 0, s, plan);
+        hipLaunchKernelGGL(k_gemv_scalar, dim3(blocks), dim3(256), 0, s, plan);
     return (int)hipGetLastError();
 }
 
