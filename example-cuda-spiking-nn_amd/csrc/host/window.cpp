@@ -2,7 +2,10 @@
 // T steps from a snapshot of a bn_sim, the window loss, and its gradient by the adjoint with
 // checkpoints every S steps (forward stores the state at each segment start; the reverse pass
 // recomputes a segment with per-step records, walks it backwards, then adds the segment's
-// x (outer) lambda_g products to dW).
+// x (outer) lambda_g products to dW).  The forward records the last segment itself, and the
+// records of the other segments are doubled, so that the recompute of segment s-1 runs on a
+// second stream while segment s is walked backwards: two independent chains of launches in one
+// graph, which share only read-only data (weights, parameters).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -27,13 +30,15 @@ struct bn_window {
     float rate_tau = 5e-3f, beta = 1e-3f;
     float g_leak = 0.0f, tau_decay_e = 0.0f;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;        // the backward's recompute chain (captured into the graph)
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<void*> allocs;
     float* W[4] = {};
     uint32_t ld[4] = {}, pre[4] = {}, post[4] = {};
     WinGroup grp[2]{};
     WinState st{}, snap{};
     std::vector<WinState> ckpt;
-    WinRecords rec{};
+    WinRecords rec[3]{};               // records of segment s: rec_of(s)
     GemvPlan gemv{};
     bn_device_clock clock{};
     int32_t* vrec_slot = nullptr;
@@ -79,6 +84,9 @@ struct bn_window {
         if (fwd_graph) (void)hipGraphDestroy(fwd_graph);
         if (bwd_graph) (void)hipGraphDestroy(bwd_graph);
         for (void* p : allocs) (void)hipFree(p);
+        for (hipEvent_t e : {fork, join})
+            if (e) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -100,10 +108,11 @@ struct bn_window {
         s.xI = alloc<float>(NI, what);
         return s;
     }
-    void copy_state(const WinState& dst, const WinState& src) {
+    void copy_state(const WinState& dst, const WinState& src, hipStream_t s) {
         WinCopyPlan p{dst, src, NE, NI, N};
-        BN_HIP_CHECK((hipError_t)launch_win_copy(p, stream), "window state copy launch");
+        BN_HIP_CHECK((hipError_t)launch_win_copy(p, s), "window state copy launch");
     }
+    void copy_state(const WinState& dst, const WinState& src) { copy_state(dst, src, stream); }
 
     void apply_params() {
         WinParams h{};
@@ -112,12 +121,16 @@ struct bn_window {
         BN_HIP_CHECK(hipMemcpyAsync(d_params, &h, sizeof h, hipMemcpyHostToDevice, stream), "window params");
         BN_HIP_CHECK(hipStreamSynchronize(stream), "window params sync");
     }
-    WinForwardPlan fwd_plan(uint32_t t, bool record) const {
+    // The last segment's records are written by the forward and read by every backward pass;
+    // the other segments are recomputed into two alternating sets.
+    const WinRecords& rec_of(uint32_t seg) const { return seg + 1 == n_seg ? rec[2] : rec[seg & 1]; }
+    // record: per-step records for the backward; outputs: spike counts, recorded V, raster.
+    WinForwardPlan fwd_plan(uint32_t t, bool record, bool outputs) const {
         WinForwardPlan p{};
         p.g[0] = grp[0];
         p.g[1] = grp[1];
         p.st = st;
-        if (record) p.rec = rec;
+        if (record) p.rec = rec_of(t / S);
         p.prm = d_params;
         p.NE = NE, p.NI = NI, p.N = N;
         p.slot = t % S;
@@ -128,16 +141,22 @@ struct bn_window {
         p.K = cfg.postsynaptic_kernel_normalization_value;
         p.clock = clock;
         p.vrec_slot = vrec_slot;
-        if (!record) {
+        if (outputs) {
             p.counts = d_counts + 2 * (size_t)t;
             p.v_rec = d_vrec + (size_t)nv * t;
             p.raster = d_raster + (size_t)N * t;
         }
         return p;
     }
-    void step_forward(uint32_t t, bool record) {
-        BN_HIP_CHECK((hipError_t)launch_gemv(gemv, stream), "window propagate launch");
-        BN_HIP_CHECK((hipError_t)launch_win_forward(fwd_plan(t, record), stream), "window forward launch");
+    void step_forward(uint32_t t, bool record, bool outputs, hipStream_t s) {
+        BN_HIP_CHECK((hipError_t)launch_gemv(gemv, s), "window propagate launch");
+        BN_HIP_CHECK((hipError_t)launch_win_forward(fwd_plan(t, record, outputs), s), "window forward launch");
+    }
+    // Segment seg from its checkpoint, with per-step records into rec_of(seg).
+    void recompute(uint32_t seg, hipStream_t s) {
+        const uint32_t t0 = seg * S, t1 = std::min(T, t0 + S);
+        copy_state(st, ckpt[seg], s);
+        for (uint32_t t = t0; t < t1; ++t) step_forward(t, true, false, s);
     }
 
     template <typename F>
@@ -165,7 +184,7 @@ struct bn_window {
                 copy_state(st, snap);
                 for (uint32_t t = 0; t < T; ++t) {
                     if (t % S == 0) copy_state(ckpt[t / S], st);
-                    step_forward(t, false);
+                    step_forward(t, t / S + 1 == n_seg, true, stream);
                 }
             });
         BN_HIP_CHECK(hipEventRecord(ev[0], stream), "hipEventRecord");
@@ -213,8 +232,18 @@ struct bn_window {
         const double dt = cfg.timestep, a = std::exp(-dt / (double)rate_tau);
         for (int64_t seg = (int64_t)n_seg - 1; seg >= 0; --seg) {
             const uint32_t t0 = (uint32_t)seg * S, t1 = std::min(T, t0 + S), slots = t1 - t0;
-            copy_state(st, ckpt[seg]);
-            for (uint32_t t = t0; t < t1; ++t) step_forward(t, true);
+            const WinRecords& rec = rec_of((uint32_t)seg);
+            // Segment seg-1 is recomputed on the side stream meanwhile (the last segment was
+            // recorded by the forward).  It writes the state, the propagate partials and
+            // rec_of(seg - 1); the fork comes after segment seg+1's reverse walk and dW update
+            // (the last readers of those records) and after segment seg's recompute (the last
+            // user of the state and partials).
+            if (seg > 0) {
+                BN_HIP_CHECK(hipEventRecord(fork, stream), "backward fork");
+                BN_HIP_CHECK(hipStreamWaitEvent(side, fork, 0), "backward fork wait");
+                recompute((uint32_t)seg - 1, side);
+                BN_HIP_CHECK(hipEventRecord(join, side), "backward join");
+            }
             for (int64_t sl = (int64_t)slots - 1; sl >= 0; --sl) {
                 const uint32_t t = t0 + (uint32_t)sl;
                 WinReversePlan rp{};
@@ -224,7 +253,7 @@ struct bn_window {
                 rp.prm = d_params;
                 for (int c = 0; c < 4; ++c) rp.W[c] = W[c], rp.ld[c] = ld[c];
                 rp.NE = NE, rp.NI = NI, rp.N = N, rp.slot = (uint32_t)sl;
-                const WinForwardPlan fp = fwd_plan(t, true);
+                const WinForwardPlan fp = fwd_plan(t, true, false);
                 rp.decay_e_r = fp.decay_e_r, rp.decay_i_r = fp.decay_i_r, rp.decay_i_d = fp.decay_i_d;
                 rp.K = fp.K;
                 rp.theta = cfg.v_threshold;
@@ -259,6 +288,7 @@ struct bn_window {
                 op.post_offset = e_post ? 0 : NE;
                 BN_HIP_CHECK((hipError_t)launch_win_outer(op, stream), "window outer launch");
             }
+            if (seg > 0) BN_HIP_CHECK(hipStreamWaitEvent(stream, join, 0), "backward join wait");
         }
     }
 
@@ -305,7 +335,10 @@ void build_window(bn_window* w, bn_sim* sim, const bn_window_spec* spec) {
     w->g_leak = c.g_leak, w->tau_decay_e = c.tau_decay_e;
     const uint32_t N = w->N, NE = w->NE, NI = w->NI, T = w->T, S = w->S;
     BN_HIP_CHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking), "hipStreamCreate");
+    BN_HIP_CHECK(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking), "hipStreamCreate");
     for (hipEvent_t& e : w->ev) BN_HIP_CHECK(hipEventCreate(&e), "hipEventCreate");
+    for (hipEvent_t* e : {&w->fork, &w->join})
+        BN_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
     // Weights: the simulation's current ones, rows padded to 64 floats.
     const uint32_t pres[4] = {NE, NE, NI, NI}, posts[4] = {NE, NI, NE, NI};
     for (int k = 0; k < 4; ++k) {
@@ -420,19 +453,20 @@ void build_window(bn_window* w, bn_sim* sim, const bn_window_spec* spec) {
     w->grp[0].fin_i = FinalizeSource{w->gemv.c[2].partials, w->gemv.c[2].n_chunks, w->ld[2]};
     w->grp[1].fin_e = FinalizeSource{w->gemv.c[1].partials, w->gemv.c[1].n_chunks, w->ld[1]};
     w->grp[1].fin_i = FinalizeSource{w->gemv.c[3].partials, w->gemv.c[3].n_chunks, w->ld[3]};
-    // Records of one segment, outputs, adjoint buffers.
-    WinRecords& r = w->rec;
-    r.Vprev = w->alloc<float>((size_t)S * N, "records");
-    r.ge = w->alloc<float>((size_t)S * N, "records");
-    r.gi = w->alloc<float>((size_t)S * N, "records");
-    r.s = w->alloc<float>((size_t)S * N, "records");
-    r.refr = w->alloc<uint8_t>((size_t)S * N, "records");
-    r.dE = w->alloc<float>((size_t)S * NE, "records");
-    r.rE = w->alloc<float>((size_t)S * NE, "records");
-    r.dI = w->alloc<float>((size_t)S * NI, "records");
-    r.rI = w->alloc<float>((size_t)S * NI, "records");
-    r.le = w->alloc<float>((size_t)S * N, "records");
-    r.li = w->alloc<float>((size_t)S * N, "records");
+    // Records of three segments (rec_of), outputs, adjoint buffers.
+    for (WinRecords& r : w->rec) {
+        r.Vprev = w->alloc<float>((size_t)S * N, "records");
+        r.ge = w->alloc<float>((size_t)S * N, "records");
+        r.gi = w->alloc<float>((size_t)S * N, "records");
+        r.s = w->alloc<float>((size_t)S * N, "records");
+        r.refr = w->alloc<uint8_t>((size_t)S * N, "records");
+        r.dE = w->alloc<float>((size_t)S * NE, "records");
+        r.rE = w->alloc<float>((size_t)S * NE, "records");
+        r.dI = w->alloc<float>((size_t)S * NI, "records");
+        r.rI = w->alloc<float>((size_t)S * NI, "records");
+        r.le = w->alloc<float>((size_t)S * N, "records");
+        r.li = w->alloc<float>((size_t)S * N, "records");
+    }
     w->nv = spec->n_vrec;
     std::vector<int32_t> slot(N, -1);
     for (uint32_t k = 0; k < w->nv; ++k) {
