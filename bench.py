@@ -373,11 +373,14 @@ def bench_adjoint(args) -> int:
     s_per_1k = dt / args.steps * 1000.0 / T
     n_w = sum(w.size for w in base)
     n = sim.N
-    # Algorithmic bytes per window: the weight set read 3 times per step (forward and recompute
-    # propagate, transposed product of the reverse step) plus 160 B of state per neuron per
-    # propagate pass, and dW read-modify-written once per segment.
+    # Algorithmic bytes per window: the weight set read by every forward step, every reverse
+    # step (transposed product) and every recomputed step (all but the last segment, which the
+    # forward records), plus 160 B of state per neuron per pass, and dW read-modify-written once
+    # per segment.
     b_step = 4 * n_w + 160 * n
-    alg_bytes = 3 * T * b_step + (T // S) * 2 * 4 * n_w
+    n_seg = -(-T // S)
+    recomputed = (n_seg - 1) * S
+    alg_bytes = (2 * T + recomputed) * b_step + n_seg * 2 * 4 * n_w
     achieved = alg_bytes / (dt / args.steps) / 1e9
     cpu = None if args.no_cpu_baseline else cpu_baseline_adjoint(sim, win, base, args.cpu_seconds)
     gnorm = float(sum((d.astype("float64") ** 2).sum() for d in g["dW"]) ** 0.5)
