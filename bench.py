@@ -83,6 +83,11 @@ def main() -> int:
     cfg = eng.config_dump(args.config)
     flags = ["no_monitors", "quiet"] + (["lazy_normalize"] if args.normalization == "lazy" else [])
     sim = eng.Simulation(args.config, flags=flags)
+    # A long run: 500 steps per hipGraph (the library's default of 100 suits callers that advance
+    # in shorter chunks; leftover steps run eagerly).  Measured 0.8% faster on the 3,125-neuron
+    # network and 0.3% on the 10,125-neuron one.
+    steps_per_graph = 500 if args.steps >= 500 and args.warmup % 500 == 0 else 100
+    sim.set_steps_per_graph(steps_per_graph)
     info = sim.info
     if args.warmup:
         sim.run_timed(args.warmup)
@@ -133,6 +138,7 @@ def main() -> int:
         "config": {"workload": f"{name} ({n_e:,} E + {n_i:,} I, 4 dense FP32 matrices, {4 * n_w / 1e6:.1f} MB, "
                                f"triplet STDP + iSTDP, seed {cfg['rng_seed']}, {args.normalization} normalisation)",
                    "neurons": n_total, "weight_entries": n_w, "normalization": args.normalization,
+                   "steps_per_graph": steps_per_graph,
                    "kernels_per_step": int(info.kernels_per_step), "parallelism": "single GPU",
                    "rate_hz_last_step": round(rate_hz, 2)},
         "roofline": {"bound": "hbm", "kernel": "k_gemv (propagate)", "achieved": round(achieved, 1),
