@@ -171,6 +171,35 @@ def test_lazy_burst_bit_exact(tmp_path):
     sim.close()
 
 
+@pytest.mark.parametrize("sequential", [False, True], ids=["overlapped", "sequential"])
+def test_spike_list_overflow_bit_exact(tmp_path, sequential):
+    # Every E neuron spikes in the same step (a 300 nA pulse moves V by 150 mV per step): 4,900
+    # spiking pre neurons overflow the 4,096-entry list of spiking rows, and 4,900 spiking posts
+    # overflow the list of the quiet I->E rows, so both fall back to scanning the spike bits.
+    n_e = 70 * 70
+    cfg = small_config(n_e_side=70, n_i_side=35, steps=200)
+    cfg["stimulus_selected_input_ids"] = list(range(n_e))
+    cfg["stimulus_selected_input_post_group_offsets"] = [0] * n_e
+    cfg["stimulus_start_clock_steps"] = [12]
+    cfg["stimulus_end_clock_steps"] = [14]
+    cfg["stimulus_input_values"] = [300e-9]
+    path = _write(tmp_path, cfg)
+    flags = ("no_monitors", "quiet", "lazy_normalize") + (("sequential_graph",) if sequential else ())
+    sim = engine().Simulation(path, flags=flags)
+    sim.set_steps_per_graph(10)
+    orc = oracle_for(path, sim, lazy=True)
+    most = 0
+    for chunk in range(5):  # whole graphs; the burst falls in the second
+        sim.run(10)
+        for _ in range(10):
+            orc.step(1)
+            most = max(most, int(orc.state()["spikes"][:n_e].sum()))
+        assert_state_equal(sim.state(), orc.state(), 10 * chunk + 9)
+    _check_traces_and_weights(sim, orc)
+    assert most > 4096, most
+    sim.close()
+
+
 def test_workload_3125_raster_1000_steps(tmp_path):
     cfg = load_config("net3125_stdp.json")
     path = _write(tmp_path, cfg)
