@@ -252,12 +252,12 @@ def test_istdp_isolated_pair_hand_computed():
     # LTD of pre row 1 with all post traces 0: w += -ltd_A * (0 - ltd_B), except j == 1 (index skip)
     exp = np.full(ne, 1e-9)
     exp[np.arange(ne) != 1] += 2e-12 * 0.25
-    assert np.allclose(w[1], exp, rtol=1e-6) and w[1, 1] == np.float32(1e-9)
+    assert np.allclose(w[1], exp, rtol=1e-6, atol=0) and w[1, 1] == np.float32(1e-9)
     assert np.all(w[[0, 2, 3]] == np.float32(1e-9))
     sim.step(8)
     w2 = sim.weights(2)
     # LTP of post column 5 at step 9: w[i,5] += ltp_A * pre_i, pre_1 = dpre^(9-1)
-    assert np.isclose(w2[1, 5] - w[1, 5], 3e-12 * dpre ** 8, rtol=1e-4)
+    assert np.isclose(w2[1, 5] - w[1, 5], 3e-12 * dpre ** 8, rtol=1e-4, atol=0)
     assert np.all(w2[[0, 2, 3], 5] == np.float32(1e-9))
 
 
@@ -282,11 +282,105 @@ def test_triplet_ltp_delta_from_weight_ratios():
     after = sim.weights(0).astype(np.float64)
     # normalisation rescales whole rows, so the ratio to an untouched column isolates the LTP delta
     delta = (after[3, 9] / after[3, 1] - before[3, 9] / before[3, 1]) * before[3, 1]
-    assert np.isclose(delta, 1e-11 * dpre ** 6, rtol=1e-3)
+    assert np.isclose(delta, 1e-11 * dpre ** 6, rtol=1e-3, atol=0)
     others = [i for i in range(ne) if i not in (1, 3, 9)]
     r_after = after[others, 9] / after[others, 1]
     r_before = before[others, 9] / before[others, 1]
-    assert np.allclose(r_after, r_before, rtol=1e-5)  # pre traces 0: no change
+    assert np.allclose(r_after, r_before, rtol=1e-5, atol=0)  # pre traces 0: no change
+
+
+def _decay(cfg, tau_key):
+    return float(np.exp(np.float32(-1.0) * np.float32(cfg["timestep"]) / np.float32(cfg[tau_key])))
+
+
+def _row_delta(before, after, i, j, j_ref):
+    """Change of entry (i, j) in the units of `before`, with row i's rescaling divided out through
+    an untouched column j_ref of the same row (the row normalisation scales whole rows)."""
+    return after[i, j] / after[i, j_ref] * before[i, j_ref] - before[i, j]
+
+
+@pytest.mark.parametrize("lazy", [False, True], ids=["faithful", "lazy"])
+def test_triplet_ltd_and_triplet_terms_hand_computed(lazy):
+    # kernel_triplet_plasticity_update.cu:54-93 with every coefficient nonzero.  E neuron 3 spikes
+    # at steps 1 and 41, E neuron 9 at step 11 (one-step 1 uA pulses one step earlier).  At
+    # step 41 (traces decayed first, increments after, :16-50 and :95-115):
+    #   LTD of pre row 3:  w[3][9] += -post_9 * (ltd_A + ltd_B * tpre_3)   (post_9 = y_fast of 9)
+    #   LTP of post col 3: w[9][3] +=  pre_9 * (ltp_A + ltp_B * tpost_3)   (tpost_3 = y_slow of 3)
+    # with post_9 = dpost^30, tpre_3 = dtpre^40, pre_9 = dpre^30, tpost_3 = dtpost^40.  The four
+    # trace time constants differ, so swapping pre/post or fast/slow traces, dropping a B term or
+    # using the wrong neuron's trace moves a delta by 5-60%.
+    cfg = _quiet_config()
+    cfg["e_e_plasticity_enabled"] = True
+    cfg.update({"e_e_plasticity_ltp_A": 1e-12, "e_e_plasticity_ltp_B": 3e-12, "e_e_plasticity_ltd_A": 1e-12,
+                "e_e_plasticity_ltd_B": 2e-12, "e_e_plasticity_weight_max": 1e-8})
+    ne, ni = 16, 4
+    rng = np.random.default_rng(5)
+    w_ee = rng.uniform(0.5e-9, 1e-9, (ne, ne)).astype(np.float32)
+    np.fill_diagonal(w_ee, 0)
+    ws = [w_ee, np.zeros((ne, ni), np.float32), np.zeros((ni, ne), np.float32), np.zeros((ni, ni), np.float32)]
+    p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy)
+    sim = O.OracleSim(p, ws, stim=_stim(cfg, [(3, 0), (9, 10), (3, 40)]))
+    sim.step(41)
+    assert [int(np.nonzero(sim.state()["spikes"])[0].size)] == [0]
+    before = sim.weights(0).astype(np.float64)
+    sim.step(1)
+    assert list(np.nonzero(sim.state()["spikes"])[0]) == [3]
+    after = sim.weights(0).astype(np.float64)
+    dpre, dpost = _decay(cfg, "e_e_plasticity_tau_ltp"), _decay(cfg, "e_e_plasticity_tau_ltd")
+    dtpre, dtpost = _decay(cfg, "e_e_plasticity_triplet_tau_ltp"), _decay(cfg, "e_e_plasticity_triplet_tau_ltd")
+    ltd = -(dpost ** 30) * (1e-12 + 2e-12 * dtpre ** 40)
+    ltp = dpre ** 30 * (1e-12 + 3e-12 * dtpost ** 40)
+    assert np.isclose(_row_delta(before, after, 3, 9, 1), ltd, rtol=2e-4, atol=0)
+    assert np.isclose(_row_delta(before, after, 9, 3, 1), ltp, rtol=2e-4, atol=0)
+    # Every other entry of rows 3 and 9 keeps its ratio to an untouched column (post traces of neurons that
+    # never spiked are 0; the diagonal is skipped by index), as do all other rows.
+    for i in range(ne):
+        ref = 1 if i != 1 else 2
+        for j in range(ne):
+            if i == j or j == ref or (i, j) in ((3, 9), (9, 3)):
+                continue
+            assert abs(_row_delta(before, after, i, j, ref)) <= 1e-6 * before[i, j] + 1e-30, (i, j)
+    # The trace increments come after the updates: the triplet traces of neuron 3 now hold the
+    # decayed old value plus 1.
+    pre, post, tpre, tpost = sim.plasticity_traces(0)
+    assert np.isclose(tpre[3], dtpre ** 40 + 1.0, rtol=1e-5, atol=0) and np.isclose(tpost[3], dtpost ** 40 + 1.0, rtol=1e-5, atol=0)
+    assert np.isclose(pre[9], dpre ** 30, rtol=1e-5, atol=0) and np.isclose(post[9], dpost ** 30, rtol=1e-5, atol=0)
+
+
+def test_eif_spike_cap_and_refractory_sequence():
+    # kernel_neuron_group_evolve.cu:92-118.  A one-step 1 uA pulse seen at step 1 lifts V by about
+    # 0.5 V: the step must cap V at v_threshold (not V + dv), set refractory = R and spike = 1;
+    # for the next R steps V is held at v_reset with the counter running down R-1 .. 0 and no
+    # spike; the step after that integrates freely from v_reset (FP64 hand computation).
+    cfg = _quiet_config()
+    p, shapes = _oracle(cfg)
+    sim = O.OracleSim(p, [np.zeros(s, np.float32) for s in shapes], stim=_stim(cfg, [(0, 0)]))
+    R = cfg["refractory_steps"]
+    f32 = np.float32
+    sim.step(1)
+    st = sim.state()
+    assert st["spikes"].sum() == 0 and st["V"][0] == f32(cfg["v_rest"])
+    sim.step(1)
+    st = sim.state()
+    assert st["spikes"][0] == 1 and st["spikes"][1:].sum() == 0
+    assert st["V"][0] == f32(cfg["v_threshold"]) and st["refr"][0] == R
+    for k in range(1, R + 1):
+        sim.step(1)
+        st = sim.state()
+        assert st["spikes"][0] == 0, k
+        assert st["V"][0] == f32(cfg["v_reset"]), k
+        assert st["refr"][0] == R - k, k
+    sim.step(1)
+    st = sim.state()
+    gl, C, dT = cfg["g_leak"], cfg["membrane_capacitance"], cfg["depolarization_slope_factor"]
+    VT, vr, dt = cfg["v_effective_spike_threshold"], cfg["v_rest"], cfg["timestep"]
+    v = float(f32(cfg["v_reset"]))
+    expected = v + dt / C * (-gl * (v - vr) + gl * dT * math.exp((v - VT) / dT))
+    assert st["spikes"][0] == 0 and st["refr"][0] == 0
+    assert abs(st["V"][0] - expected) <= 2e-6 * abs(expected)
+    assert st["V"][0] != f32(cfg["v_reset"])
+    # Without the pulse nothing spikes and V stays exactly at rest.
+    assert np.all(st["V"][1:] == f32(cfg["v_rest"]))
 
 
 def test_triplet_normalisation_restores_column_sum_targets():
@@ -304,7 +398,7 @@ def test_triplet_normalisation_restores_column_sum_targets():
         sim.step(3)
         w = sim.weights(0).astype(np.float64)
         ok = w_ee.sum(axis=1) > 0
-        assert np.allclose(w.sum(axis=1)[ok], target[ok], rtol=1e-6)
+        assert np.allclose(w.sum(axis=1)[ok], target[ok], rtol=1e-6, atol=0)
 
 
 def test_lazy_normalisation_tracks_faithful():
