@@ -48,7 +48,10 @@ def main() -> int:
     ap.add_argument("--networks", type=int, default=48)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (plain 10000, batched 5001)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (plain 2000, batched 0)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before the timed ones (plain 100, batched 0)")
+    ap.add_argument("--settle", type=int, default=2000,
+                    help="plain: steps run first to bring the network from its initial state (no spikes yet) to its "
+                         "settled firing, before the warm-up")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--config", default=os.path.join(REPO, "configs", "net3125_stdp.json"))
     ap.add_argument("--normalization", choices=("lazy", "faithful"), default="lazy")
@@ -77,22 +80,36 @@ def main() -> int:
         args.warmup = 1 if args.warmup is None else args.warmup
         return bench_adjoint(args)
     args.steps = 10000 if args.steps is None else args.steps
-    args.warmup = 2000 if args.warmup is None else args.warmup
+    args.warmup = 100 if args.warmup is None else args.warmup
 
     eng = importlib.import_module("example-cuda-spiking-nn_amd")
     cfg = eng.config_dump(args.config)
-    flags = ["no_monitors", "quiet"] + (["lazy_normalize"] if args.normalization == "lazy" else [])
+    # Monitors on: the step writes the spike-bit and voltage rings every step, as the reference's
+    # step historicizes every step (record() file I/O only happens in System::run, not here).
+    flags = ["quiet"] + (["lazy_normalize"] if args.normalization == "lazy" else [])
     sim = eng.Simulation(args.config, flags=flags)
-    # A long run: 500 steps per hipGraph (the library's default of 100 suits callers that advance
-    # in shorter chunks; leftover steps run eagerly).  Measured 0.8% faster on the 3,125-neuron
-    # network and 0.3% on the 10,125-neuron one.
-    steps_per_graph = 500 if args.steps >= 500 and args.warmup % 500 == 0 else 100
+    # Steps per hipGraph: 500 for long runs (0.8% faster than 100 on the 3,125-neuron network,
+    # 0.3% on the 10,125-neuron one); a shorter timed run is one graph of its own length, rounded
+    # down to an even count (the overlapped form), any odd step eager.
+    steps_per_graph = 500 if args.steps >= 500 else max(2, args.steps - args.steps % 2)
     sim.set_steps_per_graph(steps_per_graph)
     info = sim.info
+    if args.settle:
+        sim.run_timed(args.settle)
     if args.warmup:
         sim.run_timed(args.warmup)
+    before = sim.get_info()
+    first_timed = int(before.clock_step)
     ms = sim.run_timed(args.steps)  # synchronises before and after, HIP events on the engine stream
+    graph_launches = int(sim.get_info().graph_launches) - int(before.graph_launches)
     steps_per_s = args.steps / (ms / 1000.0)
+    # Firing over the timed window, from the device spike ring (the last 10,000 steps at most).
+    window = min(args.steps, 10000)
+    raster = sim.spike_raster(first_timed + args.steps - window, window)
+    rate_e = float(raster[:, : info.num_excitatory_neurons].mean()) / cfg["timestep"]
+    rate_i = float(raster[:, info.num_excitatory_neurons:].mean()) / cfg["timestep"]
+    rate_all = float(raster.mean()) / cfg["timestep"]
+    del raster
 
     # Dominant kernel: HIP events around back-to-back propagate (GEMV) launches over the current
     # weights (the simulation's pending partial sums are saved and restored by the engine).
@@ -101,12 +118,13 @@ def main() -> int:
     n_w = sum(int(info.weight_pre[k]) * int(info.weight_post[k]) for k in range(4))
     gemv_bytes = 4 * n_w + 4 * sum(int(info.weight_pre[k]) for k in range(4))
     achieved = gemv_bytes / k_avg_s / 1e9
-    state = sim.state()
-    rate_hz = float(state["spikes"].mean()) / cfg["timestep"]
     sim.close()
     # The applicable read bandwidth for this weight set: a plain streaming read kernel over a
     # working set of the same size (MALL-resident below 256 MB, HBM-bound above).
     read_gbs = eng.measure_read_bandwidth(4 * n_w, 50)
+    # The same working set read 200 times inside one launch: the steady-state rate of the level
+    # of the memory hierarchy that holds it, without launch ramp and drain.
+    read_gbs_one = eng.measure_read_bandwidth_one_launch(4 * n_w, 200)
 
     n_total = int(info.total_neurons)
     name = os.path.splitext(os.path.basename(args.config))[0]
@@ -129,6 +147,7 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle_steps": args.settle,
         "ms_per_step": round(ms / args.steps, 5),
         "higher_is_better": True,
         "scaling": "weak",
@@ -138,14 +157,17 @@ def main() -> int:
         "config": {"workload": f"{name} ({n_e:,} E + {n_i:,} I, 4 dense FP32 matrices, {4 * n_w / 1e6:.1f} MB, "
                                f"triplet STDP + iSTDP, seed {cfg['rng_seed']}, {args.normalization} normalisation)",
                    "neurons": n_total, "weight_entries": n_w, "normalization": args.normalization,
-                   "steps_per_graph": steps_per_graph,
+                   "steps_per_graph": steps_per_graph, "graph_launches_timed": graph_launches,
                    "kernels_per_step": int(info.kernels_per_step), "parallelism": "single GPU",
-                   "rate_hz_last_step": round(rate_hz, 2)},
+                   "timed_window_rate_hz": {"all": round(rate_all, 3), "excitatory": round(rate_e, 3),
+                                            "inhibitory": round(rate_i, 3), "steps": window},
+                   "first_timed_step": first_timed},
         "roofline": {"bound": "hbm", "kernel": "k_gemv (propagate)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "algorithmic_bytes_per_launch": gemv_bytes,
                      "avg_kernel_us": round(k_avg_s * 1e6, 2),
                      "measured_read_GBs_same_working_set": round(read_gbs, 1),
+                     "measured_read_GBs_same_working_set_one_launch_200_passes": round(read_gbs_one, 1),
                      "frac_of_measured_read": round(achieved / read_gbs, 4),
                      "step_bytes_B_step": 4 * n_w + 160 * n_total,
                      "step_GBs": round((4 * n_w + 160 * n_total) * steps_per_s / 1e9, 1),
@@ -312,6 +334,9 @@ def bench_uneven(args) -> int:
             per_size[str(n_e)] = {"networks": len(idx), "plain_single_steps_per_s": round(plain_steps / (pms / 1000.0), 1),
                                   "gemv_row_block": tuned.get(n_e, (None,))[0]}
         multi.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_uneven(cfgs, rates, args.cpu_seconds)
     net_steps_per_s = len(cfgs) * args.steps / (ms / 1000.0)
     seq_s_per_step = sum(len(v) / per_size[str(n_e)]["plain_single_steps_per_s"] for n_e, v in rates.items())
     seq_rate = len(cfgs) / seq_s_per_step
@@ -341,7 +366,7 @@ def bench_uneven(args) -> int:
         "roofline": {"bound": "hbm", "kernel": "k_gemv_multi (propagate, all networks)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": gemv_bytes, "avg_kernel_us": round(k_ms * 1000.0, 2)},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -466,6 +491,47 @@ def cpu_baseline_adjoint(sim, win, base, seconds: float) -> dict:
     return {"value": round(dt / steps * 1000.0, 3), "unit": "s/1k steps", "cores": threads, "kind": "port",
             "sample": f"{steps} steps of the window (forward + reverse) from the same start state, FP64 oracle "
                       f"adjoint, {threads} OpenMP threads, {dt:.1f} s"}
+
+
+def cpu_baseline_uneven(cfgs, by_size: dict, seconds: float) -> dict:
+    """The uneven batch on the oracle's FP32 OpenMP loop: one network of each size class stepped
+    from its initial state for a share of the time budget (every GEMV row visited), the networks
+    of the batch then run one after the other: network-steps/s = networks / sum of the seconds per
+    step of each network."""
+    import tempfile
+
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py
+
+    eng = importlib.import_module("example-cuda-spiking-nn_amd")
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    per_size, s_per_step, parts = {}, 0.0, []
+    budget = seconds / max(1, len(by_size))
+    with tempfile.TemporaryDirectory() as d:
+        for n_e, idx in sorted(by_size.items()):
+            c = dict(cfgs[idx[0]])
+            c.pop("gemv_row_blocks", None)
+            path = os.path.join(d, f"cpu_{n_e}.json")
+            json.dump(c, open(path, "w"))
+            cfg = eng.config_dump(path)
+            weights = [eng.generate_weight_matrix(path, k) for k in range(4)]
+            p = oracle_py.params_from_config(cfg, lazy_normalize=True, dense_gemv=True)
+            orc = oracle_py.OracleSim(p, weights, precision=32)
+            orc.step(1)
+            t0 = time.perf_counter()
+            n = 0
+            while time.perf_counter() - t0 < budget:
+                orc.step(1)
+                n += 1
+            dt = time.perf_counter() - t0
+            orc.close()
+            per_size[str(n_e)] = round(n / dt, 2)
+            s_per_step += len(idx) * dt / n
+            parts.append(f"N_E {n_e}: {n} steps in {dt:.1f} s")
+    return {"value": round(len(cfgs) / s_per_step, 3), "unit": "network-steps/s", "cores": threads, "kind": "port",
+            "per_size_steps_per_s": per_size,
+            "sample": f"one network of each size from its initial state, FP32 oracle (lazy normalisation), "
+                      f"{threads} OpenMP threads ({'; '.join(parts)}); the {len(cfgs)} networks one after the other"}
 
 
 def cpu_baseline_batched(base: dict, seconds: float) -> dict:

@@ -20,5 +20,6 @@ from .engine import (  # noqa: F401
     generate_weight_matrix,
     lib,
     measure_read_bandwidth,
+    measure_read_bandwidth_one_launch,
     poisson_table,
 )

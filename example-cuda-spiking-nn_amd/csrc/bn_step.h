@@ -250,5 +250,8 @@ int launch_materialize_g(float* g_dst, const float* g_base, const FinalizeSource
 int launch_add_increments(float* dst, const float* src, const uint32_t* spikes, uint32_t n, bn_stream_t s);
 int launch_fill_u32(uint32_t* p, uint32_t value, size_t n, bn_stream_t s);
 int launch_read_probe(const void* buf, uint64_t bytes, float* sink, uint32_t blocks, bn_stream_t s);
+// `passes` passes over the buffer in one launch of `blocks` blocks (each pass from a rotated start).
+int launch_read_probe_passes(const void* buf, uint64_t bytes, uint32_t passes, float* sink, uint32_t blocks,
+                             bn_stream_t s);
 
 }  // namespace bn

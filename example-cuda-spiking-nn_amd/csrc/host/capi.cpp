@@ -208,6 +208,24 @@ int create(Config cfg, uint32_t flags, const std::string& config_path, bn_sim** 
     return BN_OK;
 }
 
+// Two HIP events destroyed on every exit path (timing helpers throw through guarded()).
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() {
+        BN_HIP_CHECK(hipEventCreate(&a), "hipEventCreate");
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            throw Error(BN_ERR_HIP, "hipEventCreate: failed");
+        }
+    }
+    ~EventPair() {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+};
+
 template <typename T>
 void d2h(T* dst, const T* src, size_t n, hipStream_t s) {
     if (dst) BN_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s), "D2H state");
@@ -307,6 +325,8 @@ int bn_sim_get_info(bn_sim* sim, bn_sim_info* info) {
     info->flags = sim->flags;
     info->kernels_per_step = sim->system->kernels_per_step();
     info->device_bytes = S.total_allocated_bytes();
+    info->steps_per_graph = sim->system->graph_steps();
+    info->graph_launches = sim->system->graph_launches();
     return BN_OK;
 }
 
@@ -393,24 +413,31 @@ int bn_sim_get_plasticity_traces(bn_sim* sim, uint32_t off, float* pre, float* p
     });
 }
 
+int bn_sim_get_spike_raster(bn_sim* sim, uint32_t first_step, uint32_t num_steps, uint32_t* dst) {
+    if (!sim || (!dst && num_steps)) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_get_spike_raster: null argument");
+    return guarded([&] {
+        if (!sim->system->fused()) throw Error(BN_ERR_RUNTIME, "spike raster: the component path keeps [neuron][slot] histories");
+        BN_HIP_CHECK(hipStreamSynchronize(sim->state->stream()), "raster sync");
+        const std::vector<uint32_t> rows =
+            sim->state->download_spike_ring_rows(first_step, num_steps, sim->system->clock_step());
+        std::memcpy(dst, rows.data(), rows.size() * sizeof(uint32_t));
+    });
+}
+
 int bn_sim_run_timed(bn_sim* sim, uint32_t num_steps, float* ms) {
     if (!sim || !ms) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_run_timed: null argument");
     return guarded([&] {
         System& sys = *sim->system;
         hipStream_t s = sim->state->stream();
-        hipEvent_t a, b;
-        BN_HIP_CHECK(hipEventCreate(&a), "hipEventCreate");
-        BN_HIP_CHECK(hipEventCreate(&b), "hipEventCreate");
+        EventPair ev;
         BN_HIP_CHECK(hipStreamSynchronize(s), "pre-timing sync");
-        BN_HIP_CHECK(hipEventRecord(a, s), "hipEventRecord");
+        BN_HIP_CHECK(hipEventRecord(ev.a, s), "hipEventRecord");
         if (sys.fused()) sys.launch_fused_steps(num_steps);
         else
             for (uint32_t k = 0; k < num_steps; ++k) sys.step_the_clock(sys.clock_step());
-        BN_HIP_CHECK(hipEventRecord(b, s), "hipEventRecord");
-        BN_HIP_CHECK(hipEventSynchronize(b), "hipEventSynchronize");
-        BN_HIP_CHECK(hipEventElapsedTime(ms, a, b), "hipEventElapsedTime");
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
+        BN_HIP_CHECK(hipEventRecord(ev.b, s), "hipEventRecord");
+        BN_HIP_CHECK(hipEventSynchronize(ev.b), "hipEventSynchronize");
+        BN_HIP_CHECK(hipEventElapsedTime(ms, ev.a, ev.b), "hipEventElapsedTime");
         sim->state->check_status();
     });
 }
@@ -444,7 +471,8 @@ int bn_sim_load_state(bn_sim* sim, const char* path) {
     });
 }
 
-int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s) {
+namespace {
+int measure_read_bandwidth(uint64_t bytes, uint32_t repeats, bool in_one_launch, float* gb_per_s) {
     if (!gb_per_s || bytes < 16 || repeats == 0)
         return set_error(BN_ERR_INVALID_ARGUMENT, "bn_measure_read_bandwidth: bad argument");
     return guarded([&] {
@@ -470,8 +498,12 @@ int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s)
             BN_HIP_CHECK(hipEventCreate(&b), "hipEventCreate");
             BN_HIP_CHECK((hipError_t)launch_read_probe(buf, bytes, sink, blocks, s), "probe launch");
             BN_HIP_CHECK(hipEventRecord(a, s), "hipEventRecord");
-            for (uint32_t k = 0; k < repeats; ++k)
-                BN_HIP_CHECK((hipError_t)launch_read_probe(buf, bytes, sink, blocks, s), "probe launch");
+            if (in_one_launch) {
+                BN_HIP_CHECK((hipError_t)launch_read_probe_passes(buf, bytes, repeats, sink, blocks, s), "probe launch");
+            } else {
+                for (uint32_t k = 0; k < repeats; ++k)
+                    BN_HIP_CHECK((hipError_t)launch_read_probe(buf, bytes, sink, blocks, s), "probe launch");
+            }
             BN_HIP_CHECK(hipEventRecord(b, s), "hipEventRecord");
             BN_HIP_CHECK(hipEventSynchronize(b), "hipEventSynchronize");
             float ms = 0.0f;
@@ -483,6 +515,15 @@ int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s)
         }
         cleanup();
     });
+}
+}  // namespace
+
+int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s) {
+    return measure_read_bandwidth(bytes, repeats, false, gb_per_s);
+}
+
+int bn_measure_read_bandwidth_one_launch(uint64_t bytes, uint32_t passes, float* gb_per_s) {
+    return measure_read_bandwidth(bytes, passes, true, gb_per_s);
 }
 
 int bn_sim_set_steps_per_graph(bn_sim* sim, uint32_t steps_per_graph) {

@@ -374,6 +374,31 @@ const std::vector<uint32_t>& GpuSimulationState::spike_ring_snapshot(uint32_t cl
     return ring_snapshot_;
 }
 
+// Rows [first_step, first_step + num_steps) of the spike-bit ring: step s lives in slot
+// s % interval until the slot is written again or the ring is flushed by a record step.
+std::vector<uint32_t> GpuSimulationState::download_spike_ring_rows(uint32_t first_step, uint32_t num_steps,
+                                                                   uint32_t clock_step) const {
+    if (!spike_ring_) throw Error(BN_ERR_RUNTIME, "spike raster: no spike ring (the simulation runs without monitors)");
+    const uint32_t T = opt_.recording_interval_steps;
+    const uint64_t end = (uint64_t)first_step + num_steps;
+    if (end > clock_step) throw Error(BN_ERR_OUT_OF_RANGE, "spike raster: steps past the last executed step");
+    if ((uint64_t)clock_step - first_step > T)
+        throw Error(BN_ERR_OUT_OF_RANGE, "spike raster: steps older than one recording interval were overwritten");
+    if (ring_snapshot_step_ >= 0 && (int64_t)first_step <= ring_snapshot_step_)
+        throw Error(BN_ERR_OUT_OF_RANGE, "spike raster: steps up to the last record step were flushed to the recorders");
+    std::vector<uint32_t> host((size_t)num_steps * words_);
+    for (uint32_t k = 0; k < num_steps;) {
+        const uint32_t slot = (first_step + k) % T;
+        const uint32_t run = std::min(num_steps - k, T - slot);
+        BN_HIP_CHECK(hipMemcpyAsync(host.data() + (size_t)k * words_, spike_ring_ + (size_t)slot * words_,
+                                    (size_t)run * words_ * 4, hipMemcpyDeviceToHost, stream_),
+                     "D2H spike ring rows");
+        k += run;
+    }
+    BN_HIP_CHECK(hipStreamSynchronize(stream_), "spike ring rows sync");
+    return host;
+}
+
 std::vector<float> GpuSimulationState::download_and_flush_v_ring() {
     const size_t elems = (size_t)std::max<uint32_t>(n_vrec(), 1) * opt_.recording_interval_steps;
     std::vector<float> host(elems);

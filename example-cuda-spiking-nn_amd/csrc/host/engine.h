@@ -123,6 +123,7 @@ public:
     // zeroed), and [interval][n_vrec] voltages.
     const std::vector<uint32_t>& spike_ring_snapshot(uint32_t clock_step);
     std::vector<float> download_and_flush_v_ring();
+    std::vector<uint32_t> download_spike_ring_rows(uint32_t first_step, uint32_t num_steps, uint32_t clock_step) const;
     // Dense [pre][post] true weights (lazy normalisation: fl32(s_i * M_ij)).
     std::vector<float> download_weight_matrix(uint32_t weight_matrix_idx) const;
     std::string lookup_weight_matrix_name(uint32_t weight_matrix_idx) const;
@@ -404,6 +405,8 @@ public:
     // graph only (plans and state are untouched).
     void set_steps_per_graph(uint32_t steps_per_graph);
     uint32_t steps_per_graph() const { return steps_per_graph_; }
+    uint32_t graph_steps() const { return fused_ && use_graph_ ? steps_per_graph_ : 0; }
+    uint32_t graph_launches() const { return graph_launches_; }
     bool fused() const { return fused_; }
     void launch_fused_steps(uint32_t n);           // enqueue n steps (graph or eager)
     void launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end);
@@ -453,6 +456,7 @@ private:
     std::vector<std::unique_ptr<Monitor>> monitors_;
     std::vector<std::function<void(uint32_t)>> step_callbacks_;
     uint32_t clock_step_ = 0;
+    uint32_t graph_launches_ = 0;
 
     bool fused_ = false, use_graph_ = false;
     uint32_t steps_per_graph_ = 1;

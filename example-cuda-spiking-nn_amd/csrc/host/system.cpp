@@ -426,8 +426,10 @@ void System::launch_fused_steps(uint32_t n) {
     hipStream_t s = state_.stream();
     uint32_t k = 0;
     if (use_graph_)
-        for (; k + steps_per_graph_ <= n; k += steps_per_graph_)
+        for (; k + steps_per_graph_ <= n; k += steps_per_graph_) {
             BN_HIP_CHECK(hipGraphLaunch(graph_exec_, s), "hipGraphLaunch");
+            ++graph_launches_;
+        }
     for (; k < n; ++k) enqueue_fused_step(nullptr, nullptr);
     clock_step_ += n;
 }

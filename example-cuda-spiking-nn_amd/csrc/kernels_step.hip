@@ -1379,6 +1379,57 @@ __global__ __launch_bounds__(256) void k_read_probe(const float4* __restrict__ p
     if (a == 12345.678f) sink[blockIdx.x] = a;  // practically never; the sum must not be dead
 }
 
+// Many passes over the buffer inside one launch (no launch ramp or drain between passes).  Pass
+// p starts at element p * (n4 / 8 + 4099) (mod n4): blocks read a different part of the buffer
+// every pass, so a block's XCD does not find last pass's lines in its own L2 and the reads are
+// served past the L2 (from the MALL when the buffer fits it).  Accesses stay coalesced except
+// where a wave straddles the wrap.
+template <bool kNT>
+__global__ __launch_bounds__(256) void k_read_probe_passes(const float4* __restrict__ p, size_t n4, uint32_t passes,
+                                                           float* sink) {
+    float acc[kProbeLoads] = {};
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t rot = n4 / 8 + 4099;
+    size_t base = 0;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+        for (; i + (kProbeLoads - 1) * stride < n4; i += kProbeLoads * stride) {
+            float4 v[kProbeLoads];
+#pragma unroll
+            for (uint32_t u = 0; u < kProbeLoads; ++u) {
+                size_t k = i + u * stride + base;
+                if (k >= n4) k -= n4;
+                v[u] = load_w4<kNT>(reinterpret_cast<const float*>(p + k));
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kProbeLoads; ++u) acc[u] += (v[u].x + v[u].y) + (v[u].z + v[u].w);
+        }
+        for (; i < n4; i += stride) {
+            size_t k = i + base;
+            if (k >= n4) k -= n4;
+            acc[0] += (p[k].x + p[k].y) + (p[k].z + p[k].w);
+        }
+        base += rot;
+        while (base >= n4) base -= n4;
+    }
+    float a = 0.0f;
+#pragma unroll
+    for (uint32_t u = 0; u < kProbeLoads; ++u) a += acc[u];
+    if (a == 12345.678f) sink[blockIdx.x] = a;  // practically never; the sum must not be dead
+}
+
+int launch_read_probe_passes(const void* buf, uint64_t bytes, uint32_t passes, float* sink, uint32_t blocks,
+                             bn_stream_t s) {
+    const uint64_t n4 = bytes / 16;
+    if (bytes > (192ull << 20))
+        hipLaunchKernelGGL(k_read_probe_passes<true>, dim3(blocks), dim3(256), 0, s, static_cast<const float4*>(buf),
+                           (size_t)n4, passes, sink);
+    else
+        hipLaunchKernelGGL(k_read_probe_passes<false>, dim3(blocks), dim3(256), 0, s, static_cast<const float4*>(buf),
+                           (size_t)n4, passes, sink);
+    return (int)hipGetLastError();
+}
+
 // `blocks` bounds the grid (and the sink's length); fewer are launched when kProbeLoads loads
 // per thread cover the buffer in one pass.
 int launch_read_probe(const void* buf, uint64_t bytes, float* sink, uint32_t blocks, bn_stream_t s) {

@@ -42,6 +42,7 @@ class SimInfo(C.Structure):
         ("num_plasticities", C.c_uint32), ("weight_matrix_ld", C.c_uint32 * 4),
         ("gemv_row_block", C.c_uint32 * 4), ("weight_pre", C.c_uint32 * 4), ("weight_post", C.c_uint32 * 4),
         ("flags", C.c_uint32), ("kernels_per_step", C.c_uint32), ("device_bytes", C.c_uint64),
+        ("steps_per_graph", C.c_uint32), ("graph_launches", C.c_uint32),
     ]
 
 
@@ -110,7 +111,7 @@ EXPORTED_SYMBOLS = [
     "bn_multi_set_schedule", "bn_multi_set_streams", "bn_window_create", "bn_window_destroy", "bn_window_set_params",
     "bn_window_get_weights", "bn_window_set_targets", "bn_window_forward", "bn_window_backward",
     "bn_window_get_info", "bn_window_get_gradient", "bn_measure_read_bandwidth", "bn_sim_save_state",
-    "bn_sim_load_state",
+    "bn_sim_load_state", "bn_sim_get_spike_raster", "bn_measure_read_bandwidth_one_launch",
 ]
 
 _lib = None
@@ -146,9 +147,11 @@ def lib() -> C.CDLL:
     L.bn_sim_get_connection_traces.argtypes = [P, U, fp, fp, fp]
     L.bn_sim_get_plasticity_traces.argtypes = [P, U, fp, fp, fp, fp]
     L.bn_sim_run_timed.argtypes = [P, U, fp]
+    L.bn_sim_get_spike_raster.argtypes = [P, U, U, up]
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
     L.bn_sim_set_steps_per_graph.argtypes = [P, U]
     L.bn_measure_read_bandwidth.argtypes = [C.c_uint64, U, fp]
+    L.bn_measure_read_bandwidth_one_launch.argtypes = [C.c_uint64, U, fp]
     L.bn_sim_save_state.argtypes = [P, C.c_char_p]
     L.bn_sim_load_state.argtypes = [P, C.c_char_p]
     L.bn_config_dump_json.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -217,6 +220,13 @@ def measure_read_bandwidth(nbytes: int, repeats: int = 50) -> float:
     """GB/s of a plain streaming read over an `nbytes` device buffer, warm (see the C ABI)."""
     gbs = C.c_float(0)
     _check(lib().bn_measure_read_bandwidth(nbytes, repeats, C.byref(gbs)))
+    return gbs.value
+
+
+def measure_read_bandwidth_one_launch(nbytes: int, passes: int = 100) -> float:
+    """GB/s of `passes` streaming passes over an `nbytes` device buffer inside one launch."""
+    gbs = C.c_float(0)
+    _check(lib().bn_measure_read_bandwidth_one_launch(nbytes, passes, C.byref(gbs)))
     return gbs.value
 
 
@@ -332,6 +342,15 @@ class Simulation:
         out = np.zeros((self.info.weight_pre[idx], self.info.weight_post[idx]), np.float32)
         _check(lib().bn_sim_get_weights(self.h, idx, _fp(out)))
         return out
+
+    def spike_raster(self, first_step: int, num_steps: int) -> np.ndarray:
+        """[num_steps][total_neurons] 0/1 spikes of the given steps from the device spike ring
+        (monitors on; see bn_sim_get_spike_raster)."""
+        words = (self.N + 31) // 32
+        bits = np.zeros((num_steps, words), np.uint32)
+        _check(lib().bn_sim_get_spike_raster(self.h, first_step, num_steps, _up(bits)))
+        flat = np.unpackbits(bits.view(np.uint8).reshape(num_steps, words * 4), axis=1, bitorder="little")
+        return flat[:, : self.N].astype(np.uint32)
 
     def connection_traces(self, idx: int):
         n = self.info.weight_pre[idx]

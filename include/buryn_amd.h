@@ -310,6 +310,8 @@ typedef struct bn_sim_info {
     uint32_t flags;
     uint32_t kernels_per_step;       /* launches in one fused step */
     uint64_t device_bytes;           /* device memory held by the simulation */
+    uint32_t steps_per_graph;        /* steps per captured graph of the fused step (0: no graph) */
+    uint32_t graph_launches;         /* captured graphs launched so far */
 } bn_sim_info;
 
 /* Host pointers to fill; any may be NULL.  Arrays are indexed by global neuron id (length
@@ -348,6 +350,12 @@ int bn_sim_get_connection_traces(bn_sim* sim, uint32_t idx, float* r, float* d, 
 /* Plasticity traces (length total_neurons each, global ids); any may be NULL. */
 int bn_sim_get_plasticity_traces(bn_sim* sim, uint32_t plasticity_offset, float* pre, float* post,
                                  float* triplet_pre, float* triplet_post);
+/* Spike bits of steps [first_step, first_step + num_steps) from the device ring the fused step
+ * writes every step (the monitors' spike history; needs monitors on): dst [num_steps][words],
+ * words = ceil(total_neurons / 32), bit (g & 31) of word g >> 5 = global neuron g spiked.  The
+ * steps must have run, lie within the last recording interval, and follow the last record step
+ * (whose record flushes the ring). */
+int bn_sim_get_spike_raster(bn_sim* sim, uint32_t first_step, uint32_t num_steps, uint32_t* dst);
 /* Times `num_steps` steps between two HIP events on the simulation stream; *ms = elapsed. */
 int bn_sim_run_timed(bn_sim* sim, uint32_t num_steps, float* ms);
 /* Steps per captured graph of the fused step (no counterpart in the reference, whose host loop
@@ -374,6 +382,10 @@ int bn_sim_load_state(bn_sim* sim, const char* path);
  * warm-up pass (a working set under the 256 MB MALL stays cache-resident between passes).  GB/s
  * from HIP events; allocates and frees its buffer. */
 int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s);
+/* The same read rate with all `passes` passes inside one launch (4,096 blocks of 256 threads, each
+ * pass starting from a rotated offset so that no XCD re-reads its own L2 lines): the steady-state
+ * streaming rate without launch ramp and drain.  Measurement aid, no counterpart in the reference. */
+int bn_measure_read_bandwidth_one_launch(uint64_t bytes, uint32_t passes, float* gb_per_s);
 /* Writes the JSON of the parsed config (all fields, after defaults/overrides/prefixing). */
 int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, size_t* needed);
 

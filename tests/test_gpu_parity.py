@@ -48,13 +48,32 @@ def _check_traces_and_weights(sim, orc):
 
 @pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
 def test_fused_step_bit_exact(tmp_path, graph):
+    # graph: whole captured graphs of 20 steps (step 0, a record step, runs alone first), the state
+    # compared after every graph and the raster of every step from the device spike ring;
+    # eager: no graph at all, the state compared after every step.
     path = _write(tmp_path, small_config(steps=600))
-    flags = ("no_monitors", "quiet") + (() if graph else ("no_graph",))
+    flags = ("quiet",) if graph else ("no_monitors", "quiet", "no_graph")
     sim = engine().Simulation(path, flags=flags)
     orc = oracle_for(path, sim)
-    _run_lockstep(sim, orc, 300)
+    if graph:
+        sim.set_steps_per_graph(20)
+        sim.run(1)
+        orc.step(1)
+        step = 1
+        for _ in range(15):
+            sim.run(20)
+            raster = []
+            for _ in range(20):
+                orc.step(1)
+                raster.append(orc.state()["spikes"])
+            assert np.array_equal(sim.spike_raster(step, 20), np.array(raster)), f"raster of steps {step}..{step + 19}"
+            step += 20
+            assert_state_equal(sim.state(), orc.state(), step - 1)
+        info = sim.get_info()
+        assert info.steps_per_graph == 20 and info.graph_launches == 15  # every step after 0 ran in graphs
+    else:
+        _run_lockstep(sim, orc, 300)
     _check_traces_and_weights(sim, orc)
-    assert sim.state()["spikes"].sum() >= 0
     sim.close()
 
 
@@ -200,26 +219,130 @@ def test_spike_list_overflow_bit_exact(tmp_path, sequential):
     sim.close()
 
 
-def test_workload_3125_raster_1000_steps(tmp_path):
+def _fp64_bars(g, o64, o64_spikes, ok, v_t):
+    """Per-step comparison with the FP64 oracle: (V relative error over the neurons in the
+    contracting regime, g_e and g_i errors relative to the step's largest |g|).  `ok` marks the
+    neurons whose FP64 V stayed below V_T since their last spike (or the start); it is updated."""
+    v, ref_v = g["membrane_potentials"].astype(np.float64), o64["V"]
+    ok &= (ref_v < v_t) | (o64_spikes == 1)
+    v_rel = float(np.max(np.abs(v - ref_v)[ok] / np.abs(ref_v[ok]))) if ok.any() else 0.0
+    ok |= o64_spikes == 1  # a spike restarts the trajectory from the same capped / reset values
+    g_rel = []
+    for a, b in (("g_excitatory", "ge"), ("g_inhibitory", "gi")):
+        ref = o64[b]
+        scale = max(float(np.max(np.abs(ref))), 1e-30)
+        g_rel.append(float(np.max(np.abs(g[a].astype(np.float64) - ref))) / scale)
+    return v_rel, g_rel[0], g_rel[1]
+
+
+@pytest.mark.parametrize("normalization", ["faithful", "lazy"])
+def test_workload_3125_raster_1000_steps(tmp_path, normalization):
+    # The 3,125-neuron workload stepped one step at a time for 1,000 steps: every step bit-exact
+    # with the FP32 oracle, and against the FP64 oracle (an independent accuracy reference):
+    # identical rasters, g_e and g_i within 1e-5 of the step's largest conductance, V within 1e-5
+    # relative for every neuron whose FP64 V stayed below V_T since its last spike.  Above V_T the
+    # EIF exponential multiplies a difference by about 1 + dt/C g_L exp((V - V_T)/Delta_T) per
+    # step (300 at V = -30 mV), so an FP32 evaluation of the step drifts from FP64 there by up to
+    # ~1e-2 relative before the spike caps V at the same v_threshold; those neuron-steps are
+    # counted, not bounded.
     cfg = load_config("net3125_stdp.json")
     path = _write(tmp_path, cfg)
-    sim = engine().Simulation(path, flags=("no_monitors", "quiet"))
-    orc = oracle_for(path, sim)
+    lazy = normalization == "lazy"
+    sim = engine().Simulation(path, flags=("no_monitors", "quiet") + (("lazy_normalize",) if lazy else ()))
+    orc = oracle_for(path, sim, lazy=lazy)
+    o64 = oracle_for(path, sim, precision=64)
     n_spikes = 0
-    worst = 0.0
+    ok = np.ones(sim.N, bool)
+    worst = [0.0, 0.0, 0.0]
+    checked = 0
     for step in range(1000):
         sim.run(1)
         orc.step(1)
+        o64.step(1)
         g, o = sim.state(), orc.state()
-        assert np.array_equal(g["spikes"], o["spikes"]), f"raster differs at step {step}"
-        for a, b in (("membrane_potentials", "V"), ("g_excitatory", "ge"), ("g_inhibitory", "gi")):
-            den = np.maximum(np.abs(o[b]), 1e-30)
-            worst = max(worst, float(np.max(np.abs(g[a] - o[b]) / den)))
+        assert_state_equal(g, o, step, fields=("membrane_potentials", "g_excitatory", "g_inhibitory", "spikes",
+                                               "refractory_counter"))
+        sp64 = o64.state()["spikes"]
+        assert np.array_equal(g["spikes"], sp64), f"raster differs from FP64 at step {step}"
+        bars = _fp64_bars(g, o64.state64(), sp64, ok, cfg["v_effective_spike_threshold"])
+        worst = [max(a, b) for a, b in zip(worst, bars)]
+        checked += int(ok.sum())
         n_spikes += int(g["spikes"].sum())
-    assert worst <= 1e-5, worst
+    print(f"FP64 bars over 1,000 steps: V {worst[0]:.3g} (on {checked / (1000 * sim.N):.2f} of neuron-steps), "
+          f"g_e {worst[1]:.3g}, g_i {worst[2]:.3g}; {n_spikes} spikes")
+    assert worst[0] <= 1e-5 and worst[1] <= 1e-5 and worst[2] <= 1e-5, worst
+    assert checked >= 0.5 * 1000 * sim.N
     assert n_spikes > 0
     w_gpu, w_orc = sim.weights(0), orc.weights(0)
     assert np.array_equal(w_gpu, w_orc)
+    sim.close()
+
+
+def _graph_lockstep(sim, orc, first_step, graphs, spg):
+    """`graphs` whole graphs of `spg` steps from `first_step`: the raster of every step from the
+    device spike ring and the state after every graph, bit for bit against the oracle."""
+    step = first_step
+    spikes = 0
+    for _ in range(graphs):
+        sim.run(spg)
+        raster = []
+        for _ in range(spg):
+            orc.step(1)
+            raster.append(orc.state()["spikes"])
+        raster = np.array(raster)
+        got = sim.spike_raster(step, spg)
+        if not np.array_equal(got, raster):
+            bad = np.nonzero((got != raster).any(axis=1))[0]
+            raise AssertionError(f"raster differs at steps {step + bad[:5]}")
+        spikes += int(raster.sum())
+        step += spg
+        assert_state_equal(sim.state(), orc.state(), step - 1)
+    return spikes
+
+
+@pytest.mark.parametrize("spg", [500, 100])
+def test_bench_form_3125_graphs(tmp_path, spg):
+    # The form the benchmark times: lazy normalisation, overlapped graphs of 500 (and the library
+    # default 100) steps with the monitors' spike ring, 1,000 steps bit-exact against the FP32
+    # lazy oracle (raster of every step, state after every graph, traces and weights at the end).
+    cfg = load_config("net3125_stdp.json")
+    path = _write(tmp_path, cfg)
+    sim = engine().Simulation(path, flags=("quiet", "lazy_normalize"))
+    sim.set_steps_per_graph(spg)
+    assert sim.get_info().kernels_per_step == 2  # the overlapped form
+    orc = oracle_for(path, sim, lazy=True)
+    sim.run(1)  # step 0 is a record step and runs alone
+    orc.step(1)
+    spikes = _graph_lockstep(sim, orc, 1, 1000 // spg, spg)
+    assert sim.get_info().graph_launches == 1000 // spg
+    _check_traces_and_weights(sim, orc)
+    assert spikes > 0
+    sim.close()
+
+
+def test_net10125_graphs(tmp_path):
+    # The 10,125-neuron workload (410 MB of weights, non-temporal weight loads, long row chunks):
+    # lazy normalisation, overlapped graphs of 250 steps, 1,001 steps against the FP32 lazy oracle.
+    cfg = load_config("net10125_stdp.json")
+    path = _write(tmp_path, cfg)
+    sim = engine().Simulation(path, flags=("quiet", "lazy_normalize"))
+    sim.set_steps_per_graph(250)
+    assert sim.get_info().kernels_per_step == 2
+    orc = oracle_for(path, sim, lazy=True)
+    sim.run(1)
+    orc.step(1)
+    spikes = _graph_lockstep(sim, orc, 1, 4, 250)
+    assert sim.get_info().graph_launches == 4
+    print(f"net10125: {spikes} spikes in steps 1..1000; row blocks {tuple(sim.info.gemv_row_block)}")
+    assert spikes > 0
+    for k in range(4):
+        for a, b, name in zip(sim.connection_traces(k), orc.connection_traces(k), ("r", "d", "x")):
+            assert np.array_equal(a, b), f"connection {k} trace {name}"
+    for off in range(sim.info.num_plasticities):
+        for a, b in zip(sim.plasticity_traces(off), orc.plasticity_traces(off)):
+            assert np.array_equal(a, b)
+    for k in (0, 2):  # the plastic matrices
+        assert np.array_equal(sim.weights(k), orc.weights(k)), f"weights {k}"
     sim.close()
 
 
@@ -230,7 +353,12 @@ def test_read_bandwidth_probe():
     for nbytes in (1 << 20, 39_062_500 // 16 * 16, 256 << 20):
         gbs = eng.measure_read_bandwidth(nbytes, 5)
         assert 10.0 < gbs < 20000.0, (nbytes, gbs)
+    for nbytes in (4096, 39_062_500 // 16 * 16, 410_062_500 // 16 * 16):
+        gbs = eng.measure_read_bandwidth_one_launch(nbytes, 20)
+        assert 1.0 < gbs < 40000.0, (nbytes, gbs)
     with pytest.raises(eng.BnError):
         eng.measure_read_bandwidth(8, 1)
     with pytest.raises(eng.BnError):
         eng.measure_read_bandwidth(1 << 20, 0)
+    with pytest.raises(eng.BnError):
+        eng.measure_read_bandwidth_one_launch(1 << 20, 0)
