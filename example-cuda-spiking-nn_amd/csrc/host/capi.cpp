@@ -100,6 +100,7 @@ void build(bn_sim* sim, const std::string& config_path) {
     }
     opt.lazy_normalize = (flags & BN_SIM_LAZY_NORMALIZE) && c.e_e_plasticity_enabled;
     opt.overlap_steps = !(flags & BN_SIM_SEQUENTIAL_GRAPH);
+    opt.reference_layout = flags & BN_SIM_REFERENCE_LAYOUT;
     opt.rng_seed = c.background_noise_seed >= 0 ? (int64_t)c.background_noise_seed : (int64_t)c.rng_seed;
     for (int k = 0; k < 4; ++k) opt.gemv_row_block[k] = c.gemv_row_blocks[k];
     opt.gemv_nontemporal = c.gemv_nontemporal_loads;

@@ -123,12 +123,13 @@ GpuSimulationState::GpuSimulationState(uint32_t total_simulation_length, uint32_
     bdi_ = alloc<float>(N, "background_inhibitory_d_traces");
     clock_ = alloc<bn_device_clock>(1, "clock");
     status_ = alloc<uint32_t>(1, "status");
-    // Weights: EE, EI, IE, II, row-major [pre][post] with a 64-float (256 B) aligned row stride.
+    // Weights: EE, EI, IE, II, row-major [pre][post] with a 64-float (256 B) aligned row stride
+    // (or the reference's unpadded rows).
     const uint32_t pres[4] = {ne_, ne_, ni_, ni_}, posts[4] = {ne_, ni_, ne_, ni_};
     for (int k = 0; k < 4; ++k) {
         pre_[k] = pres[k];
         post_[k] = posts[k];
-        ld_[k] = round_up(posts[k], 64);
+        ld_[k] = opt_.reference_layout ? posts[k] : round_up(posts[k], 64);
         row_block_[k] = opt_.gemv_row_block[k] ? opt_.gemv_row_block[k] : default_row_block(pres, posts);
         w_[k] = alloc<float>((size_t)pres[k] * ld_[k], "weight_matrix");
         tr_r_[k] = alloc<float>(pres[k], "postsynaptic_r_traces");

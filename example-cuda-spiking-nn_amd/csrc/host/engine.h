@@ -37,6 +37,7 @@ struct StateOptions {
     std::array<uint32_t, 4> gemv_row_block = {0, 0, 0, 0};  // 0 = heuristic
     int gemv_nontemporal = -1;  // weight loads past the caches: 1 yes, 0 no, -1 when W >> MALL
     bool overlap_steps = true;  // captured graphs overlap plasticity(t-1) with evolve(t)
+    bool reference_layout = false;  // weight rows unpadded (row stride = post), as the reference allocates
 };
 
 class GpuSimulationState {

@@ -291,9 +291,12 @@ enum {
     BN_SIM_COMPONENT_PATH = 1u << 3,    /* step through the component launchers (drop-in path) */
     BN_SIM_NO_GRAPH = 1u << 4,          /* launch the fused step eagerly instead of a hipGraph */
     BN_SIM_QUIET = 1u << 5,             /* no progress prints */
-    BN_SIM_SEQUENTIAL_GRAPH = 1u << 6   /* graphs run each step's launches one after the other
+    BN_SIM_SEQUENTIAL_GRAPH = 1u << 6,  /* graphs run each step's launches one after the other
                                          * instead of overlapping the plasticity of step t-1
                                          * with the evolve of step t (same results) */
+    BN_SIM_REFERENCE_LAYOUT = 1u << 7   /* weight rows unpadded (row stride = post), as the
+                                         * reference allocates them (same results, slower GEMV
+                                         * where post % 4 != 0) */
 };
 
 typedef struct bn_sim_info {
@@ -393,6 +396,103 @@ int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, siz
  * matrix idx for the given config, without writing files. */
 int bn_generate_weight_matrix(const char* config_path, uint32_t weight_matrix_idx, float* dst,
                               uint32_t* pre, uint32_t* post);
+
+/* -------------------------------------------------------------------------------------
+ * Device state object: replaces GpuSimulationState (gpu_core/gpu_simulation_state.h:19-103,
+ * gpu_simulation_state.cu:35-104) for a reference-side caller that keeps its own components and
+ * calls the drop-in launchers above.  Every device buffer of one network is allocated once by
+ * bn_state_create and owned by the state; the caller builds its *View structs from
+ * bn_state_buffers as the reference's make_gpu_view functions do from the d_*() getters.  The
+ * Poisson tables, the GEMV scratch (gemv_partials) and the device clock (in place of the
+ * per-neuron cuRAND states) are parts of the state.
+ * ------------------------------------------------------------------------------------- */
+typedef struct bn_state bn_state;
+
+enum {
+    BN_STATE_REFERENCE_LAYOUT = 1u << 0, /* weight rows unpadded: row stride = post (the reference's) */
+    BN_STATE_HISTORIES = 1u << 1         /* [neuron][slot] spike and voltage histories of
+                                            recording_interval_steps slots (the monitors') */
+};
+
+typedef struct bn_state_buffers {
+    uint32_t total_neurons;
+    uint32_t num_excitatory_neurons;
+    uint32_t num_inhibitory_neurons;
+    uint32_t recording_interval_steps;
+    /* neuron state, global ids (gpu_simulation_state.h:38-50) */
+    float* membrane_potentials;
+    float* g_excitatory;
+    float* g_inhibitory;
+    float* i_stimulus;
+    uint32_t* refractory_counter;
+    uint32_t* spikes;
+    float* background_g_excitatory_synaptic_conductances;
+    float* background_g_inhibitory_synaptic_conductances;
+    float* background_excitatory_postsynaptic_r_traces;
+    float* background_excitatory_postsynaptic_d_traces;
+    float* background_inhibitory_postsynaptic_r_traces;
+    float* background_inhibitory_postsynaptic_d_traces;
+    bn_device_clock* clock;             /* in place of d_neuron_rng_states */
+    /* per weight matrix 0 EE, 1 EI, 2 IE, 3 II (gpu_simulation_state.h:52-67) */
+    float* weight_matrix[4];            /* [pre][post], row stride weight_matrix_ld */
+    uint32_t weight_matrix_ld[4];
+    uint32_t weight_pre[4];
+    uint32_t weight_post[4];
+    float* postsynaptic_r_traces[4];
+    float* postsynaptic_d_traces[4];
+    float* summed_postsynaptic_conductances[4];
+    double* gemv_partials[4];           /* ConnectionView::gemv_partials */
+    uint32_t gemv_row_block[4];
+    /* stimulus (gpu_simulation_state.h:69-72) */
+    uint32_t* timestep_event_count;
+    uint32_t* stimulus_event_neuron_ids;
+    uint32_t* stimulus_event_post_group_offsets;
+    float* stimulus_event_amplitude_changes;
+    /* plasticity, [num_plasticities][total_neurons] each (gpu_simulation_state.h:74-78) */
+    float* pre_traces;
+    float* post_traces;
+    float* triplet_pre_traces;
+    float* triplet_post_traces;
+    float* pre_weight_sums;
+    /* monitors (gpu_simulation_state.h:80-81); NULL without BN_STATE_HISTORIES */
+    uint32_t* spike_history;
+    float* membrane_potential_history;
+} bn_state_buffers;
+
+int bn_state_create(uint32_t total_simulation_length, uint32_t num_excitatory_neurons,
+                    uint32_t num_inhibitory_neurons, uint32_t num_stimulus_events, uint32_t num_plasticities,
+                    uint64_t rng_seed, uint32_t flags, bn_state** out);
+void bn_state_destroy(bn_state* state);
+int bn_state_get_buffers(bn_state* state, bn_state_buffers* out);
+/* The stream every upload/download of the state runs on (launchers may use it or any other). */
+int bn_state_get_stream(bn_state* state, bn_stream_t* out);
+int bn_state_total_allocated_bytes(bn_state* state, uint64_t* bytes);
+/* Poisson table of group (0 E, 1 I) and kind (0 excitatory, 1 inhibitory background) for
+ * `mean`; *table and *len receive the device table for the NeuronGroupView (len 0: no draws). */
+int bn_state_poisson_table(bn_state* state, uint32_t group, uint32_t kind, float mean, const uint32_t** table,
+                           uint32_t* len);
+/* upload_*_view equivalents (gpu_simulation_state.h:84-88): host arrays into the view's slices
+ * of the state, with the reference's bounds and size checks.  NULL arrays upload zeros. */
+int bn_state_upload_neuron_group_view(bn_state* state, const NeuronGroupView* view, const bn_neuron_state_host* host);
+/* weights: dense [pre][post] host array (non-negative, gpu_simulation_state.cu:221-229). */
+int bn_state_upload_connection_view(bn_state* state, const ConnectionView* view, const float* weights,
+                                    const float* r_traces, const float* d_traces, const float* x);
+int bn_state_upload_stimulus_view(bn_state* state, const StimulusView* view, const uint32_t* timestep_event_count,
+                                  uint32_t num_timesteps, const uint32_t* neuron_ids, const uint32_t* post_group_offsets,
+                                  const float* amplitude_changes, uint32_t num_events);
+/* pre_traces, triplet_pre_traces, pre_weight_sums: pre_group_num_neurons each; the post traces post_group_num_neurons. */
+int bn_state_upload_plasticity_view(bn_state* state, const PlasticityView* view, const float* pre_traces,
+                                    const float* post_traces, const float* triplet_pre_traces,
+                                    const float* triplet_post_traces, const float* pre_weight_sums);
+/* download_* equivalents (gpu_simulation_state.h:90-93): [num_neurons][num_timesteps] slices of
+ * the histories (zeroed after the copy) and the dense [pre][post] weights. */
+int bn_state_download_and_flush_spike_history(bn_state* state, uint32_t offset, uint32_t num_neurons,
+                                              uint32_t num_timesteps, uint32_t* dst);
+int bn_state_download_and_flush_membrane_potential_history(bn_state* state, uint32_t offset, uint32_t num_neurons,
+                                                           uint32_t num_timesteps, float* dst);
+int bn_state_download_weight_matrix(bn_state* state, uint32_t weight_matrix_idx, float* dst);
+/* Synchronous copy of `bytes` from a device address of the state to host memory. */
+int bn_state_download(bn_state* state, const void* device_src, void* host_dst, size_t bytes);
 
 /* -------------------------------------------------------------------------------------
  * Batched shared-weight form: B instances of one network whose weights are frozen
