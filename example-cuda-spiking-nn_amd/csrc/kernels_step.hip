@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "bn_device.h"
@@ -1682,10 +1683,9 @@ extern "C" int bn_neuron_group_evolve(NeuronGroupView v, bn_stream_t s) {
     return BN_OK;
 }
 
-extern "C" int bn_connection_propagate(ConnectionView v, bn_stream_t s) {
-    if (v.pre_group_num_neurons == 0 || v.post_group_num_neurons == 0) return BN_OK;
-    if (!v.gemv_partials || v.gemv_row_block == 0)
-        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_connection_propagate: gemv_partials/gemv_row_block not set");
+// kernel_connection_propagate.cu:8-94 for one connection: traces, the FP64 chunk partials of the
+// GEMV into `partials` ([ceil(pre / row_block)][ld] doubles), then g += the partials in order.
+static int connection_propagate(ConnectionView v, double* partials, uint32_t row_block, bn_stream_t s) {
     hipLaunchKernelGGL(k_connection_traces, dim3(cdiv(v.pre_group_num_neurons, 256)), dim3(256), 0, s, v);
     BN_HIP_TRY(hipGetLastError(), "kernel_connection_traces launch");
     GemvPlan P{};
@@ -1693,11 +1693,11 @@ extern "C" int bn_connection_propagate(ConnectionView v, bn_stream_t s) {
     C.W = v.weight_matrix;
     C.x = v.summed_postsynaptic_conductances;
     C.row_scale = nullptr;
-    C.partials = v.gemv_partials;
+    C.partials = partials;
     C.ld = v.weight_matrix_ld ? v.weight_matrix_ld : v.post_group_num_neurons;
     C.n_pre = v.pre_group_num_neurons;
     C.n_post = v.post_group_num_neurons;
-    C.row_block = v.gemv_row_block;
+    C.row_block = row_block;
     C.n_chunks = cdiv(C.n_pre, C.row_block);
     C.n_coltiles = cdiv(C.n_post, kGemvTileCols);
     C.tile_begin = 0;
@@ -1706,9 +1706,67 @@ extern "C" int bn_connection_propagate(ConnectionView v, bn_stream_t s) {
     P.total_tiles = C.tile_end;
     BN_HIP_TRY((hipError_t)launch_gemv(P, s), "gemv connection accumulate");
     float* target = v.pre_group_is_inhibtory ? v.post_group_g_inhibitory : v.post_group_g_excitatory;
-    FinalizeSource src{v.gemv_partials, C.n_chunks, C.ld};
+    FinalizeSource src{partials, C.n_chunks, C.ld};
     BN_HIP_TRY((hipError_t)launch_materialize_g(target, target, &src, 1, C.n_post, s), "gemv finalize");
     return BN_OK;
+}
+
+extern "C" int bn_connection_propagate(ConnectionView v, bn_stream_t s) {
+    if (v.pre_group_num_neurons == 0 || v.post_group_num_neurons == 0) return BN_OK;
+    if (!v.gemv_partials || v.gemv_row_block == 0)
+        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_connection_propagate: gemv_partials/gemv_row_block not set");
+    return connection_propagate(v, v.gemv_partials, v.gemv_row_block, s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Allocation-free propagate: the FP64 GEMV partials live in `scratch`, a file-scope device array
+// of kNoAllocScratchDoubles doubles (8 MiB per GPU, held until the process ends), instead of the
+// view's gemv_partials.  Host code reaches the array only through the address that
+// hipGetSymbolAddress returns on each device (the array's name is a host-side stand-in there);
+// it is looked up once per device, with that device current, on the entry point's first call
+// there.  Every call on one GPU shares that GPU's array: calls that overlap on one GPU (two
+// streams, two threads) race on it.  The kernels write every partial they later read, so
+// nothing needs clearing between calls.
+// ---------------------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchDoubles = size_t(1) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+static int scratch_on_current_device(double** out) {
+    int dev = -1;
+    BN_HIP_TRY(hipGetDevice(&dev), "bn_connection_propagate_no_alloc: hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return set_error(BN_ERR_RUNTIME, "bn_connection_propagate_no_alloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        double* p = nullptr;
+        BN_HIP_TRY(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)),
+                   "bn_connection_propagate_no_alloc: hipGetSymbolAddress(scratch)");
+        if (!p) return set_error(BN_ERR_RUNTIME, "bn_connection_propagate_no_alloc: null scratch address");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return BN_OK;
+}
+
+extern "C" int bn_connection_propagate_no_alloc(ConnectionView v, bn_stream_t s) {
+    if (v.pre_group_num_neurons == 0 || v.post_group_num_neurons == 0) return BN_OK;
+    if (v.gemv_row_block == 0)
+        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_connection_propagate_no_alloc: gemv_row_block not set");
+    const uint64_t ld = v.weight_matrix_ld ? v.weight_matrix_ld : v.post_group_num_neurons;
+    const uint64_t need = (uint64_t)cdiv(v.pre_group_num_neurons, v.gemv_row_block) * ld;
+    if (need > kNoAllocScratchDoubles)
+        return set_error(BN_ERR_INVALID_ARGUMENT,
+                         "bn_connection_propagate_no_alloc: ceil(pre / gemv_row_block) * row stride = " +
+                             std::to_string(need) + " partial sums exceed the scratch of " +
+                             std::to_string(kNoAllocScratchDoubles) + " (use a larger gemv_row_block)");
+    double* partials = nullptr;
+    const int rc = scratch_on_current_device(&partials);
+    if (rc != BN_OK) return rc;
+    return connection_propagate(v, partials, v.gemv_row_block, s);
 }
 
 extern "C" int bn_stimulus_apply(StimulusView v, uint32_t next_event_idx, uint32_t n, bn_stream_t s) {

@@ -244,6 +244,15 @@ int bn_neuron_group_evolve(NeuronGroupView view, bn_stream_t stream);
  * chunks of gemv_row_block rows, then the chunk sums are added to g in chunk order). */
 int bn_connection_propagate(ConnectionView view, bn_stream_t stream);
 
+/* No counterpart in the reference: bn_connection_propagate for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  view.gemv_partials is ignored: the FP64 partial sums go to a scratch array that the
+ * library holds per GPU (8 MiB, 1,048,576 doubles, until the process ends).  Same kernels, same
+ * results as bn_connection_propagate with the same gemv_row_block.  Rejected with
+ * BN_ERR_INVALID_ARGUMENT when ceil(pre / gemv_row_block) * row stride exceeds 1,048,576.  Calls
+ * no allocator.  Calls that overlap on one GPU (other streams or threads) race on the scratch. */
+int bn_connection_propagate_no_alloc(ConnectionView view, bn_stream_t stream);
+
 /* Replaces kernel_stimulus_apply_launch (stimuli/kernel_stimulus_apply.h:8-10). */
 int bn_stimulus_apply(StimulusView view, uint32_t next_event_idx, uint32_t num_events_to_apply,
                       bn_stream_t stream);
