@@ -126,6 +126,28 @@ double canonical_row_sum(const float* row, uint32_t n) {
     return ((wave[0] + wave[1]) + wave[2]) + wave[3];
 }
 
+// Sum over one 256-column tile of a row in the persistent step's order: lane l sums its four
+// columns from 0.0, the 64 lane sums combined by the xor butterfly (32,16,8,4,2,1).  val(j)
+// gives the term of column j (columns >= n contribute nothing).
+template <typename Val>
+double tile_sum(uint32_t q, uint32_t n, Val val) {
+    double v[64];
+    for (uint32_t l = 0; l < 64; ++l) {
+        double a = 0.0;
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t j = q * 256 + 4 * l + k;
+            if (j < n) a += val(j);
+        }
+        v[l] = a;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        double nv[64];
+        for (int l = 0; l < 64; ++l) nv[l] = v[l] + v[l ^ off];
+        std::memcpy(v, nv, sizeof v);
+    }
+    return v[0];
+}
+
 // ---------------------------------------------------------------------------------------
 // Simulation
 // ---------------------------------------------------------------------------------------
@@ -353,7 +375,8 @@ struct Sim {
                 // the chunks cut into kFinParts contiguous parts of ceil(chunks/kFinParts),
                 // each part summed in chunk order from 0.0; then g + part_0 + part_1 + ...
                 const uint32_t chunks = (C.n_pre + C.row_block - 1) / C.row_block;
-                const uint32_t per = (chunks + kFinParts - 1) / kFinParts;
+                const uint32_t per =
+                    p.gemv_part_chunks[c] ? p.gemv_part_chunks[c] : (chunks + kFinParts - 1) / kFinParts;
                 double qsum[NB];
                 for (uint32_t c0 = 0; c0 < chunks; c0 += per) {
                     const uint32_t c1 = std::min(c0 + per, chunks);
@@ -486,6 +509,8 @@ struct Sim {
         const uint32_t count = (uint32_t)posts.size();
         const uint32_t per = (count + kQuietParts - 1) / kQuietParts;
         std::vector<double> dq((size_t)kQuietParts * np, 0.0);
+        // change[(e, i)] = (double)M' - (double)M of the LTP entry (i, posts[e]) (tile order).
+        std::vector<double> change(p.lazy_tile_order ? (size_t)count * np : 0, 0.0);
         for (uint32_t e = 0; e < count; ++e) {
             const uint32_t j = posts[e], q = e / per;
             for (uint32_t i = 0; i < np; ++i) {
@@ -494,8 +519,14 @@ struct Sim {
                 const float w = clampw<float>(P.s[i] * m + (float)ltp_delta(P, i, j), (float)P.wmin, (float)P.wmax);
                 const float mn = w / P.s[i];
                 dq[(size_t)q * np + i] += (double)mn - (double)m;
+                if (p.lazy_tile_order) change[(size_t)e * np + i] = (double)mn - (double)m;
                 m = mn;
             }
+        }
+        change_ = std::move(change);
+        if (p.lazy_tile_order) {
+            lazy_tile_order_sums(P, M, np, nq, pre_sp, post_sp, posts);
+            return;
         }
         for (uint32_t i = 0; i < np; ++i) {
             float* row = &M[(size_t)i * nq];
@@ -512,6 +543,41 @@ struct Sim {
                 for (uint32_t q = 0; q < kQuietParts; ++q)
                     if (q * per < count) Rn += dq[(size_t)q * np + i];
                 P.Rs[i] = Rn;
+            }
+            if (P.Rs[i] > 0.0) P.s[i] = (float)((double)P.target[i] / P.Rs[i]);
+        }
+    }
+
+    std::vector<double> change_;  // LTP changes of the last lazy update, [spiking post][row]
+
+    // The persistent step's order of the lazy row sums (see oracle.h).  The LTP pass above has
+    // already updated the spiking-post columns (change_ holds each entry's change).
+    void lazy_tile_order_sums(Plast& P, float* M, uint32_t np, uint32_t nq, const uint32_t* pre_sp,
+                              const uint32_t* post_sp, const std::vector<uint32_t>& posts) {
+        const uint32_t ntiles = (nq + 255) / 256;
+        std::vector<int32_t> post_rank(nq, -1);
+        for (uint32_t e = 0; e < posts.size(); ++e) post_rank[posts[e]] = (int32_t)e;
+        (void)post_sp;
+        for (uint32_t i = 0; i < np; ++i) {
+            float* row = &M[(size_t)i * nq];
+            if (pre_sp[i] == 1) {
+                if (P.s[i] != 0.0f)
+                    for (uint32_t j = 0; j < nq; ++j) {
+                        if (i == j) continue;
+                        const float w = clampw<float>(P.s[i] * row[j] + (float)ltd_delta(P, i, j), (float)P.wmin, (float)P.wmax);
+                        row[j] = w / P.s[i];
+                    }
+                double rs = 0.0;
+                for (uint32_t q = 0; q < ntiles; ++q) rs += tile_sum(q, nq, [&](uint32_t j) { return (double)row[j]; });
+                P.Rs[i] = rs;
+            } else {
+                double rs = P.Rs[i];
+                for (uint32_t q = 0; q < ntiles; ++q)
+                    rs += tile_sum(q, nq, [&](uint32_t j) {
+                        const int32_t e = post_rank[j];
+                        return e < 0 ? 0.0 : change_[(size_t)e * np + i];
+                    });
+                P.Rs[i] = rs;
             }
             if (P.Rs[i] > 0.0) P.s[i] = (float)((double)P.target[i] / P.Rs[i]);
         }

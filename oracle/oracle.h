@@ -22,6 +22,13 @@
  *     changes (double)M'_ij - (double)M_ij: the list of spiking posts (increasing post order)
  *     is cut into 4 contiguous parts of ceil(count/4), each part's changes summed in list order
  *     from 0.0, the parts added to R_i in order; then s_i = fl32(target_i / R_i) when R_i > 0.
+ *   - persistent step (gemv_part_chunks = 8, lazy_tile_order = 1): the chunk sums are combined in
+ *     parts of exactly 8 chunks (the 8 waves of one block, summed in chunk order); the lazy row
+ *     sums follow the column tiles of 256 columns: in tile q, lane l (0..63) sums columns
+ *     q*256 + 4l .. q*256 + 4l + 3 in order from 0.0, the 64 lane sums are combined by the xor
+ *     butterfly (32,16,8,4,2,1), giving T_q; a row whose pre neuron spiked gets R_i = 0.0 + T_0 +
+ *     T_1 + ... over the new row; another row sums its LTP changes (double)M'_ij - (double)M_ij
+ *     the same way per tile and gets R_i = R_i + dT_0 + dT_1 + ....
  * precision 32 reproduces the engine bit for bit; precision 64 runs every quantity in FP64
  * (libm exp, plain sequential sums) as an independent accuracy reference.
  *
@@ -79,6 +86,8 @@ typedef struct oracle_params {
     uint32_t gemv_row_block[4];   /* EE, EI, IE, II */
     uint32_t lazy_normalize;      /* 1: W_EE = diag(s) M with per-row scale factors */
     uint32_t dense_gemv;          /* 1: visit every pre row (no skip of x = 0 rows; same result) */
+    uint32_t gemv_part_chunks[4]; /* chunks per combination part (0: ceil(chunks / 8)) */
+    uint32_t lazy_tile_order;     /* 1: lazy row sums in the column-tile order (see above) */
 } oracle_params;
 
 typedef struct oracle_sim oracle_sim;

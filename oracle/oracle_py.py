@@ -43,7 +43,7 @@ class Params(C.Structure):
         ("i_e_ltp_A", C.c_float), ("i_e_ltd_A", C.c_float), ("i_e_ltd_B", C.c_float), ("i_e_tau_ltp", C.c_float),
         ("i_e_tau_ltd", C.c_float), ("i_e_weight_min", C.c_float), ("i_e_weight_max", C.c_float),
         ("gemv_row_block", C.c_uint32 * 4), ("lazy_normalize", C.c_uint32),
-        ("dense_gemv", C.c_uint32),
+        ("dense_gemv", C.c_uint32), ("gemv_part_chunks", C.c_uint32 * 4), ("lazy_tile_order", C.c_uint32),
     ]
 
 
@@ -120,7 +120,8 @@ def _up(a):
 
 
 def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=False,
-                       rng_instance=0, dense_gemv=False) -> Params:
+                       rng_instance=0, dense_gemv=False, gemv_part_chunks=(0, 0, 0, 0),
+                       lazy_tile_order=False) -> Params:
     """Builds oracle parameters from a parsed config (the engine's config dump)."""
     p = Params()
     p.num_excitatory_neurons = cfg["num_excitatory_neurons"]
@@ -154,6 +155,9 @@ def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=Fa
         p.gemv_row_block[k] = int(gemv_row_block[k])
     p.lazy_normalize = int(lazy_normalize)
     p.dense_gemv = int(dense_gemv)
+    for k in range(4):
+        p.gemv_part_chunks[k] = int(gemv_part_chunks[k])
+    p.lazy_tile_order = int(lazy_tile_order)
     return p
 
 

@@ -299,8 +299,9 @@ def _row_delta(before, after, i, j, j_ref):
     return after[i, j] / after[i, j_ref] * before[i, j_ref] - before[i, j]
 
 
-@pytest.mark.parametrize("lazy", [False, True], ids=["faithful", "lazy"])
-def test_triplet_ltd_and_triplet_terms_hand_computed(lazy):
+@pytest.mark.parametrize("lazy,tile", [(False, False), (True, False), (True, True)],
+                         ids=["faithful", "lazy", "lazy-tile-order"])
+def test_triplet_ltd_and_triplet_terms_hand_computed(lazy, tile):
     # kernel_triplet_plasticity_update.cu:54-93 with every coefficient nonzero.  E neuron 3 spikes
     # at steps 1 and 41, E neuron 9 at step 11 (one-step 1 uA pulses one step earlier).  At
     # step 41 (traces decayed first, increments after, :16-50 and :95-115):
@@ -318,7 +319,8 @@ def test_triplet_ltd_and_triplet_terms_hand_computed(lazy):
     w_ee = rng.uniform(0.5e-9, 1e-9, (ne, ne)).astype(np.float32)
     np.fill_diagonal(w_ee, 0)
     ws = [w_ee, np.zeros((ne, ni), np.float32), np.zeros((ni, ne), np.float32), np.zeros((ni, ni), np.float32)]
-    p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy)
+    p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy, lazy_tile_order=tile,
+                             gemv_part_chunks=(8, 8, 8, 8) if tile else (0, 0, 0, 0))
     sim = O.OracleSim(p, ws, stim=_stim(cfg, [(3, 0), (9, 10), (3, 40)]))
     sim.step(41)
     assert [int(np.nonzero(sim.state()["spikes"])[0].size)] == [0]
@@ -392,13 +394,47 @@ def test_triplet_normalisation_restores_column_sum_targets():
     w_ee = w_ee.astype(np.float32)
     ws = [w_ee, np.zeros((ne, ni), np.float32), np.zeros((ni, ne), np.float32), np.zeros((ni, ni), np.float32)]
     target = np.array([np.float32(np.sum(w_ee[:, j], dtype=np.float32)) for j in range(ne)])  # column sums
-    for lazy in (False, True):
-        p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy)
+    for lazy, tile in ((False, False), (True, False), (True, True)):
+        p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy, lazy_tile_order=tile)
         sim = O.OracleSim(p, ws)
         sim.step(3)
         w = sim.weights(0).astype(np.float64)
         ok = w_ee.sum(axis=1) > 0
         assert np.allclose(w.sum(axis=1)[ok], target[ok], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("tile", [False, True], ids=["list-order", "tile-order"])
+def test_lazy_row_sums_restore_targets_with_spikes(tile):
+    # Lazy normalisation keeps R_i = sum_j M_ij incrementally (spiking rows re-summed, other rows
+    # plus their LTP changes); the true weights s_i * M_ij of every row must then sum to the
+    # row's target (the initial column sum, triplet_plasticity.cpp:22-30) after every step,
+    # through a stimulus-driven burst with hundreds of LTP/LTD updates per step.
+    cfg = small_config(n_e_side=20, n_i_side=10, steps=300)
+    cfg["stimulus_selected_input_ids"] = list(range(0, 400, 3))
+    cfg["stimulus_selected_input_post_group_offsets"] = [0] * len(cfg["stimulus_selected_input_ids"])
+    cfg["stimulus_start_clock_steps"] = [10, 60]
+    cfg["stimulus_end_clock_steps"] = [30, 90]
+    cfg["stimulus_input_values"] = [2e-9, 2e-9]
+    ne, ni = 400, 100
+    rng = np.random.default_rng(11)
+    shapes = [(ne, ne), (ne, ni), (ni, ne), (ni, ni)]
+    wmax = [0.5e-9, 1e-9, 2e-9, 2e-9]
+    ws = [(rng.uniform(0, w, sh) * (rng.uniform(size=sh) < 0.3)).astype(np.float32) for sh, w in zip(shapes, wmax)]
+    target = np.add.accumulate(ws[0], axis=0, dtype=np.float32)[-1].astype(np.float64)
+    from helpers import stimulus_schedule
+    p = O.params_from_config(cfg, gemv_row_block=(4, 4, 4, 4), lazy_normalize=True, lazy_tile_order=tile,
+                             gemv_part_chunks=(8, 8, 8, 8) if tile else (0, 0, 0, 0))
+    sim = O.OracleSim(p, ws, stim=stimulus_schedule(cfg))
+    spikes = 0
+    for step in range(120):
+        sim.step(1)
+        spikes += int(sim.state()["spikes"][:ne].sum())
+        if step % 10 == 9:
+            w = sim.weights(0).astype(np.float64)
+            ok = target > 0
+            rel = np.abs(w.sum(axis=1)[ok] - target[ok]) / target[ok]
+            assert np.max(rel) <= 2e-6, (step, float(np.max(rel)))
+    assert spikes > 100
 
 
 def test_lazy_normalisation_tracks_faithful():
@@ -412,12 +448,19 @@ def test_lazy_normalisation_tracks_faithful():
     stim = None
     a = O.OracleSim(O.params_from_config(eng_cfg, gemv_row_block=(32, 32, 32, 32)), ws, stim=stim)
     b = O.OracleSim(O.params_from_config(eng_cfg, gemv_row_block=(32, 32, 32, 32), lazy_normalize=True), ws, stim=stim)
-    worst = 0.0
+    # The persistent step's orders: chunk sums in parts of 8, lazy row sums in column-tile order.
+    c = O.OracleSim(O.params_from_config(eng_cfg, gemv_row_block=(4, 4, 4, 4), lazy_normalize=True,
+                                         gemv_part_chunks=(8, 8, 8, 8), lazy_tile_order=True), ws, stim=stim)
+    worst = [0.0, 0.0]
     for step in range(300):
         a.step(1)
         b.step(1)
-        sa, sb = a.state(), b.state()
-        worst = max(worst, float(np.max(np.abs(sa["V"] - sb["V"]) / np.abs(sa["V"]))))
-    wa, wb = a.weights(0).astype(np.float64), b.weights(0).astype(np.float64)
+        c.step(1)
+        sa, sb, sc = a.state(), b.state(), c.state()
+        worst[0] = max(worst[0], float(np.max(np.abs(sa["V"] - sb["V"]) / np.abs(sa["V"]))))
+        worst[1] = max(worst[1], float(np.max(np.abs(sa["V"] - sc["V"]) / np.abs(sa["V"]))))
+        assert np.array_equal(sa["spikes"], sc["spikes"])
+    wa, wb, wc = (x.weights(0).astype(np.float64) for x in (a, b, c))
     assert np.max(np.abs(wa - wb)) <= 1e-5 * np.max(np.abs(wa))
-    assert worst <= 1e-5
+    assert np.max(np.abs(wa - wc)) <= 1e-5 * np.max(np.abs(wa))
+    assert worst[0] <= 1e-5 and worst[1] <= 1e-5, worst
