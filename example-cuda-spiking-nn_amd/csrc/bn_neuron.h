@@ -176,7 +176,7 @@ __device__ __forceinline__ T* uni(T* p) {
 
 // One thread, every part in order (component path and g materialisation).
 __device__ __forceinline__ float finalize_g(float g, const FinalizeSource& s, uint32_t j) {
-    const uint32_t per = fin_part_size(s.chunks);
+    const uint32_t per = s.per ? s.per : fin_part_size(s.chunks);
     double acc = (double)g;
     for (uint32_t c0 = 0; per && c0 < s.chunks; c0 += per) acc += fin_part_sum(s, j, c0, min(c0 + per, s.chunks));
     return (float)acc;

@@ -52,6 +52,7 @@ struct FinalizeSource {
     const double* partials;  // [chunks][ld], indexed by post-local id
     uint32_t chunks;
     uint32_t ld;
+    uint32_t per;            // chunks per combination part (0: ceil(chunks / kFinParts))
 };
 
 struct EvolveGroup {

@@ -192,6 +192,10 @@ void build(bn_sim* sim, const std::string& config_path) {
         for (auto& m : sys.monitors()) m->quiet = quiet;
     }
     if (!component) sys.enable_fused(!(flags & BN_SIM_NO_GRAPH), kDefaultStepsPerGraph);
+    if (flags & BN_SIM_PERSISTENT) {
+        if (component) throw Error(BN_ERR_INVALID_ARGUMENT, "the persistent step is a fused step (not the component path)");
+        sys.enable_persistent({0, 0, 0, 0});
+    }
     BN_HIP_CHECK(hipMalloc(&sim->d_tmp_ge, S.total_neurons() * sizeof(float)), "hipMalloc g temp");
     BN_HIP_CHECK(hipMalloc(&sim->d_tmp_gi, S.total_neurons() * sizeof(float)), "hipMalloc g temp");
     BN_HIP_CHECK(hipStreamSynchronize(S.stream()), "build sync");
@@ -328,6 +332,13 @@ int bn_sim_get_info(bn_sim* sim, bn_sim_info* info) {
     info->device_bytes = S.total_allocated_bytes();
     info->steps_per_graph = sim->system->graph_steps();
     info->graph_launches = sim->system->graph_launches();
+    info->persistent_blocks = sim->system->persistent_blocks();
+    info->persistent_items = sim->system->persistent_items();
+    if (sim->system->persistent()) {
+        info->gemv_part_chunks = kPWaves;
+        info->kernels_per_step = 1;
+        for (uint32_t k = 0; k < 4; ++k) info->gemv_row_block[k] = sim->system->persistent_rows(k);
+    }
     return BN_OK;
 }
 
@@ -527,6 +538,19 @@ int bn_measure_read_bandwidth_one_launch(uint64_t bytes, uint32_t passes, float*
     return measure_read_bandwidth(bytes, passes, true, gb_per_s);
 }
 
+int bn_sim_set_persistent_rows(bn_sim* sim, const uint32_t rows[4]) {
+    if (!sim || !rows) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_persistent_rows: null argument");
+    return guarded([&] {
+        if (!sim->system->persistent()) throw Error(BN_ERR_RUNTIME, "bn_sim_set_persistent_rows: not a persistent simulation");
+        sim->system->enable_persistent({rows[0], rows[1], rows[2], rows[3]});
+    });
+}
+
+int bn_sim_set_persistent_wait_limit(bn_sim* sim, uint32_t polls) {
+    if (!sim) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_persistent_wait_limit: null simulation");
+    return guarded([&] { sim->system->set_persistent_wait_limit(polls); });
+}
+
 int bn_sim_set_steps_per_graph(bn_sim* sim, uint32_t steps_per_graph) {
     if (!sim) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_steps_per_graph: null simulation");
     return guarded([&] { sim->system->set_steps_per_graph(steps_per_graph); });
@@ -537,6 +561,8 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
     return guarded([&] {
         System& sys = *sim->system;
         if (!sys.fused()) throw Error(BN_ERR_RUNTIME, "kernel timing needs the fused step");
+        if (sys.persistent())
+            throw Error(BN_ERR_RUNTIME, "kernel timing: the persistent step is one launch per run (time bn_sim_run_timed)");
         hipStream_t s = sim->state->stream();
         hipEvent_t ev[4];
         for (auto& e : ev) BN_HIP_CHECK(hipEventCreate(&e), "hipEventCreate");

@@ -439,6 +439,9 @@ void GpuSimulationState::check_status() {
     BN_HIP_CHECK(hipMemcpyAsync(&st, status_, 4, hipMemcpyDeviceToHost, stream_), "D2H status");
     BN_HIP_CHECK(hipStreamSynchronize(stream_), "status sync");
     if (st == 1u) throw Error(BN_ERR_RUNTIME, "Stimulus event indexing exceeded uploaded event buffers");
+    if (st == 2u)
+        throw Error(BN_ERR_RUNTIME, "persistent step: a block waited past its limit and the launch was abandoned "
+                                    "(the simulation state is no longer valid)");
 }
 
 // ======================================================================================

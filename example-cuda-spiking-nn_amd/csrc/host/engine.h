@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "../bn_persist.h"
 #include "../bn_step.h"
 #include "buryn_amd.h"
 #include "generation.h"
@@ -436,6 +437,17 @@ public:
     // Trace set used by the fused step for connection idx (EE/EI share one, IE/II another).
     const TraceSet* trace_set_for_connection(uint32_t idx) const;
 
+    // Persistent step: one cooperative launch runs a whole run of steps (bn_persist.h).  rows:
+    // rows per wave chunk for each connection (0: chosen from the network's shape and the
+    // device's co-resident blocks).  Re-plans only; the state is untouched.
+    void enable_persistent(const std::array<uint32_t, 4>& rows);
+    bool persistent() const { return persist_; }
+    uint32_t persistent_blocks() const { return persist_ ? persist_plan_.blocks : 0u; }
+    uint32_t persistent_rows(uint32_t c) const { return persist_ ? persist_plan_.c[c].rb : 0u; }
+    uint32_t persistent_items() const { return persist_items_; }
+    // Polls a block of the persistent step makes before it abandons a wait (and the launch).
+    void set_persistent_wait_limit(uint32_t polls);
+
     uint32_t clock_step() const { return clock_step_; }
     bool quiet = false;
     const std::vector<std::unique_ptr<NeuronGroup>>& neuron_groups() const { return neuron_groups_; }
@@ -446,6 +458,9 @@ public:
 private:
     void build_plans();
     void build_overlap_plans();
+    void build_event_starts();
+    void launch_persistent_steps(uint32_t n);
+    void* persist_alloc(size_t bytes);
     void enqueue_fused_step(hipEvent_t gemv_begin, hipEvent_t gemv_end);
     void enqueue_overlapped_steps(uint32_t n);
 
@@ -483,6 +498,16 @@ private:
     uint32_t* d_ev_start_ = nullptr;
     bool overlap_built_ = false;
     std::vector<void*> staging_;
+    // Persistent step
+    bool persist_ = false;
+    PPlan persist_plan_{};
+    PPlan* d_persist_plan_ = nullptr;
+    uint32_t* persist_counters_ = nullptr;
+    size_t persist_counter_words_ = 0;
+    uint32_t persist_nontemporal_ = 0;
+    uint32_t persist_items_ = 0;
+    uint32_t persist_wait_limit_ = 1u << 22;  // about 4 s of polling
+    std::vector<void*> persist_bufs_;
 };
 
 }  // namespace bn

@@ -13,6 +13,7 @@ System::~System() {
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : staging_) (void)hipFree(p);
+    for (void* p : persist_bufs_) (void)hipFree(p);
     if (pre_list_buf_) (void)hipFree(pre_list_buf_);
 }
 
@@ -291,6 +292,24 @@ void System::prepare_overlap(uint32_t steps_per_graph) {
     plast_final_.tail.advance = steps_per_graph;
 }
 
+// First event of every step (prefix sums of timestep_event_count): the stimulus events of any
+// step found without the host cursor.
+void System::build_event_starts() {
+    if (d_ev_start_) return;
+    const StimTail& st = plast_step_.tail;
+    if (!st.timestep_event_count || !st.num_timesteps) return;
+    std::vector<uint32_t> counts(st.num_timesteps), start(st.num_timesteps + 1, 0);
+    BN_HIP_CHECK(hipMemcpy(counts.data(), st.timestep_event_count, counts.size() * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost), "event counts");
+    for (uint32_t t = 0; t < st.num_timesteps; ++t) start[t + 1] = start[t] + counts[t];
+    void* p = nullptr;
+    BN_HIP_CHECK(hipMalloc(&p, start.size() * sizeof(uint32_t)), "hipMalloc event starts");
+    staging_.push_back(p);
+    d_ev_start_ = static_cast<uint32_t*>(p);
+    BN_HIP_CHECK(hipMemcpy(d_ev_start_, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+                 "event starts");
+}
+
 void System::build_overlap_plans() {
     hipStream_t s = state_.stream();
     overlap_built_ = true;
@@ -334,14 +353,8 @@ void System::build_overlap_plans() {
     plast_final_.tail = plast_step_.tail;
     plast_final_.tail.active = 1;
     const StimTail& st = plast_step_.tail;
-    if (st.timestep_event_count && st.num_timesteps) {
-        std::vector<uint32_t> counts(st.num_timesteps), start(st.num_timesteps + 1, 0);
-        BN_HIP_CHECK(hipMemcpy(counts.data(), st.timestep_event_count, counts.size() * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost), "event counts");
-        for (uint32_t t = 0; t < st.num_timesteps; ++t) start[t + 1] = start[t] + counts[t];
-        d_ev_start_ = static_cast<uint32_t*>(alloc(start.size() * sizeof(uint32_t)));
-        BN_HIP_CHECK(hipMemcpy(d_ev_start_, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
-                     "event starts");
+    build_event_starts();
+    if (d_ev_start_) {
         for (EvolvePlan* e : {&evolve_even_, &evolve_odd_}) {
             e->ev_start = d_ev_start_;
             e->ev_ids = st.event_ids;
@@ -417,12 +430,20 @@ void System::capture_graph() {
 void System::set_steps_per_graph(uint32_t steps_per_graph) {
     if (!fused_) throw Error(BN_ERR_RUNTIME, "steps per graph: the component path has no graph");
     if (steps_per_graph == 0) throw Error(BN_ERR_INVALID_ARGUMENT, "steps per graph must be at least 1");
+    if (persist_) {  // the persistent step launches whole runs, no graphs
+        steps_per_graph_ = steps_per_graph;
+        return;
+    }
     BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "pre-capture sync");
     steps_per_graph_ = steps_per_graph;
     capture_graph();
 }
 
 void System::launch_fused_steps(uint32_t n) {
+    if (persist_) {
+        launch_persistent_steps(n);
+        return;
+    }
     hipStream_t s = state_.stream();
     uint32_t k = 0;
     if (use_graph_)
@@ -441,6 +462,27 @@ void System::launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end)
 
 void System::materialize_g(float* d_ge_out, float* d_gi_out) {
     hipStream_t s = state_.stream();
+    if (persist_) {
+        // Pending segment partials of the last persistent step: g + P_0 + P_1 + ... in order.
+        const PPlan& P = persist_plan_;
+        for (uint32_t g = 0; g < P.n_groups; ++g) {
+            const NeuronGroupView& v = P.g[g].v;
+            for (uint32_t side = 0; side < 2; ++side) {
+                const uint32_t c = side == 0 ? P.g[g].src_e : P.g[g].src_i;
+                FinalizeSource src{};
+                uint32_t n_src = 0;
+                if (c != kPNone) {
+                    src = FinalizeSource{P.c[c].partials, P.c[c].nseg, P.c[c].ld, 1u};
+                    n_src = 1;
+                }
+                BN_HIP_CHECK((hipError_t)launch_materialize_g((side == 0 ? d_ge_out : d_gi_out) + v.offset,
+                                                              side == 0 ? v.g_excitatory : v.g_inhibitory, &src, n_src,
+                                                              v.num_neurons, s),
+                             "materialize g (persistent)");
+            }
+        }
+        return;
+    }
     for (uint32_t g = 0; g < evolve_.n_groups || (!fused_ && g < neuron_groups_.size()); ++g) {
         const NeuronGroupView& v = fused_ ? evolve_.g[g].v : neuron_groups_[g]->gpu_view();
         const uint32_t ne = fused_ ? evolve_.g[g].n_fin_e : 0, ni = fused_ ? evolve_.g[g].n_fin_i : 0;

@@ -24,6 +24,8 @@ BURYN_FLAGS = {
     "no_graph": 1 << 4,
     "quiet": 1 << 5,
     "sequential_graph": 1 << 6,
+    "reference_layout": 1 << 7,
+    "persistent": 1 << 8,
 }
 
 
@@ -43,6 +45,7 @@ class SimInfo(C.Structure):
         ("gemv_row_block", C.c_uint32 * 4), ("weight_pre", C.c_uint32 * 4), ("weight_post", C.c_uint32 * 4),
         ("flags", C.c_uint32), ("kernels_per_step", C.c_uint32), ("device_bytes", C.c_uint64),
         ("steps_per_graph", C.c_uint32), ("graph_launches", C.c_uint32),
+        ("persistent_blocks", C.c_uint32), ("persistent_items", C.c_uint32), ("gemv_part_chunks", C.c_uint32),
     ]
 
 
@@ -112,6 +115,7 @@ EXPORTED_SYMBOLS = [
     "bn_window_get_weights", "bn_window_set_targets", "bn_window_forward", "bn_window_backward",
     "bn_window_get_info", "bn_window_get_gradient", "bn_measure_read_bandwidth", "bn_sim_save_state",
     "bn_sim_load_state", "bn_sim_get_spike_raster", "bn_measure_read_bandwidth_one_launch",
+    "bn_sim_set_persistent_rows", "bn_sim_set_persistent_wait_limit",
 ]
 
 _lib = None
@@ -148,6 +152,8 @@ def lib() -> C.CDLL:
     L.bn_sim_get_plasticity_traces.argtypes = [P, U, fp, fp, fp, fp]
     L.bn_sim_run_timed.argtypes = [P, U, fp]
     L.bn_sim_get_spike_raster.argtypes = [P, U, U, up]
+    L.bn_sim_set_persistent_rows.argtypes = [P, up]
+    L.bn_sim_set_persistent_wait_limit.argtypes = [P, U]
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
     L.bn_sim_set_steps_per_graph.argtypes = [P, U]
     L.bn_measure_read_bandwidth.argtypes = [C.c_uint64, U, fp]
@@ -317,6 +323,15 @@ class Simulation:
     def set_steps_per_graph(self, n: int) -> None:
         """Steps per captured graph of the fused step (default 100; even: overlapped form)."""
         _check(lib().bn_sim_set_steps_per_graph(self.h, n))
+
+    def set_persistent_rows(self, rows) -> None:
+        """Rows per wave chunk of the persistent step per connection (int or 4 ints; 0 = chosen)."""
+        r = np.array(rows if hasattr(rows, "__len__") else [rows] * 4, np.uint32)
+        _check(lib().bn_sim_set_persistent_rows(self.h, _up(r)))
+        self.info = self.get_info()
+
+    def set_persistent_wait_limit(self, polls: int) -> None:
+        _check(lib().bn_sim_set_persistent_wait_limit(self.h, polls))
 
     def time_propagate_kernel(self, n: int):
         k, s = C.c_float(0), C.c_float(0)

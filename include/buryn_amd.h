@@ -303,9 +303,16 @@ enum {
     BN_SIM_SEQUENTIAL_GRAPH = 1u << 6,  /* graphs run each step's launches one after the other
                                          * instead of overlapping the plasticity of step t-1
                                          * with the evolve of step t (same results) */
-    BN_SIM_REFERENCE_LAYOUT = 1u << 7   /* weight rows unpadded (row stride = post), as the
+    BN_SIM_REFERENCE_LAYOUT = 1u << 7,  /* weight rows unpadded (row stride = post), as the
                                          * reference allocates them (same results, slower GEMV
                                          * where post % 4 != 0) */
+    BN_SIM_PERSISTENT = 1u << 8         /* the persistent step: one cooperative launch per run of
+                                         * steps; plasticity applied inside the weight pass, the
+                                         * evolve run by the block that completes a column tile.
+                                         * Needs BN_SIM_LAZY_NORMALIZE when E->E normalises.  Its
+                                         * summation orders (bn_sim_info.gemv_part_chunks = 8,
+                                         * lazy row sums by 256-column tile) differ from the
+                                         * graph step's, so results differ in rounding only. */
 };
 
 typedef struct bn_sim_info {
@@ -324,6 +331,10 @@ typedef struct bn_sim_info {
     uint64_t device_bytes;           /* device memory held by the simulation */
     uint32_t steps_per_graph;        /* steps per captured graph of the fused step (0: no graph) */
     uint32_t graph_launches;         /* captured graphs launched so far */
+    uint32_t persistent_blocks;      /* blocks of the persistent step's launch (0: not persistent) */
+    uint32_t persistent_items;       /* its (connection, column tile, row segment) work items */
+    uint32_t gemv_part_chunks;       /* chunk sums per combination part (0: ceil(chunks / 8));
+                                        the persistent step's gemv_row_block is its chunk height */
 } bn_sim_info;
 
 /* Host pointers to fill; any may be NULL.  Arrays are indexed by global neuron id (length
@@ -381,6 +392,14 @@ int bn_sim_set_steps_per_graph(bn_sim* sim, uint32_t steps_per_graph);
  * weights; the pending propagate partials are saved before and restored after, so the state of
  * the simulation is unchanged by this part. */
 int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_ms, float* step_ms);
+/* Rows per wave chunk of the persistent step for each connection (0 = chosen from the shape and
+ * the device); re-plans the work only.  Changes the GEMV summation order (results move by
+ * rounding).  Persistent simulations only. */
+int bn_sim_set_persistent_rows(bn_sim* sim, const uint32_t rows[4]);
+/* Polls a block of the persistent step makes while it waits for the previous step's evolves
+ * before it gives up: the launch then ends early on every block, and the next synchronisation
+ * reports BN_ERR_RUNTIME (the state is no longer valid).  Default 4,194,304 (about 4 s). */
+int bn_sim_set_persistent_wait_limit(bn_sim* sim, uint32_t polls);
 /* State snapshot / resume (no counterpart in the reference): every device buffer of the
  * simulation (neuron and background state, weights, connection and plasticity traces, the
  * pending propagate partials, lazy row sums and scales, monitor rings, the device clock and
