@@ -10,12 +10,19 @@
 //              spiking post columns, LTD of spiking pre rows, in the reference's LTP-then-LTD
 //              order per entry) after using their old values, and, for a lazily normalised
 //              matrix, the tile's part of each row's change of R (column-tile order).
+//              With resident weights (one item per block, chunks of at most 24 rows) the
+//              entries stay in the registers of their thread for the whole launch and only
+//              the rows that changed are stored, at the end of the launch.
 //   tails      the block that finishes the last item of a column tile of a post group runs the
 //              evolve of step t+1 for those neurons (partials combined, background, EIF,
 //              connection and plasticity traces, spike bits, rings); the block that finishes the
 //              last column tile of a segment of the lazy matrix updates R_i and s_i of its rows.
 //   ordering   a step's tiles start once every tail of the previous step has finished
 //              (one counter of finished tails); counters only grow within a launch.
+//   coherence  what one block writes for another (partials, row-sum parts, neuron state,
+//              traces, row scales, spike bits) moves through agent-coherent loads and stores;
+//              a block completes its stores before it raises a counter.  No fence writes back
+//              or invalidates an L2.  Weight entries stay with the thread that owns them.
 // Buffers read by one step while the next step's evolve writes them are doubled (x, plasticity
 // traces: evolve k of a launch of n writes buffer (n-1-k) & 1, 0 being the state's own) or
 // tripled (spike bits, set with atomic OR after being cleared two steps ahead: buffer
@@ -32,6 +39,7 @@ constexpr int kPWaves = 8;                  // waves per block = chunks per segm
 constexpr int kPThreads = kPWaves * 64;     // 512
 constexpr int kPCols = 256;                 // columns per tile (64 lanes x float4)
 constexpr int kPBatch = 8;                  // rows of a chunk loaded together
+constexpr uint32_t kPMaxRows = 128;         // rows per chunk at most (a segment is staged in LDS)
 constexpr uint32_t kPNone = 0xFFFFFFFFu;
 
 struct PConn {
@@ -42,6 +50,8 @@ struct PConn {
     uint32_t ld, n_pre, n_post, pre_off, post_off;
     uint32_t rb, nseg, ncol;    // rows per chunk, segments of kPWaves*rb rows, column tiles
     uint32_t post_group;        // index into PPlan::g
+    uint32_t pre_group;
+    uint32_t ts;                // index into PPlan::ts: the trace set whose x it reads
     uint32_t plast;             // index into PPlan::pl, or kPNone
 };
 
@@ -93,6 +103,8 @@ struct PPlan {
     uint32_t lazy_pl;           // the lazily normalised plasticity, or kPNone
     uint32_t n_rowtails;        // segments of its matrix (row tails per step)
     // schedule
+    uint32_t resident_rows;     // 0: weights streamed each step; else one item per block, its
+                                // rows held in registers (a chunk of at most this many rows)
     const PItem* items;
     const uint32_t* block_begin;    // [blocks + 1]
     uint32_t blocks;
@@ -118,12 +130,31 @@ struct PPlan {
     const float* ev_amps;
     uint32_t ev_steps, ev_count;
     float* i_stimulus;          // global
+    // Resident weights: tagged 64-bit words (a 32-bit value and the step that produced it), in
+    // three slots by step (kernels_persist_ll.hip).
+    const uint32_t* block_tile;         // [blocks] column tile the block evolves, or kPNone
+    uint64_t* ll_x[kMaxTraceSets];      // [3][n_pre] x of the trace set's neurons
+    uint64_t* ll_pl[kMaxPlast][4];      // [3][n] pre, slow pre ([3][n_pre]), post, slow post ([3][n_post])
+    uint64_t* ll_spk;                   // [3][n_coltiles][8] spike flags of a tile, 32 a word
+    uint64_t* ll_part[kMaxConns];       // [3][nseg][ld][2] segment partials (low, high word)
+    uint64_t* ll_dr;                    // [3][ncol][n_pre][2] lazy matrix: row tile sums
+    uint32_t* done;                     // [0] steps finished, summed over the blocks; [1] first
+                                        // evolves finished
 };
+
+// Limits of the resident-weight kernel.
+constexpr uint32_t kLSegMax = kPWaves * 32;   // rows of a segment (chunks of at most 32 rows)
+constexpr uint32_t kLPartMax = 32;            // segment partials of a group's two sources
+constexpr uint32_t kLDrJobs = 6;              // row tile sums polled per thread
 
 // n steps from the state (cooperative launch of host_plan.blocks x kPThreads threads; the
 // counters, the spike bits of the first two evolves and their ring slots must be zero).
 int launch_persistent(const PPlan* d_plan, const PPlan& host_plan, uint32_t n, uint32_t nontemporal, bn_stream_t s);
-// Largest number of co-resident blocks of the persistent kernel on the current device.
-int persistent_max_blocks(uint32_t* blocks);
+// Largest number of co-resident blocks of the persistent kernel (resident_rows 0: streamed
+// weights; 16, 24: resident weights) on the current device.
+int persistent_max_blocks(uint32_t resident_rows, uint32_t* blocks);
+int persistent_ll_max_blocks(uint32_t resident_rows, uint32_t* blocks);
+int launch_persistent_ll(const PPlan* d_plan, const PPlan& host_plan, uint32_t n, bn_stream_t s);
+constexpr uint32_t kPResidentRows[] = {16, 24};
 
 }  // namespace bn

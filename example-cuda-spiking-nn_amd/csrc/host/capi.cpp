@@ -334,6 +334,7 @@ int bn_sim_get_info(bn_sim* sim, bn_sim_info* info) {
     info->graph_launches = sim->system->graph_launches();
     info->persistent_blocks = sim->system->persistent_blocks();
     info->persistent_items = sim->system->persistent_items();
+    info->persistent_resident_rows = sim->system->persistent_resident_rows();
     if (sim->system->persistent()) {
         info->gemv_part_chunks = kPWaves;
         info->kernels_per_step = 1;
@@ -544,6 +545,11 @@ int bn_sim_set_persistent_rows(bn_sim* sim, const uint32_t rows[4]) {
         if (!sim->system->persistent()) throw Error(BN_ERR_RUNTIME, "bn_sim_set_persistent_rows: not a persistent simulation");
         sim->system->enable_persistent({rows[0], rows[1], rows[2], rows[3]});
     });
+}
+
+int bn_sim_set_persistent_resident(bn_sim* sim, int allow) {
+    if (!sim) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_persistent_resident: null simulation");
+    return guarded([&] { sim->system->set_persistent_resident(allow != 0); });
 }
 
 int bn_sim_set_persistent_wait_limit(bn_sim* sim, uint32_t polls) {

@@ -445,6 +445,10 @@ public:
     uint32_t persistent_blocks() const { return persist_ ? persist_plan_.blocks : 0u; }
     uint32_t persistent_rows(uint32_t c) const { return persist_ ? persist_plan_.c[c].rb : 0u; }
     uint32_t persistent_items() const { return persist_items_; }
+    // Rows per chunk of the resident-weight kernel (0: weights streamed each step).
+    uint32_t persistent_resident_rows() const { return persist_ ? persist_plan_.resident_rows : 0u; }
+    // Allow (default) or forbid resident weights; the plan is rebuilt.
+    void set_persistent_resident(bool allow);
     // Polls a block of the persistent step makes before it abandons a wait (and the launch).
     void set_persistent_wait_limit(uint32_t polls);
 
@@ -506,8 +510,12 @@ private:
     size_t persist_counter_words_ = 0;
     uint32_t persist_nontemporal_ = 0;
     uint32_t persist_items_ = 0;
+    bool persist_resident_ = true;
+    std::array<uint32_t, 4> persist_rows_{};
     uint32_t persist_wait_limit_ = 1u << 22;  // about 4 s of polling
     std::vector<void*> persist_bufs_;
+    std::vector<std::pair<void*, size_t>> persist_ll_bufs_;  // tagged-word buffers (64-bit words)
+    uint32_t persist_ll_next_ = 0;                          // the clock after the last resident launch
 };
 
 }  // namespace bn

@@ -46,6 +46,7 @@ class SimInfo(C.Structure):
         ("flags", C.c_uint32), ("kernels_per_step", C.c_uint32), ("device_bytes", C.c_uint64),
         ("steps_per_graph", C.c_uint32), ("graph_launches", C.c_uint32),
         ("persistent_blocks", C.c_uint32), ("persistent_items", C.c_uint32), ("gemv_part_chunks", C.c_uint32),
+        ("persistent_resident_rows", C.c_uint32),
     ]
 
 
@@ -115,7 +116,7 @@ EXPORTED_SYMBOLS = [
     "bn_window_get_weights", "bn_window_set_targets", "bn_window_forward", "bn_window_backward",
     "bn_window_get_info", "bn_window_get_gradient", "bn_measure_read_bandwidth", "bn_sim_save_state",
     "bn_sim_load_state", "bn_sim_get_spike_raster", "bn_measure_read_bandwidth_one_launch",
-    "bn_sim_set_persistent_rows", "bn_sim_set_persistent_wait_limit",
+    "bn_sim_set_persistent_rows", "bn_sim_set_persistent_wait_limit", "bn_sim_set_persistent_resident",
 ]
 
 _lib = None
@@ -154,6 +155,7 @@ def lib() -> C.CDLL:
     L.bn_sim_get_spike_raster.argtypes = [P, U, U, up]
     L.bn_sim_set_persistent_rows.argtypes = [P, up]
     L.bn_sim_set_persistent_wait_limit.argtypes = [P, U]
+    L.bn_sim_set_persistent_resident.argtypes = [P, I]
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
     L.bn_sim_set_steps_per_graph.argtypes = [P, U]
     L.bn_measure_read_bandwidth.argtypes = [C.c_uint64, U, fp]
@@ -328,6 +330,11 @@ class Simulation:
         """Rows per wave chunk of the persistent step per connection (int or 4 ints; 0 = chosen)."""
         r = np.array(rows if hasattr(rows, "__len__") else [rows] * 4, np.uint32)
         _check(lib().bn_sim_set_persistent_rows(self.h, _up(r)))
+        self.info = self.get_info()
+
+    def set_persistent_resident(self, allow: bool) -> None:
+        """Allow or forbid register-resident weights in the persistent step (re-plans)."""
+        _check(lib().bn_sim_set_persistent_resident(self.h, 1 if allow else 0))
         self.info = self.get_info()
 
     def set_persistent_wait_limit(self, polls: int) -> None:

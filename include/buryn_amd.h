@@ -335,6 +335,9 @@ typedef struct bn_sim_info {
     uint32_t persistent_items;       /* its (connection, column tile, row segment) work items */
     uint32_t gemv_part_chunks;       /* chunk sums per combination part (0: ceil(chunks / 8));
                                         the persistent step's gemv_row_block is its chunk height */
+    uint32_t persistent_resident_rows; /* persistent step with the weights held in registers for
+                                        the whole launch: rows per chunk of its kernel (0: the
+                                        weights are streamed from memory every step) */
 } bn_sim_info;
 
 /* Host pointers to fill; any may be NULL.  Arrays are indexed by global neuron id (length
@@ -400,6 +403,12 @@ int bn_sim_set_persistent_rows(bn_sim* sim, const uint32_t rows[4]);
  * before it gives up: the launch then ends early on every block, and the next synchronisation
  * reports BN_ERR_RUNTIME (the state is no longer valid).  Default 4,194,304 (about 4 s). */
 int bn_sim_set_persistent_wait_limit(bn_sim* sim, uint32_t polls);
+/* Allow (1, the default) or forbid (0) resident weights in the persistent step: when every work
+ * item fits a block of its own with chunks of at most 32 rows, each thread keeps its weight
+ * entries in registers for the whole launch (the plasticity updates them there) and stores the
+ * rows that changed when the launch ends.  Same results either way; re-plans the work.
+ * Persistent simulations only. */
+int bn_sim_set_persistent_resident(bn_sim* sim, int allow);
 /* State snapshot / resume (no counterpart in the reference): every device buffer of the
  * simulation (neuron and background state, weights, connection and plasticity traces, the
  * pending propagate partials, lazy row sums and scales, monitor rings, the device clock and

@@ -1,7 +1,7 @@
 """GPU: the persistent step (BN_SIM_PERSISTENT) against the oracle in its summation orders.
 
-One cooperative launch per run of steps: weights streamed once per step with the plasticity
-applied in the same pass, evolves run by the blocks that complete a column tile, lazy row sums
+One cooperative launch per run of steps: weights streamed once per step (or held in registers for
+the whole launch) with the plasticity applied in the same pass, evolves run by the blocks that complete a column tile, lazy row sums
 in column-tile order.  The oracle restates those orders (chunk sums in parts of 8, lazy row sums
 by 256-column tile), so the bar is bit-identical: the raster of every step (device spike ring),
 the state after every launch, traces and weights, for launches of 1 to 1,000 steps.
@@ -57,16 +57,21 @@ def _launches(sim, orc, sizes, first_step=1):
     return spikes
 
 
-def _persistent(path, lazy=True):
+def _persistent(path, lazy=True, resident=True):
     flags = ("quiet", "persistent") + (("lazy_normalize",) if lazy else ())
     sim = engine().Simulation(path, flags=flags)
+    if not resident:
+        sim.set_persistent_resident(False)
     info = sim.get_info()
     assert info.persistent_blocks > 0 and info.gemv_part_chunks == 8
+    if not resident:
+        assert info.persistent_resident_rows == 0
     return sim
 
 
+@pytest.mark.parametrize("resident", [True, False], ids=["resident", "streamed"])
 @pytest.mark.parametrize("which", ["plastic", "burst", "no-plasticity", "istdp-only"])
-def test_persistent_small_bit_exact(tmp_path, which):
+def test_persistent_small_bit_exact(tmp_path, which, resident):
     cfg = small_config(steps=800, plastic=which != "no-plasticity")
     if which == "burst":
         cfg["stimulus_selected_input_ids"] = list(range(0, 400, 2))
@@ -77,7 +82,9 @@ def test_persistent_small_bit_exact(tmp_path, which):
     if which == "istdp-only":
         cfg["e_e_plasticity_enabled"] = False
     path = _write(tmp_path, cfg)
-    sim = _persistent(path)
+    sim = _persistent(path, resident=resident)
+    if resident:
+        assert sim.get_info().persistent_resident_rows > 0
     orc = oracle_for(path, sim, lazy=True)
     sim.run(1)  # step 0 is a record step and runs as a launch of its own
     orc.step(1)
@@ -92,10 +99,12 @@ def test_persistent_small_bit_exact(tmp_path, which):
 def test_persistent_row_choices_bit_exact(tmp_path):
     # Other chunk heights (segments of 8 x rb rows): the oracle follows each.
     path = _write(tmp_path, small_config(steps=600))
-    for rows in ([2, 2, 2, 2], [5, 3, 7, 1], [64, 64, 64, 64]):
+    for rows in ([2, 2, 2, 2], [5, 3, 7, 1], [24, 24, 24, 24], [64, 64, 64, 64]):
         sim = _persistent(path)
         sim.set_persistent_rows(rows)
-        print("rows", rows, "->", tuple(sim.get_info().gemv_row_block), sim.get_info().persistent_items, "items")
+        info = sim.get_info()
+        print("rows", rows, "->", tuple(info.gemv_row_block), info.persistent_items, "items, resident rows",
+              info.persistent_resident_rows)
         orc = oracle_for(path, sim, lazy=True)
         sim.run(1)
         orc.step(1)
@@ -104,10 +113,15 @@ def test_persistent_row_choices_bit_exact(tmp_path):
         sim.close()
 
 
-def test_persistent_workload_3125(tmp_path):
+@pytest.mark.parametrize("resident", [True, False], ids=["resident", "streamed"])
+def test_persistent_workload_3125(tmp_path, resident):
     # The benchmarked network: launches of 1,000 and 499 steps, every step's raster.
     path = _write(tmp_path, load_config("net3125_stdp.json"))
-    sim = _persistent(path)
+    sim = _persistent(path, resident=resident)
+    info = sim.get_info()
+    print(f"net3125 persistent: {info.persistent_blocks} blocks, {info.persistent_items} items, rows "
+          f"{tuple(info.gemv_row_block)}, resident rows {info.persistent_resident_rows}")
+    assert (info.persistent_resident_rows > 0) == resident
     orc = oracle_for(path, sim, lazy=True)
     sim.run(1)
     orc.step(1)
@@ -133,9 +147,11 @@ def test_persistent_workload_10125(tmp_path):
 
 
 def test_persistent_snapshot_resume(tmp_path):
-    # Pending segment partials and the lazy sums travel with the snapshot.
+    # Pending segment partials and the lazy sums travel with the snapshot; the resident weights
+    # are in memory once a launch has ended.
     path = _write(tmp_path, small_config(steps=600))
     a = _persistent(path)
+    assert a.get_info().persistent_resident_rows > 0
     a.run(1)
     a.run(120)
     snap = os.path.join(tmp_path, "snap.bin")
@@ -152,12 +168,13 @@ def test_persistent_snapshot_resume(tmp_path):
     b.close()
 
 
-def test_persistent_wait_limit_abandons_the_launch(tmp_path):
+@pytest.mark.parametrize("resident", [True, False], ids=["resident", "streamed"])
+def test_persistent_wait_limit_abandons_the_launch(tmp_path, resident):
     # A wait limit of zero polls: a block that finds the previous step's evolves unfinished gives
     # up, every block leaves, and the error surfaces at the next synchronisation (the host forces
     # the condition; every kernel still exits normally).
     path = _write(tmp_path, small_config(steps=600))
-    sim = _persistent(path)
+    sim = _persistent(path, resident=resident)
     sim.run(1)
     sim.set_persistent_wait_limit(0)
     with pytest.raises(engine().BnError, match="waited past its limit"):
