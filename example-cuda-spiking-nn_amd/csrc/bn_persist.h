@@ -136,8 +136,8 @@ struct PPlan {
     uint64_t* ll_x[kMaxTraceSets];      // [3][n_pre] x of the trace set's neurons
     uint64_t* ll_pl[kMaxPlast][4];      // [3][n] pre, slow pre ([3][n_pre]), post, slow post ([3][n_post])
     uint64_t* ll_spk;                   // [3][n_coltiles][8] spike flags of a tile, 32 a word
-    uint64_t* ll_part[kMaxConns];       // [3][nseg][ld][2] segment partials (low, high word)
-    uint64_t* ll_dr;                    // [3][ncol][n_pre][2] lazy matrix: row tile sums
+    uint64_t* ll_part[kMaxConns];       // [3][nseg][2][ld] segment partials (low, high words)
+    uint64_t* ll_dr;                    // [3][ncol][2][n_pre] lazy matrix: row tile sums (low, high words)
     uint32_t* done;                     // [0] steps finished, summed over the blocks; [1] first
                                         // evolves finished
 };

@@ -62,26 +62,28 @@ enum : uint32_t {
 constexpr uint32_t kLPollPart = 9;    // segment partials an evolve polls per thread and round
 
 struct LShared {
+    // Small arrays first: their addresses stay within the 64 KB a DS instruction's offset reaches
+    // from one base register.
+    float ev[kEvVars][kPCols];              // evolver: its neurons' state (see l_state)
+    float gsum[kPCols];
+    uint32_t heads[8];                      // evolver: the first 4 entries of its group's Poisson tables (e, i)
+    float xs[kLSegMax], pre[kLSegMax], fac[kLSegMax];
+    float post[kPCols], tpost[kPCols];      // post traces of the column tile
+    float s[kLSegMax];                      // lazy matrix: the segment's row scales (kept across steps)
+    float target[kLSegMax];                 // and normalisation targets
+    double xd[kLSegMax];                    // x_i * s_i (x_i without row scales) in FP64
+    double dr[kLSegMax];                    // this item's row tile sums
+    double R[kLSegMax];                     // lazy matrix: the segment's row sums (kept across steps)
+    uint32_t rbits[kLSegMax / 32], rbits_prev[kLSegMax / 32];
+    uint32_t rwords[kLSegMax / 32 + 2];     // spike words (32 neurons of the pre group) covering the rows
+    uint32_t cbits[8];                      // spike words of the column tile
+    uint32_t abort;
     double red[kPWaves][kPCols];            // chunk sums
     union {
         double drin[kLDrJobs * kPThreads];  // staging: the previous step's row tile sums [tile][row]
         double pv[kLPartMax][kPCols];       // evolve: the tile's segment partials [segment][neuron]
         v4f pbuf[kPWaves][8 * 64];          // plasticity: 8 resident rows of each wave
     };
-    double dr[kLSegMax];                    // this item's row tile sums
-    double R[kLSegMax];                     // lazy matrix: the segment's row sums (kept across steps)
-    float s[kLSegMax];                      // and row scales
-    float target[kLSegMax];                 // and normalisation targets
-    double xd[kLSegMax];                    // x_i * s_i (x_i without row scales) in FP64
-    float xs[kLSegMax], pre[kLSegMax], fac[kLSegMax];
-    float post[kPCols], tpost[kPCols];      // post traces of the column tile
-    uint32_t rbits[kLSegMax / 32], rbits_prev[kLSegMax / 32];
-    uint32_t rwords[kLSegMax / 32 + 2];     // spike words (32 neurons of the pre group) covering the rows
-    uint32_t cbits[8];                      // spike words of the column tile
-    float gsum[kPCols];
-    float ev[kEvVars][kPCols];              // evolver: its neurons' state (see l_state)
-    uint32_t heads[8];                      // evolver: the first 4 entries of its group's Poisson tables (e, i)
-    uint32_t abort;
 };
 
 // Polls this thread's words (bit i of `want`: word addr(i), tag tag_lo for i < 3 else tag_hi)
@@ -265,8 +267,8 @@ __device__ __forceinline__ bool l_evolve(const PPlan& P, LShared& S, uint32_t gi
     }
     // Slot reuse: every block has finished launch step ke - 3 (its items read the slot's records).
     if (!l_wait(P, S, P.done, ke >= 3 ? (ke - 2u) * P.blocks : 0u, done_seen)) return false;
-    if (half == 1) return true;
     const uint32_t sl = l_slot(a);
+    if (half == 0) {
     uint32_t spike = 0;
     if (valid) {
         const float ge = g, gi_ = S.gsum[t];
@@ -379,6 +381,7 @@ __device__ __forceinline__ bool l_evolve(const PPlan& P, LShared& S, uint32_t gi
         const uint32_t word = (threadIdx.x >> 5);  // 0..7 within the tile
         l_put(P.ll_spk + ((size_t)sl * P.n_coltiles + G.col_base + q) * 8u + word, a,
               lane == 0 ? (uint32_t)m : (uint32_t)(m >> 32));
+    }
     }
     return true;
 }
@@ -571,9 +574,71 @@ __device__ __forceinline__ void l_item_back(const PPlan& P, LShared& S, uint32_t
     }
     const uint32_t nrows = c1 > c0 ? c1 - c0 : 0u;
     chunk_bits &= nrows >= 32u ? 0xFFFFFFFFu : ((1u << nrows) - 1u);
-    if (lazy && lane == 0)
-        for (uint32_t r = c0; r < c1; ++r) S.dr[r] = 0.0;
-    if (any_col || chunk_bits) {
+    const uint32_t nr = c1 > c0 ? c1 - c0 : 0u;
+    if (lazy && lane < nr) S.dr[c0 + lane] = 0.0;
+    // The usual case: no row of the chunk spiked and at most two lanes hold the spiking columns.
+    // Those lanes' rows go through LDS and lane r of the wave updates row r (LTP of the spiking
+    // columns), so the update runs across the wave rather than down one lane; a row's tile sum is
+    // the sum of the two lanes' changes (the only non-zero terms of the lane reduction).
+    const uint64_t col_lanes = __ballot(colspike != 0u);
+    const uint32_t ncl = (uint32_t)__popcll(col_lanes);
+    if (any_col && !chunk_bits && ncl <= 2u) {
+        v4f* T = &S.pbuf[wave][0];
+        const uint32_t la = (uint32_t)__ffsll((long long)col_lanes) - 1u;
+        const uint64_t rest = col_lanes & (col_lanes - 1ull);
+        const uint32_t lb = rest ? (uint32_t)__ffsll((long long)rest) - 1u : 64u;
+        const uint32_t cs_a = (uint32_t)__shfl((int)colspike, (int)la, 64);
+        const uint32_t cs_b = lb < 64u ? (uint32_t)__shfl((int)colspike, (int)lb, 64) : 0u;
+        if (lane == la || lane == lb) {
+            const uint32_t base = lane == la ? 0u : 32u;
+#pragma unroll
+            for (uint32_t kk = 0; kk < (uint32_t)kRes; ++kk)
+                if (c0 + kk < c1) T[base + kk] = w[kk];
+        }
+        uint32_t t = 0;
+        if (lane < nr) {
+            const uint32_t r = c0 + lane, i = s0 + r;
+            const float si = lazy ? S.s[r] : 1.0f;
+            double rs = 0.0;
+            if (!lazy || si != 0.0f) {
+                const float pre_i = S.pre[r];
+#pragma unroll
+                for (uint32_t h = 0; h < 2; ++h) {
+                    const uint32_t cs = h == 0 ? cs_a : cs_b;
+                    if (!cs) continue;
+                    const uint32_t lh = h == 0 ? la : lb;
+                    const uint32_t jh = col * kPCols + 4u * lh;
+                    v4f m4 = T[32u * h + lane];
+                    double rh = 0.0;
+#pragma unroll
+                    for (uint32_t c4 = 0; c4 < 4; ++c4) {
+                        if (!((cs >> c4) & 1u) || jh + c4 == i) continue;
+                        const float m = m4[c4];
+                        const float fl = triplet ? D.p.ltp_A + (D.p.ltp_B * S.tpost[4u * lh + c4]) : 0.0f;
+                        const float delta = triplet ? pre_i * fl : rule.ltp_A * pre_i;
+                        const float val = p_apply(lazy, m, si, delta, rule.weight_min, rule.weight_max);
+                        rh += (double)val - (double)m;
+                        m4[c4] = val;
+                        t |= 1u << h;
+                    }
+                    T[32u * h + lane] = m4;
+                    rs = h == 0 ? rh : rs + rh;
+                }
+            }
+            if (lazy) S.dr[r] = rs;
+        }
+        const uint32_t rows_a = (uint32_t)__ballot(t & 1u), rows_b = (uint32_t)__ballot((t >> 1) & 1u);
+        if (lane == la || lane == lb) {
+            const uint32_t base = lane == la ? 0u : 32u;
+#pragma unroll
+            for (uint32_t kk = 0; kk < (uint32_t)kRes; ++kk)
+                if (c0 + kk < c1) w[kk] = T[base + kk];
+            dirty |= lane == la ? rows_a : rows_b;
+        }
+    } else if (any_col || chunk_bits) {
+        // Rows through LDS 8 at a time: a row whose pre neuron spiked takes the whole row update
+        // (LTP then LTD, every column); any other row only the LTP of the spiking columns, on the
+        // lanes that hold them.
         v4f* pb = &S.pbuf[wave][0];
 #pragma unroll
         for (uint32_t b8 = 0; b8 < (uint32_t)kRes; b8 += 8) {
@@ -587,10 +652,49 @@ __device__ __forceinline__ void l_item_back(const PPlan& P, LShared& S, uint32_t
                 if (r >= c1) break;
                 const bool sp = (chunk_bits >> (b8 + kk)) & 1u;
                 if (!sp && !any_col) continue;
-                double lsum = 0.0;
+                const float si = lazy ? S.s[r] : 1.0f;
+                const uint32_t i = s0 + r;
                 v4f m4 = pb[kk * 64u + lane];
-                const uint32_t touched = p_plast_row(rule, lazy, triplet, sp, s0 + r, lazy ? S.s[r] : 1.0f, S.pre[r],
-                                                     S.fac[r], j0, active, C.n_post, colspike, post_j, fac_ltp, m4, lsum);
+                double lsum = 0.0;
+                uint32_t touched = 0;
+                if (sp) {
+                    touched = p_plast_row(rule, lazy, triplet, true, i, si, S.pre[r], S.fac[r], j0, active, C.n_post,
+                                          colspike, post_j, fac_ltp, m4, lsum);
+                } else {
+                    double rs = 0.0;
+                    if (colspike && (!lazy || si != 0.0f)) {
+                        const float pre_i = S.pre[r];
+#pragma unroll
+                        for (uint32_t c4 = 0; c4 < 4; ++c4) {
+                            if (!((colspike >> c4) & 1u) || j0 + c4 == i) continue;
+                            const float m = m4[c4];
+                            const float delta = triplet ? pre_i * fac_ltp[c4] : rule.ltp_A * pre_i;
+                            const float val = p_apply(lazy, m, si, delta, rule.weight_min, rule.weight_max);
+                            rs += (double)val - (double)m;
+                            m4[c4] = val;
+                            touched |= 1u << c4;
+                        }
+                    }
+                    if (lazy) {
+                        // The lanes' sums combined as the butterfly of the row update combines
+                        // them; with at most two non-zero lanes that is their sum.
+                        const uint64_t nz = __ballot(rs != 0.0);
+                        if (__popcll(nz) <= 2) {
+                            double t = 0.0;
+                            if (nz) {
+                                const int l0 = __ffsll((long long)nz) - 1;
+                                t = __shfl(rs, l0, 64);
+                                const uint64_t rest = nz & (nz - 1);
+                                if (rest) t = t + __shfl(rs, __ffsll((long long)rest) - 1, 64);
+                            }
+                            lsum = t;
+                        } else {
+#pragma unroll
+                            for (int off = 32; off >= 1; off >>= 1) rs += __shfl_xor(rs, off, 64);
+                            lsum = rs;
+                        }
+                    }
+                }
                 if (touched) {
                     pb[kk * 64u + lane] = m4;
                     dirty |= 1u << (b8 + kk);
