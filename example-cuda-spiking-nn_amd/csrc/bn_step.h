@@ -131,10 +131,13 @@ struct PreCompact {
     uint32_t* counts;        // [kMaxPlast]
 };
 
+constexpr uint32_t kGemvPartChunksMax = 16;   // chunks per part a part-sum block holds in LDS
+
 struct GemvPlan {
     GemvConn c[kMaxConns];
     uint32_t n_conn;
-    uint32_t total_tiles;    // one wave per tile
+    uint32_t parts;          // 1: blocks of part sums (tile = (part, 256 columns)), partials row p = part p
+    uint32_t total_tiles;    // one wave per tile (parts: one block per tile)
     uint32_t nontemporal;    // stream the weights past the caches (weight set >> MALL)
     PreCompact pc;           // pc.list == nullptr: no compaction block
 };

@@ -191,6 +191,10 @@ void build(bn_sim* sim, const std::string& config_path) {
         sys.add_monitor(std::make_unique<WeightRecorder>(*ee, files ? c.e_e_weight_recording_file : "", S));
         for (auto& m : sys.monitors()) m->quiet = quiet;
     }
+    if (flags & BN_SIM_GEMV_PART_SUMS) {
+        if (component) throw Error(BN_ERR_INVALID_ARGUMENT, "part sums are a fused-step option (not the component path)");
+        sys.set_gemv_part_sums(true);
+    }
     if (!component) sys.enable_fused(!(flags & BN_SIM_NO_GRAPH), kDefaultStepsPerGraph);
     if (flags & BN_SIM_PERSISTENT) {
         if (component) throw Error(BN_ERR_INVALID_ARGUMENT, "the persistent step is a fused step (not the component path)");
@@ -335,6 +339,7 @@ int bn_sim_get_info(bn_sim* sim, bn_sim_info* info) {
     info->persistent_blocks = sim->system->persistent_blocks();
     info->persistent_items = sim->system->persistent_items();
     info->persistent_resident_rows = sim->system->persistent_resident_rows();
+    info->gemv_part_sums = sim->system->gemv_part_sums() ? 1u : 0u;
     if (sim->system->persistent()) {
         info->gemv_part_chunks = kPWaves;
         info->kernels_per_step = 1;

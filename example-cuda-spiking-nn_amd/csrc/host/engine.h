@@ -413,6 +413,10 @@ public:
     void launch_fused_steps(uint32_t n);           // enqueue n steps (graph or eager)
     void launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end);
     const GemvPlan& gemv_plan() const { return gemv_; }
+    // The fused propagate writes part sums (GemvPlan::parts) when the shapes allow; set before
+    // enable_fused.
+    void set_gemv_part_sums(bool on) { gemv_part_sums_ = on; }
+    bool gemv_part_sums() const { return gemv_.parts != 0; }
     const EvolvePlan& evolve_plan() const { return evolve_; }
     const PlastStepPlan& plast_step_plan() const { return plast_step_; }
     bool has_norm_tail() const { return tail_.norm_rows != 0; }
@@ -484,6 +488,7 @@ private:
     hipGraph_t graph_ = nullptr;
     EvolvePlan evolve_{};
     GemvPlan gemv_{};
+    bool gemv_part_sums_ = false;
     PlastPlan plast_{};
     PlastStepPlan plast_step_{};
     TailPlan tail_{};

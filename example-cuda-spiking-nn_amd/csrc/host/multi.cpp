@@ -114,6 +114,8 @@ struct bn_multi {
         for (bn_sim* s : sims) {
             System& sys = *s->system;
             if (!sys.fused()) throw Error(BN_ERR_RUNTIME, "multi-network step needs the fused step of every network");
+            if (sys.gemv_part_sums())
+                throw Error(BN_ERR_INVALID_ARGUMENT, "multi-network step: a network was created with BN_SIM_GEMV_PART_SUMS");
             if (sys.has_norm_tail())
                 throw Error(BN_ERR_INVALID_ARGUMENT,
                             "multi-network step supports lazy or no normalisation (BN_SIM_LAZY_NORMALIZE)");

@@ -96,6 +96,20 @@ void System::build_plans() {
     tail_ = TailPlan{};
     conn_trace_set_.assign(connections_.size(), -1);
 
+    // Part sums: every connection's rows padded to 16 bytes and at most kGemvPartChunksMax
+    // chunks per part.
+    bool parts = gemv_part_sums_;
+    for (size_t c = 0; c < connections_.size() && parts; ++c) {
+        const ConnectionView& v = connections_[c]->gpu_view();
+        const uint32_t chunks = (v.pre_group_num_neurons + v.gemv_row_block - 1) / v.gemv_row_block;
+        const uint32_t per = (chunks + kFinParts - 1) / kFinParts;
+        if (v.weight_matrix_ld % 4 != 0 || (uintptr_t)v.weight_matrix % 16 != 0 || per > kGemvPartChunksMax) parts = false;
+    }
+    auto part_count = [](uint32_t chunks) {
+        const uint32_t per = (chunks + kFinParts - 1) / kFinParts;
+        return per ? (chunks + per - 1) / per : 0u;
+    };
+
     // Groups and the GEMV partial sources that finalise their conductances.
     for (size_t g = 0; g < neuron_groups_.size(); ++g) evolve_.g[g].v = neuron_groups_[g]->gpu_view();
     evolve_.n_groups = (uint32_t)neuron_groups_.size();
@@ -106,8 +120,9 @@ void System::build_plans() {
         while (pg < evolve_.n_groups && evolve_.g[pg].v.offset != v.post_group_offset) ++pg;
         if (pg == evolve_.n_groups) throw Error(BN_ERR_INVALID_ARGUMENT, "connection post group not found");
         EvolveGroup& G = evolve_.g[pg];
-        const FinalizeSource src{v.gemv_partials, (v.pre_group_num_neurons + v.gemv_row_block - 1) / v.gemv_row_block,
-                                 v.weight_matrix_ld};
+        const uint32_t chunks = (v.pre_group_num_neurons + v.gemv_row_block - 1) / v.gemv_row_block;
+        // (part sums: one row per part, each combined as a part of one chunk)
+        const FinalizeSource src{v.gemv_partials, parts ? part_count(chunks) : chunks, v.weight_matrix_ld};
         if (v.pre_group_is_inhibtory) {
             if (G.n_fin_i >= (uint32_t)kMaxSources) throw Error(BN_ERR_INVALID_ARGUMENT, "too many inhibitory sources");
             G.fin_i[G.n_fin_i++] = src;
@@ -216,9 +231,10 @@ void System::build_plans() {
         C.n_chunks = (C.n_pre + C.row_block - 1) / C.row_block;
         C.n_coltiles = (C.n_post + kGemvTileCols - 1) / kGemvTileCols;
         C.tile_begin = tiles;
-        tiles += C.n_chunks * C.n_coltiles;
+        tiles += (parts ? part_count(C.n_chunks) : C.n_chunks) * C.n_coltiles;
         C.tile_end = tiles;
     }
+    gemv_.parts = parts ? 1u : 0u;
     gemv_.n_conn = (uint32_t)connections_.size();
     gemv_.total_tiles = tiles;
     // Weight sets well above the 256 MB MALL are streamed with non-temporal loads; smaller

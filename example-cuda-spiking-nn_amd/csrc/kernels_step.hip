@@ -502,10 +502,11 @@ __device__ __forceinline__ float4 load_w4(const float* p) {
     }
 }
 
+// Sums of rows [i0, i1) of the lane's 4 columns j0 .. j0+3, each in FP64 from 0.0 in row order.
 template <bool kScaled, bool kNT>
-__device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const float* __restrict__ x,
-                                          const float* __restrict__ s, double* __restrict__ partials,
-                                          uint32_t ld, uint32_t i0, uint32_t i1, uint32_t j0, uint32_t chunk) {
+__device__ __forceinline__ void gemv_tile_sums(const float* __restrict__ W, const float* __restrict__ x,
+                                               const float* __restrict__ s, uint32_t ld, uint32_t i0, uint32_t i1,
+                                               uint32_t j0, double& o0, double& o1, double& o2, double& o3) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     const float* wp = W + (size_t)i0 * ld + j0;
     uint32_t i = i0;
@@ -535,6 +536,18 @@ __device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const flo
         a3 += xd * (double)w.w;
         wp += ld;
     }
+    o0 = a0;
+    o1 = a1;
+    o2 = a2;
+    o3 = a3;
+}
+
+template <bool kScaled, bool kNT>
+__device__ __forceinline__ void gemv_tile(const float* __restrict__ W, const float* __restrict__ x,
+                                          const float* __restrict__ s, double* __restrict__ partials,
+                                          uint32_t ld, uint32_t i0, uint32_t i1, uint32_t j0, uint32_t chunk) {
+    double a0, a1, a2, a3;
+    gemv_tile_sums<kScaled, kNT>(W, x, s, ld, i0, i1, j0, a0, a1, a2, a3);
     double* out = partials + (size_t)chunk * ld + j0;
     out[0] = a0;
     out[1] = a1;
@@ -571,6 +584,61 @@ __global__ __launch_bounds__(256) void k_gemv(GemvPlan P) {
         return;
     }
     gemv_block<kNT>(P, blockIdx.x);
+}
+
+// Part sums (the fused step's propagate): a block of kFinParts waves owns one part of a
+// connection's row chunks -- part p is chunks [p * per, min((p + 1) * per, chunks)), per =
+// ceil(chunks / kFinParts), the parts the evolve combines -- and one 256-column tile.  Each wave
+// sums its chunks (chunk w, w + kFinParts, ... of the part) into LDS; the block adds the part's
+// chunk sums in chunk order from 0.0, which is the part sum the evolve forms from the chunk
+// partials, and stores it as row p.  The evolve then reads one row per part instead of one per
+// chunk (same arithmetic).
+template <bool kNT>
+__device__ __forceinline__ void gemv_part_block(const GemvPlan& P, uint32_t blk) {
+    __shared__ double cs[kGemvPartChunksMax][kGemvTileCols];
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (blk >= P.total_tiles) return;
+    uint32_t ci = 0;
+    while (ci + 1 < P.n_conn && blk >= P.c[ci].tile_end) ++ci;
+    const GemvConn& C = P.c[ci];
+    const uint32_t t = blk - C.tile_begin;
+    const uint32_t part = t / C.n_coltiles;
+    const uint32_t q = t - part * C.n_coltiles;
+    const uint32_t per = fin_part_size(C.n_chunks);
+    const uint32_t c0 = part * per, c1 = min(c0 + per, C.n_chunks);
+    const uint32_t j0 = q * kGemvTileCols + 4 * lane;
+    for (uint32_t c = c0 + wave; c < c1; c += (uint32_t)kFinParts) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        const uint32_t i0 = c * C.row_block, i1 = min(i0 + C.row_block, C.n_pre);
+        if (j0 < C.n_post) {
+            if (C.row_scale) gemv_tile_sums<true, kNT>(C.W, C.x, C.row_scale, C.ld, i0, i1, j0, a0, a1, a2, a3);
+            else gemv_tile_sums<false, kNT>(C.W, C.x, nullptr, C.ld, i0, i1, j0, a0, a1, a2, a3);
+        }
+        double* row = cs[c - c0];
+        row[4 * lane + 0] = a0;
+        row[4 * lane + 1] = a1;
+        row[4 * lane + 2] = a2;
+        row[4 * lane + 3] = a3;
+    }
+    __syncthreads();
+    if (threadIdx.x < (uint32_t)kGemvTileCols) {
+        const uint32_t j = q * kGemvTileCols + threadIdx.x;
+        if (j < C.n_post) {
+            double acc = 0.0;
+            for (uint32_t c = c0; c < c1; ++c) acc += cs[c - c0][threadIdx.x];
+            C.partials[(size_t)part * C.ld + j] = acc;
+        }
+    }
+}
+
+template <bool kNT>
+__global__ __launch_bounds__(kFinParts * 64) void k_gemv_parts(GemvPlan P) {
+    if (P.pc.list && blockIdx.x == gridDim.x - 1) {
+        compact_pre_to_global(P.pc);
+        return;
+    }
+    gemv_part_block<kNT>(P, blockIdx.x);
 }
 
 template <bool kNT>
@@ -1474,6 +1542,15 @@ int launch_evolve_plast(const EvolvePlan& ev, const PlastStepPlan* plast, bn_str
 
 int launch_gemv(const GemvPlan& plan, bn_stream_t s) {
     if (plan.total_tiles == 0 && !plan.pc.list) return 0;
+    if (plan.parts) {
+        // One part tile per block, and the compaction block last when the plan has one.
+        const uint32_t blocks = plan.total_tiles + (plan.pc.list ? 1u : 0u);
+        if (plan.nontemporal)
+            hipLaunchKernelGGL(k_gemv_parts<true>, dim3(blocks), dim3(kFinParts * 64), 0, s, plan);
+        else
+            hipLaunchKernelGGL(k_gemv_parts<false>, dim3(blocks), dim3(kFinParts * 64), 0, s, plan);
+        return (int)hipGetLastError();
+    }
     // Four tiles per block, and the compaction block last when the plan has one.
     const uint32_t blocks = cdiv(plan.total_tiles, 4) + (plan.pc.list ? 1u : 0u);
     if (gemv_vector_ok(plan) && plan.nontemporal)

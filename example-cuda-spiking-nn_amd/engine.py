@@ -26,6 +26,7 @@ BURYN_FLAGS = {
     "sequential_graph": 1 << 6,
     "reference_layout": 1 << 7,
     "persistent": 1 << 8,
+    "gemv_part_sums": 1 << 9,
 }
 
 
@@ -46,7 +47,7 @@ class SimInfo(C.Structure):
         ("flags", C.c_uint32), ("kernels_per_step", C.c_uint32), ("device_bytes", C.c_uint64),
         ("steps_per_graph", C.c_uint32), ("graph_launches", C.c_uint32),
         ("persistent_blocks", C.c_uint32), ("persistent_items", C.c_uint32), ("gemv_part_chunks", C.c_uint32),
-        ("persistent_resident_rows", C.c_uint32),
+        ("persistent_resident_rows", C.c_uint32), ("gemv_part_sums", C.c_uint32),
     ]
 
 

@@ -306,13 +306,18 @@ enum {
     BN_SIM_REFERENCE_LAYOUT = 1u << 7,  /* weight rows unpadded (row stride = post), as the
                                          * reference allocates them (same results, slower GEMV
                                          * where post % 4 != 0) */
-    BN_SIM_PERSISTENT = 1u << 8         /* the persistent step: one cooperative launch per run of
+    BN_SIM_PERSISTENT = 1u << 8,        /* the persistent step: one cooperative launch per run of
                                          * steps; plasticity applied inside the weight pass, the
                                          * evolve run by the block that completes a column tile.
                                          * Needs BN_SIM_LAZY_NORMALIZE when E->E normalises.  Its
                                          * summation orders (bn_sim_info.gemv_part_chunks = 8,
                                          * lazy row sums by 256-column tile) differ from the
                                          * graph step's, so results differ in rounding only. */
+    BN_SIM_GEMV_PART_SUMS = 1u << 9     /* the fused step's propagate reduces each part of a
+                                         * connection's row chunks (the 8 parts the evolve
+                                         * combines) inside a block, so the evolve reads one FP64
+                                         * row per part instead of one per chunk.  Same results;
+                                         * not for bn_multi networks. */
 };
 
 typedef struct bn_sim_info {
@@ -338,6 +343,8 @@ typedef struct bn_sim_info {
     uint32_t persistent_resident_rows; /* persistent step with the weights held in registers for
                                         the whole launch: rows per chunk of its kernel (0: the
                                         weights are streamed from memory every step) */
+    uint32_t gemv_part_sums;         /* 1: the fused propagate writes one FP64 row per part of
+                                        each connection's row chunks (BN_SIM_GEMV_PART_SUMS) */
 } bn_sim_info;
 
 /* Host pointers to fill; any may be NULL.  Arrays are indexed by global neuron id (length

@@ -77,12 +77,15 @@ def test_fused_step_bit_exact(tmp_path, graph):
     sim.close()
 
 
-@pytest.mark.parametrize("mode", ["lazy", "lazy-sequential", "faithful", "burst"])
+@pytest.mark.parametrize("mode", ["lazy", "lazy-sequential", "faithful", "burst", "lazy-parts", "faithful-parts",
+                                  "burst-parts"])
 def test_graph_steps_bit_exact(tmp_path, mode):
     # Whole graphs of 10 steps: the overlapped form (plasticity of step t-1 in the launch of the
     # evolve of step t, staging copies, stimulus in the propagate launch) and the sequential
     # one must both leave exactly the oracle's state after every graph; faithful normalisation
     # always runs sequentially.
+    parts = mode.endswith("-parts")
+    mode = mode.replace("-parts", "")
     if mode == "burst":
         cfg = small_config(n_e_side=24, n_i_side=12, steps=400)
         cfg["stimulus_selected_input_ids"] = list(range(300))
@@ -95,8 +98,11 @@ def test_graph_steps_bit_exact(tmp_path, mode):
     path = _write(tmp_path, cfg)
     lazy = mode != "faithful"
     flags = ("no_monitors", "quiet") + (("lazy_normalize",) if lazy else ()) + \
-        (("sequential_graph",) if mode == "lazy-sequential" else ())
+        (("sequential_graph",) if mode == "lazy-sequential" else ()) + (("gemv_part_sums",) if parts else ())
     sim = engine().Simulation(path, flags=flags)
+    # Part sums: the propagate adds each part's chunk sums in the order the evolve would (same
+    # results, one partial row per part).
+    assert sim.get_info().gemv_part_sums == (1 if parts else 0)
     sim.set_steps_per_graph(10)
     orc = oracle_for(path, sim, lazy=lazy)
     for chunk in range(25):
@@ -137,9 +143,11 @@ def test_steps_per_graph_bit_exact(tmp_path, spg):
     sim.close()
 
 
-@pytest.mark.parametrize("row_blocks,nontemporal", [([8, 8, 16, 8], 1), ([3, 5, 7, 1], 0), ([512, 512, 1, 2], 1)],
-                         ids=["rb8-nt", "rb-odd", "rb-mixed-nt"])
-def test_gemv_tilings_bit_exact(tmp_path, row_blocks, nontemporal):
+@pytest.mark.parametrize("row_blocks,nontemporal,parts", [([8, 8, 16, 8], 1, False), ([3, 5, 7, 1], 0, False),
+                                                          ([512, 512, 1, 2], 1, False), ([8, 8, 16, 8], 1, True),
+                                                          ([6, 5, 7, 3], 0, True)],
+                         ids=["rb8-nt", "rb-odd", "rb-mixed-nt", "rb8-nt-part-sums", "rb-odd-part-sums"])
+def test_gemv_tilings_bit_exact(tmp_path, row_blocks, nontemporal, parts):
     # Row blocks change the number of chunk partials per post neuron (1 to 134 here, up to 17
     # per combination part) and so the order of the FP64 sums; the oracle follows the same
     # tiling.  Non-temporal weight loads must not change any value.
@@ -147,8 +155,9 @@ def test_gemv_tilings_bit_exact(tmp_path, row_blocks, nontemporal):
     cfg["gemv_row_blocks"] = row_blocks
     cfg["gemv_nontemporal_loads"] = nontemporal
     path = _write(tmp_path, cfg)
-    sim = engine().Simulation(path, flags=("no_monitors", "quiet", "lazy_normalize"))
+    sim = engine().Simulation(path, flags=("no_monitors", "quiet", "lazy_normalize") + (("gemv_part_sums",) if parts else ()))
     assert tuple(sim.info.gemv_row_block) == tuple(row_blocks)
+    assert sim.get_info().gemv_part_sums == (1 if parts else 0)
     orc = oracle_for(path, sim, lazy=True)
     _run_lockstep(sim, orc, 150, check_every=5)
     _check_traces_and_weights(sim, orc)
@@ -300,14 +309,16 @@ def _graph_lockstep(sim, orc, first_step, graphs, spg):
     return spikes
 
 
-@pytest.mark.parametrize("spg", [500, 100])
-def test_bench_form_3125_graphs(tmp_path, spg):
+@pytest.mark.parametrize("spg,parts", [(500, False), (100, False), (500, True)],
+                         ids=["500", "100", "500-part-sums"])
+def test_bench_form_3125_graphs(tmp_path, spg, parts):
     # The form the benchmark times: lazy normalisation, overlapped graphs of 500 (and the library
     # default 100) steps with the monitors' spike ring, 1,000 steps bit-exact against the FP32
     # lazy oracle (raster of every step, state after every graph, traces and weights at the end).
     cfg = load_config("net3125_stdp.json")
     path = _write(tmp_path, cfg)
-    sim = engine().Simulation(path, flags=("quiet", "lazy_normalize"))
+    sim = engine().Simulation(path, flags=("quiet", "lazy_normalize") + (("gemv_part_sums",) if parts else ()))
+    assert sim.get_info().gemv_part_sums == (1 if parts else 0)
     sim.set_steps_per_graph(spg)
     assert sim.get_info().kernels_per_step == 2  # the overlapped form
     orc = oracle_for(path, sim, lazy=True)
@@ -320,12 +331,14 @@ def test_bench_form_3125_graphs(tmp_path, spg):
     sim.close()
 
 
-def test_net10125_graphs(tmp_path):
+@pytest.mark.parametrize("parts", [False, True], ids=["chunk-partials", "part-sums"])
+def test_net10125_graphs(tmp_path, parts):
     # The 10,125-neuron workload (410 MB of weights, non-temporal weight loads, long row chunks):
     # lazy normalisation, overlapped graphs of 250 steps, 1,001 steps against the FP32 lazy oracle.
     cfg = load_config("net10125_stdp.json")
     path = _write(tmp_path, cfg)
-    sim = engine().Simulation(path, flags=("quiet", "lazy_normalize"))
+    sim = engine().Simulation(path, flags=("quiet", "lazy_normalize") + (("gemv_part_sums",) if parts else ()))
+    assert sim.get_info().gemv_part_sums == (1 if parts else 0)
     sim.set_steps_per_graph(250)
     assert sim.get_info().kernels_per_step == 2
     orc = oracle_for(path, sim, lazy=True)
