@@ -46,7 +46,8 @@ def main() -> int:
                          "batched-uneven: 48 plastic networks of uneven size; adjoint: gradient of a "
                          "1,000-step window of the 3,125-neuron network (plasticity off)")
     ap.add_argument("--networks", type=int, default=48)
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="plain: keep the configured GEMV row blocks; batched: keep the default tiles and schedule")
     ap.add_argument("--steps", type=int, default=None, help="timed steps (plain 10000, batched 5001)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps before the timed ones (plain 100, batched 0)")
     ap.add_argument("--settle", type=int, default=2000,
@@ -55,6 +56,8 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--config", default=os.path.join(REPO, "configs", "net3125_stdp.json"))
     ap.add_argument("--normalization", choices=("lazy", "faithful"), default="lazy")
+    ap.add_argument("--gemv-row-blocks", default=None,
+                    help="plain: GEMV row blocks EE,EI,IE,II (e.g. 48,48,32,32) instead of the autotuner's choice")
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -83,11 +86,27 @@ def main() -> int:
     args.warmup = 100 if args.warmup is None else args.warmup
 
     eng = importlib.import_module("example-cuda-spiking-nn_amd")
-    cfg = eng.config_dump(args.config)
     # Monitors on: the step writes the spike-bit and voltage rings every step, as the reference's
     # step historicizes every step (record() file I/O only happens in System::run, not here).
     flags = ["quiet"] + (["lazy_normalize"] if args.normalization == "lazy" else [])
-    sim = eng.Simulation(args.config, flags=flags)
+    config_path, tune = args.config, None
+    if args.gemv_row_blocks:
+        import tempfile
+        c = json.load(open(args.config))
+        c["gemv_row_blocks"] = [int(r) for r in args.gemv_row_blocks.split(",")]
+        config_path = os.path.join(tempfile.mkdtemp(prefix="bn_bench_rows_"), os.path.basename(args.config))
+        json.dump(c, open(config_path, "w"))
+    elif not args.no_autotune:
+        # GEMV row blocks (the propagate's tile shape and FP64 summation order): the default and a
+        # few variants, each settled and timed briefly; the timed run below uses the fastest.
+        import tempfile
+        wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+        tune_dir = tempfile.mkdtemp(prefix="bn_bench_rows_")
+        config_path, best_rows, tune_us = wl.autotune_plain_row_blocks(eng, args.config, tune_dir, flags)
+        tune = {"chosen": best_rows, "us_per_step": {k: round(v, 3) for k, v in tune_us.items()},
+                "settle_steps": 1000, "timed_steps": 3000}
+    cfg = eng.config_dump(config_path)
+    sim = eng.Simulation(config_path, flags=flags)
     # Steps per hipGraph: 500 for long runs (0.8% faster than 100 on the 3,125-neuron network,
     # 0.3% on the 10,125-neuron one); a shorter timed run is one graph of its own length, rounded
     # down to an even count (the overlapped form), any odd step eager.
@@ -132,12 +151,13 @@ def main() -> int:
     tpath = os.path.join(REPO, "profiles", f"k_gemv_{name}_hbm_traffic.json")
     if os.path.exists(tpath):
         t = json.load(open(tpath))
-        if t.get("neurons") == n_total and t.get("weight_entries") == n_w:
+        if (t.get("neurons") == n_total and t.get("weight_entries") == n_w
+                and t.get("gemv_row_blocks") == [int(r) for r in info.gemv_row_block]):
             traffic = t.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.config, cfg, args.cpu_seconds, info)
+        cpu = cpu_baseline(config_path, cfg, args.cpu_seconds, info)
 
     n_e, n_i = int(cfg["num_excitatory_neurons"]), int(cfg["num_inhibitory_neurons"])
     line = {
@@ -159,6 +179,7 @@ def main() -> int:
                    "neurons": n_total, "weight_entries": n_w, "normalization": args.normalization,
                    "steps_per_graph": steps_per_graph, "graph_launches_timed": graph_launches,
                    "kernels_per_step": int(info.kernels_per_step), "parallelism": "single GPU",
+                   "gemv_row_blocks": [int(r) for r in info.gemv_row_block], "row_block_autotune": tune,
                    "timed_window_rate_hz": {"all": round(rate_all, 3), "excitatory": round(rate_e, 3),
                                             "inhibitory": round(rate_i, 3), "steps": window},
                    "first_timed_step": first_timed},
