@@ -164,8 +164,17 @@ void build(bn_sim* sim, const std::string& config_path) {
         p.triplet_pre_trace_decay = std::exp(-1.0f * dt / c.e_e_plasticity_triplet_tau_ltp);
         p.triplet_post_trace_decay = std::exp(-1.0f * dt / c.e_e_plasticity_triplet_tau_ltd);
         p.plasticity_offset = off;
-        sys.add_plasticity(std::make_unique<TripletPlasticity>(*ee, p, S));
-        validate_plasticity_params(p, true, "e_e excitatory Triplet plasticity");
+        if (c.e_e_plasticity_rule == "pair") {
+            // The reference's pair-STDP component on E->E (its PlasticityParams use ltp_A, ltd_A,
+            // the two trace decays and the weight bounds only).
+            p.ltp_B = p.ltd_B = 0.0f;
+            p.triplet_pre_trace_decay = p.triplet_post_trace_decay = 0.0f;
+            sys.add_plasticity(std::make_unique<StdpPlasticity>(*ee, p, S));
+            validate_plasticity_params(p, false, "e_e excitatory pair Stdp plasticity");
+        } else {
+            sys.add_plasticity(std::make_unique<TripletPlasticity>(*ee, p, S));
+            validate_plasticity_params(p, true, "e_e excitatory Triplet plasticity");
+        }
         ++off;
     }
     if (c.i_e_plasticity_enabled) {

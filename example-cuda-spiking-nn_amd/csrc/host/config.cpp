@@ -231,6 +231,9 @@ Config from_value(const Value& j) {
     if (c.gemv_row_blocks.size() != 4) throw std::runtime_error("gemv_row_blocks must hold 4 values (EE, EI, IE, II)");
     c.gemv_nontemporal_loads = value_or<int>(j, "gemv_nontemporal_loads", c.gemv_nontemporal_loads);
     c.background_noise_seed = value_or<long long>(j, "background_noise_seed", c.background_noise_seed);
+    c.e_e_plasticity_rule = string_or(j, "e_e_plasticity_rule", c.e_e_plasticity_rule);
+    if (c.e_e_plasticity_rule != "triplet" && c.e_e_plasticity_rule != "pair")
+        throw std::runtime_error("e_e_plasticity_rule must be \"triplet\" or \"pair\"");
 
     const std::string& od = c.output_directory;
     c.excitatory_locations_file = prefix_output_path(od, c.excitatory_locations_file);
@@ -351,6 +354,7 @@ std::string config_to_json(const Config& c) {
     d.ua("gemv_row_blocks", c.gemv_row_blocks);
     d.i("gemv_nontemporal_loads", c.gemv_nontemporal_loads);
     d.i("background_noise_seed", c.background_noise_seed);
+    S(e_e_plasticity_rule)
 #undef F
 #undef U
 #undef S

@@ -135,6 +135,11 @@ struct Config {
     // extension: key of the background-noise Philox streams; -1 uses rng_seed (which also seeds
     // the connectivity), so one weight set can be run with several noise streams.
     long long background_noise_seed = -1;
+    // extension: the E->E learning rule.  "triplet" (the reference's wiring, main_buryn.cpp:286-305)
+    // or "pair": the reference's StdpPlasticity component (stdp_plasticity.cpp,
+    // kernel_stdp_plasticity_update.cu) on E->E with e_e_plasticity_ltp_A, _ltd_A, _tau_ltp,
+    // _tau_ltd, _weight_min and _weight_max (no triplet terms, no row normalisation).
+    std::string e_e_plasticity_rule = "triplet";
 };
 
 Config load_config_json(const std::string& path);

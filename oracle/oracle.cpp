@@ -151,7 +151,7 @@ double tile_sum(uint32_t q, uint32_t n, Val val) {
 // ---------------------------------------------------------------------------------------
 // Simulation
 // ---------------------------------------------------------------------------------------
-enum { kTriplet = 0, kIstdp = 1 };
+enum { kTriplet = 0, kIstdp = 1, kPair = 2 };
 
 template <typename R>
 struct Sim {
@@ -240,13 +240,15 @@ struct Sim {
         }
         if (p.e_e_plasticity_enabled) {
             Plast P{};
-            P.kind = kTriplet;
+            // main_buryn.cpp:286-305 (triplet) or the reference's StdpPlasticity component.
+            P.kind = p.e_e_pair_stdp ? kPair : kTriplet;
             P.c = 0;
             P.ltp_A = (R)p.e_e_ltp_A, P.ltd_A = (R)p.e_e_ltd_A, P.ltp_B = (R)p.e_e_ltp_B, P.ltd_B = (R)p.e_e_ltd_B;
             P.pre_decay = dec(p.timestep, p.e_e_tau_ltp);
             P.post_decay = dec(p.timestep, p.e_e_tau_ltd);
             P.tpre_decay = dec(p.timestep, p.e_e_triplet_tau_ltp);
             P.tpost_decay = dec(p.timestep, p.e_e_triplet_tau_ltd);
+            if (P.kind == kPair) P.ltp_B = P.ltd_B = P.tpre_decay = P.tpost_decay = (R)0;
             P.wmin = (R)p.e_e_weight_min, P.wmax = (R)p.e_e_weight_max;
             plast.push_back(P);
         }
@@ -420,10 +422,11 @@ struct Sim {
 
     R ltp_delta(const Plast& P, uint32_t i, uint32_t j) const {
         if (P.kind == kTriplet) return P.pre[i] * (P.ltp_A + (P.ltp_B * P.tpost[j]));
-        return P.ltp_A * P.pre[i];
+        return P.ltp_A * P.pre[i];  // kernel_istdp_plasticity_update.cu:55, kernel_stdp_plasticity_update.cu:55
     }
     R ltd_delta(const Plast& P, uint32_t i, uint32_t j) const {
         if (P.kind == kTriplet) return (R)-1.0f * P.post[j] * (P.ltd_A + (P.ltd_B * P.tpre[i]));
+        if (P.kind == kPair) return (R)-1.0f * P.ltd_A * P.post[j];  // kernel_stdp_plasticity_update.cu:77
         return (R)-1.0f * P.ltd_A * (P.post[j] - P.ltd_B);
     }
 

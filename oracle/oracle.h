@@ -32,6 +32,11 @@
  * precision 32 reproduces the engine bit for bit; precision 64 runs every quantity in FP64
  * (libm exp, plain sequential sums) as an independent accuracy reference.
  *
+ *   - pair STDP (e_e_pair_stdp = 1): the reference's StdpPlasticity on E->E
+ *     (kernel_stdp_plasticity_update.cu:16-137): decay pre then post traces with flush, LTP of
+ *     every column of a spiking post (w += ltp_A * pre_i, clamp), LTD of every row of a spiking
+ *     pre (w += (-1 * ltd_A) * post_j, clamp), self entries i == j skipped, then the +1 trace
+ *     increments; no triplet terms, no normalisation.
  * Parity pins (tests/test_oracle_*.py): Philox known-answer vectors of the published method and
  * rocRAND's engine; expf against libm/FP64 within 2 ulp; Poisson table against scipy's CDF;
  * traces against the closed form K(d^k - r^k); EIF single step against an FP64 hand
@@ -88,6 +93,7 @@ typedef struct oracle_params {
     uint32_t dense_gemv;          /* 1: visit every pre row (no skip of x = 0 rows; same result) */
     uint32_t gemv_part_chunks[4]; /* chunks per combination part (0: ceil(chunks / 8)) */
     uint32_t lazy_tile_order;     /* 1: lazy row sums in the column-tile order (see above) */
+    uint32_t e_e_pair_stdp;       /* 1: pair STDP on E->E (kernel_stdp_plasticity_update.cu) */
 } oracle_params;
 
 typedef struct oracle_sim oracle_sim;

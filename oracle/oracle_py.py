@@ -44,6 +44,7 @@ class Params(C.Structure):
         ("i_e_tau_ltd", C.c_float), ("i_e_weight_min", C.c_float), ("i_e_weight_max", C.c_float),
         ("gemv_row_block", C.c_uint32 * 4), ("lazy_normalize", C.c_uint32),
         ("dense_gemv", C.c_uint32), ("gemv_part_chunks", C.c_uint32 * 4), ("lazy_tile_order", C.c_uint32),
+        ("e_e_pair_stdp", C.c_uint32),
     ]
 
 
@@ -158,6 +159,7 @@ def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=Fa
     for k in range(4):
         p.gemv_part_chunks[k] = int(gemv_part_chunks[k])
     p.lazy_tile_order = int(lazy_tile_order)
+    p.e_e_pair_stdp = int(cfg.get("e_e_plasticity_rule", "triplet") == "pair")
     return p
 
 

@@ -13,6 +13,10 @@ FP32 oracle, gradients against the oracle's FP64 adjoint.
   FP32-rounding-sized difference of the trajectory (a 1e-7 relative change of the start
   voltages moves the FP64 surrogate gradient by ~5e-3 and the exact one by ~2e-4), so that
   comparison is held to 5e-2 (surrogate) and 5e-3 (exact).
+- Subthreshold regime (a spike-free window after early pulse-driven spikes, traces still
+  decaying): there the problem is well conditioned, and the FP32 engine's gradients lie within
+  1e-4 relative of the FP64-forward adjoint in both modes and, for the exact derivative, of FP64
+  central differences (h = 1e-6 relative) of the FP64 forward loss.
 - Checkpointing: the gradient does not depend on the checkpoint interval (ragged last segment
   included) beyond FP32 summation order.
 - torch.autograd: WindowLoss returns the library's gradients, scaled by the incoming gradient.
@@ -216,6 +220,104 @@ def test_window_gradient_independent_of_checkpoint_interval(tmp_path):
         assert g["dg_leak"] == g0["dg_leak"] and g["dtau_decay_e"] == g0["dtau_decay_e"]
         for k in range(4):
             assert _norm_rel(g["dW"][k], g0["dW"][k].astype(np.float64)) <= 1e-6, k
+    sim.close()
+
+
+def _subthreshold(tmp_path):
+    """A small network whose every neuron is driven over threshold by a 2 nA pulse early on
+    (every other E neuron and every I neuron, steps 5-60, each fires once) and that is
+    subthreshold afterwards under a weak background (no spike from step 100 to 400 in the FP32
+    oracle), with the connection traces of the early spikes still decaying: a spike-free window
+    whose gradients flow through V, g and the traces of both populations."""
+    cfg = small_config(n_e_side=12, n_i_side=6, steps=600, stimulus=False, plastic=False, seed=11)
+    ne, ni = 144, 36
+    cfg["group_e_background_excitatory_scaling_factor"] = 0.3e-9
+    cfg["group_i_background_excitatory_scaling_factor"] = 0.3e-9
+    cfg["stimulus_selected_input_ids"] = list(range(0, ne, 2)) + list(range(ni))
+    cfg["stimulus_selected_input_post_group_offsets"] = [0] * (ne // 2) + [ne] * ni
+    cfg["stimulus_start_clock_steps"] = [5]
+    cfg["stimulus_end_clock_steps"] = [60]
+    cfg["stimulus_input_values"] = [2e-9]
+    path = os.path.join(str(tmp_path), "sub.json")
+    json.dump(cfg, open(path, "w"))
+    return path
+
+
+@pytest.mark.parametrize("beta", [0.0, 1e-3])
+def test_window_subthreshold_matches_fp64_forward_and_differences(tmp_path, beta):
+    # The check the gradient interface is specified by, on the GPU: in a spike-free window the
+    # FP32 engine's gradients must lie within 1e-4 relative of the FP64-forward adjoint (its own
+    # FP64 trajectory from the same start, oracle/adjoint.cpp) and, for the exact derivative
+    # (beta = 0: nothing flips, so it is the derivative of the loss), of FP64 central
+    # differences of that loss, h = 1e-6 relative.
+    path = _subthreshold(tmp_path)
+    sim = engine().Simulation(path)
+    warm, T = 100, 120
+    sim.run(warm)
+    assert not np.any(sim.state()["i_stimulus"]), "the pulse has ended"
+    rE, dE, _ = sim.connection_traces(0)
+    rI, dI, _ = sim.connection_traces(2)
+    assert np.all((dE - rE)[::2] > 0) and np.all(dI - rI > 0), "every pulsed neuron fired before the window"
+    state = _oracle_state(sim)
+    ids = np.arange(0, sim.NE, 3, dtype=np.uint32)
+    rng = np.random.default_rng(12)
+    tr = rng.uniform(0, 20, (T, 2))
+    tv = rng.uniform(-0.070, -0.060, (T, ids.size))
+    win = engine().Window(sim, T, ids, checkpoint_interval=50, surrogate_beta=beta)
+    win.set_targets(tr, tv)
+    out = win.forward()
+    g = win.backward()
+    assert out["raster"].sum() == 0, "the window must be spike-free"
+    dump = engine().config_dump(path)
+    params = oracle_py.params_from_config(dump)
+    gl, td = float(dump["g_leak"]), float(dump["tau_decay_e"])
+    W = [win.weights(k).astype(np.float64) for k in range(4)]
+    kw = dict(rate_tau=5e-3, beta=beta, target_rates=tr, target_v=tv)
+    ref = oracle_py.window(params, W, gl, td, state, T, ids, **kw)
+    assert ref["spikes"].sum() == 0
+    assert out["loss"] == pytest.approx(ref["loss"], rel=1e-5)
+    print(f"beta {beta}: loss {out['loss']:.6g}, |dW| " +
+          " ".join(f"{np.linalg.norm(ref['dW'][k]):.3g}" for k in range(4)), flush=True)
+    for k in range(4):
+        n = np.linalg.norm(ref["dW"][k])
+        if n == 0:
+            assert not np.any(g["dW"][k]), k
+            continue
+        err = _norm_rel(g["dW"][k], ref["dW"][k])
+        print(f"  dW[{k}] norm-relative error vs FP64 forward adjoint {err:.2e}", flush=True)
+        assert err <= 1e-4, (k, err)
+    assert g["dg_leak"] == pytest.approx(ref["dg_leak"], rel=1e-4)
+    assert g["dtau_decay_e"] == pytest.approx(ref["dtau_decay_e"], rel=1e-4)
+    assert np.linalg.norm(ref["dW"][0]) > 0 and np.linalg.norm(ref["dW"][2]) > 0
+    if beta == 0.0:
+        def loss_at(W2, gl2, td2):
+            r = oracle_py.window(params, W2, gl2, td2, state, T, ids, grad=False, **kw)
+            assert r["spikes"].sum() == 0
+            return r["loss"]
+
+        checks = []
+        for name, value in (("g_leak", gl), ("tau_decay_e", td)):
+            h = 1e-6 * abs(value)
+            up = loss_at(W, gl + h, td) if name == "g_leak" else loss_at(W, gl, td + h)
+            dn = loss_at(W, gl - h, td) if name == "g_leak" else loss_at(W, gl, td - h)
+            checks.append((name, (up - dn) / (2 * h), g["dg_leak"] if name == "g_leak" else g["dtau_decay_e"]))
+        for c in (0, 2):  # the matrices onto recorded (E) posts
+            mag = np.abs(ref["dW"][c]) * (W[c] > 0)
+            picks = np.argsort(mag.ravel())[::-1][:6]
+            scale = np.abs(W[c][W[c] > 0]).mean()
+            for f in picks:
+                i, j = divmod(int(f), W[c].shape[1])
+                h = 1e-6 * max(abs(W[c][i, j]), scale)
+                Wp = [w.copy() for w in W]
+                Wp[c][i, j] += h
+                up = loss_at(Wp, gl, td)
+                Wp[c][i, j] -= 2 * h
+                dn = loss_at(Wp, gl, td)
+                checks.append((f"W{c}[{i},{j}]", (up - dn) / (2 * h), float(g["dW"][c][i, j])))
+        for name, fd, gpu in checks:
+            print(f"  {name}: central difference {fd:.9g}, GPU {gpu:.9g}", flush=True)
+            assert abs(gpu - fd) <= 1e-4 * abs(fd), (name, fd, gpu)
+    win.close()
     sim.close()
 
 

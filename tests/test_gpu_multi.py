@@ -1,6 +1,7 @@
 """Several plastic networks of uneven size in one launch per phase (bn_multi_*): every network
 must compute bit for bit what it computes alone (same plans, same kernels), so each is checked
-against a plain run of its own config: spikes every step, and state, traces and weights."""
+against a plain run of its own config (spikes every step, and state, traces and weights) and,
+one network per size class of the uneven workload, directly against the FP32 oracle."""
 import importlib
 import json
 import os
@@ -8,7 +9,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import engine, small_config
+from helpers import assert_state_equal, engine, oracle_for, small_config
 
 pytestmark = pytest.mark.gpu
 
@@ -30,6 +31,19 @@ def _check_same(a, b, label):
     for off in range(a.info.num_plasticities):
         for x, y in zip(a.plasticity_traces(off), b.plasticity_traces(off)):
             assert np.array_equal(x, y), f"{label}: plasticity traces {off}"
+
+
+def _check_traces_and_weights(sim, orc):
+    for k in range(4):
+        for a, b, name in zip(sim.connection_traces(k), orc.connection_traces(k), ("r", "d", "x")):
+            assert np.array_equal(a, b), f"connection {k} trace {name}"
+    for off in range(sim.info.num_plasticities):
+        for a, b, name in zip(sim.plasticity_traces(off), orc.plasticity_traces(off),
+                              ("pre", "post", "triplet_pre", "triplet_post")):
+            assert np.array_equal(a, b), f"plasticity {off} trace {name}"
+    for k in range(4):
+        w_gpu, w_orc = sim.weights(k), orc.weights(k)
+        assert np.array_equal(w_gpu, w_orc), f"weights {k}: {np.count_nonzero(w_gpu != w_orc)} entries differ"
 
 
 def test_multi_networks_match_their_plain_runs(tmp_path):
@@ -71,6 +85,38 @@ def test_multi_uneven_workload_networks(tmp_path):
         p.run(100)
         _check_same(multi.network(k), p, f"uneven network {k} (N_E {cfgs[k]['num_excitatory_neurons']})")
         p.close()
+    multi.close()
+
+
+def test_multi_uneven_networks_match_the_oracle(tmp_path):
+    # One network of each size class of the uneven plastic workload (N_E 400, 900, 1,600,
+    # 2,500, their real connectivity and seeds) stepped inside the batch for 600 steps and
+    # checked directly against the FP32 lazy-normalisation oracle on that network's config and
+    # GEMV row blocks: state every 50 steps, then traces and all four weight matrices, bit for bit.
+    wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+    all48 = wl.uneven_plastic_configs(count=48, steps=300)
+    picks = {}
+    for k, c in enumerate(all48):
+        picks.setdefault(c["num_excitatory_neurons"], k)
+    order = [picks[n] for n in sorted(picks)]
+    assert [all48[k]["num_excitatory_neurons"] for k in order] == [400, 900, 1600, 2500]
+    paths = [_write(tmp_path, all48[k], f"o{k}.json") for k in order]
+    multi = engine().MultiSimulation(paths, flags=("lazy_normalize",))
+    orcs = [oracle_for(p, multi.network(k), lazy=True) for k, p in enumerate(paths)]
+    for chunk in range(12):  # the networks start firing between steps 330 and 410
+        multi.run(50)
+        for k, orc in enumerate(orcs):
+            orc.step(50)
+            assert_state_equal(multi.network(k).state(), orc.state(), 50 * chunk + 49)
+    fired = 0
+    for k, orc in enumerate(orcs):
+        net = multi.network(k)
+        print(f"network {order[k]} (N_E {all48[order[k]]['num_excitatory_neurons']}): row blocks "
+              f"{list(net.get_info().gemv_row_block)}", flush=True)
+        _check_traces_and_weights(net, orc)
+        fired += int(net.plasticity_traces(0)[0].sum() > 0)
+        orc.close()
+    assert fired == 4, "every network's E neurons should have fired within 600 steps"
     multi.close()
 
 

@@ -164,6 +164,43 @@ def test_gemv_tilings_bit_exact(tmp_path, row_blocks, nontemporal, parts):
     sim.close()
 
 
+@pytest.mark.parametrize("form", ["graph", "eager", "component"])
+def test_pair_stdp_bit_exact(tmp_path, form):
+    # E->E under the reference's pair-STDP component (e_e_plasticity_rule "pair") with iSTDP on
+    # I->E: the overlapped graphs, the eager fused step and the component path
+    # (bn_stdp_plasticity_update) against the oracle's pair rule, with a burst so that spiking
+    # rows and columns meet.
+    cfg = small_config(n_e_side=24, n_i_side=12, steps=600)
+    cfg["e_e_plasticity_rule"] = "pair"
+    cfg["e_e_plasticity_ltp_A"] = 3e-12
+    cfg["e_e_plasticity_ltd_A"] = 2e-12
+    cfg["stimulus_selected_input_ids"] = list(range(0, 576, 3))
+    cfg["stimulus_selected_input_post_group_offsets"] = [0] * 192
+    cfg["stimulus_start_clock_steps"] = [30]
+    cfg["stimulus_end_clock_steps"] = [45]
+    cfg["stimulus_input_values"] = [4e-9]
+    path = _write(tmp_path, cfg)
+    flags = {"graph": ("quiet",), "eager": ("no_monitors", "quiet", "no_graph"),
+             "component": ("no_monitors", "quiet", "component_path")}[form]
+    sim = engine().Simulation(path, flags=flags)
+    assert sim.info.num_plasticities == 2
+    orc = oracle_for(path, sim)
+    w0 = sim.weights(0)
+    if form == "graph":
+        sim.set_steps_per_graph(20)
+        sim.run(1)
+        orc.step(1)
+        for k in range(12):
+            sim.run(20)
+            orc.step(20)
+            assert_state_equal(sim.state(), orc.state(), 20 * k + 20)
+    else:
+        _run_lockstep(sim, orc, 200, check_every=5, component=form == "component")
+    _check_traces_and_weights(sim, orc)
+    assert np.count_nonzero(sim.weights(0) != w0) > 1000, "the burst should move many E->E weights"
+    sim.close()
+
+
 def test_component_path_bit_exact(tmp_path):
     path = _write(tmp_path, small_config(steps=600))
     sim = engine().Simulation(path, flags=("no_monitors", "quiet", "component_path"))

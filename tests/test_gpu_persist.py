@@ -70,7 +70,7 @@ def _persistent(path, lazy=True, resident=True):
 
 
 @pytest.mark.parametrize("resident", [True, False], ids=["resident", "streamed"])
-@pytest.mark.parametrize("which", ["plastic", "burst", "no-plasticity", "istdp-only"])
+@pytest.mark.parametrize("which", ["plastic", "burst", "no-plasticity", "istdp-only", "pair-stdp"])
 def test_persistent_small_bit_exact(tmp_path, which, resident):
     cfg = small_config(steps=800, plastic=which != "no-plasticity")
     if which == "burst":
@@ -81,6 +81,8 @@ def test_persistent_small_bit_exact(tmp_path, which, resident):
         cfg["stimulus_input_values"] = [4e-9, 3e-9]
     if which == "istdp-only":
         cfg["e_e_plasticity_enabled"] = False
+    if which == "pair-stdp":
+        cfg["e_e_plasticity_rule"] = "pair"  # no normalised matrix
     path = _write(tmp_path, cfg)
     sim = _persistent(path, resident=resident)
     if resident:

@@ -349,6 +349,62 @@ def test_triplet_ltd_and_triplet_terms_hand_computed(lazy, tile):
     assert np.isclose(pre[9], dpre ** 30, rtol=1e-5, atol=0) and np.isclose(post[9], dpost ** 30, rtol=1e-5, atol=0)
 
 
+def test_pair_stdp_hand_computed():
+    # The reference's pair-STDP component (kernel_stdp_plasticity_update.cu:16-137) on E->E:
+    # E neuron 3 spikes at steps 1 and 41, E neuron 9 at step 11.  Pre and post traces have
+    # different time constants and ltp_A != ltd_A, so swapping the traces, the rates or the
+    # roles of row and column moves a delta by 10-80%.
+    #   step 11: LTP of column 9: w[3][9] += ltp_A * pre_3,  pre_3  = dpre^10
+    #            LTD of row 9:    w[9][3] += -ltd_A * post_3, post_3 = dpost^10
+    #   step 41: LTD of row 3:    w[3][9] += -ltd_A * post_9, post_9 = dpost^30
+    #            LTP of column 3: w[9][3] += ltp_A * pre_9,  pre_9  = dpre^30
+    # w[3][9] starts just below weight_max (the step-11 LTP clamps to it) and w[9][3] just above
+    # weight_min = 0 (the step-11 LTD clamps to it); every other entry stays bit-identical (no
+    # triplet terms, no normalisation, the diagonal skipped by index).
+    cfg = _quiet_config()
+    cfg["e_e_plasticity_enabled"] = True
+    cfg.update({"e_e_plasticity_rule": "pair", "e_e_plasticity_ltp_A": 1e-12, "e_e_plasticity_ltd_A": 3e-12,
+                "e_e_plasticity_ltp_B": 5e-12, "e_e_plasticity_ltd_B": 7e-12,  # unused by the pair rule
+                "e_e_plasticity_tau_ltp": 0.02, "e_e_plasticity_tau_ltd": 0.005,
+                "e_e_plasticity_weight_min": 0.0, "e_e_plasticity_weight_max": 1.0005e-9})
+    ne, ni = 16, 4
+    rng = np.random.default_rng(8)
+    w_ee = rng.uniform(0.5e-9, 1e-9, (ne, ne)).astype(np.float32)
+    np.fill_diagonal(w_ee, 0)
+    w_ee[3, 9] = 1e-9
+    w_ee[9, 3] = 1e-13
+    ws = [w_ee, np.zeros((ne, ni), np.float32), np.zeros((ni, ne), np.float32), np.zeros((ni, ni), np.float32)]
+    p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8))
+    assert p.e_e_pair_stdp == 1
+    sim = O.OracleSim(p, ws, stim=_stim(cfg, [(3, 0), (9, 10), (3, 40)]))
+    dpre, dpost = _decay(cfg, "e_e_plasticity_tau_ltp"), _decay(cfg, "e_e_plasticity_tau_ltd")
+    wmax = float(np.float32(1.0005e-9))
+    sim.step(11)
+    b1 = sim.weights(0)
+    sim.step(1)
+    assert list(np.nonzero(sim.state()["spikes"])[0]) == [9]
+    a1 = sim.weights(0)
+    assert a1[3, 9] == np.float32(wmax)  # 1e-9 + ltp_A * dpre^10 clamped to weight_max
+    assert a1[9, 3] == 0.0  # 1e-13 - ltd_A * dpost^10 clamped to weight_min
+    assert 1e-9 + 1e-12 * dpre ** 10 > wmax and 1e-13 - 3e-12 * dpost ** 10 < 0
+    mask = np.ones_like(a1, bool)
+    mask[3, 9] = mask[9, 3] = False
+    assert np.array_equal(a1[mask], b1[mask])
+    sim.step(29)
+    b2 = sim.weights(0)
+    sim.step(1)
+    assert list(np.nonzero(sim.state()["spikes"])[0]) == [3]
+    a2 = sim.weights(0).astype(np.float64)
+    assert np.isclose(a2[3, 9] - wmax, -3e-12 * dpost ** 30, rtol=2e-4, atol=0)
+    assert np.isclose(a2[9, 3], 1e-12 * dpre ** 30, rtol=1e-6, atol=0)
+    assert np.array_equal(a2[mask], b2[mask].astype(np.float64))
+    # Trace increments after the updates; no triplet traces.
+    pre, post, tpre, tpost = sim.plasticity_traces(0)
+    assert np.isclose(pre[3], dpre ** 40 + 1.0, rtol=1e-5, atol=0) and np.isclose(post[3], dpost ** 40 + 1.0, rtol=1e-5, atol=0)
+    assert np.isclose(pre[9], dpre ** 30, rtol=1e-5, atol=0) and np.isclose(post[9], dpost ** 30, rtol=1e-5, atol=0)
+    assert not np.any(tpre) and not np.any(tpost)
+
+
 def test_eif_spike_cap_and_refractory_sequence():
     # kernel_neuron_group_evolve.cu:92-118.  A one-step 1 uA pulse seen at step 1 lifts V by about
     # 0.5 V: the step must cap V at v_threshold (not V + dv), set refractory = R and spike = 1;
