@@ -209,6 +209,10 @@ void build(bn_sim* sim, const std::string& config_path) {
         if (component) throw Error(BN_ERR_INVALID_ARGUMENT, "the persistent step is a fused step (not the component path)");
         sys.enable_persistent({0, 0, 0, 0});
     }
+    if (flags & BN_SIM_RESIDENT) {
+        if (component) throw Error(BN_ERR_INVALID_ARGUMENT, "the resident step is a fused step (not the component path)");
+        sys.enable_resident();
+    }
     BN_HIP_CHECK(hipMalloc(&sim->d_tmp_ge, S.total_neurons() * sizeof(float)), "hipMalloc g temp");
     BN_HIP_CHECK(hipMalloc(&sim->d_tmp_gi, S.total_neurons() * sizeof(float)), "hipMalloc g temp");
     BN_HIP_CHECK(hipStreamSynchronize(S.stream()), "build sync");
@@ -348,6 +352,13 @@ int bn_sim_get_info(bn_sim* sim, bn_sim_info* info) {
     info->persistent_blocks = sim->system->persistent_blocks();
     info->persistent_items = sim->system->persistent_items();
     info->persistent_resident_rows = sim->system->persistent_resident_rows();
+    if (sim->system->resident()) {
+        const RPlan& R = sim->system->resident_plan();
+        info->resident_blocks = R.blocks;
+        info->resident_columns = kRCols;
+        info->resident_e_waves = R.e_waves;
+        info->resident_rows = R.rows;
+    }
     info->gemv_part_sums = sim->system->gemv_part_sums() ? 1u : 0u;
     if (sim->system->persistent()) {
         info->gemv_part_chunks = kPWaves;
@@ -581,8 +592,8 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
     return guarded([&] {
         System& sys = *sim->system;
         if (!sys.fused()) throw Error(BN_ERR_RUNTIME, "kernel timing needs the fused step");
-        if (sys.persistent())
-            throw Error(BN_ERR_RUNTIME, "kernel timing: the persistent step is one launch per run (time bn_sim_run_timed)");
+        if (sys.persistent() || sys.resident())
+            throw Error(BN_ERR_RUNTIME, "kernel timing: this step is one launch per run (time bn_sim_run_timed)");
         hipStream_t s = sim->state->stream();
         hipEvent_t ev[4];
         for (auto& e : ev) BN_HIP_CHECK(hipEventCreate(&e), "hipEventCreate");

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../bn_persist.h"
+#include "../bn_resident.h"
 #include "../bn_step.h"
 #include "buryn_amd.h"
 #include "generation.h"
@@ -456,6 +457,12 @@ public:
     // Polls a block of the persistent step makes before it abandons a wait (and the launch).
     void set_persistent_wait_limit(uint32_t polls);
 
+    // Resident step: one cooperative launch per run, the weights in registers for the whole run
+    // (bn_resident.h).  Builds the plan; the state is untouched.
+    void enable_resident();
+    bool resident() const { return resident_; }
+    const RPlan& resident_plan() const { return resident_plan_; }
+
     uint32_t clock_step() const { return clock_step_; }
     bool quiet = false;
     const std::vector<std::unique_ptr<NeuronGroup>>& neuron_groups() const { return neuron_groups_; }
@@ -468,6 +475,7 @@ private:
     void build_overlap_plans();
     void build_event_starts();
     void launch_persistent_steps(uint32_t n);
+    void launch_resident_steps(uint32_t n);
     void* persist_alloc(size_t bytes);
     void enqueue_fused_step(hipEvent_t gemv_begin, hipEvent_t gemv_end);
     void enqueue_overlapped_steps(uint32_t n);
@@ -521,6 +529,12 @@ private:
     std::vector<void*> persist_bufs_;
     std::vector<std::pair<void*, size_t>> persist_ll_bufs_;  // tagged-word buffers (64-bit words)
     uint32_t persist_ll_next_ = 0;                          // the clock after the last resident launch
+    // Resident step
+    bool resident_ = false;
+    RPlan resident_plan_{};
+    RPlan* d_resident_plan_ = nullptr;
+    std::vector<std::pair<void*, size_t>> resident_bufs_;  // tagged-word buffers (64-bit words)
+    uint32_t resident_next_ = 0;                           // the clock after the last launch
 };
 
 }  // namespace bn

@@ -409,11 +409,16 @@ void System::enable_persistent(const std::array<uint32_t, 4>& rows) {
 }
 
 void System::set_persistent_wait_limit(uint32_t polls) {
-    if (!persist_) throw Error(BN_ERR_RUNTIME, "not a persistent simulation");
+    if (!persist_ && !resident_) throw Error(BN_ERR_RUNTIME, "not a persistent or resident simulation");
     BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "persistent plan sync");
     persist_wait_limit_ = polls;
-    persist_plan_.wait_limit = polls;
-    BN_HIP_CHECK(hipMemcpy(d_persist_plan_, &persist_plan_, sizeof(PPlan), hipMemcpyHostToDevice), "persistent plan");
+    if (persist_) {
+        persist_plan_.wait_limit = polls;
+        BN_HIP_CHECK(hipMemcpy(d_persist_plan_, &persist_plan_, sizeof(PPlan), hipMemcpyHostToDevice), "persistent plan");
+    } else {
+        resident_plan_.wait_limit = polls;
+        BN_HIP_CHECK(hipMemcpy(d_resident_plan_, &resident_plan_, sizeof(RPlan), hipMemcpyHostToDevice), "resident plan");
+    }
 }
 
 // Runs of at most kPersistMaxSteps steps per launch; before each launch the counters, the spike

@@ -446,7 +446,7 @@ void System::capture_graph() {
 void System::set_steps_per_graph(uint32_t steps_per_graph) {
     if (!fused_) throw Error(BN_ERR_RUNTIME, "steps per graph: the component path has no graph");
     if (steps_per_graph == 0) throw Error(BN_ERR_INVALID_ARGUMENT, "steps per graph must be at least 1");
-    if (persist_) {  // the persistent step launches whole runs, no graphs
+    if (persist_ || resident_) {  // these steps launch whole runs, no graphs
         steps_per_graph_ = steps_per_graph;
         return;
     }
@@ -458,6 +458,10 @@ void System::set_steps_per_graph(uint32_t steps_per_graph) {
 void System::launch_fused_steps(uint32_t n) {
     if (persist_) {
         launch_persistent_steps(n);
+        return;
+    }
+    if (resident_) {
+        launch_resident_steps(n);
         return;
     }
     hipStream_t s = state_.stream();
@@ -478,6 +482,19 @@ void System::launch_fused_step_eager(hipEvent_t gemv_begin, hipEvent_t gemv_end)
 
 void System::materialize_g(float* d_ge_out, float* d_gi_out) {
     hipStream_t s = state_.stream();
+    if (resident_) {
+        // The resident step stores g after the propagate itself: no pending partials.
+        for (uint32_t g = 0; g < 2; ++g) {
+            const NeuronGroupView& v = resident_plan_.g[g];
+            BN_HIP_CHECK((hipError_t)launch_materialize_g(d_ge_out + v.offset, v.g_excitatory, nullptr, 0,
+                                                          v.num_neurons, s),
+                         "materialize g_e (resident)");
+            BN_HIP_CHECK((hipError_t)launch_materialize_g(d_gi_out + v.offset, v.g_inhibitory, nullptr, 0,
+                                                          v.num_neurons, s),
+                         "materialize g_i (resident)");
+        }
+        return;
+    }
     if (persist_) {
         // Pending segment partials of the last persistent step: g + P_0 + P_1 + ... in order.
         const PPlan& P = persist_plan_;

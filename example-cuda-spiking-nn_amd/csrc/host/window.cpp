@@ -502,9 +502,9 @@ extern "C" {
 int bn_window_create(bn_sim* sim, const bn_window_spec* spec, bn_window** out) {
     if (!sim || !spec || !out) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_window_create: null argument");
     *out = nullptr;
-    if (sim->system && sim->system->persistent())
+    if (sim->system && (sim->system->persistent() || sim->system->resident()))
         return set_error(BN_ERR_INVALID_ARGUMENT,
-                         "bn_window_create: the window starts from the graph step's state (not BN_SIM_PERSISTENT)");
+                         "bn_window_create: the window starts from the graph step's state (not BN_SIM_PERSISTENT or BN_SIM_RESIDENT)");
     auto w = std::make_unique<bn_window>();
     const int rc = guarded([&] { build_window(w.get(), sim, spec); });
     if (rc) return rc;

@@ -1,0 +1,738 @@
+// The resident step (bn_resident.h): one cooperative launch runs n steps of a network whose
+// weights stay in the registers of the co-resident blocks for the whole launch.
+//
+// Per step a, every block (owning C post columns of one group, all pre rows of them):
+//   poll      the spike words of step a of every block (one wave) and, for the rows of the
+//             lazily normalised matrix, their scales of step a (the rows' threads);
+//   rows      each thread advances its rows' connection traces and plasticity pre traces from
+//             the spike bits (kernel_connection_propagate.cu:8-46, the trace step of
+//             kernel_triplet_plasticity_update.cu:16-50 with the previous step's +1 first);
+//   propagate FP64 column sums of x_i (s_i) W_ij over the thread's rows, the lane butterfly,
+//             the waves in order: g of the block's neurons (kernel_connection_propagate.cu:79-94);
+//   plastic   LTP of the block's spiking columns and LTD of the spiking rows on the entries in
+//             registers, LTP then LTD per entry (kernel_triplet_plasticity_update.cu:54-93,
+//             kernel_istdp_plasticity_update.cu:42-80, kernel_stdp_plasticity_update.cu);
+//             the lazy matrix's row-sum parts go to the rows' owners;
+//   evolve    the block's neurons for step a + 1 (kernel_neuron_group_evolve.cu:11-130, the
+//             stimulus events of step a first, stimulus.cpp:46-60) and their spike word;
+//   owner     for its own rows of the lazy matrix: the parts of the row sums, R_i, s_i
+//             (kernel_triplet_plasticity_update.cu:117-145 kept as W = diag(s) M), the scales
+//             of step a + 1.
+// Before the first step the block loads its entries, its rows' traces and its neurons' state
+// and runs the evolve of the launch's first step; after the last it stores them back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "bn_device.h"
+#include "bn_errors.h"
+#include "bn_neuron.h"
+#include "bn_persist_dev.h"
+#include "bn_resident.h"
+
+namespace bn {
+
+namespace {
+
+__device__ __forceinline__ uint32_t r_tag(uint32_t a) { return a + 1u; }  // 0: never written
+__device__ __forceinline__ uint32_t r_slot(uint32_t a) { return a % 3u; }
+__device__ __forceinline__ uint64_t r_word(uint32_t a, uint32_t bits) { return ((uint64_t)r_tag(a) << 32) | bits; }
+
+template <int K>
+struct RShared {
+    // The block's neurons (its post columns).
+    float V[kRCols], istim[kRCols], re[kRCols], ri[kRCols], de[kRCols], di[kRCols];
+    float bge[kRCols], bgi[kRCols], ge[kRCols], gi[kRCols];
+    uint32_t refr[kRCols], spike[kRCols];
+    float post[3][kRCols];             // post traces at the step: E->E fast, E->E slow, I->E
+    int32_t vrec[kRCols];              // slot in the membrane-potential ring, or -1
+    double R[kRCols];                  // lazy E->E: the block's rows' sums, scales, targets
+    float s[kRCols], target[kRCols];
+    uint32_t heads[8];                 // first four entries of the group's Poisson tables (e, i)
+    uint32_t abort;
+    uint32_t ncontrib;
+    // The threads' rows, [row][thread].
+    float r[K][kRThreads], d[K][kRThreads], pre[K][kRThreads], tpre[K][kRThreads], sc[K][kRThreads];
+    float xs[K][kRThreads];            // the rows' x_i (s_i) of the step, FP32
+    uint32_t spk[kRMaxBlocks];         // spike words of the step, block order
+    uint32_t contrib[kRMaxBlocks];     // E-post blocks with a spiking column (owner's polls)
+    double red[kRWaves][kRCols];       // the waves' column sums
+    double part[kRCols][kRMaxBlocks];  // owner: the polled row-sum parts, [row][rank]
+};
+
+// Polls up to M words per thread (bit m of `want`: the word at addr(m)) until each carries the
+// tag, handing its low half to sink(m, bits).  Every thread of the block calls it; false for
+// every thread when a wait ran past its limit here or in another block.
+template <int M, int K, typename Addr, typename Sink>
+__device__ __forceinline__ bool r_poll(const RPlan& P, RShared<K>& S, uint32_t want, uint32_t tag, Addr addr,
+                                       Sink sink) {
+    uint32_t pending = want;
+    for (uint32_t it = 0; pending; ++it) {
+        uint64_t v[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = (pending >> m) & 1u ? ld_c(addr(m)) : 0ull;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (((pending >> m) & 1u) && (uint32_t)(v[m] >> 32) == tag) {
+                sink(m, (uint32_t)v[m]);
+                pending &= ~(1u << m);
+            }
+        if (!pending) break;
+        if (it >= P.wait_limit) {
+            max_c(P.status, 2u);
+            S.abort = 1u;
+            break;
+        }
+        if ((it & 15u) == 15u && ld_c(P.status) >= 2u) {
+            S.abort = 1u;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __syncthreads();
+    return S.abort == 0u;
+}
+
+// Spike bit of global neuron gid in the step's block words.
+template <int K>
+__device__ __forceinline__ uint32_t r_spiked(const RPlan& P, const RShared<K>& S, uint32_t gid) {
+    const uint32_t lid = gid < P.ne ? gid : gid - P.ne;
+    const uint32_t blk = (gid < P.ne ? 0u : P.be) + lid / kRCols;
+    return (S.spk[blk] >> (lid % kRCols)) & 1u;
+}
+
+// Word wd of the step's spike bits in global ids (bit g & 31 of word g >> 5), from the block
+// words: an E block word holds 16 aligned neurons, an I block word the 16 from ne + 16 * (its
+// I index), which need not be aligned to the words.
+template <int K>
+__device__ __forceinline__ uint32_t r_global_word(const RPlan& P, const RShared<K>& S, uint32_t wd) {
+    const uint32_t lo = 32u * wd, hi = min(lo + 32u, P.n);
+    uint32_t word = 0;
+    if (lo < P.ne) {
+        word = S.spk[lo / (uint32_t)kRCols];
+        if (lo + (uint32_t)kRCols < P.ne) word |= S.spk[lo / (uint32_t)kRCols + 1u] << kRCols;
+    }
+    const uint32_t ilo = max(lo, P.ne);
+    if (ilo < hi)
+        for (uint32_t l = (ilo - P.ne) & ~(uint32_t)(kRCols - 1); l < hi - P.ne; l += (uint32_t)kRCols) {
+            const uint32_t bw = S.spk[P.be + l / (uint32_t)kRCols];
+            const int sh = (int)(P.ne + l) - (int)lo;   // bit of I neuron l in the word (-15 .. 31)
+            word |= sh >= 0 ? bw << sh : bw >> -sh;
+        }
+    return word;
+}
+
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ uint32_t r_nth_bit(uint32_t m, uint32_t k) {
+    uint32_t pos = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < (uint32_t)kRCols; ++b) {
+        const bool set = (m >> b) & 1u;
+        if (set && k == 0u) pos = b;
+        if (set) --k;
+    }
+    return pos;
+}
+
+// ---------------------------------------------------------------------------------------
+// Evolve of the block's neurons for step a: threads [0, ncols).  Events [e0, e1) are the
+// stimulus events of step a - 1 (none for the launch's first step: those are in the state).
+// Returns the thread's spike; the spike word of the block is published by the caller.
+// ---------------------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ uint32_t r_evolve(const RPlan& P, RShared<K>& S, const NeuronGroupView& v, bool epost,
+                                             uint32_t j0, uint32_t a, uint32_t e0, uint32_t e1,
+                                             const bn_device_clock& clk) {
+    const uint32_t c = threadIdx.x;
+    const uint32_t gid = v.offset + j0 + c;
+    float istim = S.istim[c];
+    // stimulus.cpp:46-60 for this neuron: the events in list order.
+    for (uint32_t e = e0; e < e1; ++e)
+        if (P.ev_ids[e] + P.ev_offsets[e] == gid) istim += P.ev_amps[e];
+    // Background (kernel_neuron_group_evolve.cu:11-83).
+    uint32_t he[4], hi[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+        he[q] = S.heads[q];
+        hi[q] = S.heads[4 + q];
+    }
+    float re = flush_trace(S.re[c] * v.background_excitatory_postsynaptic_r_trace_decay);
+    float ri = flush_trace(S.ri[c] * v.background_inhibitory_postsynaptic_r_trace_decay);
+    float de = flush_trace(S.de[c] * v.background_excitatory_postsynaptic_d_trace_decay);
+    float di = flush_trace(S.di[c] * v.background_inhibitory_postsynaptic_d_trace_decay);
+    uint32_t ke = 0, ki = 0;
+    if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
+        const U32x4 r = background_draws(clk, a, gid, v.rng_instance);
+        if (v.mean_background_excitatory_spikes > 0.0f)
+            ke = poisson_lookup(r.x, v.poisson_table_excitatory, v.poisson_table_excitatory_len, he);
+        if (v.mean_background_inhibitory_spikes > 0.0f)
+            ki = poisson_lookup(r.y, v.poisson_table_inhibitory, v.poisson_table_inhibitory_len, hi);
+    }
+    if (ke > 0u) {
+        const float inc = (float)ke;
+        re += inc;
+        de += inc;
+    }
+    const float bg_ge = (de - re) * v.background_excitatory_scaling_factor;
+    if (ki > 0u) {
+        const float inc = (float)ki;
+        ri += inc;
+        di += inc;
+    }
+    const float bg_gi = (di - ri) * v.background_inhibitory_scaling_factor;
+    // EIF (kernel_neuron_group_evolve.cu:85-130) on the g of the last propagate.
+    float V = S.V[c];
+    uint32_t refr = S.refr[c], spike = 0;
+    eif_step(v, S.ge[c], S.gi[c], istim, V, refr, spike);
+    // Post traces of the plasticities (E neurons): the previous step's +1, then the decay.
+    if (epost) {
+        const float inc = S.spike[c] == 1u ? 1.0f : 0.0f;
+        if (P.ee_kind != kRNone) {
+            S.post[0][c] = flush_trace((S.spike[c] == 1u ? S.post[0][c] + inc : S.post[0][c]) * P.ee_dec[2]);
+            if (P.ee_kind == kPlastTriplet)
+                S.post[1][c] = flush_trace((S.spike[c] == 1u ? S.post[1][c] + inc : S.post[1][c]) * P.ee_dec[3]);
+        }
+        if (P.ie_kind != kRNone)
+            S.post[2][c] = flush_trace((S.spike[c] == 1u ? S.post[2][c] + inc : S.post[2][c]) * P.ie_dec[1]);
+    }
+    S.V[c] = V;
+    S.refr[c] = refr;
+    S.spike[c] = spike;
+    S.istim[c] = istim;
+    S.re[c] = re;
+    S.ri[c] = ri;
+    S.de[c] = de;
+    S.di[c] = di;
+    S.bge[c] = bg_ge;
+    S.bgi[c] = bg_gi;
+    if (P.v_ring && S.vrec[c] >= 0) gp(P.v_ring)[(size_t)(a % P.interval) * P.n_vrec + (uint32_t)S.vrec[c]] = V;
+    return spike;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// The launch.  K: rows per thread (the plan's `rows`).
+// ---------------------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict__ plan, uint32_t n) {
+    __shared__ RShared<K> S;
+    const RPlan& P = *plan;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const uint32_t b = blockIdx.x;
+    const bool epost = b < P.be;
+    const uint32_t j0 = (epost ? b : b - P.be) * (uint32_t)kRCols;   // first post-local column
+    const uint32_t npost = epost ? P.ne : P.ni;
+    const uint32_t ncols = min((uint32_t)kRCols, npost - j0);
+    const NeuronGroupView& G = P.g[epost ? 0 : 1];
+    // The thread's rows: pre group `side`, rows tl, tl + T, ...
+    const bool eside = wave < P.e_waves;
+    const uint32_t side = eside ? 0u : 1u;
+    const uint32_t T = 64u * (eside ? P.e_waves : (uint32_t)kRWaves - P.e_waves);
+    const uint32_t tl = eside ? tid : tid - 64u * P.e_waves;
+    const uint32_t npre = eside ? P.ne : P.ni;
+    const uint32_t pre_base = eside ? 0u : P.ne;                    // global id of pre-local 0
+    const uint32_t conn = epost ? (eside ? 0u : 2u) : (eside ? 1u : 3u);
+    const uint32_t pkind = !epost ? kRNone : (eside ? P.ee_kind : P.ie_kind);
+    const bool lazy = epost && eside && P.ee_lazy;
+    const bool triplet = pkind == kPlastTriplet;
+    const PlasticityParams& PP = eside ? P.ee_p : P.ie_p;
+    const bn_device_clock clk = *P.clock;
+    const uint32_t t0 = clk.clock_step;
+    if (tid == 0) {
+        S.abort = 0u;
+        S.ncontrib = 0u;
+    }
+    // The entries of the block's columns in the thread's rows (padding columns and rows: 0).
+    float w[K][kRCols];
+    {
+        const BN_GAS float* W = gp(P.W[conn]);
+        const uint32_t ld = P.ld[conn];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t i = tl + T * (uint32_t)k;
+            const bool ok = i < npre;
+            const BN_GAS v4f* src = reinterpret_cast<const BN_GAS v4f*>(W + (size_t)(ok ? i : 0u) * ld + j0);
+#pragma unroll
+            for (int q = 0; q < kRCols / 4; ++q) {
+                const v4f x = ok ? src[q] : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) w[k][4 * q + e] = (uint32_t)(4 * q + e) < ncols ? x[e] : 0.0f;
+            }
+        }
+    }
+    // The rows' traces and scales; bit k of sp_prev: row k's neuron spiked at the step before.
+    uint32_t sp_prev = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t i = tl + T * (uint32_t)k;
+        const bool ok = i < npre;
+        S.r[k][tid] = ok ? gp(P.r[side])[i] : 0.0f;
+        S.d[k][tid] = ok ? gp(P.d[side])[i] : 0.0f;
+        S.pre[k][tid] = ok && pkind != kRNone ? gp(eside ? P.ee_tr[0] : P.ie_tr[0])[i] : 0.0f;
+        S.tpre[k][tid] = ok && triplet ? gp(P.ee_tr[1])[i] : 0.0f;
+        S.sc[k][tid] = ok && lazy ? gp(P.s)[i] : 1.0f;
+        if (ok && gp(P.spikes)[pre_base + i] == 1u) sp_prev |= 1u << k;
+    }
+    // The block's neurons.
+    if (tid < ncols) {
+        const uint32_t lid = j0 + tid;
+        const NeuronGroupView& v = G;
+        S.V[tid] = gp(v.membrane_potentials)[lid];
+        S.refr[tid] = gp(v.refractory_counter)[lid];
+        S.istim[tid] = gp(v.i_stimulus)[lid];
+        S.re[tid] = gp(v.background_excitatory_postsynaptic_r_traces)[lid];
+        S.ri[tid] = gp(v.background_inhibitory_postsynaptic_r_traces)[lid];
+        S.de[tid] = gp(v.background_excitatory_postsynaptic_d_traces)[lid];
+        S.di[tid] = gp(v.background_inhibitory_postsynaptic_d_traces)[lid];
+        S.ge[tid] = gp(v.g_excitatory)[lid];
+        S.gi[tid] = gp(v.g_inhibitory)[lid];
+        S.spike[tid] = gp(P.spikes)[v.offset + lid];
+        S.post[0][tid] = epost && P.ee_kind != kRNone ? gp(P.ee_tr[2])[lid] : 0.0f;
+        S.post[1][tid] = epost && P.ee_kind == kPlastTriplet ? gp(P.ee_tr[3])[lid] : 0.0f;
+        S.post[2][tid] = epost && P.ie_kind != kRNone ? gp(P.ie_tr[1])[lid] : 0.0f;
+        S.vrec[tid] = P.v_ring ? gp(P.vrec_index)[v.offset + lid] : -1;
+        if (epost && P.ee_lazy) {
+            S.R[tid] = gp(P.R)[lid];
+            S.s[tid] = gp(P.s)[lid];
+            S.target[tid] = gp(P.target)[lid];
+        }
+    }
+    if (tid < 4) {
+        S.heads[tid] = tid < G.poisson_table_excitatory_len ? gp(G.poisson_table_excitatory)[tid] : 0u;
+        S.heads[4 + tid] = tid < G.poisson_table_inhibitory_len ? gp(G.poisson_table_inhibitory)[tid] : 0u;
+    }
+    // Every load of the state has returned before the block publishes anything: another block
+    // stores its part of the state back after the launch's last step, which can follow this
+    // block's first spike word closely.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // The evolve of the launch's first step.
+    if (wave == 0) {
+        const uint32_t sp = tid < ncols ? r_evolve(P, S, G, epost, j0, t0, 0u, 0u, clk) : 0u;
+        const uint32_t bits = (uint32_t)__ballot(sp == 1u);
+        if (lane == 0) st_c(P.spk + (size_t)r_slot(t0) * P.blocks + b, r_word(t0, bits));
+    }
+    uint32_t ev0 = 0, ev1 = 0;   // stimulus events of step a (applied by the evolve of step a + 1)
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t a = t0 + k;
+        if (P.ev_steps && a < P.ev_steps) {  // loaded now, used after the poll
+            ev0 = P.ev_start[a];
+            ev1 = min(P.ev_start[a + 1], P.ev_count);
+        } else {
+            ev0 = ev1 = 0;
+        }
+        // ---- poll: every block's spike word of step a; the lazy rows' scales of step a.
+        {
+            uint32_t want = 0;
+            if (wave == 0)
+                for (uint32_t m = 0; m < kRMaxBlocks / 64; ++m)
+                    if (lane + 64u * m < P.blocks) want |= 1u << m;
+            if (lazy && k > 0)
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    if (tl + T * (uint32_t)q < npre) want |= 1u << (4 + q);
+            const uint32_t sl = r_slot(a);
+            auto addr = [&](int m) -> const uint64_t* {
+                if (m < 4) return P.spk + (size_t)sl * P.blocks + lane + 64u * (uint32_t)m;
+                return P.sx + (size_t)sl * P.ne + tl + T * (uint32_t)(m - 4);
+            };
+            auto sink = [&](int m, uint32_t bits) {
+                if (m < 4) S.spk[lane + 64u * (uint32_t)m] = bits;
+                else S.sc[m - 4][tid] = __uint_as_float(bits);
+            };
+            if (!r_poll<4 + K>(P, S, want, r_tag(a), addr, sink)) return;
+        }
+        // ---- the monitors' spike ring and (last step) the state's spike bits: block b writes
+        // global word b (there are fewer words than blocks).
+        if (tid == 0 && b < P.words && (P.spike_ring || k + 1 == n)) {
+            const uint32_t word = r_global_word(P, S, b);
+            if (P.spike_ring) gp(P.spike_ring)[(size_t)(a % P.interval) * P.words + b] = word;
+            if (k + 1 == n) gp(P.bits)[b] = word;
+        }
+        // ---- rows: connection traces with the spikes of step a, plasticity pre traces (the
+        // previous step's +1 first), x_i (s_i) for the propagate.
+        uint32_t sp_now = 0;
+        {
+            const float dr = P.decay_r[side], dd = P.decay_d[side], Kn = P.K[side];
+            const float dpre = eside ? P.ee_dec[0] : P.ie_dec[0], dtpre = P.ee_dec[1];
+            float rv[K], dv[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const uint32_t i = tl + T * (uint32_t)q;
+                const uint32_t ic = i < npre ? i : 0u;
+                sp_now |= (i < npre ? r_spiked(P, S, pre_base + ic) : 0u) << q;
+                rv[q] = S.r[q][tid];
+                dv[q] = S.d[q][tid];
+            }
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const bool sp = (sp_now >> q) & 1u;
+                float d = flush_trace(dv[q] * dd);
+                float r = flush_trace(rv[q] * dr);
+                if (sp) {
+                    r += Kn;
+                    d += Kn;
+                }
+                S.d[q][tid] = d;
+                S.r[q][tid] = r;
+                const float x = d - r;
+                S.xs[q][tid] = lazy ? x * S.sc[q][tid] : x;
+            }
+            if (pkind != kRNone) {
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const bool was = (sp_prev >> q) & 1u;
+                    const float pr = S.pre[q][tid];
+                    S.pre[q][tid] = flush_trace((was ? pr + 1.0f : pr) * dpre);
+                    if (triplet) {
+                        const float tp = S.tpre[q][tid];
+                        S.tpre[q][tid] = flush_trace((was ? tp + 1.0f : tp) * dtpre);
+                    }
+                }
+            }
+        }
+        // ---- propagate: column sums over the thread's rows (FP64 from 0.0, row order), the
+        // lane butterfly (32,16,8,4,2,1) run as a reduce-scatter, the waves in order.
+        {
+            double acc[kRCols];
+#pragma unroll
+            for (int c = 0; c < kRCols; ++c) acc[c] = 0.0;
+            // (The FP64 product of two FP32 values is exact, so fma(x, w, acc) rounds as acc + x * w.
+            // Rows past the group hold x = 0 and entries 0: they add +0.)
+#pragma unroll
+            for (int q = 0; q < K; ++q) {
+                const double xq = (double)S.xs[q][tid];
+#pragma unroll
+                for (int c = 0; c < kRCols; ++c) acc[c] = fma(xq, (double)w[q][c], acc[c]);
+            }
+            // The entries count as rewritten here (an empty register constraint, no instruction):
+            // otherwise the compiler keeps their FP64 conversions from the sums above alive until
+            // the plasticity below converts the same values again, twice the entries' registers.
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+#pragma unroll
+                for (int c = 0; c < kRCols; ++c) asm volatile("" : "+v"(w[q][c]));
+            double u8[8], u4[4], u2[2];
+            const bool h32 = lane & 32u, h16 = lane & 16u, h8 = lane & 8u, h4 = lane & 4u;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const double send = h32 ? acc[m] : acc[m + 8];
+                const double keep = h32 ? acc[m + 8] : acc[m];
+                u8[m] = keep + __shfl_xor(send, 32, 64);
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const double send = h16 ? u8[m] : u8[m + 4];
+                const double keep = h16 ? u8[m + 4] : u8[m];
+                u4[m] = keep + __shfl_xor(send, 16, 64);
+            }
+#pragma unroll
+            for (int m = 0; m < 2; ++m) {
+                const double send = h8 ? u4[m] : u4[m + 2];
+                const double keep = h8 ? u4[m + 2] : u4[m];
+                u2[m] = keep + __shfl_xor(send, 8, 64);
+            }
+            double u1;
+            {
+                const double send = h4 ? u2[0] : u2[1];
+                const double keep = h4 ? u2[1] : u2[0];
+                u1 = keep + __shfl_xor(send, 4, 64);
+            }
+            u1 = u1 + __shfl_xor(u1, 2, 64);
+            u1 = u1 + __shfl_xor(u1, 1, 64);
+            if ((lane & 3u) == 0u) {
+                const uint32_t col = (h32 ? 8u : 0u) + (h16 ? 4u : 0u) + (h8 ? 2u : 0u) + (h4 ? 1u : 0u);
+                S.red[wave][col] = u1;
+            }
+        }
+        __syncthreads();
+        if (tid < ncols) {
+            double ge = (double)S.bge[tid];
+            for (uint32_t q = 0; q < P.e_waves; ++q) ge += S.red[q][tid];
+            double gi = (double)S.bgi[tid];
+            for (uint32_t q = P.e_waves; q < (uint32_t)kRWaves; ++q) gi += S.red[q][tid];
+            S.ge[tid] = (float)ge;
+            S.gi[tid] = (float)gi;
+        }
+        // ---- plasticity of step a on the entries in registers: LTP of the block's spiking
+        // columns, then LTD of the thread's spiking rows, each pulled out of the register array
+        // by a select chain (a loop over columns or rows instead of every entry unrolled keeps
+        // the loop's code within the instruction cache).
+        const uint32_t own = __builtin_amdgcn_readfirstlane(S.spk[b]);   // the block's spiking columns
+        if (pkind != kRNone && (own != 0u || sp_now != 0u)) {
+            const float wmin = PP.weight_min, wmax = PP.weight_max;
+            double change[K];
+#pragma unroll
+            for (int q = 0; q < K; ++q) change[q] = 0.0;
+            for (uint32_t cm = own; cm; cm &= cm - 1u) {
+                const uint32_t c = (uint32_t)__builtin_ctz(cm);   // uniform
+                const uint32_t j = j0 + c;
+                const float fac = triplet ? PP.ltp_A + (PP.ltp_B * S.post[1][c]) : PP.ltp_A;
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const uint32_t i = tl + T * (uint32_t)q;
+                    const float si = lazy ? S.sc[q][tid] : 1.0f;
+                    float m = w[q][0];
+#pragma unroll
+                    for (int cc = 1; cc < kRCols; ++cc) m = c == (uint32_t)cc ? w[q][cc] : m;
+                    float val = m;
+                    if (i < npre && j != i && (!lazy || si != 0.0f)) {
+                        const float pre_i = S.pre[q][tid];
+                        const float delta = triplet ? pre_i * fac : PP.ltp_A * pre_i;
+                        val = p_apply(lazy, m, si, delta, wmin, wmax);
+                    }
+#pragma unroll
+                    for (int cc = 0; cc < kRCols; ++cc) w[q][cc] = c == (uint32_t)cc ? val : w[q][cc];
+                    change[q] += (double)val - (double)m;
+                }
+            }
+            // The lazy matrix's row-sum parts of the rows whose neuron did not spike (their LTP
+            // changes, when a column spiked); a spiking row's part follows its LTD below.
+            if (lazy && own != 0u) {
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    const uint32_t i = tl + T * (uint32_t)q;
+                    if (i >= npre || ((sp_now >> q) & 1u)) continue;
+                    const uint64_t bits = (uint64_t)__double_as_longlong(change[q]);
+                    uint64_t* dst = P.dr + ((size_t)r_slot(a) * P.be + b) * 2u * P.ne + i;
+                    st_c(dst, r_word(a, (uint32_t)bits));
+                    st_c(dst + P.ne, r_word(a, (uint32_t)(bits >> 32)));
+                }
+            }
+            const float* post_j = eside ? S.post[0] : S.post[2];
+            for (uint32_t rm = sp_now; rm; rm &= rm - 1u) {
+                const uint32_t qs = (uint32_t)__builtin_ctz(rm);
+                const uint32_t i = tl + T * qs;
+                const float si = lazy ? S.sc[qs][tid] : 1.0f;
+                float row[kRCols];
+#pragma unroll
+                for (int c = 0; c < kRCols; ++c) {
+                    row[c] = w[0][c];
+#pragma unroll
+                    for (int q = 1; q < K; ++q) row[c] = qs == (uint32_t)q ? w[q][c] : row[c];
+                }
+                if (!lazy || si != 0.0f) {
+                    const float fac_i = triplet ? PP.ltd_A + (PP.ltd_B * S.tpre[qs][tid]) : 0.0f;
+#pragma unroll
+                    for (int c = 0; c < kRCols; ++c) {
+                        const uint32_t j = j0 + (uint32_t)c;
+                        if ((uint32_t)c >= ncols || j == i) continue;
+                        float dl;
+                        if (triplet) dl = -1.0f * post_j[c] * fac_i;
+                        else if (pkind == kPlastIstdp) dl = -1.0f * PP.ltd_A * (post_j[c] - PP.ltd_B);
+                        else dl = -1.0f * PP.ltd_A * post_j[c];
+                        row[c] = p_apply(lazy, row[c], si, dl, wmin, wmax);
+                    }
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
+#pragma unroll
+                        for (int c = 0; c < kRCols; ++c) w[q][c] = qs == (uint32_t)q ? row[c] : w[q][c];
+                }
+                if (lazy) {
+                    double part = 0.0;
+#pragma unroll
+                    for (int c = 0; c < kRCols; ++c)
+                        if ((uint32_t)c < ncols) part += (double)row[c];
+                    const uint64_t bits = (uint64_t)__double_as_longlong(part);
+                    uint64_t* dst = P.dr + ((size_t)r_slot(a) * P.be + b) * 2u * P.ne + i;
+                    st_c(dst, r_word(a, (uint32_t)bits));
+                    st_c(dst + P.ne, r_word(a, (uint32_t)(bits >> 32)));
+                }
+            }
+        }
+        // ---- evolve of step a + 1 (with the events of step a) and its spike word.
+        __syncthreads();
+        if (k + 1 < n && wave == 0) {
+            const uint32_t sp = tid < ncols ? r_evolve(P, S, G, epost, j0, a + 1, ev0, ev1, clk) : 0u;
+            const uint32_t bits = (uint32_t)__ballot(sp == 1u);
+            if (lane == 0) st_c(P.spk + (size_t)r_slot(a + 1) * P.blocks + b, r_word(a + 1, bits));
+        }
+        // ---- owner: the row sums of the block's rows of the lazy matrix after step a's
+        // plasticity, their scales for step a + 1.
+        if (epost && P.ee_lazy) {
+            // Contributors: the E-post blocks with a spiking column, in block order.
+            if (wave == 0) {
+                uint32_t cnt = 0;
+                for (uint32_t m = 0; m < kRMaxBlocks / 64; ++m) {
+                    const uint32_t q = lane + 64u * m;
+                    const bool is = q < P.be && S.spk[q] != 0u;
+                    const uint64_t bal = __ballot(is);
+                    if (is) S.contrib[cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = q;
+                    cnt += (uint32_t)__popcll(bal);
+                }
+                if (lane == 0) S.ncontrib = cnt;
+            }
+            __syncthreads();
+            // Items: (quiet row, contributor rank) for the rows whose neuron did not spike, then
+            // (spiking row, E-post block) for the others; thread t polls items t, t + 512, ...
+            const uint32_t nc = S.ncontrib;
+            const uint32_t rows_mask = (1u << ncols) - 1u;
+            const uint32_t spm = own & rows_mask, qm = ~own & rows_mask;
+            const uint32_t nq = (uint32_t)__popc(qm), ns = (uint32_t)__popc(spm);
+            const uint32_t nquiet = nq * nc, total = nquiet + ns * P.be;
+            const uint32_t sl = r_slot(a);
+            for (uint32_t base = 0; base < total; base += 4u * kRThreads) {
+                uint32_t row_of[4], rank_of[4], blk_of[4];
+                uint32_t want = 0;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const uint32_t it = base + tid + kRThreads * (uint32_t)m;
+                    uint32_t row = 0, rank = 0, blk = 0;
+                    if (it < nquiet) {
+                        row = r_nth_bit(qm, it / nc);
+                        rank = it % nc;
+                        blk = S.contrib[rank];
+                    } else if (it < total) {
+                        const uint32_t i2 = it - nquiet;
+                        row = r_nth_bit(spm, i2 / P.be);
+                        rank = blk = i2 % P.be;
+                    }
+                    row_of[m] = row;
+                    rank_of[m] = rank;
+                    blk_of[m] = blk;
+                    if (it < total) want |= 3u << (2 * m);
+                }
+                auto addr = [&](int m) -> const uint64_t* {
+                    return P.dr + (((size_t)sl * P.be + blk_of[m >> 1]) * 2u + (uint32_t)(m & 1)) * P.ne + j0 +
+                           row_of[m >> 1];
+                };
+                auto sink = [&](int m, uint32_t bits) {
+                    reinterpret_cast<uint32_t*>(&S.part[row_of[m >> 1]][rank_of[m >> 1]])[m & 1] = bits;
+                };
+                if (!r_poll<8>(P, S, want, r_tag(a), addr, sink)) return;
+            }
+            if (tid < ncols) {
+                const bool row_sp = (own >> tid) & 1u;
+                double R = row_sp ? 0.0 : S.R[tid];
+                const uint32_t cnt = row_sp ? P.be : nc;
+                for (uint32_t u = 0; u < cnt; ++u) R += S.part[tid][u];
+                S.R[tid] = R;
+                float sv = S.s[tid];
+                if (R > 0.0) sv = (float)((double)S.target[tid] / R);
+                S.s[tid] = sv;
+                if (k + 1 < n) st_c(P.sx + (size_t)r_slot(a + 1) * P.ne + j0 + tid, r_word(a + 1, __float_as_uint(sv)));
+            }
+        }
+        sp_prev = sp_now;
+    }
+    // ---- after the last step: the block's neurons (with the last step's stimulus events), its
+    // rows' sums and scales, the rows' traces (block 0 holds every row), the plastic entries.
+    if (tid < ncols) {
+        const uint32_t lid = j0 + tid;
+        const NeuronGroupView& v = G;
+        const uint32_t gid = v.offset + lid;
+        float istim = S.istim[tid];
+        for (uint32_t e = ev0; e < ev1; ++e)   // the events of the last step
+            if (P.ev_ids[e] + P.ev_offsets[e] == gid) istim += P.ev_amps[e];
+        gp(v.membrane_potentials)[lid] = S.V[tid];
+        gp(v.refractory_counter)[lid] = S.refr[tid];
+        gp(v.i_stimulus)[lid] = istim;
+        gp(v.background_excitatory_postsynaptic_r_traces)[lid] = S.re[tid];
+        gp(v.background_inhibitory_postsynaptic_r_traces)[lid] = S.ri[tid];
+        gp(v.background_excitatory_postsynaptic_d_traces)[lid] = S.de[tid];
+        gp(v.background_inhibitory_postsynaptic_d_traces)[lid] = S.di[tid];
+        gp(v.background_g_excitatory_synaptic_conductances)[lid] = S.bge[tid];
+        gp(v.background_g_inhibitory_synaptic_conductances)[lid] = S.bgi[tid];
+        gp(v.g_excitatory)[lid] = S.ge[tid];
+        gp(v.g_inhibitory)[lid] = S.gi[tid];
+        gp(P.spikes)[gid] = S.spike[tid];
+        if (epost && P.ee_kind != kRNone) {
+            gp(P.ee_tr[2])[lid] = S.post[0][tid];
+            if (P.ee_kind == kPlastTriplet) gp(P.ee_tr[3])[lid] = S.post[1][tid];
+        }
+        if (epost && P.ie_kind != kRNone) gp(P.ie_tr[1])[lid] = S.post[2][tid];
+        if (epost && P.ee_lazy) {
+            gp(P.R)[lid] = S.R[tid];
+            gp(P.s)[lid] = S.s[tid];
+        }
+    }
+    if (b == 0) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const uint32_t i = tl + T * (uint32_t)q;
+            if (i >= npre) continue;
+            gp(P.r[side])[i] = S.r[q][tid];
+            gp(P.d[side])[i] = S.d[q][tid];
+            gp(P.x[side])[i] = S.d[q][tid] - S.r[q][tid];
+            if (pkind != kRNone) gp(eside ? P.ee_tr[0] : P.ie_tr[0])[i] = S.pre[q][tid];
+            if (triplet) gp(P.ee_tr[1])[i] = S.tpre[q][tid];
+        }
+    }
+    if (pkind != kRNone) {
+        BN_GAS float* W = gp(P.W[conn]);
+        const uint32_t ld = P.ld[conn];
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            const uint32_t i = tl + T * (uint32_t)q;
+            if (i >= npre) continue;
+#pragma unroll
+            for (int c = 0; c < kRCols; ++c)
+                if ((uint32_t)c < ncols) W[(size_t)i * ld + j0 + c] = w[q][c];
+        }
+    }
+}
+
+// The clock after n steps (one thread, after the launch): step and stimulus cursor.
+__global__ void k_resident_clock(const RPlan* __restrict__ plan, uint32_t n) {
+    const RPlan& P = *plan;
+    bn_device_clock c = *P.clock;
+    c.clock_step += n;
+    if (P.ev_steps) c.next_event_idx = P.ev_start[min(c.clock_step, P.ev_steps)];
+    *P.clock = c;
+}
+
+namespace {
+const void* resident_kernel(uint32_t rows) {
+    switch (rows) {
+        case 1: return reinterpret_cast<const void*>(k_resident<1>);
+        case 2: return reinterpret_cast<const void*>(k_resident<2>);
+        case 3: return reinterpret_cast<const void*>(k_resident<3>);
+        case 4: return reinterpret_cast<const void*>(k_resident<4>);
+        case 5: return reinterpret_cast<const void*>(k_resident<5>);
+        case 6: return reinterpret_cast<const void*>(k_resident<6>);
+        case 7: return reinterpret_cast<const void*>(k_resident<7>);
+        case 8: return reinterpret_cast<const void*>(k_resident<8>);
+        default: return nullptr;
+    }
+}
+}  // namespace
+
+void resident_shape(uint32_t ne, uint32_t ni, uint32_t* rows, uint32_t* e_waves) {
+    uint32_t best = 0, best_w = 0;
+    for (uint32_t we = 1; we < (uint32_t)kRWaves; ++we) {
+        const uint32_t ke = (ne + 64u * we - 1u) / (64u * we);
+        const uint32_t ki = (ni + 64u * (kRWaves - we) - 1u) / (64u * (kRWaves - we));
+        const uint32_t k = std::max(ke, ki);
+        if (best == 0 || k < best) {
+            best = k;
+            best_w = we;
+        }
+    }
+    *rows = best <= (uint32_t)kRMaxRows ? best : 0u;
+    *e_waves = best_w;
+}
+
+int resident_max_blocks(uint32_t rows, uint32_t* blocks) {
+    const void* f = resident_kernel(rows);
+    if (!f) return (int)hipErrorInvalidValue;
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, kRThreads, 0);
+    *blocks = e == hipSuccess ? (uint32_t)std::max(0, cus * per_cu) : 0u;
+    return (int)e;
+}
+
+int launch_resident(const RPlan* d_plan, const RPlan& hp, uint32_t n, bn_stream_t s) {
+    const void* f = resident_kernel(hp.rows);
+    if (!f || n == 0) return (int)hipErrorInvalidValue;
+    void* args[] = {(void*)&d_plan, (void*)&n};
+    hipError_t e = hipLaunchCooperativeKernel(f, dim3(hp.blocks), dim3(kRThreads), args, 0, s);
+    if (e != hipSuccess) return (int)e;
+    hipLaunchKernelGGL(k_resident_clock, dim3(1), dim3(1), 0, s, d_plan, n);
+    return (int)hipGetLastError();
+}
+
+}  // namespace bn

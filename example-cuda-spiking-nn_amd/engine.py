@@ -27,6 +27,7 @@ BURYN_FLAGS = {
     "reference_layout": 1 << 7,
     "persistent": 1 << 8,
     "gemv_part_sums": 1 << 9,
+    "resident": 1 << 10,
 }
 
 
@@ -48,6 +49,8 @@ class SimInfo(C.Structure):
         ("steps_per_graph", C.c_uint32), ("graph_launches", C.c_uint32),
         ("persistent_blocks", C.c_uint32), ("persistent_items", C.c_uint32), ("gemv_part_chunks", C.c_uint32),
         ("persistent_resident_rows", C.c_uint32), ("gemv_part_sums", C.c_uint32),
+        ("resident_blocks", C.c_uint32), ("resident_columns", C.c_uint32), ("resident_e_waves", C.c_uint32),
+        ("resident_rows", C.c_uint32),
     ]
 
 

@@ -313,11 +313,21 @@ enum {
                                          * summation orders (bn_sim_info.gemv_part_chunks = 8,
                                          * lazy row sums by 256-column tile) differ from the
                                          * graph step's, so results differ in rounding only. */
-    BN_SIM_GEMV_PART_SUMS = 1u << 9     /* the fused step's propagate reduces each part of a
+    BN_SIM_GEMV_PART_SUMS = 1u << 9,    /* the fused step's propagate reduces each part of a
                                          * connection's row chunks (the 8 parts the evolve
                                          * combines) inside a block, so the evolve reads one FP64
                                          * row per part instead of one per chunk.  Same results;
                                          * not for bn_multi networks. */
+    BN_SIM_RESIDENT = 1u << 10          /* the resident step: one cooperative launch per run of
+                                         * steps with every weight in the registers of its blocks
+                                         * (each block owns 16 post columns and evolves those
+                                         * neurons); only spike words, lazy row scales and
+                                         * row-sum parts cross blocks.  Needs
+                                         * BN_SIM_LAZY_NORMALIZE when E->E normalises, padded rows,
+                                         * and a network whose blocks fit the device (about 4,000
+                                         * neurons).  Its summation orders (bn_sim_info.resident_*)
+                                         * differ from the graph step's: results differ in
+                                         * rounding only. */
 };
 
 typedef struct bn_sim_info {
@@ -345,6 +355,10 @@ typedef struct bn_sim_info {
                                         weights are streamed from memory every step) */
     uint32_t gemv_part_sums;         /* 1: the fused propagate writes one FP64 row per part of
                                         each connection's row chunks (BN_SIM_GEMV_PART_SUMS) */
+    uint32_t resident_blocks;        /* blocks of the resident step's launch (0: not resident) */
+    uint32_t resident_columns;       /* its post columns per block */
+    uint32_t resident_e_waves;       /* its waves (of 8) holding excitatory pre rows */
+    uint32_t resident_rows;          /* its pre rows per thread */
 } bn_sim_info;
 
 /* Host pointers to fill; any may be NULL.  Arrays are indexed by global neuron id (length

@@ -126,6 +126,16 @@ double canonical_row_sum(const float* row, uint32_t n) {
     return ((wave[0] + wave[1]) + wave[2]) + wave[3];
 }
 
+// The xor butterfly (32,16,8,4,2,1) over 64 lane values: the value every lane holds at the end.
+double butterfly64(double (&v)[64]) {
+    for (int off = 32; off >= 1; off >>= 1) {
+        double nv[64];
+        for (int l = 0; l < 64; ++l) nv[l] = v[l] + v[l ^ off];
+        std::memcpy(v, nv, sizeof v);
+    }
+    return v[0];
+}
+
 // Sum over one 256-column tile of a row in the persistent step's order: lane l sums its four
 // columns from 0.0, the 64 lane sums combined by the xor butterfly (32,16,8,4,2,1).  val(j)
 // gives the term of column j (columns >= n contribute nothing).
@@ -372,7 +382,32 @@ struct Sim {
             const uint32_t j0 = (uint32_t)b * NB, j1 = std::min(j0 + NB, C.n_post);
             double acc[NB], part[NB];
             for (uint32_t j = j0; j < j1; ++j) acc[j - j0] = (double)g[C.post_off + j];
-            if (kF32) {
+            if (kF32 && p.resident_columns) {
+                // Resident step order: thread t of the T = 64 * waves threads of this pre group
+                // sums rows t, t + T, ... from 0.0; the lane sums of a wave by the xor
+                // butterfly; g + wave_0 + wave_1 + ... in wave order.
+                const uint32_t waves = C.pre_g == 0 ? p.resident_e_waves : 8 - p.resident_e_waves;
+                const uint32_t T = 64 * waves;
+                static thread_local std::vector<double> lane;
+                lane.assign((size_t)64 * NB, 0.0);
+                for (uint32_t w = 0; w < waves; ++w) {
+                    std::fill(lane.begin(), lane.end(), 0.0);
+                    for (uint32_t l = 0; l < 64; ++l) {
+                        double* a = &lane[(size_t)l * NB];
+                        for (uint32_t i = 64 * w + l; i < C.n_pre; i += T) {
+                            const double xi = (double)xs[i];
+                            if (xi == 0.0 && !p.dense_gemv) continue;  // adds +0 in the engine: no effect
+                            const R* row = &C.W[(size_t)i * C.n_post];
+                            for (uint32_t j = j0; j < j1; ++j) a[j - j0] += xi * (double)row[j];
+                        }
+                    }
+                    for (uint32_t j = j0; j < j1; ++j) {
+                        double v[64];
+                        for (uint32_t l = 0; l < 64; ++l) v[l] = lane[(size_t)l * NB + (j - j0)];
+                        acc[j - j0] += butterfly64(v);
+                    }
+                }
+            } else if (kF32) {
                 // Engine order: chunk partials (row_block rows each, increasing i, from 0.0);
                 // the chunks cut into kFinParts contiguous parts of ceil(chunks/kFinParts),
                 // each part summed in chunk order from 0.0; then g + part_0 + part_1 + ...
@@ -513,7 +548,8 @@ struct Sim {
         const uint32_t per = (count + kQuietParts - 1) / kQuietParts;
         std::vector<double> dq((size_t)kQuietParts * np, 0.0);
         // change[(e, i)] = (double)M' - (double)M of the LTP entry (i, posts[e]) (tile order).
-        std::vector<double> change(p.lazy_tile_order ? (size_t)count * np : 0, 0.0);
+        const bool keep_changes = p.lazy_tile_order || p.resident_columns;
+        std::vector<double> change(keep_changes ? (size_t)count * np : 0, 0.0);
         for (uint32_t e = 0; e < count; ++e) {
             const uint32_t j = posts[e], q = e / per;
             for (uint32_t i = 0; i < np; ++i) {
@@ -522,11 +558,15 @@ struct Sim {
                 const float w = clampw<float>(P.s[i] * m + (float)ltp_delta(P, i, j), (float)P.wmin, (float)P.wmax);
                 const float mn = w / P.s[i];
                 dq[(size_t)q * np + i] += (double)mn - (double)m;
-                if (p.lazy_tile_order) change[(size_t)e * np + i] = (double)mn - (double)m;
+                if (keep_changes) change[(size_t)e * np + i] = (double)mn - (double)m;
                 m = mn;
             }
         }
         change_ = std::move(change);
+        if (p.resident_columns) {
+            lazy_resident_sums(P, M, np, nq, pre_sp, post_sp, posts);
+            return;
+        }
         if (p.lazy_tile_order) {
             lazy_tile_order_sums(P, M, np, nq, pre_sp, post_sp, posts);
             return;
@@ -580,6 +620,47 @@ struct Sim {
                         const int32_t e = post_rank[j];
                         return e < 0 ? 0.0 : change_[(size_t)e * np + i];
                     });
+                P.Rs[i] = rs;
+            }
+            if (P.Rs[i] > 0.0) P.s[i] = (float)((double)P.target[i] / P.Rs[i]);
+        }
+    }
+
+    // The resident step's order of the lazy row sums (see oracle.h): blocks of C post columns.
+    void lazy_resident_sums(Plast& P, float* M, uint32_t np, uint32_t nq, const uint32_t* pre_sp,
+                            const uint32_t* post_sp, const std::vector<uint32_t>& posts) {
+        const uint32_t Cb = p.resident_columns;
+        const uint32_t nblk = (nq + Cb - 1) / Cb;
+        std::vector<int32_t> post_rank(nq, -1);
+        for (uint32_t e = 0; e < posts.size(); ++e) post_rank[posts[e]] = (int32_t)e;
+        for (uint32_t i = 0; i < np; ++i) {
+            float* row = &M[(size_t)i * nq];
+            if (pre_sp[i] == 1) {
+                if (P.s[i] != 0.0f)
+                    for (uint32_t j = 0; j < nq; ++j) {
+                        if (i == j) continue;
+                        const float w = clampw<float>(P.s[i] * row[j] + (float)ltd_delta(P, i, j), (float)P.wmin, (float)P.wmax);
+                        row[j] = w / P.s[i];
+                    }
+                double rs = 0.0;
+                for (uint32_t b = 0; b < nblk; ++b) {
+                    double pb = 0.0;
+                    for (uint32_t j = b * Cb; j < std::min(b * Cb + Cb, nq); ++j) pb += (double)row[j];
+                    rs += pb;
+                }
+                P.Rs[i] = rs;
+            } else {
+                double rs = P.Rs[i];
+                for (uint32_t b = 0; b < nblk; ++b) {
+                    bool any = false;
+                    double db = 0.0;
+                    for (uint32_t j = b * Cb; j < std::min(b * Cb + Cb, nq); ++j)
+                        if (post_sp[j] == 1) {
+                            any = true;
+                            db += change_[(size_t)post_rank[j] * np + i];
+                        }
+                    if (any) rs += db;
+                }
                 P.Rs[i] = rs;
             }
             if (P.Rs[i] > 0.0) P.s[i] = (float)((double)P.target[i] / P.Rs[i]);

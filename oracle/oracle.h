@@ -29,6 +29,15 @@
  *     butterfly (32,16,8,4,2,1), giving T_q; a row whose pre neuron spiked gets R_i = 0.0 + T_0 +
  *     T_1 + ... over the new row; another row sums its LTP changes (double)M'_ij - (double)M_ij
  *     the same way per tile and gets R_i = R_i + dT_0 + dT_1 + ....
+ *   - resident step (resident_columns = C > 0, resident_e_waves = WE): the GEMV of a post column
+ *     sums the excitatory pre rows over T = 64*WE threads and the inhibitory ones over
+ *     T = 64*(8 - WE): thread t (wave t / 64, lane t % 64) sums rows t, t + T, t + 2T, ... in
+ *     order from 0.0, the 64 lane sums of a wave are combined by the xor butterfly
+ *     (32,16,8,4,2,1), and g' = fl32(((double)g + wave_0) + wave_1 + ...) in wave order.  The lazy
+ *     row sums follow blocks of C post columns (block b: columns C*b .. C*b + C - 1): a row whose
+ *     pre neuron spiked gets R_i = 0.0 + P_0 + P_1 + ..., P_b the block's new entries summed in
+ *     column order from 0.0; another row gets R_i = R_i + D_b + ... over the blocks b holding a
+ *     spiking post, D_b its LTP changes (double)M'_ij - (double)M_ij summed in column order.
  * precision 32 reproduces the engine bit for bit; precision 64 runs every quantity in FP64
  * (libm exp, plain sequential sums) as an independent accuracy reference.
  *
@@ -94,6 +103,8 @@ typedef struct oracle_params {
     uint32_t gemv_part_chunks[4]; /* chunks per combination part (0: ceil(chunks / 8)) */
     uint32_t lazy_tile_order;     /* 1: lazy row sums in the column-tile order (see above) */
     uint32_t e_e_pair_stdp;       /* 1: pair STDP on E->E (kernel_stdp_plasticity_update.cu) */
+    uint32_t resident_columns;    /* > 0: the resident step's orders (see above), post columns per block */
+    uint32_t resident_e_waves;    /* its waves (of 8) holding excitatory pre rows */
 } oracle_params;
 
 typedef struct oracle_sim oracle_sim;

@@ -44,7 +44,7 @@ class Params(C.Structure):
         ("i_e_tau_ltd", C.c_float), ("i_e_weight_min", C.c_float), ("i_e_weight_max", C.c_float),
         ("gemv_row_block", C.c_uint32 * 4), ("lazy_normalize", C.c_uint32),
         ("dense_gemv", C.c_uint32), ("gemv_part_chunks", C.c_uint32 * 4), ("lazy_tile_order", C.c_uint32),
-        ("e_e_pair_stdp", C.c_uint32),
+        ("e_e_pair_stdp", C.c_uint32), ("resident_columns", C.c_uint32), ("resident_e_waves", C.c_uint32),
     ]
 
 
@@ -122,8 +122,9 @@ def _up(a):
 
 def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=False,
                        rng_instance=0, dense_gemv=False, gemv_part_chunks=(0, 0, 0, 0),
-                       lazy_tile_order=False) -> Params:
-    """Builds oracle parameters from a parsed config (the engine's config dump)."""
+                       lazy_tile_order=False, resident=(0, 0)) -> Params:
+    """Builds oracle parameters from a parsed config (the engine's config dump).  resident:
+    (post columns per block, waves of excitatory rows) of the resident step's orders, (0, 0) off."""
     p = Params()
     p.num_excitatory_neurons = cfg["num_excitatory_neurons"]
     p.num_inhibitory_neurons = cfg["num_inhibitory_neurons"]
@@ -160,6 +161,7 @@ def params_from_config(cfg: dict, gemv_row_block=(0, 0, 0, 0), lazy_normalize=Fa
         p.gemv_part_chunks[k] = int(gemv_part_chunks[k])
     p.lazy_tile_order = int(lazy_tile_order)
     p.e_e_pair_stdp = int(cfg.get("e_e_plasticity_rule", "triplet") == "pair")
+    p.resident_columns, p.resident_e_waves = int(resident[0]), int(resident[1])
     return p
 
 

@@ -100,11 +100,14 @@ def oracle_for(config_path: str, sim=None, precision=32, lazy=False, row_block=N
     cfg = eng.config_dump(config_path)
     if row_block is None:
         row_block = tuple(sim.get_info().gemv_row_block) if sim is not None else (0, 0, 0, 0)
-    # The persistent step's orders: chunk sums in parts of 8, lazy row sums by column tile.
-    persistent = sim is not None and sim.get_info().persistent_blocks > 0
+    # The persistent step's orders: chunk sums in parts of 8, lazy row sums by column tile; the
+    # resident step's: rows over lanes and waves, lazy row sums by blocks of columns.
+    info = sim.get_info() if sim is not None else None
+    persistent = info is not None and info.persistent_blocks > 0
+    resident = (info.resident_columns, info.resident_e_waves) if info is not None and info.resident_blocks else (0, 0)
     p = oracle_py.params_from_config(cfg, gemv_row_block=row_block, lazy_normalize=lazy,
                                      gemv_part_chunks=(8, 8, 8, 8) if persistent else (0, 0, 0, 0),
-                                     lazy_tile_order=persistent)
+                                     lazy_tile_order=persistent, resident=resident)
     weights = [eng.generate_weight_matrix(config_path, k) for k in range(4)]
     stim = stimulus_schedule(cfg)
     return oracle_py.OracleSim(p, weights, precision=precision, stim=stim)

@@ -214,8 +214,11 @@ def test_gemv_matches_fp64():
     ne, ni = 100, 25
     shapes = [(ne, ne), (ne, ni), (ni, ne), (ni, ni)]
     ws = [(rng.uniform(0, 1e-9, s) * (rng.uniform(size=s) < 0.3)).astype(np.float32) for s in shapes]
-    for rb in ((7, 7, 7, 7), (100, 100, 25, 25)):
-        p = O.params_from_config(cfg, gemv_row_block=rb)
+    # Chunked orders of two row blocks, and the resident step's orders (columns per block, waves
+    # of excitatory rows): every term must enter each sum exactly once.
+    for rb, res in (((7, 7, 7, 7), (0, 0)), ((100, 100, 25, 25), (0, 0)), ((0, 0, 0, 0), (16, 6)),
+                    ((0, 0, 0, 0), (4, 1))):
+        p = O.params_from_config(cfg, gemv_row_block=rb, resident=res)
         sched = [(n, 0) for n in (0, 3, 17, 50, 99)] + [(ne + 2, 0), (ne + 20, 0)]
         sim = O.OracleSim(p, ws, stim=_stim(cfg, sched))
         sim.step(6)
@@ -299,9 +302,9 @@ def _row_delta(before, after, i, j, j_ref):
     return after[i, j] / after[i, j_ref] * before[i, j_ref] - before[i, j]
 
 
-@pytest.mark.parametrize("lazy,tile", [(False, False), (True, False), (True, True)],
-                         ids=["faithful", "lazy", "lazy-tile-order"])
-def test_triplet_ltd_and_triplet_terms_hand_computed(lazy, tile):
+@pytest.mark.parametrize("lazy,tile,res", [(False, False, 0), (True, False, 0), (True, True, 0), (True, False, 4)],
+                         ids=["faithful", "lazy", "lazy-tile-order", "lazy-resident-order"])
+def test_triplet_ltd_and_triplet_terms_hand_computed(lazy, tile, res):
     # kernel_triplet_plasticity_update.cu:54-93 with every coefficient nonzero.  E neuron 3 spikes
     # at steps 1 and 41, E neuron 9 at step 11 (one-step 1 uA pulses one step earlier).  At
     # step 41 (traces decayed first, increments after, :16-50 and :95-115):
@@ -320,7 +323,7 @@ def test_triplet_ltd_and_triplet_terms_hand_computed(lazy, tile):
     np.fill_diagonal(w_ee, 0)
     ws = [w_ee, np.zeros((ne, ni), np.float32), np.zeros((ni, ne), np.float32), np.zeros((ni, ni), np.float32)]
     p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy, lazy_tile_order=tile,
-                             gemv_part_chunks=(8, 8, 8, 8) if tile else (0, 0, 0, 0))
+                             gemv_part_chunks=(8, 8, 8, 8) if tile else (0, 0, 0, 0), resident=(res, 1 if res else 0))
     sim = O.OracleSim(p, ws, stim=_stim(cfg, [(3, 0), (9, 10), (3, 40)]))
     sim.step(41)
     assert [int(np.nonzero(sim.state()["spikes"])[0].size)] == [0]
@@ -450,8 +453,9 @@ def test_triplet_normalisation_restores_column_sum_targets():
     w_ee = w_ee.astype(np.float32)
     ws = [w_ee, np.zeros((ne, ni), np.float32), np.zeros((ni, ne), np.float32), np.zeros((ni, ni), np.float32)]
     target = np.array([np.float32(np.sum(w_ee[:, j], dtype=np.float32)) for j in range(ne)])  # column sums
-    for lazy, tile in ((False, False), (True, False), (True, True)):
-        p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy, lazy_tile_order=tile)
+    for lazy, tile, res in ((False, False, 0), (True, False, 0), (True, True, 0), (True, False, 4)):
+        p = O.params_from_config(cfg, gemv_row_block=(8, 8, 8, 8), lazy_normalize=lazy, lazy_tile_order=tile,
+                                 resident=(res, 3 if res else 0))
         sim = O.OracleSim(p, ws)
         sim.step(3)
         w = sim.weights(0).astype(np.float64)
@@ -459,8 +463,8 @@ def test_triplet_normalisation_restores_column_sum_targets():
         assert np.allclose(w.sum(axis=1)[ok], target[ok], rtol=1e-6, atol=0)
 
 
-@pytest.mark.parametrize("tile", [False, True], ids=["list-order", "tile-order"])
-def test_lazy_row_sums_restore_targets_with_spikes(tile):
+@pytest.mark.parametrize("tile,res", [(False, 0), (True, 0), (False, 16)], ids=["list-order", "tile-order", "resident-order"])
+def test_lazy_row_sums_restore_targets_with_spikes(tile, res):
     # Lazy normalisation keeps R_i = sum_j M_ij incrementally (spiking rows re-summed, other rows
     # plus their LTP changes); the true weights s_i * M_ij of every row must then sum to the
     # row's target (the initial column sum, triplet_plasticity.cpp:22-30) after every step,
@@ -479,7 +483,7 @@ def test_lazy_row_sums_restore_targets_with_spikes(tile):
     target = np.add.accumulate(ws[0], axis=0, dtype=np.float32)[-1].astype(np.float64)
     from helpers import stimulus_schedule
     p = O.params_from_config(cfg, gemv_row_block=(4, 4, 4, 4), lazy_normalize=True, lazy_tile_order=tile,
-                             gemv_part_chunks=(8, 8, 8, 8) if tile else (0, 0, 0, 0))
+                             gemv_part_chunks=(8, 8, 8, 8) if tile else (0, 0, 0, 0), resident=(res, 1 if res else 0))
     sim = O.OracleSim(p, ws, stim=stimulus_schedule(cfg))
     spikes = 0
     for step in range(120):
@@ -507,16 +511,21 @@ def test_lazy_normalisation_tracks_faithful():
     # The persistent step's orders: chunk sums in parts of 8, lazy row sums in column-tile order.
     c = O.OracleSim(O.params_from_config(eng_cfg, gemv_row_block=(4, 4, 4, 4), lazy_normalize=True,
                                          gemv_part_chunks=(8, 8, 8, 8), lazy_tile_order=True), ws, stim=stim)
-    worst = [0.0, 0.0]
+    # The resident step's orders: rows over thread lanes and waves, lazy row sums by column blocks.
+    d = O.OracleSim(O.params_from_config(eng_cfg, lazy_normalize=True, resident=(16, 6)), ws, stim=stim)
+    worst = [0.0, 0.0, 0.0]
     for step in range(300):
         a.step(1)
         b.step(1)
         c.step(1)
-        sa, sb, sc = a.state(), b.state(), c.state()
+        d.step(1)
+        sa, sb, sc, sd = a.state(), b.state(), c.state(), d.state()
         worst[0] = max(worst[0], float(np.max(np.abs(sa["V"] - sb["V"]) / np.abs(sa["V"]))))
         worst[1] = max(worst[1], float(np.max(np.abs(sa["V"] - sc["V"]) / np.abs(sa["V"]))))
-        assert np.array_equal(sa["spikes"], sc["spikes"])
-    wa, wb, wc = (x.weights(0).astype(np.float64) for x in (a, b, c))
+        worst[2] = max(worst[2], float(np.max(np.abs(sa["V"] - sd["V"]) / np.abs(sa["V"]))))
+        assert np.array_equal(sa["spikes"], sc["spikes"]) and np.array_equal(sa["spikes"], sd["spikes"])
+    wa, wb, wc, wd = (x.weights(0).astype(np.float64) for x in (a, b, c, d))
     assert np.max(np.abs(wa - wb)) <= 1e-5 * np.max(np.abs(wa))
     assert np.max(np.abs(wa - wc)) <= 1e-5 * np.max(np.abs(wa))
-    assert worst[0] <= 1e-5 and worst[1] <= 1e-5, worst
+    assert np.max(np.abs(wa - wd)) <= 1e-5 * np.max(np.abs(wa))
+    assert max(worst) <= 1e-5, worst
