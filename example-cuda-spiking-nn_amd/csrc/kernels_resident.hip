@@ -44,19 +44,19 @@ struct RShared {
     float V[kRCols], istim[kRCols], re[kRCols], ri[kRCols], de[kRCols], di[kRCols];
     float bge[kRCols], bgi[kRCols], ge[kRCols], gi[kRCols];
     uint32_t refr[kRCols], spike[kRCols];
-    float post[3][kRCols];             // post traces at the step: E->E fast, E->E slow, I->E
+    float post[2][3][kRCols];          // post traces of step a in [a & 1]: E->E fast, E->E slow, I->E
     int32_t vrec[kRCols];              // slot in the membrane-potential ring, or -1
     double R[kRCols];                  // lazy E->E: the block's rows' sums, scales, targets
     float s[kRCols], target[kRCols];
     uint32_t heads[8];                 // first four entries of the group's Poisson tables (e, i)
+    float rk[2][5];                    // per pre group: r decay, d decay, K, pre decay, slow pre decay
     uint32_t abort;
-    uint32_t ncontrib;
     // The threads' rows, [row][thread].
     float r[K][kRThreads], d[K][kRThreads], pre[K][kRThreads], tpre[K][kRThreads], sc[K][kRThreads];
     float xs[K][kRThreads];            // the rows' x_i (s_i) of the step, FP32
     uint32_t spk[kRMaxBlocks];         // spike words of the step, block order
-    uint32_t contrib[kRMaxBlocks];     // E-post blocks with a spiking column (owner's polls)
     double red[kRWaves][kRCols];       // the waves' column sums
+    float zy[kRWaves][2][kRCols];      // per wave and step: LTP factors Z_c, LTD factors Y_c
     double part[kRCols][kRMaxBlocks];  // owner: the polled row-sum parts, [row][rank]
 };
 
@@ -92,6 +92,66 @@ __device__ __forceinline__ bool r_poll(const RPlan& P, RShared<K>& S, uint32_t w
     __syncthreads();
     return S.abort == 0u;
 }
+
+// As r_poll for the threads that call it, without the closing barrier: the caller's next
+// block barrier is followed by the check of S.abort.
+template <int M, int K, typename Addr, typename Sink>
+__device__ __forceinline__ void r_poll_nb(const RPlan& P, RShared<K>& S, uint32_t want, uint32_t tag, Addr addr,
+                                          Sink sink) {
+    uint32_t pending = want;
+    for (uint32_t it = 0; pending; ++it) {
+        uint64_t v[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) v[m] = (pending >> m) & 1u ? ld_c(addr(m)) : 0ull;
+#pragma unroll
+        for (int m = 0; m < M; ++m)
+            if (((pending >> m) & 1u) && (uint32_t)(v[m] >> 32) == tag) {
+                sink(m, (uint32_t)v[m]);
+                pending &= ~(1u << m);
+            }
+        if (!pending) break;
+        if (it >= P.wait_limit) {
+            max_c(P.status, 2u);
+            S.abort = 1u;
+            break;
+        }
+        if ((it & 15u) == 15u && ld_c(P.status) >= 2u) {
+            S.abort = 1u;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Index of the k-th set bit (k < popcount) of the 256-bit mask m[0..3].
+__device__ __forceinline__ uint32_t r_nth_set256(const uint64_t (&m)[4], uint32_t k) {
+    uint32_t base = 0;
+    uint64_t x = m[0];
+#pragma unroll
+    for (int w = 0; w < 3; ++w) {
+        const uint32_t c = (uint32_t)__popcll(x);
+        if (k < c) break;
+        k -= c;
+        base += 64u;
+        x = m[w + 1];
+    }
+#pragma unroll
+    for (uint32_t width = 32; width >= 1; width >>= 1) {
+        const uint64_t lo = x & ((1ull << width) - 1ull);
+        const uint32_t c = (uint32_t)__popcll(lo);
+        if (k >= c) {
+            k -= c;
+            base += width;
+            x >>= width;
+        } else {
+            x = lo;
+        }
+    }
+    return base;
+}
+
+// Global id of pre-local row 0 of the E rows (0) or the I rows (ne).
+__device__ __forceinline__ uint32_t pre_base_of(const RPlan& P, bool eside) { return eside ? 0u : P.ne; }
 
 // Spike bit of global neuron gid in the step's block words.
 template <int K>
@@ -141,9 +201,8 @@ __device__ __forceinline__ uint32_t r_nth_bit(uint32_t m, uint32_t k) {
 // ---------------------------------------------------------------------------------------
 template <int K>
 __device__ __forceinline__ uint32_t r_evolve(const RPlan& P, RShared<K>& S, const NeuronGroupView& v, bool epost,
-                                             uint32_t j0, uint32_t a, uint32_t e0, uint32_t e1,
+                                             uint32_t j0, uint32_t c, uint32_t a, uint32_t e0, uint32_t e1,
                                              const bn_device_clock& clk) {
-    const uint32_t c = threadIdx.x;
     const uint32_t gid = v.offset + j0 + c;
     float istim = S.istim[c];
     // stimulus.cpp:46-60 for this neuron: the events in list order.
@@ -186,14 +245,14 @@ __device__ __forceinline__ uint32_t r_evolve(const RPlan& P, RShared<K>& S, cons
     eif_step(v, S.ge[c], S.gi[c], istim, V, refr, spike);
     // Post traces of the plasticities (E neurons): the previous step's +1, then the decay.
     if (epost) {
-        const float inc = S.spike[c] == 1u ? 1.0f : 0.0f;
+        const bool was = S.spike[c] == 1u;
+        const float(&pp)[3][kRCols] = S.post[(a - 1u) & 1u];
+        float(&pn)[3][kRCols] = S.post[a & 1u];
         if (P.ee_kind != kRNone) {
-            S.post[0][c] = flush_trace((S.spike[c] == 1u ? S.post[0][c] + inc : S.post[0][c]) * P.ee_dec[2]);
-            if (P.ee_kind == kPlastTriplet)
-                S.post[1][c] = flush_trace((S.spike[c] == 1u ? S.post[1][c] + inc : S.post[1][c]) * P.ee_dec[3]);
+            pn[0][c] = flush_trace((was ? pp[0][c] + 1.0f : pp[0][c]) * P.ee_dec[2]);
+            if (P.ee_kind == kPlastTriplet) pn[1][c] = flush_trace((was ? pp[1][c] + 1.0f : pp[1][c]) * P.ee_dec[3]);
         }
-        if (P.ie_kind != kRNone)
-            S.post[2][c] = flush_trace((S.spike[c] == 1u ? S.post[2][c] + inc : S.post[2][c]) * P.ie_dec[1]);
+        if (P.ie_kind != kRNone) pn[2][c] = flush_trace((was ? pp[2][c] + 1.0f : pp[2][c]) * P.ie_dec[1]);
     }
     S.V[c] = V;
     S.refr[c] = refr;
@@ -212,8 +271,14 @@ __device__ __forceinline__ uint32_t r_evolve(const RPlan& P, RShared<K>& S, cons
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
-// The launch.  K: rows per thread (the plan's `rows`).
+// The launch.  K: rows per thread (the plan's `rows`).  Roles within a step: the last wave
+// finalises g and evolves the block's neurons (the spike word every block waits for) while the
+// other waves apply the plasticity and, in E-post blocks, poll their rows' row-sum parts.
 // ---------------------------------------------------------------------------------------
+typedef float v16f __attribute__((ext_vector_type(16)));
+static_assert(kRCols == 16, "a row of the block's entries is one 16-wide vector");
+constexpr uint32_t kREvolveWave = kRWaves - 1;
+
 template <int K>
 __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict__ plan, uint32_t n) {
     __shared__ RShared<K> S;
@@ -232,20 +297,17 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
     const uint32_t T = 64u * (eside ? P.e_waves : (uint32_t)kRWaves - P.e_waves);
     const uint32_t tl = eside ? tid : tid - 64u * P.e_waves;
     const uint32_t npre = eside ? P.ne : P.ni;
-    const uint32_t pre_base = eside ? 0u : P.ne;                    // global id of pre-local 0
     const uint32_t conn = epost ? (eside ? 0u : 2u) : (eside ? 1u : 3u);
     const uint32_t pkind = !epost ? kRNone : (eside ? P.ee_kind : P.ie_kind);
     const bool lazy = epost && eside && P.ee_lazy;
     const bool triplet = pkind == kPlastTriplet;
     const PlasticityParams& PP = eside ? P.ee_p : P.ie_p;
+    const bool evolver = wave == kREvolveWave;
     const bn_device_clock clk = *P.clock;
     const uint32_t t0 = clk.clock_step;
-    if (tid == 0) {
-        S.abort = 0u;
-        S.ncontrib = 0u;
-    }
+    if (tid == 0) S.abort = 0u;
     // The entries of the block's columns in the thread's rows (padding columns and rows: 0).
-    float w[K][kRCols];
+    v16f w[K];
     {
         const BN_GAS float* W = gp(P.W[conn]);
         const uint32_t ld = P.ld[conn];
@@ -273,12 +335,13 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
         S.pre[k][tid] = ok && pkind != kRNone ? gp(eside ? P.ee_tr[0] : P.ie_tr[0])[i] : 0.0f;
         S.tpre[k][tid] = ok && triplet ? gp(P.ee_tr[1])[i] : 0.0f;
         S.sc[k][tid] = ok && lazy ? gp(P.s)[i] : 1.0f;
-        if (ok && gp(P.spikes)[pre_base + i] == 1u) sp_prev |= 1u << k;
+        if (ok && gp(P.spikes)[pre_base_of(P, eside) + i] == 1u) sp_prev |= 1u << k;
     }
-    // The block's neurons.
+    // The block's neurons (post traces in the buffer of the step before the launch).
     if (tid < ncols) {
         const uint32_t lid = j0 + tid;
         const NeuronGroupView& v = G;
+        float(&pp)[3][kRCols] = S.post[(t0 - 1u) & 1u];
         S.V[tid] = gp(v.membrane_potentials)[lid];
         S.refr[tid] = gp(v.refractory_counter)[lid];
         S.istim[tid] = gp(v.i_stimulus)[lid];
@@ -289,9 +352,9 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
         S.ge[tid] = gp(v.g_excitatory)[lid];
         S.gi[tid] = gp(v.g_inhibitory)[lid];
         S.spike[tid] = gp(P.spikes)[v.offset + lid];
-        S.post[0][tid] = epost && P.ee_kind != kRNone ? gp(P.ee_tr[2])[lid] : 0.0f;
-        S.post[1][tid] = epost && P.ee_kind == kPlastTriplet ? gp(P.ee_tr[3])[lid] : 0.0f;
-        S.post[2][tid] = epost && P.ie_kind != kRNone ? gp(P.ie_tr[1])[lid] : 0.0f;
+        pp[0][tid] = epost && P.ee_kind != kRNone ? gp(P.ee_tr[2])[lid] : 0.0f;
+        pp[1][tid] = epost && P.ee_kind == kPlastTriplet ? gp(P.ee_tr[3])[lid] : 0.0f;
+        pp[2][tid] = epost && P.ie_kind != kRNone ? gp(P.ie_tr[1])[lid] : 0.0f;
         S.vrec[tid] = P.v_ring ? gp(P.vrec_index)[v.offset + lid] : -1;
         if (epost && P.ee_lazy) {
             S.R[tid] = gp(P.R)[lid];
@@ -303,20 +366,39 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
         S.heads[tid] = tid < G.poisson_table_excitatory_len ? gp(G.poisson_table_excitatory)[tid] : 0u;
         S.heads[4 + tid] = tid < G.poisson_table_inhibitory_len ? gp(G.poisson_table_inhibitory)[tid] : 0u;
     }
+    if (tid < 2) {
+        S.rk[tid][0] = P.decay_r[tid];
+        S.rk[tid][1] = P.decay_d[tid];
+        S.rk[tid][2] = P.K[tid];
+        S.rk[tid][3] = tid == 0 ? P.ee_dec[0] : P.ie_dec[0];
+        S.rk[tid][4] = P.ee_dec[1];
+    }
+    // The spike bit of the thread's row q: bit (tl % 16) of block word spk_row0 + q * spk_step
+    // (a thread's rows are 16-aligned apart, so they share the bit); vmask: rows within the group.
+    const uint32_t spk_row0 = (eside ? 0u : P.be) + tl / (uint32_t)kRCols;
+    const uint32_t spk_step = T / (uint32_t)kRCols, spk_bit = tl % (uint32_t)kRCols;
+    uint32_t vmask = 0;
+#pragma unroll
+    for (int q = 0; q < K; ++q) vmask |= (tl + T * (uint32_t)q < npre ? 1u : 0u) << q;
     // Every load of the state has returned before the block publishes anything: another block
     // stores its part of the state back after the launch's last step, which can follow this
     // block's first spike word closely.
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     // The evolve of the launch's first step.
-    if (wave == 0) {
-        const uint32_t sp = tid < ncols ? r_evolve(P, S, G, epost, j0, t0, 0u, 0u, clk) : 0u;
+    if (evolver) {
+        const uint32_t sp = lane < ncols ? r_evolve(P, S, G, epost, j0, lane, t0, 0u, 0u, clk) : 0u;
         const uint32_t bits = (uint32_t)__ballot(sp == 1u);
         if (lane == 0) st_c(P.spk + (size_t)r_slot(t0) * P.blocks + b, r_word(t0, bits));
     }
     uint32_t ev0 = 0, ev1 = 0;   // stimulus events of step a (applied by the evolve of step a + 1)
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t a = t0 + k;
+        // The row and column bases count as rewritten each step (an empty register constraint,
+        // no instruction): otherwise the compiler hoists every entry's self-skip and bounds test
+        // out of the step loop and keeps those lane masks in spilled scalar registers.
+        uint32_t tl_k = tl, j0_k = j0, ncols_k = ncols;
+        asm volatile("" : "+v"(tl_k), "+s"(j0_k), "+s"(ncols_k));
         if (P.ev_steps && a < P.ev_steps) {  // loaded now, used after the poll
             ev0 = P.ev_start[a];
             ev1 = min(P.ev_start[a + 1], P.ev_count);
@@ -329,10 +411,7 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             if (wave == 0)
                 for (uint32_t m = 0; m < kRMaxBlocks / 64; ++m)
                     if (lane + 64u * m < P.blocks) want |= 1u << m;
-            if (lazy && k > 0)
-#pragma unroll
-                for (int q = 0; q < K; ++q)
-                    if (tl + T * (uint32_t)q < npre) want |= 1u << (4 + q);
+            if (lazy && k > 0) want |= vmask << 4;
             const uint32_t sl = r_slot(a);
             auto addr = [&](int m) -> const uint64_t* {
                 if (m < 4) return P.spk + (size_t)sl * P.blocks + lane + 64u * (uint32_t)m;
@@ -355,17 +434,19 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
         // previous step's +1 first), x_i (s_i) for the propagate.
         uint32_t sp_now = 0;
         {
-            const float dr = P.decay_r[side], dd = P.decay_d[side], Kn = P.K[side];
-            const float dpre = eside ? P.ee_dec[0] : P.ie_dec[0], dtpre = P.ee_dec[1];
+            const float dr = S.rk[side][0], dd = S.rk[side][1], Kn = S.rk[side][2];
+            const float dpre = S.rk[side][3], dtpre = S.rk[side][4];
             float rv[K], dv[K];
+            uint32_t wv[K];
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                const uint32_t i = tl + T * (uint32_t)q;
-                const uint32_t ic = i < npre ? i : 0u;
-                sp_now |= (i < npre ? r_spiked(P, S, pre_base + ic) : 0u) << q;
+                wv[q] = S.spk[min(spk_row0 + spk_step * (uint32_t)q, kRMaxBlocks - 1u)];
                 rv[q] = S.r[q][tid];
                 dv[q] = S.d[q][tid];
             }
+#pragma unroll
+            for (int q = 0; q < K; ++q) sp_now |= ((wv[q] >> spk_bit) & 1u) << q;
+            sp_now &= vmask;
 #pragma unroll
             for (int q = 0; q < K; ++q) {
                 const bool sp = (sp_now >> q) & 1u;
@@ -385,11 +466,9 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                 for (int q = 0; q < K; ++q) {
                     const bool was = (sp_prev >> q) & 1u;
                     const float pr = S.pre[q][tid];
+                    const float tp = S.tpre[q][tid];
                     S.pre[q][tid] = flush_trace((was ? pr + 1.0f : pr) * dpre);
-                    if (triplet) {
-                        const float tp = S.tpre[q][tid];
-                        S.tpre[q][tid] = flush_trace((was ? tp + 1.0f : tp) * dtpre);
-                    }
+                    if (triplet) S.tpre[q][tid] = flush_trace((was ? tp + 1.0f : tp) * dtpre);
                 }
             }
         }
@@ -411,9 +490,7 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             // otherwise the compiler keeps their FP64 conversions from the sums above alive until
             // the plasticity below converts the same values again, twice the entries' registers.
 #pragma unroll
-            for (int q = 0; q < K; ++q)
-#pragma unroll
-                for (int c = 0; c < kRCols; ++c) asm volatile("" : "+v"(w[q][c]));
+            for (int q = 0; q < K; ++q) asm volatile("" : "+v"(w[q]));
             double u8[8], u4[4], u2[2];
             const bool h32 = lane & 32u, h16 = lane & 16u, h8 = lane & 8u, h4 = lane & 4u;
 #pragma unroll
@@ -448,43 +525,69 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             }
         }
         __syncthreads();
-        if (tid < ncols) {
-            double ge = (double)S.bge[tid];
-            for (uint32_t q = 0; q < P.e_waves; ++q) ge += S.red[q][tid];
-            double gi = (double)S.bgi[tid];
-            for (uint32_t q = P.e_waves; q < (uint32_t)kRWaves; ++q) gi += S.red[q][tid];
-            S.ge[tid] = (float)ge;
-            S.gi[tid] = (float)gi;
+        const uint32_t pa = a & 1u;
+        const uint32_t own = __builtin_amdgcn_readfirstlane(S.spk[b]);   // the block's spiking columns
+        // ---- evolver wave: g of the block's neurons, the evolve of step a + 1 (with the events
+        // of step a) and its spike word.
+        if (evolver) {
+            if (lane < ncols) {
+                double ge = (double)S.bge[lane];
+                for (uint32_t q = 0; q < P.e_waves; ++q) ge += S.red[q][lane];
+                double gi = (double)S.bgi[lane];
+                for (uint32_t q = P.e_waves; q < (uint32_t)kRWaves; ++q) gi += S.red[q][lane];
+                S.ge[lane] = (float)ge;
+                S.gi[lane] = (float)gi;
+            }
+            if (k + 1 < n) {
+                const uint32_t sp = lane < ncols ? r_evolve(P, S, G, epost, j0, lane, a + 1, ev0, ev1, clk) : 0u;
+                const uint32_t bits = (uint32_t)__ballot(sp == 1u);
+                if (lane == 0) st_c(P.spk + (size_t)r_slot(a + 1) * P.blocks + b, r_word(a + 1, bits));
+            }
         }
         // ---- plasticity of step a on the entries in registers: LTP of the block's spiking
-        // columns, then LTD of the thread's spiking rows, each pulled out of the register array
-        // by a select chain (a loop over columns or rows instead of every entry unrolled keeps
-        // the loop's code within the instruction cache).
-        const uint32_t own = __builtin_amdgcn_readfirstlane(S.spk[b]);   // the block's spiking columns
-        if (pkind != kRNone && (own != 0u || sp_now != 0u)) {
+        // columns (a uniform index into each row's vector), then LTD of the spiking rows (one
+        // pass per row slot that spikes on some lane of the wave).  Every rule is written as
+        //   LTP: delta = pre_i * Z_c   (triplet: Z_c = ltp_A + ltp_B * slow post_c; else ltp_A)
+        //   LTD: dl = X_i * Y_c        (triplet: X_i = ltd_A + ltd_B * slow pre_i, Y_c = -post_c;
+        //                               iSTDP: X = -ltd_A, Y_c = post_c - ltd_B; pair: X = -ltd_A,
+        //                               Y_c = post_c)
+        // (the reference's products with their factors swapped, the same roundings), and the
+        // wave's Z_c, Y_c go through its own LDS slots once per step.
+        // (Entered by whole waves: lanes 0..15 fill the wave's factors for every lane.)
+        if (pkind != kRNone && (own != 0u || __ballot(sp_now != 0u) != 0ull)) {
             const float wmin = PP.weight_min, wmax = PP.weight_max;
+            float(&zc)[kRCols] = S.zy[wave][0];
+            float(&yc)[kRCols] = S.zy[wave][1];
+            if (lane < (uint32_t)kRCols) {
+                const float(&post)[3][kRCols] = S.post[pa];
+                const float pj = post[eside ? 0 : 2][lane];
+                zc[lane] = triplet ? PP.ltp_A + (PP.ltp_B * post[1][lane]) : PP.ltp_A;
+                yc[lane] = triplet ? -pj : (pkind == kPlastIstdp ? pj - PP.ltd_B : pj);
+            }
+            const float xw = -PP.ltd_A;   // X of iSTDP and pair
             double change[K];
 #pragma unroll
             for (int q = 0; q < K; ++q) change[q] = 0.0;
             for (uint32_t cm = own; cm; cm &= cm - 1u) {
                 const uint32_t c = (uint32_t)__builtin_ctz(cm);   // uniform
-                const uint32_t j = j0 + c;
-                const float fac = triplet ? PP.ltp_A + (PP.ltp_B * S.post[1][c]) : PP.ltp_A;
+                const uint32_t j = j0_k + c;
+                const float z = zc[c];
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    const uint32_t i = tl + T * (uint32_t)q;
-                    const float si = lazy ? S.sc[q][tid] : 1.0f;
-                    float m = w[q][0];
-#pragma unroll
-                    for (int cc = 1; cc < kRCols; ++cc) m = c == (uint32_t)cc ? w[q][cc] : m;
-                    float val = m;
-                    if (i < npre && j != i && (!lazy || si != 0.0f)) {
-                        const float pre_i = S.pre[q][tid];
-                        const float delta = triplet ? pre_i * fac : PP.ltp_A * pre_i;
-                        val = p_apply(lazy, m, si, delta, wmin, wmax);
+                    const uint32_t i = tl_k + T * (uint32_t)q;
+                    const float m = w[q][c];
+                    const float delta = S.pre[q][tid] * z;
+                    float val;
+                    bool ok = ((vmask >> q) & 1u) && j != i;
+                    if (lazy) {
+                        const float si = S.sc[q][tid];
+                        ok = ok && si != 0.0f;
+                        val = clamp_weight(si * m + delta, wmin, wmax) / si;
+                    } else {
+                        val = clamp_weight(m + delta, wmin, wmax);
                     }
-#pragma unroll
-                    for (int cc = 0; cc < kRCols; ++cc) w[q][cc] = c == (uint32_t)cc ? val : w[q][cc];
+                    val = ok ? val : m;
+                    w[q][c] = val;
                     change[q] += (double)val - (double)m;
                 }
             }
@@ -493,97 +596,86 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             if (lazy && own != 0u) {
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    const uint32_t i = tl + T * (uint32_t)q;
-                    if (i >= npre || ((sp_now >> q) & 1u)) continue;
+                    const uint32_t i = tl_k + T * (uint32_t)q;
+                    if (!((vmask >> q) & 1u) || ((sp_now >> q) & 1u)) continue;
                     const uint64_t bits = (uint64_t)__double_as_longlong(change[q]);
                     uint64_t* dst = P.dr + ((size_t)r_slot(a) * P.be + b) * 2u * P.ne + i;
                     st_c(dst, r_word(a, (uint32_t)bits));
                     st_c(dst + P.ne, r_word(a, (uint32_t)(bits >> 32)));
                 }
             }
-            const float* post_j = eside ? S.post[0] : S.post[2];
-            for (uint32_t rm = sp_now; rm; rm &= rm - 1u) {
-                const uint32_t qs = (uint32_t)__builtin_ctz(rm);
-                const uint32_t i = tl + T * qs;
-                const float si = lazy ? S.sc[qs][tid] : 1.0f;
-                float row[kRCols];
 #pragma unroll
-                for (int c = 0; c < kRCols; ++c) {
-                    row[c] = w[0][c];
+            for (int q = 0; q < K; ++q) {
+                const bool sp = (sp_now >> q) & 1u;
+                if (__ballot(sp) == 0ull) continue;   // no lane of the wave has row q spiking
+                const uint32_t i = tl_k + T * (uint32_t)q;
+                float y[kRCols];
 #pragma unroll
-                    for (int q = 1; q < K; ++q) row[c] = qs == (uint32_t)q ? w[q][c] : row[c];
-                }
-                if (!lazy || si != 0.0f) {
-                    const float fac_i = triplet ? PP.ltd_A + (PP.ltd_B * S.tpre[qs][tid]) : 0.0f;
+                for (int c = 0; c < kRCols; ++c) y[c] = yc[c];
+                const float x = triplet ? PP.ltd_A + (PP.ltd_B * S.tpre[q][tid]) : xw;
+                double part = 0.0;
+                if (lazy) {
+                    const float si = S.sc[q][tid];
+                    const bool upd = sp && si != 0.0f;
 #pragma unroll
                     for (int c = 0; c < kRCols; ++c) {
-                        const uint32_t j = j0 + (uint32_t)c;
-                        if ((uint32_t)c >= ncols || j == i) continue;
-                        float dl;
-                        if (triplet) dl = -1.0f * post_j[c] * fac_i;
-                        else if (pkind == kPlastIstdp) dl = -1.0f * PP.ltd_A * (post_j[c] - PP.ltd_B);
-                        else dl = -1.0f * PP.ltd_A * post_j[c];
-                        row[c] = p_apply(lazy, row[c], si, dl, wmin, wmax);
+                        const float m = w[q][c];
+                        const float v = clamp_weight(si * m + x * y[c], wmin, wmax) / si;
+                        const bool ok = upd && (uint32_t)c < ncols_k && j0_k + (uint32_t)c != i;
+                        w[q][c] = ok ? v : m;
+                        if ((uint32_t)c < ncols_k) part += (double)w[q][c];
                     }
+                    if (sp) {
+                        const uint64_t bits = (uint64_t)__double_as_longlong(part);
+                        uint64_t* dst = P.dr + ((size_t)r_slot(a) * P.be + b) * 2u * P.ne + i;
+                        st_c(dst, r_word(a, (uint32_t)bits));
+                        st_c(dst + P.ne, r_word(a, (uint32_t)(bits >> 32)));
+                    }
+                } else {
 #pragma unroll
-                    for (int q = 0; q < K; ++q)
-#pragma unroll
-                        for (int c = 0; c < kRCols; ++c) w[q][c] = qs == (uint32_t)q ? row[c] : w[q][c];
-                }
-                if (lazy) {
-                    double part = 0.0;
-#pragma unroll
-                    for (int c = 0; c < kRCols; ++c)
-                        if ((uint32_t)c < ncols) part += (double)row[c];
-                    const uint64_t bits = (uint64_t)__double_as_longlong(part);
-                    uint64_t* dst = P.dr + ((size_t)r_slot(a) * P.be + b) * 2u * P.ne + i;
-                    st_c(dst, r_word(a, (uint32_t)bits));
-                    st_c(dst + P.ne, r_word(a, (uint32_t)(bits >> 32)));
+                    for (int c = 0; c < kRCols; ++c) {
+                        const float m = w[q][c];
+                        const float v = clamp_weight(m + x * y[c], wmin, wmax);
+                        const bool ok = sp && (uint32_t)c < ncols_k && j0_k + (uint32_t)c != i;
+                        w[q][c] = ok ? v : m;
+                    }
                 }
             }
         }
-        // ---- evolve of step a + 1 (with the events of step a) and its spike word.
-        __syncthreads();
-        if (k + 1 < n && wave == 0) {
-            const uint32_t sp = tid < ncols ? r_evolve(P, S, G, epost, j0, a + 1, ev0, ev1, clk) : 0u;
-            const uint32_t bits = (uint32_t)__ballot(sp == 1u);
-            if (lane == 0) st_c(P.spk + (size_t)r_slot(a + 1) * P.blocks + b, r_word(a + 1, bits));
-        }
-        // ---- owner: the row sums of the block's rows of the lazy matrix after step a's
-        // plasticity, their scales for step a + 1.
-        if (epost && P.ee_lazy) {
-            // Contributors: the E-post blocks with a spiking column, in block order.
-            if (wave == 0) {
-                uint32_t cnt = 0;
-                for (uint32_t m = 0; m < kRMaxBlocks / 64; ++m) {
-                    const uint32_t q = lane + 64u * m;
-                    const bool is = q < P.be && S.spk[q] != 0u;
-                    const uint64_t bal = __ballot(is);
-                    if (is) S.contrib[cnt + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = q;
-                    cnt += (uint32_t)__popcll(bal);
-                }
-                if (lane == 0) S.ncontrib = cnt;
-            }
-            __syncthreads();
-            // Items: (quiet row, contributor rank) for the rows whose neuron did not spike, then
-            // (spiking row, E-post block) for the others; thread t polls items t, t + 512, ...
-            const uint32_t nc = S.ncontrib;
-            const uint32_t rows_mask = (1u << ncols) - 1u;
+        // ---- owner (E-post blocks): the parts of the block's rows of the lazy matrix after
+        // step a's plasticity, R_i and s_i of step a + 1.  One wave does it: the first wave of I
+        // rows, which stores nothing before its polls (a wave's loads wait for its own earlier
+        // stores), and runs beside the evolver.
+        const bool owner = epost && P.ee_lazy;
+        const uint32_t owner_wave = P.e_waves < kREvolveWave ? P.e_waves : 0u;
+        if (owner && wave == owner_wave) {
+            const uint32_t rows_mask = (1u << ncols_k) - 1u;
             const uint32_t spm = own & rows_mask, qm = ~own & rows_mask;
+            // Contributors: the E-post blocks with a spiking column (block order).
+            uint64_t cmask[4];
+            uint32_t nc = 0;
+#pragma unroll
+            for (uint32_t m = 0; m < 4; ++m) {
+                const uint32_t q = lane + 64u * m;
+                cmask[m] = __ballot(q < P.be && S.spk[q] != 0u);
+                nc += (uint32_t)__popcll(cmask[m]);
+            }
+            // Items: (quiet row, contributor rank) for the rows whose neuron did not spike, then
+            // (spiking row, E-post block); lane l polls items l, l + 64, ...
             const uint32_t nq = (uint32_t)__popc(qm), ns = (uint32_t)__popc(spm);
             const uint32_t nquiet = nq * nc, total = nquiet + ns * P.be;
             const uint32_t sl = r_slot(a);
-            for (uint32_t base = 0; base < total; base += 4u * kRThreads) {
+            for (uint32_t base = 0; base < total; base += 4u * 64u) {
                 uint32_t row_of[4], rank_of[4], blk_of[4];
                 uint32_t want = 0;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
-                    const uint32_t it = base + tid + kRThreads * (uint32_t)m;
+                    const uint32_t it = base + lane + 64u * (uint32_t)m;
                     uint32_t row = 0, rank = 0, blk = 0;
                     if (it < nquiet) {
                         row = r_nth_bit(qm, it / nc);
                         rank = it % nc;
-                        blk = S.contrib[rank];
+                        blk = r_nth_set256(cmask, rank);
                     } else if (it < total) {
                         const uint32_t i2 = it - nquiet;
                         row = r_nth_bit(spm, i2 / P.be);
@@ -595,30 +687,44 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                     if (it < total) want |= 3u << (2 * m);
                 }
                 auto addr = [&](int m) -> const uint64_t* {
-                    return P.dr + (((size_t)sl * P.be + blk_of[m >> 1]) * 2u + (uint32_t)(m & 1)) * P.ne + j0 +
+                    return P.dr + (((size_t)sl * P.be + blk_of[m >> 1]) * 2u + (uint32_t)(m & 1)) * P.ne + j0_k +
                            row_of[m >> 1];
                 };
                 auto sink = [&](int m, uint32_t bits) {
                     reinterpret_cast<uint32_t*>(&S.part[row_of[m >> 1]][rank_of[m >> 1]])[m & 1] = bits;
                 };
-                if (!r_poll<8>(P, S, want, r_tag(a), addr, sink)) return;
+                r_poll_nb<8>(P, S, want, r_tag(a), addr, sink);
             }
-            if (tid < ncols) {
-                const bool row_sp = (own >> tid) & 1u;
-                double R = row_sp ? 0.0 : S.R[tid];
-                const uint32_t cnt = row_sp ? P.be : nc;
-                for (uint32_t u = 0; u < cnt; ++u) R += S.part[tid][u];
-                S.R[tid] = R;
-                float sv = S.s[tid];
-                if (R > 0.0) sv = (float)((double)S.target[tid] / R);
-                S.s[tid] = sv;
-                if (k + 1 < n) st_c(P.sx + (size_t)r_slot(a + 1) * P.ne + j0 + tid, r_word(a + 1, __float_as_uint(sv)));
+            // A spiking row: R = the xor butterfly over 64 lane sums, lane l summing the parts of
+            // blocks l, l + 64, ... in order.  Another row: R plus its contributors' parts in
+            // block order.  Then s_i = target_i / R_i (R_i > 0), published for step a + 1.
+            for (uint32_t m = 0; m < ns; ++m) {
+                const uint32_t r = r_nth_bit(spm, m);
+                double acc = 0.0;
+                for (uint32_t u = lane; u < P.be; u += 64u) acc += S.part[r][u];
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+                if (lane == r) S.R[r] = acc;
+            }
+            if (lane < ncols_k) {
+                double R = S.R[lane];
+                if (!((spm >> lane) & 1u))
+                    for (uint32_t u = 0; u < nc; ++u) R += S.part[lane][u];
+                S.R[lane] = R;
+                float sv = S.s[lane];
+                if (R > 0.0) sv = (float)((double)S.target[lane] / R);
+                S.s[lane] = sv;
+                if (k + 1 < n)
+                    st_c(P.sx + (size_t)r_slot(a + 1) * P.ne + j0_k + lane, r_word(a + 1, __float_as_uint(sv)));
             }
         }
+        __syncthreads();
+        if (S.abort) return;
         sp_prev = sp_now;
     }
     // ---- after the last step: the block's neurons (with the last step's stimulus events), its
     // rows' sums and scales, the rows' traces (block 0 holds every row), the plastic entries.
+    const uint32_t pl = (t0 + n - 1u) & 1u;
     if (tid < ncols) {
         const uint32_t lid = j0 + tid;
         const NeuronGroupView& v = G;
@@ -639,10 +745,10 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
         gp(v.g_inhibitory)[lid] = S.gi[tid];
         gp(P.spikes)[gid] = S.spike[tid];
         if (epost && P.ee_kind != kRNone) {
-            gp(P.ee_tr[2])[lid] = S.post[0][tid];
-            if (P.ee_kind == kPlastTriplet) gp(P.ee_tr[3])[lid] = S.post[1][tid];
+            gp(P.ee_tr[2])[lid] = S.post[pl][0][tid];
+            if (P.ee_kind == kPlastTriplet) gp(P.ee_tr[3])[lid] = S.post[pl][1][tid];
         }
-        if (epost && P.ie_kind != kRNone) gp(P.ie_tr[1])[lid] = S.post[2][tid];
+        if (epost && P.ie_kind != kRNone) gp(P.ie_tr[1])[lid] = S.post[pl][2][tid];
         if (epost && P.ee_lazy) {
             gp(P.R)[lid] = S.R[tid];
             gp(P.s)[lid] = S.s[tid];

@@ -642,13 +642,18 @@ struct Sim {
                         const float w = clampw<float>(P.s[i] * row[j] + (float)ltd_delta(P, i, j), (float)P.wmin, (float)P.wmax);
                         row[j] = w / P.s[i];
                     }
-                double rs = 0.0;
-                for (uint32_t b = 0; b < nblk; ++b) {
-                    double pb = 0.0;
-                    for (uint32_t j = b * Cb; j < std::min(b * Cb + Cb, nq); ++j) pb += (double)row[j];
-                    rs += pb;
+                // Block parts P_b; lane l of 64 sums P_l, P_{l+64}, ... from 0.0; xor butterfly.
+                double lane[64];
+                for (uint32_t l = 0; l < 64; ++l) {
+                    double acc = 0.0;
+                    for (uint32_t b = l; b < nblk; b += 64) {
+                        double pb = 0.0;
+                        for (uint32_t j = b * Cb; j < std::min(b * Cb + Cb, nq); ++j) pb += (double)row[j];
+                        acc += pb;
+                    }
+                    lane[l] = acc;
                 }
-                P.Rs[i] = rs;
+                P.Rs[i] = butterfly64(lane);
             } else {
                 double rs = P.Rs[i];
                 for (uint32_t b = 0; b < nblk; ++b) {

@@ -35,9 +35,10 @@
  *     order from 0.0, the 64 lane sums of a wave are combined by the xor butterfly
  *     (32,16,8,4,2,1), and g' = fl32(((double)g + wave_0) + wave_1 + ...) in wave order.  The lazy
  *     row sums follow blocks of C post columns (block b: columns C*b .. C*b + C - 1): a row whose
- *     pre neuron spiked gets R_i = 0.0 + P_0 + P_1 + ..., P_b the block's new entries summed in
- *     column order from 0.0; another row gets R_i = R_i + D_b + ... over the blocks b holding a
- *     spiking post, D_b its LTP changes (double)M'_ij - (double)M_ij summed in column order.
+ *     pre neuron spiked gets R_i = the xor butterfly over 64 lane sums, lane l summing P_l,
+ *     P_{l+64}, ... in order from 0.0, P_b the block's new entries summed in column order from
+ *     0.0; another row gets R_i = R_i + D_b + ... over the blocks b holding a spiking post, D_b
+ *     its LTP changes (double)M'_ij - (double)M_ij summed in column order.
  * precision 32 reproduces the engine bit for bit; precision 64 runs every quantity in FP64
  * (libm exp, plain sequential sums) as an independent accuracy reference.
  *
