@@ -7,10 +7,17 @@ pass of System::step_the_clock (evolve -> propagate -> stimulus -> plasticity ->
 the whole network, with the weights resident in device memory.  Metric: simulated steps per
 second.  --config configs/net10125_stdp.json runs the 10,125-neuron (410 MB) network instead.
 
-Prints ONE JSON line.  roofline = the propagate (GEMV) kernel's algorithmic bytes per launch
-(the four weight matrices plus the pre-synaptic vectors) over its average duration, measured
-with HIP events around back-to-back launches on the engine's stream; cpu_baseline = the
-oracle's FP32 C++/OpenMP step loop on this host, timed on a bounded sample of the same workload.
+The step runs as the resident step when the network fits the device (every weight in the
+registers of one cooperative launch per run; the 3,125-neuron network does), else as the
+graph-captured fused step (--step chooses).
+
+Prints ONE JSON line.  roofline: for the resident step, its one launch is the dominant kernel:
+the step's algorithmic bytes (4 x weight entries + 160 B per neuron) times the measured steps/s,
+against 8 TB/s (the weights stay in registers, so this is the HBM rate a streaming step would
+need to match it); for the graph step, the propagate (GEMV) kernel's algorithmic bytes per
+launch over its average duration, HIP events around back-to-back launches on the engine's
+stream.  cpu_baseline = the oracle's FP32 C++/OpenMP step loop on this host, timed on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -56,6 +63,14 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--config", default=os.path.join(REPO, "configs", "net3125_stdp.json"))
     ap.add_argument("--normalization", choices=("lazy", "faithful"), default="lazy")
+    ap.add_argument("--step", choices=("auto", "resident", "graph"), default="auto",
+                    help="plain: the resident step (weights in registers, one launch per run) when the network "
+                         "fits, else the graph-captured fused step")
+    ap.add_argument("--resident-launch", choices=("cooperative", "plain"), default="cooperative",
+                    help="plain, resident step: a cooperative launch (default) or an ordinary one (same kernel; "
+                         "rocprofv3 traces only the ordinary one to the end of the process)")
+    ap.add_argument("--no-graph-compare", action="store_true",
+                    help="plain, resident step: skip the short comparison run of the graph step")
     ap.add_argument("--gemv-row-blocks", default=None,
                     help="plain: GEMV row blocks EE,EI,IE,II (e.g. 48,48,32,32) instead of the autotuner's choice")
     ap.add_argument("--kernel-steps", type=int, default=100)
@@ -84,34 +99,52 @@ def main() -> int:
         return bench_adjoint(args)
     args.steps = 10000 if args.steps is None else args.steps
     args.warmup = 100 if args.warmup is None else args.warmup
+    return bench_plain(args)
 
+
+def bench_plain(args) -> int:
+    """One network: the resident step when the network fits the device (every weight in
+    registers, one cooperative launch per run), else the graph-captured fused step."""
     eng = importlib.import_module("example-cuda-spiking-nn_amd")
     # Monitors on: the step writes the spike-bit and voltage rings every step, as the reference's
     # step historicizes every step (record() file I/O only happens in System::run, not here).
     flags = ["quiet"] + (["lazy_normalize"] if args.normalization == "lazy" else [])
-    config_path, tune = args.config, None
-    if args.gemv_row_blocks:
-        import tempfile
-        c = json.load(open(args.config))
-        c["gemv_row_blocks"] = [int(r) for r in args.gemv_row_blocks.split(",")]
-        config_path = os.path.join(tempfile.mkdtemp(prefix="bn_bench_rows_"), os.path.basename(args.config))
-        json.dump(c, open(config_path, "w"))
-    elif not args.no_autotune:
-        # GEMV row blocks (the propagate's tile shape and FP64 summation order): the default and a
-        # few variants, each settled and timed briefly; the timed run below uses the fastest.
-        import tempfile
-        wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
-        tune_dir = tempfile.mkdtemp(prefix="bn_bench_rows_")
-        config_path, best_rows, tune_us = wl.autotune_plain_row_blocks(eng, args.config, tune_dir, flags)
-        tune = {"chosen": best_rows, "us_per_step": {k: round(v, 3) for k, v in tune_us.items()},
-                "settle_steps": 1000, "timed_steps": 3000}
-    cfg = eng.config_dump(config_path)
-    sim = eng.Simulation(config_path, flags=flags)
-    # Steps per hipGraph: 500 for long runs (0.8% faster than 100 on the 3,125-neuron network,
-    # 0.3% on the 10,125-neuron one); a shorter timed run is one graph of its own length, rounded
-    # down to an even count (the overlapped form), any odd step eager.
-    steps_per_graph = 500 if args.steps >= 500 else max(2, args.steps - args.steps % 2)
-    sim.set_steps_per_graph(steps_per_graph)
+    cfg = eng.config_dump(args.config)
+    name = os.path.splitext(os.path.basename(args.config))[0]
+    n_e, n_i = int(cfg["num_excitatory_neurons"]), int(cfg["num_inhibitory_neurons"])
+    resident = args.step in ("auto", "resident") and args.normalization == "lazy"
+    sim = None
+    if resident:
+        try:
+            sim = eng.Simulation(args.config, flags=flags + ["resident"])
+            sim.set_resident_cooperative(args.resident_launch == "cooperative")
+        except eng.BnError as e:
+            if args.step == "resident" or "does not fit" not in str(e):
+                raise
+            resident = False
+    config_path, tune, steps_per_graph = args.config, None, None
+    if not resident:
+        if args.gemv_row_blocks:
+            import tempfile
+            c = json.load(open(args.config))
+            c["gemv_row_blocks"] = [int(r) for r in args.gemv_row_blocks.split(",")]
+            config_path = os.path.join(tempfile.mkdtemp(prefix="bn_bench_rows_"), os.path.basename(args.config))
+            json.dump(c, open(config_path, "w"))
+        elif not args.no_autotune:
+            # GEMV row blocks (the propagate's tile shape and FP64 summation order): the default
+            # and a few variants, each settled and timed briefly; the timed run uses the fastest.
+            import tempfile
+            wl = importlib.import_module("example-cuda-spiking-nn_amd.workloads")
+            tune_dir = tempfile.mkdtemp(prefix="bn_bench_rows_")
+            config_path, best_rows, tune_us = wl.autotune_plain_row_blocks(eng, args.config, tune_dir, flags)
+            tune = {"chosen": best_rows, "us_per_step": {k: round(v, 3) for k, v in tune_us.items()},
+                    "settle_steps": 1000, "timed_steps": 3000}
+        sim = eng.Simulation(config_path, flags=flags)
+        # Steps per hipGraph: 500 for long runs (0.8% faster than 100 on the 3,125-neuron
+        # network, 0.3% on the 10,125-neuron one); a shorter timed run is one graph of its own
+        # length, rounded down to an even count (the overlapped form), any odd step eager.
+        steps_per_graph = 500 if args.steps >= 500 else max(2, args.steps - args.steps % 2)
+        sim.set_steps_per_graph(steps_per_graph)
     info = sim.info
     if args.settle:
         sim.run_timed(args.settle)
@@ -125,41 +158,80 @@ def main() -> int:
     # Firing over the timed window, from the device spike ring (the last 10,000 steps at most).
     window = min(args.steps, 10000)
     raster = sim.spike_raster(first_timed + args.steps - window, window)
-    rate_e = float(raster[:, : info.num_excitatory_neurons].mean()) / cfg["timestep"]
-    rate_i = float(raster[:, info.num_excitatory_neurons:].mean()) / cfg["timestep"]
+    rate_e = float(raster[:, :n_e].mean()) / cfg["timestep"]
+    rate_i = float(raster[:, n_e:].mean()) / cfg["timestep"]
     rate_all = float(raster.mean()) / cfg["timestep"]
     del raster
-
-    # Dominant kernel: HIP events around back-to-back propagate (GEMV) launches over the current
-    # weights (the simulation's pending partial sums are saved and restored by the engine).
-    k_ms, s_ms = sim.time_propagate_kernel(args.kernel_steps)
-    k_avg_s = k_ms / 1000.0 / args.kernel_steps
     n_w = sum(int(info.weight_pre[k]) * int(info.weight_post[k]) for k in range(4))
-    gemv_bytes = 4 * n_w + 4 * sum(int(info.weight_pre[k]) for k in range(4))
-    achieved = gemv_bytes / k_avg_s / 1e9
-    sim.close()
-    # The applicable read bandwidth for this weight set: a plain streaming read kernel over a
-    # working set of the same size (MALL-resident below 256 MB, HBM-bound above).
-    read_gbs = eng.measure_read_bandwidth(4 * n_w, 50)
-    # The same working set read 200 times inside one launch: the steady-state rate of the level
-    # of the memory hierarchy that holds it, without launch ramp and drain.
-    read_gbs_one = eng.measure_read_bandwidth_one_launch(4 * n_w, 200)
-
     n_total = int(info.total_neurons)
-    name = os.path.splitext(os.path.basename(args.config))[0]
+    step_bytes = 4 * n_w + 160 * n_total
+    gemv_bytes = 4 * n_w + 4 * sum(int(info.weight_pre[k]) for k in range(4))
+    traffic_file = os.path.join(REPO, "profiles", f"{'k_resident' if resident else 'k_gemv'}_{name}_hbm_traffic.json")
     traffic = None
-    tpath = os.path.join(REPO, "profiles", f"k_gemv_{name}_hbm_traffic.json")
-    if os.path.exists(tpath):
-        t = json.load(open(tpath))
-        if (t.get("neurons") == n_total and t.get("weight_entries") == n_w
-                and t.get("gemv_row_blocks") == [int(r) for r in info.gemv_row_block]):
-            traffic = t.get("hbm_bytes_per_launch")
+    if os.path.exists(traffic_file):
+        t = json.load(open(traffic_file))
+        if t.get("neurons") == n_total and t.get("weight_entries") == n_w and (
+                resident or t.get("gemv_row_blocks") == [int(r) for r in info.gemv_row_block]):
+            traffic = t.get("hbm_bytes_per_launch") if not resident else t.get("hbm_bytes_per_step")
+            if resident and traffic is not None:
+                traffic = int(traffic) * args.steps   # the timed launch's steps
+    graph_cmp = None
+    if resident:
+        # The dominant (only) kernel is the resident launch itself: the whole timed run.  Its
+        # algorithmic bytes are the step's (the weights read once per step and ~40 state words per
+        # neuron), although the weights stay in registers: the figure is the HBM rate a streaming
+        # step would need to match it.
+        achieved = step_bytes * steps_per_s / 1e9
+        roofline = {"bound": "hbm", "kernel": f"k_resident (one {args.resident_launch} launch per run)",
+                    "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_unit": "HBM bytes of the timed launch (rocprofv3 FETCH_SIZE + WRITE_SIZE per step of a "
+                                    "10,000-step launch, times its steps; the weights stay in registers)",
+                    "algorithmic_bytes_per_step": step_bytes,
+                    "algorithmic_bytes_per_launch": step_bytes * args.steps,
+                    "avg_kernel_us": round(ms * 1000.0, 2), "us_per_step": round(ms * 1000.0 / args.steps, 3),
+                    "resident_blocks": int(info.resident_blocks), "resident_rows": int(info.resident_rows),
+                    "step_bytes_B_step": step_bytes, "step_GBs": round(achieved, 1),
+                    "step_frac_of_peak": round(achieved / HBM_PEAK_GBS, 4)}
+        sim.close()
+        if not args.no_graph_compare:
+            # The graph-captured fused step on the same network (engine default row blocks), for
+            # comparison: settled, 2,000 steps in graphs of 500.
+            g = eng.Simulation(args.config, flags=flags)
+            g.set_steps_per_graph(500)
+            g.run_timed(args.settle or 1000)
+            gms = g.run_timed(2000)
+            graph_cmp = {"steps_per_s": round(2000 / (gms / 1000.0), 1), "us_per_step": round(gms * 1000.0 / 2000, 3),
+                         "steps": 2000, "steps_per_graph": 500}
+            g.close()
+    else:
+        # Dominant kernel: HIP events around back-to-back propagate (GEMV) launches over the
+        # current weights (the simulation's pending partial sums are saved and restored).
+        k_ms, s_ms = sim.time_propagate_kernel(args.kernel_steps)
+        k_avg_s = k_ms / 1000.0 / args.kernel_steps
+        achieved = gemv_bytes / k_avg_s / 1e9
+        sim.close()
+        # The applicable read bandwidth for this weight set: a plain streaming read kernel over a
+        # working set of the same size (MALL-resident below 256 MB, HBM-bound above), and the
+        # same working set read 200 times inside one launch (the steady rate of the level of the
+        # memory hierarchy that holds it, without launch ramp and drain).
+        read_gbs = eng.measure_read_bandwidth(4 * n_w, 50)
+        read_gbs_one = eng.measure_read_bandwidth_one_launch(4 * n_w, 200)
+        roofline = {"bound": "hbm", "kernel": "k_gemv (propagate)", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic, "algorithmic_bytes_per_launch": gemv_bytes,
+                    "avg_kernel_us": round(k_avg_s * 1e6, 2),
+                    "measured_read_GBs_same_working_set": round(read_gbs, 1),
+                    "measured_read_GBs_same_working_set_one_launch_200_passes": round(read_gbs_one, 1),
+                    "frac_of_measured_read": round(achieved / read_gbs, 4),
+                    "step_bytes_B_step": step_bytes,
+                    "step_GBs": round(step_bytes * steps_per_s / 1e9, 1),
+                    "step_frac_of_peak": round(step_bytes * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)}
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(config_path, cfg, args.cpu_seconds, info)
 
-    n_e, n_i = int(cfg["num_excitatory_neurons"]), int(cfg["num_inhibitory_neurons"])
     line = {
         "metric": f"sim steps/s ({n_total:,}-nrn STDP net)",
         "value": round(steps_per_s, 2),
@@ -177,22 +249,16 @@ def main() -> int:
         "config": {"workload": f"{name} ({n_e:,} E + {n_i:,} I, 4 dense FP32 matrices, {4 * n_w / 1e6:.1f} MB, "
                                f"triplet STDP + iSTDP, seed {cfg['rng_seed']}, {args.normalization} normalisation)",
                    "neurons": n_total, "weight_entries": n_w, "normalization": args.normalization,
+                   "step": "resident" if resident else "graph",
+                   "resident_launch": args.resident_launch if resident else None,
                    "steps_per_graph": steps_per_graph, "graph_launches_timed": graph_launches,
                    "kernels_per_step": int(info.kernels_per_step), "parallelism": "single GPU",
-                   "gemv_row_blocks": [int(r) for r in info.gemv_row_block], "row_block_autotune": tune,
+                   "gemv_row_blocks": None if resident else [int(r) for r in info.gemv_row_block],
+                   "row_block_autotune": tune, "graph_step_same_network": graph_cmp,
                    "timed_window_rate_hz": {"all": round(rate_all, 3), "excitatory": round(rate_e, 3),
                                             "inhibitory": round(rate_i, 3), "steps": window},
                    "first_timed_step": first_timed},
-        "roofline": {"bound": "hbm", "kernel": "k_gemv (propagate)", "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "algorithmic_bytes_per_launch": gemv_bytes,
-                     "avg_kernel_us": round(k_avg_s * 1e6, 2),
-                     "measured_read_GBs_same_working_set": round(read_gbs, 1),
-                     "measured_read_GBs_same_working_set_one_launch_200_passes": round(read_gbs_one, 1),
-                     "frac_of_measured_read": round(achieved / read_gbs, 4),
-                     "step_bytes_B_step": 4 * n_w + 160 * n_total,
-                     "step_GBs": round((4 * n_w + 160 * n_total) * steps_per_s / 1e9, 1),
-                     "step_frac_of_peak": round((4 * n_w + 160 * n_total) * steps_per_s / 1e9 / HBM_PEAK_GBS, 4)},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -96,8 +96,8 @@ struct RPlan {
 void resident_shape(uint32_t ne, uint32_t ni, uint32_t* rows, uint32_t* e_waves);
 // Largest number of co-resident blocks of the kernel for `rows` rows per thread.
 int resident_max_blocks(uint32_t rows, uint32_t* blocks);
-// n steps from the state (cooperative launch of host_plan.blocks blocks), then the clock
-// advance of n steps and the stimulus cursor.
-int launch_resident(const RPlan* d_plan, const RPlan& host_plan, uint32_t n, bn_stream_t s);
+// n steps from the state (one launch of host_plan.blocks blocks, cooperative or ordinary), the
+// clock advanced by n steps with its stimulus cursor.
+int launch_resident(const RPlan* d_plan, const RPlan& host_plan, uint32_t n, bool cooperative, bn_stream_t s);
 
 }  // namespace bn

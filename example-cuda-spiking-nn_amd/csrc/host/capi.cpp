@@ -515,6 +515,9 @@ int measure_read_bandwidth(uint64_t bytes, uint32_t repeats, bool in_one_launch,
         return set_error(BN_ERR_INVALID_ARGUMENT, "bn_measure_read_bandwidth: bad argument");
     return guarded([&] {
         bytes &= ~(uint64_t)15;
+        // One launch reads at most 4 GiB (well under a second at HBM rate): a dispatch running for
+        // seconds looks like a hung queue.
+        if (in_one_launch) repeats = (uint32_t)std::min<uint64_t>(repeats, std::max<uint64_t>(1, (4ull << 30) / bytes));
         void* buf = nullptr;
         float* sink = nullptr;
         hipStream_t s = nullptr;
@@ -575,6 +578,11 @@ int bn_sim_set_persistent_rows(bn_sim* sim, const uint32_t rows[4]) {
 int bn_sim_set_persistent_resident(bn_sim* sim, int allow) {
     if (!sim) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_persistent_resident: null simulation");
     return guarded([&] { sim->system->set_persistent_resident(allow != 0); });
+}
+
+int bn_sim_set_resident_cooperative(bn_sim* sim, int cooperative) {
+    if (!sim) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_resident_cooperative: null simulation");
+    return guarded([&] { sim->system->set_resident_cooperative(cooperative != 0); });
 }
 
 int bn_sim_set_persistent_wait_limit(bn_sim* sim, uint32_t polls) {

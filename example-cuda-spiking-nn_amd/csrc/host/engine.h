@@ -461,6 +461,10 @@ public:
     // (bn_resident.h).  Builds the plan; the state is untouched.
     void enable_resident();
     bool resident() const { return resident_; }
+    // Cooperative launch (default) or an ordinary one (same kernel; co-residency then rests on
+    // an idle device, the bounded waits turning a shortfall into an error).
+    void set_resident_cooperative(bool cooperative);
+    bool resident_cooperative() const { return resident_cooperative_; }
     const RPlan& resident_plan() const { return resident_plan_; }
 
     uint32_t clock_step() const { return clock_step_; }
@@ -525,7 +529,7 @@ private:
     uint32_t persist_items_ = 0;
     bool persist_resident_ = true;
     std::array<uint32_t, 4> persist_rows_{};
-    uint32_t persist_wait_limit_ = 1u << 22;  // about 4 s of polling
+    uint32_t persist_wait_limit_ = 1u << 18;  // about a quarter second of polling
     std::vector<void*> persist_bufs_;
     std::vector<std::pair<void*, size_t>> persist_ll_bufs_;  // tagged-word buffers (64-bit words)
     uint32_t persist_ll_next_ = 0;                          // the clock after the last resident launch
@@ -535,6 +539,7 @@ private:
     RPlan* d_resident_plan_ = nullptr;
     std::vector<std::pair<void*, size_t>> resident_bufs_;  // tagged-word buffers (64-bit words)
     uint32_t resident_next_ = 0;                           // the clock after the last launch
+    bool resident_cooperative_ = true;
 };
 
 }  // namespace bn

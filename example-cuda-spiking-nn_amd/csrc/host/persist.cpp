@@ -72,6 +72,10 @@ void* System::persist_alloc(size_t bytes) {
     BN_HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)), "hipMalloc persistent step");
     persist_bufs_.push_back(p);
     BN_HIP_CHECK(hipMemsetAsync(p, 0, std::max<size_t>(bytes, 16), state_.stream()), "hipMemset persistent step");
+    // The zeroing has finished before the caller fills the buffer: its plans, item tables and
+    // block maps go up by hipMemcpy on the null stream, which does not wait for the state's
+    // stream (created non-blocking), so a zeroing still queued there could land after them.
+    BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "hipMemset persistent step sync");
     return p;
 }
 

@@ -144,7 +144,12 @@ void System::enable_resident() {
     BN_HIP_CHECK(hipStreamSynchronize(S.stream()), "resident plan sync");
 }
 
-// One cooperative launch per run (its tagged words carry their steps, so consecutive launches
+void System::set_resident_cooperative(bool cooperative) {
+    if (!resident_) throw Error(BN_ERR_RUNTIME, "not a resident simulation");
+    resident_cooperative_ = cooperative;
+}
+
+// One launch per run (its tagged words carry their steps, so consecutive launches
 // need no clearing); after a state restore moved the clock back, the words are cleared first.
 void System::launch_resident_steps(uint32_t n) {
     if (n == 0) return;
@@ -153,7 +158,8 @@ void System::launch_resident_steps(uint32_t n) {
         for (const auto& b : resident_bufs_)
             BN_HIP_CHECK((hipError_t)launch_fill_u32(static_cast<uint32_t*>(b.first), 0u, b.second * 2, s),
                          "resident tagged words");
-    BN_HIP_CHECK((hipError_t)launch_resident(d_resident_plan_, resident_plan_, n, s), "resident step launch");
+    BN_HIP_CHECK((hipError_t)launch_resident(d_resident_plan_, resident_plan_, n, resident_cooperative_, s),
+                 "resident step launch");
     clock_step_ += n;
     resident_next_ = clock_step_;
 }

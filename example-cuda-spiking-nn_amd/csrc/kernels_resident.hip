@@ -306,6 +306,61 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
     const bn_device_clock clk = *P.clock;
     const uint32_t t0 = clk.clock_step;
     if (tid == 0) S.abort = 0u;
+    // The launch's loads go out together, the small ones first (the rows' traces and scales,
+    // the block's neurons), then the weights; the LDS copies of the small ones wait only for
+    // those.  Indices past a group are clamped to row 0 (the value is then discarded), so no
+    // lane branches around a load.
+    float rv[K], dv[K], pv[K], tv[K], sv[K];
+    uint32_t spv[K];
+    {
+        const BN_GAS float* pr = gp(P.r[side]);
+        const BN_GAS float* pd = gp(P.d[side]);
+        const BN_GAS uint32_t* ps = gp(P.spikes) + pre_base_of(P, eside);
+        const BN_GAS float* ppre = gp(eside ? P.ee_tr[0] : P.ie_tr[0]);
+        const BN_GAS float* ptp = gp(P.ee_tr[1]);
+        const BN_GAS float* psc = gp(P.s);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t i = tl + T * (uint32_t)k;
+            const uint32_t ii = i < npre ? i : 0u;
+            rv[k] = pr[ii];
+            dv[k] = pd[ii];
+            spv[k] = ps[ii];
+            pv[k] = pkind != kRNone ? ppre[ii] : 0.0f;
+            tv[k] = triplet ? ptp[ii] : 0.0f;
+            sv[k] = lazy ? psc[ii] : 1.0f;
+        }
+    }
+    // The block's neurons (threads [0, ncols)).
+    const bool holder = tid < ncols;
+    const uint32_t lid0 = j0 + (holder ? tid : 0u);
+    float nV = 0.0f, nistim = 0.0f, nre = 0.0f, nri = 0.0f, nde = 0.0f, ndi = 0.0f, nge = 0.0f, ngi = 0.0f;
+    float np0 = 0.0f, np1 = 0.0f, np2 = 0.0f, ns = 0.0f, ntg = 0.0f;
+    uint32_t nrefr = 0, nspike = 0;
+    int32_t nvrec = -1;
+    double nR = 0.0;
+    if (wave == 0) {
+        const NeuronGroupView& v = G;
+        nV = gp(v.membrane_potentials)[lid0];
+        nrefr = gp(v.refractory_counter)[lid0];
+        nistim = gp(v.i_stimulus)[lid0];
+        nre = gp(v.background_excitatory_postsynaptic_r_traces)[lid0];
+        nri = gp(v.background_inhibitory_postsynaptic_r_traces)[lid0];
+        nde = gp(v.background_excitatory_postsynaptic_d_traces)[lid0];
+        ndi = gp(v.background_inhibitory_postsynaptic_d_traces)[lid0];
+        nge = gp(v.g_excitatory)[lid0];
+        ngi = gp(v.g_inhibitory)[lid0];
+        nspike = gp(P.spikes)[v.offset + lid0];
+        if (epost && P.ee_kind != kRNone) np0 = gp(P.ee_tr[2])[lid0];
+        if (epost && P.ee_kind == kPlastTriplet) np1 = gp(P.ee_tr[3])[lid0];
+        if (epost && P.ie_kind != kRNone) np2 = gp(P.ie_tr[1])[lid0];
+        if (P.v_ring) nvrec = gp(P.vrec_index)[v.offset + lid0];
+        if (epost && P.ee_lazy) {
+            nR = gp(P.R)[lid0];
+            ns = gp(P.s)[lid0];
+            ntg = gp(P.target)[lid0];
+        }
+    }
     // The entries of the block's columns in the thread's rows (padding columns and rows: 0).
     v16f w[K];
     {
@@ -318,49 +373,47 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             const BN_GAS v4f* src = reinterpret_cast<const BN_GAS v4f*>(W + (size_t)(ok ? i : 0u) * ld + j0);
 #pragma unroll
             for (int q = 0; q < kRCols / 4; ++q) {
-                const v4f x = ok ? src[q] : v4f{0.0f, 0.0f, 0.0f, 0.0f};
+                const v4f x = src[q];
 #pragma unroll
-                for (int e = 0; e < 4; ++e) w[k][4 * q + e] = (uint32_t)(4 * q + e) < ncols ? x[e] : 0.0f;
+                for (int e = 0; e < 4; ++e) w[k][4 * q + e] = ok && (uint32_t)(4 * q + e) < ncols ? x[e] : 0.0f;
             }
         }
     }
     // The rows' traces and scales; bit k of sp_prev: row k's neuron spiked at the step before.
     uint32_t sp_prev = 0;
+    // Entries the launch changes: rows with an LTD (per thread) and columns with an LTP (per
+    // block); only those are stored back after the last step.
+    uint32_t dirty_rows = 0, dirty_cols = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const uint32_t i = tl + T * (uint32_t)k;
-        const bool ok = i < npre;
-        S.r[k][tid] = ok ? gp(P.r[side])[i] : 0.0f;
-        S.d[k][tid] = ok ? gp(P.d[side])[i] : 0.0f;
-        S.pre[k][tid] = ok && pkind != kRNone ? gp(eside ? P.ee_tr[0] : P.ie_tr[0])[i] : 0.0f;
-        S.tpre[k][tid] = ok && triplet ? gp(P.ee_tr[1])[i] : 0.0f;
-        S.sc[k][tid] = ok && lazy ? gp(P.s)[i] : 1.0f;
-        if (ok && gp(P.spikes)[pre_base_of(P, eside) + i] == 1u) sp_prev |= 1u << k;
+        const bool ok = tl + T * (uint32_t)k < npre;
+        S.r[k][tid] = ok ? rv[k] : 0.0f;
+        S.d[k][tid] = ok ? dv[k] : 0.0f;
+        S.pre[k][tid] = ok ? pv[k] : 0.0f;
+        S.tpre[k][tid] = ok ? tv[k] : 0.0f;
+        S.sc[k][tid] = ok ? sv[k] : 1.0f;
+        if (ok && spv[k] == 1u) sp_prev |= 1u << k;
     }
     // The block's neurons (post traces in the buffer of the step before the launch).
-    if (tid < ncols) {
-        const uint32_t lid = j0 + tid;
-        const NeuronGroupView& v = G;
+    if (holder) {
         float(&pp)[3][kRCols] = S.post[(t0 - 1u) & 1u];
-        S.V[tid] = gp(v.membrane_potentials)[lid];
-        S.refr[tid] = gp(v.refractory_counter)[lid];
-        S.istim[tid] = gp(v.i_stimulus)[lid];
-        S.re[tid] = gp(v.background_excitatory_postsynaptic_r_traces)[lid];
-        S.ri[tid] = gp(v.background_inhibitory_postsynaptic_r_traces)[lid];
-        S.de[tid] = gp(v.background_excitatory_postsynaptic_d_traces)[lid];
-        S.di[tid] = gp(v.background_inhibitory_postsynaptic_d_traces)[lid];
-        S.ge[tid] = gp(v.g_excitatory)[lid];
-        S.gi[tid] = gp(v.g_inhibitory)[lid];
-        S.spike[tid] = gp(P.spikes)[v.offset + lid];
-        pp[0][tid] = epost && P.ee_kind != kRNone ? gp(P.ee_tr[2])[lid] : 0.0f;
-        pp[1][tid] = epost && P.ee_kind == kPlastTriplet ? gp(P.ee_tr[3])[lid] : 0.0f;
-        pp[2][tid] = epost && P.ie_kind != kRNone ? gp(P.ie_tr[1])[lid] : 0.0f;
-        S.vrec[tid] = P.v_ring ? gp(P.vrec_index)[v.offset + lid] : -1;
-        if (epost && P.ee_lazy) {
-            S.R[tid] = gp(P.R)[lid];
-            S.s[tid] = gp(P.s)[lid];
-            S.target[tid] = gp(P.target)[lid];
-        }
+        S.V[tid] = nV;
+        S.refr[tid] = nrefr;
+        S.istim[tid] = nistim;
+        S.re[tid] = nre;
+        S.ri[tid] = nri;
+        S.de[tid] = nde;
+        S.di[tid] = ndi;
+        S.ge[tid] = nge;
+        S.gi[tid] = ngi;
+        S.spike[tid] = nspike;
+        pp[0][tid] = np0;
+        pp[1][tid] = np1;
+        pp[2][tid] = np2;
+        S.vrec[tid] = nvrec;
+        S.R[tid] = nR;
+        S.s[tid] = ns;
+        S.target[tid] = ntg;
     }
     if (tid < 4) {
         S.heads[tid] = tid < G.poisson_table_excitatory_len ? gp(G.poisson_table_excitatory)[tid] : 0u;
@@ -720,6 +773,10 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
         }
         __syncthreads();
         if (S.abort) return;
+        if (pkind != kRNone) {
+            dirty_rows |= sp_now;
+            dirty_cols |= own;
+        }
         sp_prev = sp_now;
     }
     // ---- after the last step: the block's neurons (with the last step's stimulus events), its
@@ -766,27 +823,36 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             if (triplet) gp(P.ee_tr[1])[i] = S.tpre[q][tid];
         }
     }
-    if (pkind != kRNone) {
+    if (pkind != kRNone && (dirty_cols != 0u || dirty_rows != 0u)) {
+        // A row whose neuron spiked, or every row when a column spiked: the row's segment (four
+        // 16-byte stores when the block has all 16 columns).
         BN_GAS float* W = gp(P.W[conn]);
         const uint32_t ld = P.ld[conn];
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             const uint32_t i = tl + T * (uint32_t)q;
-            if (i >= npre) continue;
+            if (i >= npre || (dirty_cols == 0u && !((dirty_rows >> q) & 1u))) continue;
+            BN_GAS float* dst = W + (size_t)i * ld + j0;
+            if (ncols == (uint32_t)kRCols) {
 #pragma unroll
-            for (int c = 0; c < kRCols; ++c)
-                if ((uint32_t)c < ncols) W[(size_t)i * ld + j0 + c] = w[q][c];
+                for (int c4 = 0; c4 < kRCols / 4; ++c4)
+                    reinterpret_cast<BN_GAS v4f*>(dst)[c4] =
+                        v4f{w[q][4 * c4], w[q][4 * c4 + 1], w[q][4 * c4 + 2], w[q][4 * c4 + 3]};
+            } else {
+#pragma unroll
+                for (int c = 0; c < kRCols; ++c)
+                    if ((uint32_t)c < ncols) dst[c] = w[q][c];
+            }
         }
     }
-}
-
-// The clock after n steps (one thread, after the launch): step and stimulus cursor.
-__global__ void k_resident_clock(const RPlan* __restrict__ plan, uint32_t n) {
-    const RPlan& P = *plan;
-    bn_device_clock c = *P.clock;
-    c.clock_step += n;
-    if (P.ev_steps) c.next_event_idx = P.ev_start[min(c.clock_step, P.ev_steps)];
-    *P.clock = c;
+    // The clock after the n steps (step and stimulus cursor), by block 0 at its end: every block
+    // read the clock before it published its first spike word, which block 0 has seen.
+    if (b == 0 && tid == 0) {
+        bn_device_clock c = clk;
+        c.clock_step = t0 + n;
+        if (P.ev_steps) c.next_event_idx = P.ev_start[min(c.clock_step, P.ev_steps)];
+        *P.clock = c;
+    }
 }
 
 namespace {
@@ -831,14 +897,12 @@ int resident_max_blocks(uint32_t rows, uint32_t* blocks) {
     return (int)e;
 }
 
-int launch_resident(const RPlan* d_plan, const RPlan& hp, uint32_t n, bn_stream_t s) {
+int launch_resident(const RPlan* d_plan, const RPlan& hp, uint32_t n, bool cooperative, bn_stream_t s) {
     const void* f = resident_kernel(hp.rows);
     if (!f || n == 0) return (int)hipErrorInvalidValue;
     void* args[] = {(void*)&d_plan, (void*)&n};
-    hipError_t e = hipLaunchCooperativeKernel(f, dim3(hp.blocks), dim3(kRThreads), args, 0, s);
-    if (e != hipSuccess) return (int)e;
-    hipLaunchKernelGGL(k_resident_clock, dim3(1), dim3(1), 0, s, d_plan, n);
-    return (int)hipGetLastError();
+    if (!cooperative) return (int)hipLaunchKernel(f, dim3(hp.blocks), dim3(kRThreads), args, 0, s);
+    return (int)hipLaunchCooperativeKernel(f, dim3(hp.blocks), dim3(kRThreads), args, 0, s);
 }
 
 }  // namespace bn

@@ -121,6 +121,7 @@ EXPORTED_SYMBOLS = [
     "bn_window_get_info", "bn_window_get_gradient", "bn_measure_read_bandwidth", "bn_sim_save_state",
     "bn_sim_load_state", "bn_sim_get_spike_raster", "bn_measure_read_bandwidth_one_launch",
     "bn_sim_set_persistent_rows", "bn_sim_set_persistent_wait_limit", "bn_sim_set_persistent_resident",
+    "bn_sim_set_resident_cooperative",
 ]
 
 _lib = None
@@ -160,6 +161,7 @@ def lib() -> C.CDLL:
     L.bn_sim_set_persistent_rows.argtypes = [P, up]
     L.bn_sim_set_persistent_wait_limit.argtypes = [P, U]
     L.bn_sim_set_persistent_resident.argtypes = [P, I]
+    L.bn_sim_set_resident_cooperative.argtypes = [P, I]
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
     L.bn_sim_set_steps_per_graph.argtypes = [P, U]
     L.bn_measure_read_bandwidth.argtypes = [C.c_uint64, U, fp]
@@ -340,6 +342,10 @@ class Simulation:
         """Allow or forbid register-resident weights in the persistent step (re-plans)."""
         _check(lib().bn_sim_set_persistent_resident(self.h, 1 if allow else 0))
         self.info = self.get_info()
+
+    def set_resident_cooperative(self, cooperative: bool) -> None:
+        """Cooperative (default) or ordinary launches of the resident step (same kernel)."""
+        _check(lib().bn_sim_set_resident_cooperative(self.h, 1 if cooperative else 0))
 
     def set_persistent_wait_limit(self, polls: int) -> None:
         _check(lib().bn_sim_set_persistent_wait_limit(self.h, polls))

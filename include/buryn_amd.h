@@ -420,9 +420,10 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
  * the device); re-plans the work only.  Changes the GEMV summation order (results move by
  * rounding).  Persistent simulations only. */
 int bn_sim_set_persistent_rows(bn_sim* sim, const uint32_t rows[4]);
-/* Polls a block of the persistent step makes while it waits for the previous step's evolves
- * before it gives up: the launch then ends early on every block, and the next synchronisation
- * reports BN_ERR_RUNTIME (the state is no longer valid).  Default 4,194,304 (about 4 s). */
+/* Polls a block of the persistent or resident step makes while it waits for another block's
+ * words before it gives up: the launch then ends early on every block, and the next
+ * synchronisation reports BN_ERR_RUNTIME (the state is no longer valid).  Default 262,144 (about
+ * a quarter second; a step's waits last microseconds). */
 int bn_sim_set_persistent_wait_limit(bn_sim* sim, uint32_t polls);
 /* Allow (1, the default) or forbid (0) resident weights in the persistent step: when every work
  * item fits a block of its own with chunks of at most 32 rows, each thread keeps its weight
@@ -430,6 +431,13 @@ int bn_sim_set_persistent_wait_limit(bn_sim* sim, uint32_t polls);
  * rows that changed when the launch ends.  Same results either way; re-plans the work.
  * Persistent simulations only. */
 int bn_sim_set_persistent_resident(bn_sim* sim, int allow);
+/* How the resident step (BN_SIM_RESIDENT) launches its run (no counterpart in the reference):
+ * cooperatively (1, the default: the runtime admits the launch only with every block resident at
+ * once, and orders it against other cooperative launches) or as an ordinary launch (0: about 14 us
+ * less per launch on MI355X; the blocks are co-resident only if no other kernel holds the CUs,
+ * and when one does, the bounded waits end the launch with BN_ERR_RUNTIME instead of a hang).
+ * The kernel and its results are the same.  Resident simulations only. */
+int bn_sim_set_resident_cooperative(bn_sim* sim, int cooperative);
 /* State snapshot / resume (no counterpart in the reference): every device buffer of the
  * simulation (neuron and background state, weights, connection and plasticity traces, the
  * pending propagate partials, lazy row sums and scales, monitor rings, the device clock and
@@ -445,7 +453,9 @@ int bn_sim_load_state(bn_sim* sim, const char* path);
 int bn_measure_read_bandwidth(uint64_t bytes, uint32_t repeats, float* gb_per_s);
 /* The same read rate with all `passes` passes inside one launch (4,096 blocks of 256 threads, each
  * pass starting from a rotated offset so that no XCD re-reads its own L2 lines): the steady-state
- * streaming rate without launch ramp and drain.  Measurement aid, no counterpart in the reference. */
+ * streaming rate without launch ramp and drain.  The passes are capped so that the launch reads
+ * at most 4 GiB (well under a second; a multi-second dispatch reads as a hung queue), and the
+ * rate uses the passes run.  Measurement aid, no counterpart in the reference. */
 int bn_measure_read_bandwidth_one_launch(uint64_t bytes, uint32_t passes, float* gb_per_s);
 /* Writes the JSON of the parsed config (all fields, after defaults/overrides/prefixing). */
 int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, size_t* needed);

@@ -95,6 +95,21 @@ def test_resident_small_bit_exact(tmp_path, which):
     sim.close()
 
 
+def test_resident_ordinary_launch_bit_exact(tmp_path):
+    # The same kernel launched without the cooperative API (the device is otherwise idle, so
+    # every block is resident): the same raster, state, traces and weights.
+    path = _write(tmp_path, small_config(steps=800))
+    sim = _resident(path)
+    sim.set_resident_cooperative(False)
+    orc = oracle_for(path, sim, lazy=True)
+    sim.run(1)
+    orc.step(1)
+    spikes = _launches(sim, orc, [1, 20, 200, 137])
+    _check_traces_and_weights(sim, orc)
+    assert spikes > 0
+    sim.close()
+
+
 @pytest.mark.parametrize("n_e_side", [20, 30, 40, 50])
 def test_resident_network_sizes_bit_exact(tmp_path, n_e_side):
     # The sizes of the uneven batch (400 .. 2,500 E neurons, 2 .. 7 rows per thread).
