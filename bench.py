@@ -66,9 +66,10 @@ def main() -> int:
     ap.add_argument("--step", choices=("auto", "resident", "graph"), default="auto",
                     help="plain: the resident step (weights in registers, one launch per run) when the network "
                          "fits, else the graph-captured fused step")
-    ap.add_argument("--resident-launch", choices=("cooperative", "plain"), default="cooperative",
-                    help="plain, resident step: a cooperative launch (default) or an ordinary one (same kernel; "
-                         "rocprofv3 traces only the ordinary one to the end of the process)")
+    ap.add_argument("--resident-launch", choices=("cooperative", "plain"), default="plain",
+                    help="plain, resident step: an ordinary launch (default: the bench owns the GPU, and the "
+                         "committed rocprofv3 profiles are of this form) or a cooperative one (the library's "
+                         "default; same kernel, about 14 us more per launch)")
     ap.add_argument("--no-graph-compare", action="store_true",
                     help="plain, resident step: skip the short comparison run of the graph step")
     ap.add_argument("--gemv-row-blocks", default=None,
