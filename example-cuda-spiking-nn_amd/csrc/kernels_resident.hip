@@ -458,33 +458,32 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
         } else {
             ev0 = ev1 = 0;
         }
-        // ---- poll: every block's spike word of step a; the lazy rows' scales of step a.
+        // ---- poll: every block's spike word of step a (the lazy rows' scales, which come
+        // later, are polled after the rows' traces).
         {
             uint32_t want = 0;
             if (wave == 0)
                 for (uint32_t m = 0; m < kRMaxBlocks / 64; ++m)
                     if (lane + 64u * m < P.blocks) want |= 1u << m;
-            if (lazy && k > 0) want |= vmask << 4;
             const uint32_t sl = r_slot(a);
             auto addr = [&](int m) -> const uint64_t* {
-                if (m < 4) return P.spk + (size_t)sl * P.blocks + lane + 64u * (uint32_t)m;
-                return P.sx + (size_t)sl * P.ne + tl + T * (uint32_t)(m - 4);
+                return P.spk + (size_t)sl * P.blocks + lane + 64u * (uint32_t)m;
             };
-            auto sink = [&](int m, uint32_t bits) {
-                if (m < 4) S.spk[lane + 64u * (uint32_t)m] = bits;
-                else S.sc[m - 4][tid] = __uint_as_float(bits);
-            };
-            if (!r_poll<4 + K>(P, S, want, r_tag(a), addr, sink)) return;
+            auto sink = [&](int m, uint32_t bits) { S.spk[lane + 64u * (uint32_t)m] = bits; };
+            if (!r_poll<4>(P, S, want, r_tag(a), addr, sink)) return;
         }
-        // ---- the monitors' spike ring and (last step) the state's spike bits: block b writes
-        // global word b (there are fewer words than blocks).
-        if (tid == 0 && b < P.words && (P.spike_ring || k + 1 == n)) {
-            const uint32_t word = r_global_word(P, S, b);
-            if (P.spike_ring) gp(P.spike_ring)[(size_t)(a % P.interval) * P.words + b] = word;
-            if (k + 1 == n) gp(P.bits)[b] = word;
+        // ---- the monitors' spike ring and (last step) the state's spike bits, written by the
+        // first wave of the I-post blocks (word w by I-post block w % bi): those blocks poll
+        // nothing else, and a wave's later loads would wait for these stores.
+        if (!epost && wave == 0 && (P.spike_ring || k + 1 == n)) {
+            for (uint32_t w = (b - P.be) + P.bi * lane; w < P.words; w += P.bi * 64u) {
+                const uint32_t word = r_global_word(P, S, w);
+                if (P.spike_ring) gp(P.spike_ring)[(size_t)(a % P.interval) * P.words + w] = word;
+                if (k + 1 == n) gp(P.bits)[w] = word;
+            }
         }
         // ---- rows: connection traces with the spikes of step a, plasticity pre traces (the
-        // previous step's +1 first), x_i (s_i) for the propagate.
+        // previous step's +1 first), x_i for the propagate.
         uint32_t sp_now = 0;
         {
             const float dr = S.rk[side][0], dd = S.rk[side][1], Kn = S.rk[side][2];
@@ -511,8 +510,7 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                 }
                 S.d[q][tid] = d;
                 S.r[q][tid] = r;
-                const float x = d - r;
-                S.xs[q][tid] = lazy ? x * S.sc[q][tid] : x;
+                S.xs[q][tid] = d - r;
             }
             if (pkind != kRNone) {
 #pragma unroll
@@ -525,8 +523,19 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                 }
             }
         }
-        // ---- propagate: column sums over the thread's rows (FP64 from 0.0, row order), the
-        // lane butterfly (32,16,8,4,2,1) run as a reduce-scatter, the waves in order.
+        // ---- the lazy rows' scales of step a, each thread its own rows (published by the rows'
+        // owners after the plasticity of step a - 1; the step's first launch step has them from
+        // the state).  No barrier: the propagate reads only the thread's rows, and a wait that
+        // gives up is seen at the step's last barrier.
+        if (lazy && k > 0) {
+            const uint32_t sl = r_slot(a);
+            auto addr = [&](int m) -> const uint64_t* { return P.sx + (size_t)sl * P.ne + tl_k + T * (uint32_t)m; };
+            auto sink = [&](int m, uint32_t bits) { S.sc[m][tid] = __uint_as_float(bits); };
+            r_poll_nb<K>(P, S, vmask, r_tag(a), addr, sink);
+        }
+        // ---- propagate: column sums of x_i (s_i) W_ij over the thread's rows (FP64 from 0.0,
+        // row order), the lane butterfly (32,16,8,4,2,1) run as a reduce-scatter, the waves in
+        // order.
         {
             double acc[kRCols];
 #pragma unroll
@@ -535,7 +544,8 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             // Rows past the group hold x = 0 and entries 0: they add +0.)
 #pragma unroll
             for (int q = 0; q < K; ++q) {
-                const double xq = (double)S.xs[q][tid];
+                const float xf = S.xs[q][tid];
+                const double xq = (double)(lazy ? xf * S.sc[q][tid] : xf);
 #pragma unroll
                 for (int c = 0; c < kRCols; ++c) acc[c] = fma(xq, (double)w[q][c], acc[c]);
             }
