@@ -15,10 +15,12 @@
 //   spk   per block and step, one tagged word: the spike bits of its C neurons;
 //   sx    per lazily normalised row and step, one tagged word: its scale s_i, published by the
 //         block owning neuron i (the row's R_i, s_i and target live there);
-//   dr    per E-post block, row and step, two tagged words (an FP64 value): the block's part of
-//         the row-sum update of the lazy matrix (for a row whose pre neuron spiked: the sum of
-//         its new entries over the block's columns; for another row, when one of the block's
-//         columns spiked: the sum of those columns' LTP changes), polled by the row's owner.
+//   dr    per E-post block, row and step, two tagged words (an FP64 value, low half first): the
+//         block's part of the row-sum update of the lazy matrix, polled by the row's owner.  For
+//         a row whose pre neuron did not spike, when one of the block's columns spiked: the sum
+//         of those columns' LTP changes ([slot][block][row]); for a row whose pre neuron spiked:
+//         the sum of its new entries over the block's columns ([slot][row][block], so that the
+//         owner's loads of one row's parts are contiguous).
 // A tagged word is a 32-bit value and the step that produced it in one 64-bit agent-coherent
 // store; readers poll the words they need, in three slots by step.  A block rewrites a slot only
 // after every block has published its spike word of a later step, which each block does only
@@ -87,7 +89,8 @@ struct RPlan {
     // Exchange (tagged 64-bit words, three slots by step).
     uint64_t* spk;                // [3][blocks]
     uint64_t* sx;                 // [3][ne]
-    uint64_t* dr;                 // [3][be][2][ne] (low words, then high words)
+    uint64_t* dr;                 // [3][be][ne][2]: quiet rows' parts
+    uint64_t* drs;                // [3][ne][be][2]: spiking rows' parts
     uint32_t* status;             // the state's error word
     uint32_t wait_limit;          // polls before a wait gives up
 };

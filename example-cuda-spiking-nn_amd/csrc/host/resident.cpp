@@ -134,7 +134,8 @@ void System::enable_resident() {
     };
     P.spk = words((size_t)3 * P.blocks);
     P.sx = words((size_t)3 * P.ne);
-    P.dr = P.ee_lazy ? words((size_t)3 * P.be * 2 * P.ne) : nullptr;
+    P.dr = P.ee_lazy ? words((size_t)3 * P.be * P.ne * 2) : nullptr;
+    P.drs = P.ee_lazy ? words((size_t)3 * P.ne * P.be * 2) : nullptr;
     d_resident_plan_ = static_cast<RPlan*>(persist_alloc(sizeof(RPlan)));
     BN_HIP_CHECK(hipMemcpy(d_resident_plan_, &P, sizeof(RPlan), hipMemcpyHostToDevice), "resident plan");
     resident_plan_ = P;

@@ -662,9 +662,9 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                     const uint32_t i = tl_k + T * (uint32_t)q;
                     if (!((vmask >> q) & 1u) || ((sp_now >> q) & 1u)) continue;
                     const uint64_t bits = (uint64_t)__double_as_longlong(change[q]);
-                    uint64_t* dst = P.dr + ((size_t)r_slot(a) * P.be + b) * 2u * P.ne + i;
+                    uint64_t* dst = P.dr + (((size_t)r_slot(a) * P.be + b) * P.ne + i) * 2u;
                     st_c(dst, r_word(a, (uint32_t)bits));
-                    st_c(dst + P.ne, r_word(a, (uint32_t)(bits >> 32)));
+                    st_c(dst + 1, r_word(a, (uint32_t)(bits >> 32)));
                 }
             }
 #pragma unroll
@@ -690,9 +690,9 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                     }
                     if (sp) {
                         const uint64_t bits = (uint64_t)__double_as_longlong(part);
-                        uint64_t* dst = P.dr + ((size_t)r_slot(a) * P.be + b) * 2u * P.ne + i;
+                        uint64_t* dst = P.drs + (((size_t)r_slot(a) * P.ne + i) * P.be + b) * 2u;
                         st_c(dst, r_word(a, (uint32_t)bits));
-                        st_c(dst + P.ne, r_word(a, (uint32_t)(bits >> 32)));
+                        st_c(dst + 1, r_word(a, (uint32_t)(bits >> 32)));
                     }
                 } else {
 #pragma unroll
@@ -723,36 +723,38 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                 cmask[m] = __ballot(q < P.be && S.spk[q] != 0u);
                 nc += (uint32_t)__popcll(cmask[m]);
             }
-            // Items: (quiet row, contributor rank) for the rows whose neuron did not spike, then
-            // (spiking row, E-post block); lane l polls items l, l + 64, ...
+            // Items: (contributor rank, quiet row) for the rows whose neuron did not spike, then
+            // (spiking row, E-post block); lane l polls items l, l + 64, ... (neighbouring lanes
+            // read neighbouring rows of one contributor, or neighbouring blocks of one row).
             const uint32_t nq = (uint32_t)__popc(qm), ns = (uint32_t)__popc(spm);
             const uint32_t nquiet = nq * nc, total = nquiet + ns * P.be;
             const uint32_t sl = r_slot(a);
             for (uint32_t base = 0; base < total; base += 4u * 64u) {
-                uint32_t row_of[4], rank_of[4], blk_of[4];
+                uint32_t row_of[4], rank_of[4];
+                const uint64_t* src_of[4];
                 uint32_t want = 0;
 #pragma unroll
                 for (int m = 0; m < 4; ++m) {
                     const uint32_t it = base + lane + 64u * (uint32_t)m;
-                    uint32_t row = 0, rank = 0, blk = 0;
+                    uint32_t row = 0, rank = 0;
+                    const uint64_t* src = P.dr;
                     if (it < nquiet) {
-                        row = r_nth_bit(qm, it / nc);
-                        rank = it % nc;
-                        blk = r_nth_set256(cmask, rank);
+                        row = r_nth_bit(qm, it % nq);
+                        rank = it / nq;
+                        const uint32_t blk = r_nth_set256(cmask, rank);
+                        src = P.dr + (((size_t)sl * P.be + blk) * P.ne + j0_k + row) * 2u;
                     } else if (it < total) {
                         const uint32_t i2 = it - nquiet;
                         row = r_nth_bit(spm, i2 / P.be);
-                        rank = blk = i2 % P.be;
+                        rank = i2 % P.be;
+                        src = P.drs + (((size_t)sl * P.ne + j0_k + row) * P.be + rank) * 2u;
                     }
                     row_of[m] = row;
                     rank_of[m] = rank;
-                    blk_of[m] = blk;
+                    src_of[m] = src;
                     if (it < total) want |= 3u << (2 * m);
                 }
-                auto addr = [&](int m) -> const uint64_t* {
-                    return P.dr + (((size_t)sl * P.be + blk_of[m >> 1]) * 2u + (uint32_t)(m & 1)) * P.ne + j0_k +
-                           row_of[m >> 1];
-                };
+                auto addr = [&](int m) -> const uint64_t* { return src_of[m >> 1] + (m & 1); };
                 auto sink = [&](int m, uint32_t bits) {
                     reinterpret_cast<uint32_t*>(&S.part[row_of[m >> 1]][rank_of[m >> 1]])[m & 1] = bits;
                 };
