@@ -48,7 +48,8 @@ def main() -> int:
     os.environ.setdefault("OMP_PROC_BIND", "close")
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--mode", choices=("plain", "batched-shared", "batched-uneven", "adjoint"), default="plain",
+    ap.add_argument("--mode", choices=("plain", "batched-shared", "batched-uneven", "adjoint", "recording"),
+                    default="plain",
                     help="plain: one network; batched-shared: 64 instances sharing frozen weights; "
                          "batched-uneven: 48 plastic networks of uneven size; adjoint: gradient of a "
                          "1,000-step window of the 3,125-neuron network (plasticity off)")
@@ -98,6 +99,10 @@ def main() -> int:
         args.steps = 5 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         return bench_adjoint(args)
+    if args.mode == "recording":
+        args.steps = 30000 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        return bench_recording(args)
     args.steps = 10000 if args.steps is None else args.steps
     args.warmup = 100 if args.warmup is None else args.warmup
     return bench_plain(args)
@@ -261,6 +266,65 @@ def bench_plain(args) -> int:
                    "first_timed_step": first_timed},
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_recording(args) -> int:
+    """The recordings' cost on the plain workload: the reference's files (the E and I spike
+    files every 10,000 steps, one voltage file per step, the E->E weight dump every 100,000 steps)
+    written by the writer-thread pipeline, on the calling thread at each record step, and the
+    same run without files.  Step 0 (a record step with the weight dump) runs first, untimed;
+    the timed run covers `steps` steps and their record steps."""
+    import shutil
+    import tempfile
+    eng = importlib.import_module("example-cuda-spiking-nn_amd")
+    cfg = json.load(open(args.config))
+    name = os.path.splitext(os.path.basename(args.config))[0]
+    flags = ["quiet"] + (["lazy_normalize", "resident"] if args.step != "graph" else ["lazy_normalize"])
+    forms = {}
+    for form in ("no files", "pipeline", "calling thread"):
+        d = tempfile.mkdtemp(prefix="bn_rec_")
+        c = dict(cfg)
+        c["output_directory"] = os.path.join(d, "out")   # cleaned at creation when files are written
+        path = os.path.join(d, "config.json")
+        json.dump(c, open(path, "w"))
+        sim = eng.Simulation(path, flags=flags + ([] if form == "no files" else ["write_files"]))
+        if "resident" in flags:
+            sim.set_resident_cooperative(args.resident_launch == "cooperative")
+        sim.set_async_recording(form != "calling thread")
+        sim.run(args.warmup)
+        c0, w0, r0 = sim.record_times()
+        t0 = time.perf_counter()
+        sim.run(args.steps)
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        c1, w1, r1 = sim.record_times()
+        sim.close()
+        files = sum(len(f) for _, _, f in os.walk(d))
+        shutil.rmtree(d, ignore_errors=True)
+        forms[form] = {"steps_per_s": round(args.steps / wall, 1), "wall_s": round(wall, 4),
+                       "record_steps": r1 - r0, "calling_thread_s": round(c1 - c0, 4),
+                       "writer_thread_s": round(w1 - w0, 4), "files": files}
+    base = forms["no files"]["wall_s"]
+    piped, sync = forms["pipeline"], forms["calling thread"]
+    n_rec = max(1, piped["record_steps"])
+    line = {
+        "metric": "sim steps/s with the reference's recordings written (3,125-nrn STDP net)",
+        "value": piped["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(piped["wall_s"] * 1000.0 / args.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 state/weights, f64 GEMV accumulation",
+        "data": "synthetic: seeded Philox Poisson background; files to a temporary directory",
+        "config": {"workload": f"{name}, {'resident' if 'resident' in flags else 'graph'} step, spike and voltage "
+                               f"files every {cfg.get('recording_interval_steps', 10000)} steps",
+                   "forms": forms,
+                   "record_cost_per_interval_s": {
+                       "calling_thread": round((sync["wall_s"] - base) / n_rec, 4),
+                       "pipeline_wall": round((piped["wall_s"] - base) / n_rec, 4),
+                       "pipeline_writer": round(piped["writer_thread_s"] / n_rec, 4)},
+                   "interval_compute_s": round(base / n_rec, 4)},
+        "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
     return 0

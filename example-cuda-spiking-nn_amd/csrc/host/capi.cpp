@@ -580,6 +580,21 @@ int bn_sim_set_persistent_resident(bn_sim* sim, int allow) {
     return guarded([&] { sim->system->set_persistent_resident(allow != 0); });
 }
 
+int bn_sim_set_async_recording(bn_sim* sim, int async) {
+    if (!sim) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_async_recording: null simulation");
+    return guarded([&] { sim->system->set_async_recording(async != 0); });
+}
+
+int bn_sim_record_times(bn_sim* sim, double* call_seconds, double* writer_seconds, uint32_t* record_steps) {
+    if (!sim || !call_seconds || !writer_seconds || !record_steps)
+        return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_record_times: null argument");
+    return guarded([&] {
+        *call_seconds = sim->system->record_call_seconds();
+        *writer_seconds = sim->system->record_writer_seconds();
+        *record_steps = sim->system->record_steps();
+    });
+}
+
 int bn_sim_set_resident_cooperative(bn_sim* sim, int cooperative) {
     if (!sim) return set_error(BN_ERR_INVALID_ARGUMENT, "bn_sim_set_resident_cooperative: null simulation");
     return guarded([&] { sim->system->set_resident_cooperative(cooperative != 0); });

@@ -726,42 +726,60 @@ void SpikeRecorder::historicize(uint32_t clock_step) {
 // spike_recorder.cpp:40-60 — slot 0 holds the record step itself but is labelled C - interval.
 void SpikeRecorder::record(uint32_t clock_step) {
     const uint32_t T = recording_interval_steps_;
+    if (fused) {
+        const std::vector<uint32_t>& ring = gpu_state_.spike_ring_snapshot(clock_step);
+        record_from_rings(clock_step, ring.data(), nullptr);
+        return;
+    }
     if (clock_step < T) return;
     const uint32_t first = clock_step - T;
     std::string buf;
     buf.reserve(1 << 20);
     char tmp[32];
-    if (!fused) {
-        const std::vector<uint32_t> h = gpu_state_.download_and_flush_spike_history(gpu_view_.offset, gpu_view_.num_neurons, T);
-        for (uint32_t inner = 0; inner < T; ++inner) {
-            buf += std::to_string(first + inner) + ", ";
-            for (uint32_t n = 0; n < gpu_view_.num_neurons; ++n)
-                if (h[(size_t)n * T + inner] == 1) {
-                    const int len = std::snprintf(tmp, sizeof tmp, "%u ", n);
-                    buf.append(tmp, (size_t)len);
-                }
-            buf += "\n";
-        }
-    } else {
-        const std::vector<uint32_t>& ring = gpu_state_.spike_ring_snapshot(clock_step);
-        const uint32_t W = gpu_state_.spike_words();
-        for (uint32_t inner = 0; inner < T; ++inner) {
-            buf += std::to_string(first + inner) + ", ";
-            const uint32_t* row = &ring[(size_t)inner * W];
-            for (uint32_t n = 0; n < gpu_view_.num_neurons; ++n) {
-                const uint32_t g = gpu_view_.offset + n;
-                if ((row[g >> 5] >> (g & 31)) & 1u) {
-                    const int len = std::snprintf(tmp, sizeof tmp, "%u ", n);
-                    buf.append(tmp, (size_t)len);
-                }
+    const std::vector<uint32_t> h = gpu_state_.download_and_flush_spike_history(gpu_view_.offset, gpu_view_.num_neurons, T);
+    for (uint32_t inner = 0; inner < T; ++inner) {
+        buf += std::to_string(first + inner) + ", ";
+        for (uint32_t n = 0; n < gpu_view_.num_neurons; ++n)
+            if (h[(size_t)n * T + inner] == 1) {
+                const int len = std::snprintf(tmp, sizeof tmp, "%u ", n);
+                buf.append(tmp, (size_t)len);
             }
-            buf += "\n";
-        }
+        buf += "\n";
     }
     if (output_.good()) {
         output_ << buf;
         output_.flush();
     }
+}
+
+// The same lines from the fused step's spike-bit ring ([interval][words], bit g & 31 of word
+// g >> 5 for global neuron g), the set bits of the group's words scanned in order.
+void SpikeRecorder::record_from_rings(uint32_t clock_step, const uint32_t* ring, const float*) {
+    const uint32_t T = recording_interval_steps_;
+    if (clock_step < T || !output_.good()) return;
+    const uint32_t first = clock_step - T;
+    const uint32_t W = gpu_state_.spike_words();
+    const uint32_t lo = gpu_view_.offset, hi = lo + gpu_view_.num_neurons;
+    std::string buf;
+    buf.reserve(1 << 20);
+    char tmp[32];
+    for (uint32_t inner = 0; inner < T; ++inner) {
+        buf += std::to_string(first + inner) + ", ";
+        const uint32_t* row = ring + (size_t)inner * W;
+        for (uint32_t wd = lo >> 5; wd <= (hi - 1) >> 5 && lo < hi; ++wd) {
+            uint32_t bits = row[wd];
+            while (bits) {
+                const uint32_t g = 32u * wd + (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                if (g < lo || g >= hi) continue;
+                const int len = std::snprintf(tmp, sizeof tmp, "%u ", g - lo);
+                buf.append(tmp, (size_t)len);
+            }
+        }
+        buf += "\n";
+    }
+    output_ << buf;
+    output_.flush();
 }
 
 VoltageRecorder::VoltageRecorder(const NeuronGroup& group, const std::string& output_filepath,
@@ -800,37 +818,67 @@ void VoltageRecorder::historicize(uint32_t clock_step) {
 // voltage_recorder.cpp:54-77 — one file per history slot, name index computed in uint32.
 void VoltageRecorder::record(uint32_t clock_step) {
     const uint32_t T = recording_interval_steps_;
-    if (!gpu_state_.options().reference_histories && !fused) return;
-    std::vector<float> h;
-    std::vector<int64_t> col(selected_neurons_.size());
-    uint32_t stride_n = 0;
-    if (!fused) {
-        h = gpu_state_.download_and_flush_membrane_potential_history(gpu_view_.offset, gpu_view_.num_neurons, T);
-    } else {
-        h = gpu_state_.download_and_flush_v_ring();
-        stride_n = gpu_state_.n_vrec();
-        const auto& ids = gpu_state_.options().vrec_global_ids;
-        for (size_t k = 0; k < selected_neurons_.size(); ++k) {
-            const uint32_t g = gpu_view_.offset + selected_neurons_[k];
-            col[k] = std::find(ids.begin(), ids.end(), g) - ids.begin();
-        }
+    if (fused) {
+        const std::vector<float> h = gpu_state_.download_and_flush_v_ring();
+        record_from_rings(clock_step, nullptr, h.data());
+        return;
     }
+    if (!gpu_state_.options().reference_histories) return;
+    const std::vector<float> h =
+        gpu_state_.download_and_flush_membrane_potential_history(gpu_view_.offset, gpu_view_.num_neurons, T);
+    write_files(clock_step, h.data(), 0, nullptr);
+}
+
+// From the fused step's voltage ring ([interval][n_vrec], columns in the order of the
+// simulation's recorded ids).
+void VoltageRecorder::record_from_rings(uint32_t clock_step, const uint32_t*, const float* v_ring) {
+    std::vector<int64_t> col(selected_neurons_.size());
+    const auto& ids = gpu_state_.options().vrec_global_ids;
+    for (size_t k = 0; k < selected_neurons_.size(); ++k) {
+        const uint32_t g = gpu_view_.offset + selected_neurons_[k];
+        col[k] = std::find(ids.begin(), ids.end(), g) - ids.begin();
+    }
+    write_files(clock_step, v_ring, gpu_state_.n_vrec(), col.data());
+}
+
+// The interval's files: one per history slot hs, named with k = C - T + hs + 1 in uint32.
+// Values: v[hs * stride_n + col[q]] from a ring (col given), else v[id * T + hs] (histories).
+// The files are independent, so they are written by up to 8 threads over slot ranges.
+void VoltageRecorder::write_files(uint32_t clock_step, const float* v, uint32_t stride_n, const int64_t* col) {
+    const uint32_t T = recording_interval_steps_;
     if (output_filepath_.empty()) return;
     const size_t pos = output_filepath_.rfind(".txt");
     if (!quiet) std::cout << "Recording membrane potentials at clock step " << clock_step << "...\n";
-    for (uint32_t hs = 0; hs < T; ++hs) {
-        std::string name = output_filepath_;
-        const uint32_t k = (clock_step - T) + hs + 1;
-        name.insert(pos == std::string::npos ? name.size() : pos, "_step" + std::to_string(k));
-        std::ofstream out(name, std::ios::out | std::ios::trunc);
-        if (!out.good()) return;
-        out << "neuron_id, voltage\n";
-        for (size_t q = 0; q < selected_neurons_.size(); ++q) {
-            const uint32_t id = selected_neurons_[q];
-            const float v = fused ? h[(size_t)hs * stride_n + (size_t)col[q]] : h[(size_t)id * T + hs];
-            out << id << ", " << v << "\n";
+    auto write_range = [&](uint32_t h0, uint32_t h1) {
+        std::string text;
+        char tmp[64];
+        for (uint32_t hs = h0; hs < h1; ++hs) {
+            std::string name = output_filepath_;
+            const uint32_t k = (clock_step - T) + hs + 1;
+            name.insert(pos == std::string::npos ? name.size() : pos, "_step" + std::to_string(k));
+            text.assign("neuron_id, voltage\n");
+            for (size_t q = 0; q < selected_neurons_.size(); ++q) {
+                const uint32_t id = selected_neurons_[q];
+                const float val = col ? v[(size_t)hs * stride_n + (size_t)col[q]] : v[(size_t)id * T + hs];
+                std::ostringstream os;   // the stream formatting of the reference's `out << v`
+                os << val;
+                const int len = std::snprintf(tmp, sizeof tmp, "%u, ", id);
+                text.append(tmp, (size_t)len).append(os.str()).append("\n");
+            }
+            FILE* f = std::fopen(name.c_str(), "w");
+            if (!f) return;
+            std::fwrite(text.data(), 1, text.size(), f);
+            std::fclose(f);
         }
+    };
+    const uint32_t nt = std::min<uint32_t>(8, std::max<uint32_t>(1, T / 256));
+    if (nt <= 1) {
+        write_range(0, T);
+        return;
     }
+    std::vector<std::thread> pool;
+    for (uint32_t t = 0; t < nt; ++t) pool.emplace_back(write_range, T * t / nt, T * (t + 1) / nt);
+    for (auto& th : pool) th.join();
 }
 
 WeightRecorder::WeightRecorder(const Connection& connection, const std::string& output_filepath,

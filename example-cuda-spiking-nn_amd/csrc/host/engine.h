@@ -9,11 +9,16 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <condition_variable>
 #include <cstdint>
+#include <deque>
+#include <exception>
 #include <functional>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../bn_persist.h"
@@ -127,6 +132,8 @@ public:
     const std::vector<uint32_t>& spike_ring_snapshot(uint32_t clock_step);
     std::vector<float> download_and_flush_v_ring();
     std::vector<uint32_t> download_spike_ring_rows(uint32_t first_step, uint32_t num_steps, uint32_t clock_step) const;
+    // The rings' rows up to `step` went to the recorders (their copies were queued on the stream).
+    void mark_rings_flushed(uint32_t step) { ring_snapshot_step_ = step; }
     // Dense [pre][post] true weights (lazy normalisation: fl32(s_i * M_ij)).
     std::vector<float> download_weight_matrix(uint32_t weight_matrix_idx) const;
     std::string lookup_weight_matrix_name(uint32_t weight_matrix_idx) const;
@@ -334,6 +341,11 @@ public:
     virtual ~Monitor() = default;
     virtual void historicize(uint32_t clock_step) = 0;
     virtual void record(uint32_t clock_step) = 0;
+    // Recorders of the fused step's rings: write the interval ending at clock_step from host
+    // copies of the spike-bit ring [interval][words] and the voltage ring [interval][n_vrec]
+    // (called by the recording pipeline's writer thread).
+    virtual bool from_rings() const { return false; }
+    virtual void record_from_rings(uint32_t, const uint32_t*, const float*) {}
     bool fused = false;  // rings are written by the fused step
     bool quiet = false;
 
@@ -347,6 +359,8 @@ public:
                   GpuSimulationState& gpu_state);
     void historicize(uint32_t clock_step) override;
     void record(uint32_t clock_step) override;
+    bool from_rings() const override { return fused; }
+    void record_from_rings(uint32_t clock_step, const uint32_t* spike_ring, const float* v_ring) override;
     bool good() const { return output_.good(); }
 
 private:
@@ -362,9 +376,12 @@ public:
                     uint32_t recording_interval_steps, GpuSimulationState& gpu_state);
     void historicize(uint32_t clock_step) override;
     void record(uint32_t clock_step) override;
+    bool from_rings() const override { return fused; }
+    void record_from_rings(uint32_t clock_step, const uint32_t* spike_ring, const float* v_ring) override;
     const std::vector<uint32_t>& selected_neurons() const { return selected_neurons_; }
 
 private:
+    void write_files(uint32_t clock_step, const float* v, uint32_t stride_n, const int64_t* col);
     const NeuronGroup& group_;
     std::string output_filepath_;
     std::vector<uint32_t> selected_neurons_;
@@ -381,6 +398,48 @@ public:
 private:
     const Connection& connection_;
     std::string output_filepath_;
+};
+
+// --------------------------------------------------------------------------------------
+// Recording pipeline of the fused step: at a record step the rings' copies to pinned host
+// memory (double-buffered) and their clearing are queued on the state's stream, and a writer
+// thread writes the recorders' files while the next interval's steps run.
+// --------------------------------------------------------------------------------------
+class RecordPipeline {
+public:
+    RecordPipeline(GpuSimulationState& s, std::vector<Monitor*> monitors);
+    ~RecordPipeline();
+    RecordPipeline(const RecordPipeline&) = delete;
+    RecordPipeline& operator=(const RecordPipeline&) = delete;
+    // Queue the interval ending at `step` (waits only while both host buffers are being written).
+    void submit(uint32_t step);
+    // Wait until every queued interval is written; rethrows the writer's first error.
+    void drain();
+    double writer_seconds() const { return writer_seconds_; }   // formatting and writing, all intervals
+    uint32_t intervals() const { return intervals_; }
+
+private:
+    struct Slot {
+        uint32_t* spk = nullptr;
+        float* v = nullptr;
+        hipEvent_t ready = nullptr;
+        bool busy = false;
+    };
+    void writer();
+    GpuSimulationState& s_;
+    std::vector<Monitor*> monitors_;
+    size_t spk_elems_ = 0, v_elems_ = 0;
+    Slot slot_[2];
+    int next_ = 0;
+    int device_ = 0;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::pair<int, uint32_t>> jobs_;
+    bool stop_ = false;
+    std::exception_ptr error_;
+    double writer_seconds_ = 0.0;
+    uint32_t intervals_ = 0;
+    std::thread thread_;
 };
 
 // --------------------------------------------------------------------------------------
@@ -468,6 +527,15 @@ public:
     const RPlan& resident_plan() const { return resident_plan_; }
 
     uint32_t clock_step() const { return clock_step_; }
+    // Fused step: recordings written by the pipeline's writer thread while the steps run (default)
+    // or on the calling thread at each record step, as the component path does.
+    void set_async_recording(bool on);
+    bool async_recording() const { return async_recording_; }
+    // Host seconds spent on recordings: on the calling thread at record steps, and (pipeline)
+    // in the writer thread; and the number of record steps.
+    double record_call_seconds() const { return record_call_seconds_; }
+    double record_writer_seconds() const { return pipeline_ ? pipeline_->writer_seconds() : 0.0; }
+    uint32_t record_steps() const { return record_steps_; }
     bool quiet = false;
     const std::vector<std::unique_ptr<NeuronGroup>>& neuron_groups() const { return neuron_groups_; }
     const std::vector<std::unique_ptr<Connection>>& connections() const { return connections_; }
@@ -540,6 +608,10 @@ private:
     std::vector<std::pair<void*, size_t>> resident_bufs_;  // tagged-word buffers (64-bit words)
     uint32_t resident_next_ = 0;                           // the clock after the last launch
     bool resident_cooperative_ = true;
+    bool async_recording_ = true;
+    std::unique_ptr<RecordPipeline> pipeline_;
+    double record_call_seconds_ = 0.0;
+    uint32_t record_steps_ = 0;
 };
 
 }  // namespace bn

@@ -10,6 +10,7 @@
 namespace bn {
 
 System::~System() {
+    pipeline_.reset();   // the writer finishes its queued intervals first
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : staging_) (void)hipFree(p);
@@ -48,13 +49,25 @@ void System::step_the_clock(uint32_t clock_step) {
     clock_step_ = clock_step + 1;
 }
 
-// system.cpp:65-83
+void System::set_async_recording(bool on) {
+    if (pipeline_) pipeline_->drain();
+    async_recording_ = on;
+}
+
+// system.cpp:65-83.  Fused step with the recording pipeline: a record step queues its rings'
+// copies and the writer thread writes the files while the next interval's steps run; the run
+// returns once the files are written, as the reference's does.
 void System::run(uint32_t total_clock_steps) {
     using clock = std::chrono::steady_clock;
     const auto start = clock::now();
     const uint32_t interval = state_.options().recording_interval_steps;
     uint32_t step = clock_step_;
     const uint32_t end = clock_step_ + total_clock_steps;
+    std::vector<Monitor*> ring_monitors;
+    for (auto& m : monitors_)
+        if (m->from_rings()) ring_monitors.push_back(m.get());
+    const bool piped = fused_ && async_recording_ && !ring_monitors.empty();
+    if (piped && !pipeline_) pipeline_ = std::make_unique<RecordPipeline>(state_, ring_monitors);
     while (step < end) {
         uint32_t n = 1;
         if (fused_ && step_callbacks_.empty()) {
@@ -70,12 +83,27 @@ void System::run(uint32_t total_clock_steps) {
         clock_step_ = step;
         const uint32_t last = step - 1;
         if (last % interval == 0) {
-            BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "step sync");
-            state_.check_status();
-            for (auto& m : monitors_) m->record(last);
+            const auto r0 = clock::now();
+            if (piped) {
+                pipeline_->submit(last);
+                for (auto& m : monitors_)   // others (the weight dump) read the state now
+                    if (!m->from_rings()) m->record(last);
+            } else {
+                BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "step sync");
+                state_.check_status();
+                for (auto& m : monitors_) m->record(last);
+            }
+            const std::chrono::duration<double> rt = clock::now() - r0;
+            record_call_seconds_ += rt.count();
+            ++record_steps_;
             const std::chrono::duration<float> elapsed = clock::now() - start;
             if (!quiet) std::cout << "Completed clock step: " << last << " | elapsed: " << elapsed.count() << "s\n";
         }
+    }
+    if (piped) {
+        pipeline_->drain();
+        BN_HIP_CHECK(hipStreamSynchronize(state_.stream()), "step sync");
+        state_.check_status();
     }
 }
 

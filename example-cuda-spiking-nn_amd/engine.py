@@ -121,7 +121,7 @@ EXPORTED_SYMBOLS = [
     "bn_window_get_info", "bn_window_get_gradient", "bn_measure_read_bandwidth", "bn_sim_save_state",
     "bn_sim_load_state", "bn_sim_get_spike_raster", "bn_measure_read_bandwidth_one_launch",
     "bn_sim_set_persistent_rows", "bn_sim_set_persistent_wait_limit", "bn_sim_set_persistent_resident",
-    "bn_sim_set_resident_cooperative",
+    "bn_sim_set_resident_cooperative", "bn_sim_set_async_recording", "bn_sim_record_times",
 ]
 
 _lib = None
@@ -162,6 +162,8 @@ def lib() -> C.CDLL:
     L.bn_sim_set_persistent_wait_limit.argtypes = [P, U]
     L.bn_sim_set_persistent_resident.argtypes = [P, I]
     L.bn_sim_set_resident_cooperative.argtypes = [P, I]
+    L.bn_sim_set_async_recording.argtypes = [P, I]
+    L.bn_sim_record_times.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_uint32)]
     L.bn_sim_time_propagate_kernel.argtypes = [P, U, fp, fp]
     L.bn_sim_set_steps_per_graph.argtypes = [P, U]
     L.bn_measure_read_bandwidth.argtypes = [C.c_uint64, U, fp]
@@ -346,6 +348,16 @@ class Simulation:
     def set_resident_cooperative(self, cooperative: bool) -> None:
         """Cooperative (default) or ordinary launches of the resident step (same kernel)."""
         _check(lib().bn_sim_set_resident_cooperative(self.h, 1 if cooperative else 0))
+
+    def set_async_recording(self, on: bool) -> None:
+        """Recordings written by a writer thread while the steps run (default) or at each record step."""
+        _check(lib().bn_sim_set_async_recording(self.h, 1 if on else 0))
+
+    def record_times(self):
+        """(seconds on the calling thread at record steps, seconds in the writer thread, record steps)."""
+        a, b, n = C.c_double(0), C.c_double(0), C.c_uint32(0)
+        _check(lib().bn_sim_record_times(self.h, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
 
     def set_persistent_wait_limit(self, polls: int) -> None:
         _check(lib().bn_sim_set_persistent_wait_limit(self.h, polls))

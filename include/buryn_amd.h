@@ -438,6 +438,16 @@ int bn_sim_set_persistent_resident(bn_sim* sim, int allow);
  * and when one does, the bounded waits end the launch with BN_ERR_RUNTIME instead of a hang).
  * The kernel and its results are the same.  Resident simulations only. */
 int bn_sim_set_resident_cooperative(bn_sim* sim, int cooperative);
+/* Recordings of the fused step (no counterpart in the reference, which writes them on the stepping
+ * thread, spike_recorder.cpp:40-60, voltage_recorder.cpp:54-77): written by a writer thread while
+ * the next interval's steps run (1, the default; a record step queues the rings' copies to pinned
+ * host buffers on the simulation stream, and bn_sim_run returns once the files are written) or on
+ * the calling thread at each record step (0).  The files are the same either way. */
+int bn_sim_set_async_recording(bn_sim* sim, int async);
+/* Host time spent on recordings since creation: seconds on the calling thread at record steps
+ * (waits for the device included), seconds the writer thread spent formatting and writing, and
+ * the number of record steps. */
+int bn_sim_record_times(bn_sim* sim, double* call_seconds, double* writer_seconds, uint32_t* record_steps);
 /* State snapshot / resume (no counterpart in the reference): every device buffer of the
  * simulation (neuron and background state, weights, connection and plasticity traces, the
  * pending propagate partials, lazy row sums and scales, monitor rings, the device clock and

@@ -97,3 +97,35 @@ def test_recordings_match_oracle_and_paths_agree(tmp_path):
     assert open(os.path.join(out_f, "weights", "weights_e_to_e_step0.txt")).readline() == \
         "pre_neuron_id, post_neuron_id, weight\n"
     assert os.path.exists(os.path.join(out_f, "stimulus_schedules", "timestep_event_counts.txt"))
+
+
+@pytest.mark.parametrize("step", ["graph", "resident"])
+def test_recording_pipeline_writes_the_same_files(tmp_path, step):
+    # The writer-thread pipeline (default: the rings' copies queued on the stream, the files
+    # written while the next interval runs) and recordings on the calling thread at each record
+    # step write byte-identical files; two intervals and the record step at 0.
+    flags = ("write_files", "quiet") + (("resident", "lazy_normalize") if step == "resident" else ())
+    dirs = []
+    for piped in (True, False):
+        cfg = small_config(n_e_side=12, n_i_side=6, steps=2 * T + 1)
+        cfg["output_directory"] = os.path.join(str(tmp_path), f"{step}_{piped}")
+        path = os.path.join(str(tmp_path), f"{step}_{piped}.json")
+        json.dump(cfg, open(path, "w"))
+        sim = engine().Simulation(path, flags=flags)
+        sim.set_async_recording(piped)
+        sim.run(T)
+        sim.run(T + 1)   # the record steps at 10,000 and 20,000 fall inside the second run
+        call_s, writer_s, records = sim.record_times()
+        print(f"{step} piped={piped}: {records} record steps, {call_s:.3f} s on the calling thread, "
+              f"{writer_s:.3f} s in the writer")
+        assert records == 3
+        assert (writer_s > 0.0) == piped
+        sim.close()
+        dirs.append(os.path.join(cfg["output_directory"], "recordings"))
+    names = sorted(os.listdir(dirs[0]))
+    assert names == sorted(os.listdir(dirs[1]))
+    assert len([n for n in names if n.startswith("voltages_step0_step")]) == 3 * T
+    for name in names:
+        assert filecmp.cmp(os.path.join(dirs[0], name), os.path.join(dirs[1], name), shallow=False), name
+    lines = _spike_lines(os.path.join(dirs[0], "spikes_e.txt"))
+    assert len(lines) == 2 * T
