@@ -18,6 +18,7 @@ from .engine import (  # noqa: F401
     build_library,
     config_dump,
     generate_weight_matrix,
+    set_connectivity_cache,
     lib,
     measure_read_bandwidth,
     measure_read_bandwidth_one_launch,

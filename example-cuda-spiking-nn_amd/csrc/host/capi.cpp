@@ -676,6 +676,10 @@ int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, siz
     return BN_OK;
 }
 
+int bn_set_connectivity_cache(const char* directory) {
+    return guarded([&] { set_connectivity_cache_dir(directory ? directory : ""); });
+}
+
 int bn_generate_weight_matrix(const char* config_path, uint32_t idx, float* dst, uint32_t* pre, uint32_t* post) {
     if (idx >= 4) return set_error(BN_ERR_OUT_OF_RANGE, "Invalid weight matrix index: " + std::to_string(idx));
     return guarded([&] {

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <filesystem>
@@ -86,6 +87,113 @@ struct LineWriter {
 };
 }  // namespace
 
+// ---------------------------------------------------------------------------------------
+// Binary connectivity cache (no counterpart in the reference, which regenerates every matrix on
+// every run, generate_weights.cpp:39-100).  A matrix is fixed by its locations, its generation
+// parameters, the seed and its index; with a cache directory set (set_connectivity_cache_dir,
+// or the BN_CONNECTIVITY_CACHE environment variable), each generated matrix is stored as
+// "w_<key>.bnw" (key: FNV-1a 64, over 8-byte words, of those inputs) and read back by later runs
+// instead of drawn again.  The file holds a header (magic, format, key, shape, the same hash of
+// the entries) and the FP32 entries; a file whose header or checksum does not match is regenerated and rewritten.
+// ---------------------------------------------------------------------------------------
+namespace {
+constexpr char kCacheMagic[8] = {'B', 'N', 'W', 'E', 'I', 'G', 'H', 'T'};
+constexpr uint32_t kCacheFormat = 2;
+
+std::string& cache_dir_ref() {
+    static std::string dir = [] {
+        const char* e = std::getenv("BN_CONNECTIVITY_CACHE");
+        return std::string(e ? e : "");
+    }();
+    return dir;
+}
+
+struct Fnv {   // FNV-1a 64 over 8-byte words (then the tail bytes)
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        size_t k = 0;
+        for (; k + 8 <= n; k += 8) {
+            uint64_t word;
+            std::memcpy(&word, b + k, 8);
+            h = (h ^ word) * 1099511628211ull;
+        }
+        for (; k < n; ++k) h = (h ^ b[k]) * 1099511628211ull;
+    }
+    template <typename T>
+    void add(const T& v) { add(&v, sizeof v); }
+};
+
+struct CacheHeader {
+    char magic[8];
+    uint32_t format, n_pre, n_post, pad;
+    uint64_t key, sum;
+};
+
+uint64_t matrix_key(const std::vector<Location>& pre, const std::vector<Location>& post,
+                    const WeightGenerationParams& p, int rng_seed, uint32_t matrix_idx) {
+    Fnv f;
+    f.add(kCacheFormat);
+    const uint32_t np = (uint32_t)pre.size(), nq = (uint32_t)post.size();
+    f.add(np);
+    f.add(nq);
+    f.add(pre.data(), pre.size() * sizeof(Location));
+    f.add(post.data(), post.size() * sizeof(Location));
+    for (float x : {p.x_location_length, p.y_location_length, p.max_distance, p.spread_scale, p.max_weight_value,
+                    p.probability_multiplier})
+        f.add(x);
+    for (bool x : {p.weight_value_normal_dropoff, p.probabilistic_normal_not_uniform, p.no_diagonal_connections,
+                   p.weight_value_uniform})
+        f.add((uint8_t)x);
+    f.add((int32_t)rng_seed);
+    f.add(matrix_idx);
+    return f.h;
+}
+
+bool cache_load(const std::string& path, uint64_t key, uint32_t n_pre, uint32_t n_post, std::vector<float>& w) {
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    CacheHeader h{};
+    bool ok = std::fread(&h, sizeof h, 1, f) == 1 && std::memcmp(h.magic, kCacheMagic, 8) == 0 &&
+              h.format == kCacheFormat && h.key == key && h.n_pre == n_pre && h.n_post == n_post;
+    if (ok) {
+        w.assign((size_t)n_pre * n_post, 0.0f);
+        ok = std::fread(w.data(), sizeof(float), w.size(), f) == w.size();
+        if (ok) {
+            Fnv s;
+            s.add(w.data(), w.size() * sizeof(float));
+            ok = s.h == h.sum;
+        }
+    }
+    std::fclose(f);
+    return ok;
+}
+
+void cache_store(const std::string& path, uint64_t key, uint32_t n_pre, uint32_t n_post, const std::vector<float>& w) {
+    CacheHeader h{};
+    std::memcpy(h.magic, kCacheMagic, 8);
+    h.format = kCacheFormat;
+    h.n_pre = n_pre;
+    h.n_post = n_post;
+    h.key = key;
+    Fnv s;
+    s.add(w.data(), w.size() * sizeof(float));
+    h.sum = s.h;
+    // Written under a temporary name and renamed, so a reader never sees a partial file.
+    const std::string tmp = path + ".tmp" + std::to_string((uintptr_t)&h);
+    FILE* f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return;   // an unwritable cache directory only costs the next run a regeneration
+    const bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(w.data(), sizeof(float), w.size(), f) == w.size();
+    const bool closed = std::fclose(f) == 0;
+    std::error_code ec;
+    if (ok && closed) std::filesystem::rename(tmp, path, ec);
+    if (!(ok && closed) || ec) std::filesystem::remove(tmp, ec);
+}
+}  // namespace
+
+void set_connectivity_cache_dir(const std::string& dir) { cache_dir_ref() = dir; }
+const std::string& connectivity_cache_dir() { return cache_dir_ref(); }
+
 std::vector<float> distance_2d_weights(const std::string& filename, const std::vector<Location>& pre,
                                        const std::vector<Location>& post, const WeightGenerationParams& params,
                                        int rng_seed, uint32_t matrix_idx) {
@@ -99,7 +207,22 @@ std::vector<float> distance_2d_weights(const std::string& filename, const std::v
         const char* hdr = "pre_neuron_id, post_neuron_id, weight\n";
         out->put(hdr, std::strlen(hdr));
     }
-    std::vector<float> w((size_t)n_pre * n_post, 0.0f);
+    std::vector<float> w;
+    std::string cache_path;
+    uint64_t key = 0;
+    if (!cache_dir_ref().empty()) {
+        key = matrix_key(pre, post, params, rng_seed, matrix_idx);
+        char name[40];
+        std::snprintf(name, sizeof name, "w_%016llx.bnw", (unsigned long long)key);
+        cache_path = (std::filesystem::path(cache_dir_ref()) / name).string();
+        if (cache_load(cache_path, key, n_pre, n_post, w)) {
+            if (out)
+                for (uint32_t i = 0; i < n_pre; ++i)
+                    for (uint32_t j = 0; j < n_post; ++j) out->line(i, j, w[(size_t)i * n_post + j]);
+            return w;
+        }
+    }
+    w.assign((size_t)n_pre * n_post, 0.0f);
     // The reference re-seeds one mt19937 per matrix (generate_weights.cpp:36-37).
     std::mt19937 rng(rng_seed);
     std::uniform_real_distribution<float> unif(0.0f, 1.0f);
@@ -133,6 +256,11 @@ std::vector<float> distance_2d_weights(const std::string& filename, const std::v
             w[(size_t)i * n_post + j] = weight;
             if (out) out->line(i, j, weight);
         }
+    }
+    if (!cache_path.empty()) {
+        std::error_code ec;
+        std::filesystem::create_directories(cache_dir_ref(), ec);
+        cache_store(cache_path, key, n_pre, n_post, w);
     }
     return w;
 }

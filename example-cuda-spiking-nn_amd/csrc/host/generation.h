@@ -24,6 +24,10 @@ float euclidean_distance_pbc(float i, float j, float ip, float jp, float Lx, flo
 
 // generate_weights.cpp:10-107: dense row-major [pre][post].  `filename` empty = no text file.
 // `matrix_idx` (0 EE, 1 EI, 2 IE, 3 II) only seeds the extension's weight-value stream.
+// Directory of the binary connectivity cache ("" = off; default: $BN_CONNECTIVITY_CACHE).
+void set_connectivity_cache_dir(const std::string& dir);
+const std::string& connectivity_cache_dir();
+
 std::vector<float> distance_2d_weights(const std::string& filename, const std::vector<Location>& pre,
                                        const std::vector<Location>& post, const WeightGenerationParams& params,
                                        int rng_seed, uint32_t matrix_idx);

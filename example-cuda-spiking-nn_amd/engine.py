@@ -111,7 +111,7 @@ EXPORTED_SYMBOLS = [
     "bn_sim_create_from_json_text", "bn_sim_destroy", "bn_sim_run", "bn_sim_step_components",
     "bn_sim_synchronize", "bn_sim_get_info", "bn_sim_get_neuron_state", "bn_sim_get_weights",
     "bn_sim_get_connection_traces", "bn_sim_get_plasticity_traces", "bn_sim_run_timed",
-    "bn_sim_time_propagate_kernel", "bn_sim_set_steps_per_graph", "bn_config_dump_json", "bn_generate_weight_matrix",
+    "bn_sim_time_propagate_kernel", "bn_sim_set_steps_per_graph", "bn_config_dump_json", "bn_generate_weight_matrix", "bn_set_connectivity_cache",
     "bn_batch_create", "bn_batch_destroy", "bn_batch_run", "bn_batch_synchronize", "bn_batch_run_timed",
     "bn_batch_time_gemm", "bn_batch_set_gemm_rows", "bn_batch_get_info", "bn_batch_get_state",
     "bn_batch_workload_instance", "bn_multi_create", "bn_multi_destroy", "bn_multi_run", "bn_multi_run_timed",
@@ -172,6 +172,7 @@ def lib() -> C.CDLL:
     L.bn_sim_load_state.argtypes = [P, C.c_char_p]
     L.bn_config_dump_json.argtypes = [C.c_char_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
     L.bn_generate_weight_matrix.argtypes = [C.c_char_p, U, fp, up, up]
+    L.bn_set_connectivity_cache.argtypes = [C.c_char_p]
     L.bn_batch_create.argtypes = [C.c_char_p, U, C.POINTER(BatchInstance), C.POINTER(P)]
     L.bn_batch_destroy.argtypes = [P]
     L.bn_batch_destroy.restype = None
@@ -253,6 +254,11 @@ def config_dump(path: str) -> dict:
     buf = C.create_string_buffer(need.value)
     _check(lib().bn_config_dump_json(path.encode(), buf, need.value, C.byref(need)))
     return json.loads(buf.value.decode())
+
+
+def set_connectivity_cache(directory: str | None) -> None:
+    """Directory of the binary connectivity cache (None or "": off); process-wide."""
+    _check(lib().bn_set_connectivity_cache((directory or "").encode()))
 
 
 def generate_weight_matrix(path: str, idx: int) -> np.ndarray:

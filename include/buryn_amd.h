@@ -474,6 +474,12 @@ int bn_config_dump_json(const char* config_path, char* buf, size_t buf_size, siz
  * matrix idx for the given config, without writing files. */
 int bn_generate_weight_matrix(const char* config_path, uint32_t weight_matrix_idx, float* dst,
                               uint32_t* pre, uint32_t* post);
+/* Binary connectivity cache (no counterpart in the reference, which draws every matrix again on
+ * every run, generate_weights.cpp:39-100): with a directory set, each generated weight matrix is
+ * stored there once (keyed by its locations, generation parameters, seed and index, checked by
+ * a checksum) and read back by later simulations and bn_generate_weight_matrix calls.  NULL or
+ * "" turns it off.  Process-wide; the default is the BN_CONNECTIVITY_CACHE environment variable. */
+int bn_set_connectivity_cache(const char* directory);
 
 /* -------------------------------------------------------------------------------------
  * Device state object: replaces GpuSimulationState (gpu_core/gpu_simulation_state.h:19-103,
