@@ -314,14 +314,22 @@ __global__ __launch_bounds__(256) void k_win_reverse(WinReversePlan P) {
     if (lane < nrows) neuron_reverse(P, g0 + lane, mine, in);
 }
 
-// dW[i][j] += sum_slots x_slot[i] * lambda_slot[j], x = d - r at the start of the step.  A block
-// owns a 64 x 64 tile; each thread 4 x 4 entries; FP32 sums over the segment's slots.
+// dW[i][j] += sum_slots x_slot[i] * lambda_slot[j], x = d - r at the start of the step: the
+// segment's outer products as one GEMM, dW += X^T L (X: slots x pre, L: slots x post), on the
+// FP32 matrix cores.  A block owns a 64 x 64 tile of dW and stages 16 slots of its X and L
+// columns in LDS at a time; wave q accumulates the 32 x 32 quarter (rows 32 * (q >> 1), columns
+// 32 * (q & 1)) with v_mfma_f32_32x32x2f32, two slots per operation.  Operand lanes:
+// A[m][k] = X[slot k][row m] with m = lane % 32, k = lane / 32; B[k][n] = L[slot k][column n]
+// with n = lane % 32.  Result: acc[r] is row 8 * (r / 4) + 4 * (lane / 32) + r % 4, column
+// lane % 32.  FP32 sums (the slots past the segment are staged as zeros).
+typedef float v16f_w __attribute__((ext_vector_type(16)));
 __global__ __launch_bounds__(256) void k_win_outer(WinOuterPlan P) {
     __shared__ float xs[16][64];
     __shared__ float ls[16][64];
     const uint32_t ti = blockIdx.y * 64, tj = blockIdx.x * 64;
-    const uint32_t tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    float acc[4][4] = {};
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t mi = 32u * (wave >> 1) + (lane & 31), nj = 32u * (wave & 1) + (lane & 31), kk = lane >> 5;
+    v16f_w acc = {};
     for (uint32_t s0 = 0; s0 < P.slots; s0 += 16) {
         for (uint32_t e = threadIdx.x; e < 16 * 64; e += 256) {
             const uint32_t sl = e / 64, c = e % 64;
@@ -339,29 +347,16 @@ __global__ __launch_bounds__(256) void k_win_outer(WinOuterPlan P) {
         }
         __syncthreads();
 #pragma unroll
-        for (int sl = 0; sl < 16; ++sl) {
-            float a[4], b[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                a[u] = xs[sl][ty * 4 + u];
-                b[u] = ls[sl][tx * 4 + u];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int w = 0; w < 4; ++w) acc[u][w] += a[u] * b[w];
-        }
+        for (int p = 0; p < 8; ++p)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xs[2 * p + kk][mi], ls[2 * p + kk][nj], acc, 0, 0, 0);
         __syncthreads();
     }
+    const uint32_t j = tj + nj;
+    if (j >= P.n_post) return;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const uint32_t i = ti + ty * 4 + u;
-        if (i >= P.n_pre) continue;
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const uint32_t jj = tj + tx * 4 + w;
-            if (jj < P.n_post) P.dW[(size_t)i * P.ld + jj] += acc[u][w];
-        }
+    for (int r = 0; r < 16; ++r) {
+        const uint32_t i = ti + 32u * (wave >> 1) + 8u * (r / 4) + 4u * kk + (r % 4);
+        if (i < P.n_pre) P.dW[(size_t)i * P.ld + j] += acc[r];
     }
 }
 
