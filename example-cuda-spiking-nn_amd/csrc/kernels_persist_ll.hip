@@ -395,8 +395,7 @@ __device__ __forceinline__ bool l_item_front(const PPlan& P, LShared& S, uint32_
                                              uint32_t a, uint32_t k, uint32_t n, const v4f (&w)[kRes]) {
     const PConn& C = P.c[conn];
     const PRows Rw = p_rows(C, col, seg);
-    const uint32_t s0 = Rw.s0, seg_rows = Rw.seg_rows, c0 = Rw.c0, c1 = Rw.c1, j0 = Rw.j0;
-    const bool active = Rw.active;
+    const uint32_t s0 = Rw.s0, seg_rows = Rw.seg_rows, c0 = Rw.c0, c1 = Rw.c1;
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool plastic = C.plast != kPNone;
