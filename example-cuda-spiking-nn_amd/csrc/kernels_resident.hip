@@ -657,12 +657,12 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             // The lazy matrix's row-sum parts of the rows whose neuron did not spike (their LTP
             // changes, when a column spiked); a spiking row's part follows its LTD below.
             if (lazy && own != 0u) {
+                uint64_t* const drow = P.dr + ((size_t)r_slot(a) * P.be + b) * P.ne * 2u + (size_t)tl_k * 2u;
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
-                    const uint32_t i = tl_k + T * (uint32_t)q;
                     if (!((vmask >> q) & 1u) || ((sp_now >> q) & 1u)) continue;
                     const uint64_t bits = (uint64_t)__double_as_longlong(change[q]);
-                    uint64_t* dst = P.dr + (((size_t)r_slot(a) * P.be + b) * P.ne + i) * 2u;
+                    uint64_t* dst = drow + (size_t)T * 2u * (uint32_t)q;
                     st_c(dst, r_word(a, (uint32_t)bits));
                     st_c(dst + 1, r_word(a, (uint32_t)(bits >> 32)));
                 }
