@@ -429,8 +429,9 @@ def bench_batched(args) -> int:
 def bench_uneven(args) -> int:
     """48 independent plastic networks (N_E drawn from {400, 900, 1600, 2500}, triplet STDP +
     iSTDP, own seeds), one launch per phase for all of them; the GEMV row block of each size is
-    chosen by the autotuner first.  Metric: network-steps/s.  Per size: the count and a plain
-    single-network run of that size on the GPU (the sequential alternative)."""
+    chosen by the autotuner first.  Metric: network-steps/s.  Per size: the count and plain
+    single-network runs of that size on the GPU, the graph step and the resident step; the faster
+    of the two is the sequential alternative."""
     import tempfile
 
     eng = importlib.import_module("example-cuda-spiking-nn_amd")
@@ -479,11 +480,19 @@ def bench_uneven(args) -> int:
             plain_cfg.pop("gemv_row_blocks", None)
             plain_path = os.path.join(d, f"plain_{n_e}.json")
             json.dump(plain_cfg, open(plain_path, "w"))
-            sim = eng.Simulation(plain_path, flags=("no_monitors", "quiet", "lazy_normalize"))
             plain_steps = min(args.steps, 2000)
-            pms = sim.run_timed(plain_steps)
-            sim.close()
-            per_size[str(n_e)] = {"networks": len(idx), "plain_single_steps_per_s": round(plain_steps / (pms / 1000.0), 1),
+            rates_of = {}
+            for form, extra in (("graph", ()), ("resident", ("resident",))):
+                sim = eng.Simulation(plain_path, flags=("no_monitors", "quiet", "lazy_normalize") + extra)
+                if extra:
+                    sim.set_resident_cooperative(False)
+                sim.run_timed(500)
+                pms = sim.run_timed(plain_steps)
+                sim.close()
+                rates_of[form] = round(plain_steps / (pms / 1000.0), 1)
+            per_size[str(n_e)] = {"networks": len(idx), "plain_single_steps_per_s": max(rates_of.values()),
+                                  "plain_graph_steps_per_s": rates_of["graph"],
+                                  "plain_resident_steps_per_s": rates_of["resident"],
                                   "gemv_row_block": tuned.get(n_e, (None,))[0]}
         multi.close()
     cpu = None
@@ -492,6 +501,7 @@ def bench_uneven(args) -> int:
     net_steps_per_s = len(cfgs) * args.steps / (ms / 1000.0)
     seq_s_per_step = sum(len(v) / per_size[str(n_e)]["plain_single_steps_per_s"] for n_e, v in rates.items())
     seq_rate = len(cfgs) / seq_s_per_step
+    seq_graph = len(cfgs) / sum(len(v) / per_size[str(n_e)]["plain_graph_steps_per_s"] for n_e, v in rates.items())
     gemv_bytes = 4 * n_w + 4 * n_pre
     achieved = gemv_bytes / (k_ms / 1000.0) / 1e9
     line = {
@@ -515,6 +525,8 @@ def bench_uneven(args) -> int:
                    "parallelism": "single GPU"},
         "vs_sequential_plain": round(net_steps_per_s / seq_rate, 2),
         "sequential_plain_network_steps_per_s": round(seq_rate, 1),
+        "vs_sequential_plain_graph_step": round(net_steps_per_s / seq_graph, 2),
+        "sequential_plain_graph_step_network_steps_per_s": round(seq_graph, 1),
         "roofline": {"bound": "hbm", "kernel": "k_gemv_multi (propagate, all networks)", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": gemv_bytes, "avg_kernel_us": round(k_ms * 1000.0, 2)},
