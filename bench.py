@@ -371,7 +371,8 @@ def bench_batched(args) -> int:
         ms = batch.run_timed(args.steps)
         gemm_ms = batch.time_gemm(args.kernel_steps) / args.kernel_steps
         batch.close()
-        # Sequential alternative: one plain instance (same network, plasticity off, fused step).
+        # Sequential alternative: one plain instance (same network, plasticity off), as the graph
+        # step and as the resident step; the faster of the two.
         plain_cfg = dict(base)
         plain_cfg.update({"background_noise_seed": 0, "stimulus_selected_input_ids": [int(i) for i in insts[0]["stimulus_ids"]],
                           "stimulus_selected_input_post_group_offsets": [0] * 100,
@@ -380,8 +381,13 @@ def bench_batched(args) -> int:
         ppath = os.path.join(d, "plain.json")
         json.dump(plain_cfg, open(ppath, "w"))
         sim = eng.Simulation(ppath, flags=("no_monitors", "quiet"))
-        plain_ms = sim.run_timed(args.steps)
+        graph_ms = sim.run_timed(args.steps)
         sim.close()
+        sim = eng.Simulation(ppath, flags=("no_monitors", "quiet", "lazy_normalize", "resident"))
+        sim.set_resident_cooperative(False)
+        resident_ms = sim.run_timed(args.steps)
+        sim.close()
+        plain_ms = min(graph_ms, resident_ms)
     inst_steps_per_s = args.instances * args.steps / (ms / 1000.0)
     plain_inst_steps_per_s = args.steps / (plain_ms / 1000.0)
     n_w = sum(int(info.weight_pre[k]) * int(info.weight_post[k]) for k in range(4))
@@ -413,6 +419,8 @@ def bench_batched(args) -> int:
                    "parallelism": "single GPU"},
         "vs_sequential_plain": round(inst_steps_per_s / plain_inst_steps_per_s, 2),
         "sequential_plain_instance_steps_per_s": round(plain_inst_steps_per_s, 1),
+        "sequential_plain_forms_steps_per_s": {"graph": round(args.steps / (graph_ms / 1000.0), 1),
+                                               "resident": round(args.steps / (resident_ms / 1000.0), 1)},
         "roofline": {"bound": "mfma", "kernel": "k_batch_gemm", "achieved": round(tflops, 2),
                      "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / MFMA_FP32_PEAK_TFLOPS, 4),
                      "traffic": None, "algorithmic_flops_per_launch": flops,
