@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "bn_device.h"
 #include "bn_errors.h"
@@ -629,8 +630,20 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
             }
             const float xw = -PP.ltd_A;   // X of iSTDP and pair
             double change[K];
+            // The rows' pre traces and scales, loaded together before the column loop (up to 7
+            // rows per thread; at 8 the registers are short and they are read per use).
+            constexpr bool kPreload = K <= 7;
+            float pr[K], sr[K];
 #pragma unroll
-            for (int q = 0; q < K; ++q) change[q] = 0.0;
+            for (int q = 0; q < K; ++q) {
+                change[q] = 0.0;
+                pr[q] = kPreload ? S.pre[q][tid] : 0.0f;
+                sr[q] = kPreload && lazy ? S.sc[q][tid] : 1.0f;
+            }
+            // One straight-line form over the rows, so that the rows' chains interleave: a row
+            // of a matrix that is not lazily normalised has the scale 1 (1 * m and a quotient by
+            // 1 are exact), and an entry that is not updated computes a value that the select
+            // drops.
             for (uint32_t cm = own; cm; cm &= cm - 1u) {
                 const uint32_t c = (uint32_t)__builtin_ctz(cm);   // uniform
                 const uint32_t j = j0_k + c;
@@ -639,19 +652,14 @@ __global__ __launch_bounds__(kRThreads) void k_resident(const RPlan* __restrict_
                 for (int q = 0; q < K; ++q) {
                     const uint32_t i = tl_k + T * (uint32_t)q;
                     const float m = w[q][c];
-                    const float delta = S.pre[q][tid] * z;
-                    float val;
-                    bool ok = ((vmask >> q) & 1u) && j != i;
-                    if (lazy) {
-                        const float si = S.sc[q][tid];
-                        ok = ok && si != 0.0f;
-                        val = clamp_weight(si * m + delta, wmin, wmax) / si;
-                    } else {
-                        val = clamp_weight(m + delta, wmin, wmax);
-                    }
-                    val = ok ? val : m;
-                    w[q][c] = val;
-                    change[q] += (double)val - (double)m;
+                    const float delta = (kPreload ? pr[q] : S.pre[q][tid]) * z;
+                    const float si = kPreload ? sr[q] : (lazy ? S.sc[q][tid] : 1.0f);
+                    float val = clamp_weight(si * m + delta, wmin, wmax) / si;
+                    asm volatile("" : "+v"(val));   // computed on every lane, not under a branch
+                    const bool ok = ((vmask >> q) & 1u) && j != i && si != 0.0f;
+                    const float nw = ok ? val : m;
+                    w[q][c] = nw;
+                    change[q] += (double)nw - (double)m;
                 }
             }
             // The lazy matrix's row-sum parts of the rows whose neuron did not spike (their LTP
