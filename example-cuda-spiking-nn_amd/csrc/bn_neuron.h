@@ -52,10 +52,14 @@ __device__ __forceinline__ void eif_step(const NeuronGroupView& v, float ge, flo
 
 // kernel_neuron_group_evolve.cu:11-83 on values already in registers: decay (+flush) of the
 // four background traces, the Poisson shot counts, the increments, the background conductances.
+// head_e / head_i: the tables' first four entries when the caller loaded them with its other
+// inputs (the same counts, without a dependent load in the common case), or null.
 __device__ __forceinline__ void background_advance(const NeuronGroupView& v, uint32_t gid, uint32_t step,
                                                    const bn_device_clock& clk, const uint32_t* tab_e,
                                                    const uint32_t* tab_i, float& re, float& ri, float& de,
-                                                   float& di, float& bg_ge, float& bg_gi) {
+                                                   float& di, float& bg_ge, float& bg_gi,
+                                                   const uint32_t (*head_e)[4] = nullptr,
+                                                   const uint32_t (*head_i)[4] = nullptr) {
     re = flush_trace(re * v.background_excitatory_postsynaptic_r_trace_decay);
     ri = flush_trace(ri * v.background_inhibitory_postsynaptic_r_trace_decay);
     de = flush_trace(de * v.background_excitatory_postsynaptic_d_trace_decay);
@@ -64,9 +68,11 @@ __device__ __forceinline__ void background_advance(const NeuronGroupView& v, uin
     if (v.poisson_table_excitatory_len > 0 || v.poisson_table_inhibitory_len > 0) {
         const U32x4 r = background_draws(clk, step, gid, v.rng_instance);
         if (v.mean_background_excitatory_spikes > 0.0f)
-            ke = poisson_from_table(r.x, tab_e, v.poisson_table_excitatory_len);
+            ke = head_e ? poisson_lookup(r.x, tab_e, v.poisson_table_excitatory_len, *head_e)
+                        : poisson_from_table(r.x, tab_e, v.poisson_table_excitatory_len);
         if (v.mean_background_inhibitory_spikes > 0.0f)
-            ki = poisson_from_table(r.y, tab_i, v.poisson_table_inhibitory_len);
+            ki = head_i ? poisson_lookup(r.y, tab_i, v.poisson_table_inhibitory_len, *head_i)
+                        : poisson_from_table(r.y, tab_i, v.poisson_table_inhibitory_len);
     }
     if (ke > 0u) {
         const float inc = (float)ke;

@@ -2,8 +2,8 @@
 // reverse pass of the window loss (adjoint).  Forward per step: the propagate (k_gemv, shared
 // with the plain step) then k_win_forward (the plain evolve's layout: 64 neurons per block,
 // the partial sums split over 8 waves).  Reverse per step: k_win_reverse (lambda x = W lambda g
-// of the next step, one wave per pre row, then the traces, rates and EIF adjoint of that
-// neuron); per checkpoint segment, k_win_outer adds sum_t x_{t-1} (outer) lambda g_t to dW.
+// of the next step, one block per 4 pre rows, then the traces, rates and EIF adjoint of those
+// neurons); per checkpoint segment, k_win_outer adds sum_t x_{t-1} (outer) lambda g_t to dW.
 //
 // Forward semantics: the plain step without plasticity or stimulus
 // (kernel_neuron_group_evolve.cu:11-130, kernel_connection_propagate.cu:8-94).  Reverse: the
@@ -71,9 +71,15 @@ __global__ __launch_bounds__(kEvolveNeurons * kFinParts) void k_win_forward(WinF
     } else if (wave == 1) {
         if (live) {
             const NeuronGroupView& v = exc ? G0.v : G1.v;
+            uint32_t he[4], hi[4];   // the Poisson tables' first entries, loaded with the traces
+#pragma unroll
+            for (uint32_t q = 0; q < 4; ++q) {
+                he[q] = q < v.poisson_table_excitatory_len ? v.poisson_table_excitatory[q] : 0u;
+                hi[q] = q < v.poisson_table_inhibitory_len ? v.poisson_table_inhibitory[q] : 0u;
+            }
             float re = S.bre[j], ri = S.bri[j], de = S.bde[j], di = S.bdi[j], bg_ge, bg_gi;
             background_advance(v, j, P.step, P.clock, v.poisson_table_excitatory, v.poisson_table_inhibitory, re, ri,
-                               de, di, bg_ge, bg_gi);
+                               de, di, bg_ge, bg_gi, &he, &hi);
             S.bre[j] = re, S.bri[j] = ri, S.bde[j] = de, S.bdi[j] = di;
             bgl[0][lane] = bg_ge;
             bgl[1][lane] = bg_gi;
@@ -182,12 +188,13 @@ __global__ __launch_bounds__(256) void k_win_copy(WinCopyPlan P) {
     }
 }
 
-// What the reverse step of neuron j reads besides lambda x_j (loaded before the row product so
-// that their latency overlaps it).
+// What the reverse step of neuron j reads besides lambda x_j, its accumulators included (loaded
+// before the row product so that their latency overlaps it; only this thread writes them).
 struct ReverseInputs {
     float d0, r0, Vp, ge, gi, s;
+    float ad, gl;                     // the group's d decay, g_leak (device parameters)
     uint32_t refr;
-    double carry_d, carry_r, carry_V, dv;
+    double carry_d, carry_r, carry_V, dv, lrho, acc_alpha, acc_gleak;
 };
 
 __device__ __forceinline__ ReverseInputs load_reverse_inputs(const WinReversePlan& P, uint32_t j) {
@@ -198,6 +205,11 @@ __device__ __forceinline__ ReverseInputs load_reverse_inputs(const WinReversePla
     const uint32_t n = exc ? P.NE : P.NI;
     in.d0 = (exc ? P.rec.dE : P.rec.dI)[(size_t)P.slot * n + lid];
     in.r0 = (exc ? P.rec.rE : P.rec.rI)[(size_t)P.slot * n + lid];
+    in.ad = exc ? P.prm->decay_e_d : P.decay_i_d;
+    in.gl = P.prm->g_leak;
+    in.lrho = P.lrho[exc ? 0 : 1];
+    in.acc_alpha = exc ? P.acc_alpha[j] : 0.0;
+    in.acc_gleak = P.acc_gleak[j];
     in.carry_d = P.carry_d[j];
     in.carry_r = P.carry_r[j];
     in.carry_V = P.carry_V[j];
@@ -218,15 +230,15 @@ __device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t
     const size_t k = (size_t)P.slot * P.N + j;
     // Traces: d_t = flush(d_{t-1} a_d) + K s_t, r likewise, x_t = d_t - r_t.
     const float d0 = in.d0, r0 = in.r0;
-    const float ad = exc ? P.prm->decay_e_d : P.decay_i_d, ar = exc ? P.decay_e_r : P.decay_i_r;
+    const float ad = in.ad, ar = exc ? P.decay_e_r : P.decay_i_r;
     const double ld = lx + in.carry_d;
     const double lr = -lx + in.carry_r;
     const bool nd = !(d0 * ad < kTraceFlush), nr = !(r0 * ar < kTraceFlush);
-    if (exc && nd) P.acc_alpha[j] += ld * (double)d0;
+    if (exc && nd) P.acc_alpha[j] = in.acc_alpha + ld * (double)d0;
     P.carry_d[j] = nd ? (double)ad * ld : 0.0;
     P.carry_r[j] = nr ? (double)ar * lr : 0.0;
     // Spike: traces and the smoothed population rate.
-    const double ls = (double)P.K * (ld + lr) + (exc ? P.lrho[0] * P.rate_coef_e : P.lrho[1] * P.rate_coef_i);
+    const double ls = (double)P.K * (ld + lr) + in.lrho * (exc ? P.rate_coef_e : P.rate_coef_i);
     const double lv = in.carry_V + in.dv;
     if (in.refr) {
         P.carry_V[j] = 0.0;
@@ -239,7 +251,7 @@ __device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t
     const NeuronGroupView& v = P.g[exc ? 0 : 1].v;
     const double Vp = in.Vp, ge = in.ge, gi = in.gi, s = in.s;
     // U of the step in FP64 from its recorded FP32 inputs (the spike itself is the record's).
-    const double DT = v.depolarization_slope_factor, gl = P.prm->g_leak;
+    const double DT = v.depolarization_slope_factor, gl = in.gl;
     const double ex = exp((Vp - (double)v.v_effective_spike_threshold) / DT);
     const double F = -gl * (Vp - (double)v.v_rest) + gl * DT * ex - ge * (Vp - (double)v.v_excitatory_reversal) -
                      gi * (Vp - (double)v.v_inhibitory_reversal);
@@ -258,26 +270,29 @@ __device__ __forceinline__ void neuron_reverse(const WinReversePlan& P, uint32_t
     P.lg_out_i[j] = li;
     P.rec.le[k] = (float)le;
     P.rec.li[k] = (float)li;
-    P.acc_gleak[j] += lU * kd * (-(Vp - (double)v.v_rest) + DT * ex);
+    P.acc_gleak[j] = in.acc_gleak + lU * kd * (-(Vp - (double)v.v_rest) + DT * ex);
 }
 
-// One wave per kRevRows neurons of one group: lambda x_j = sum_k W[j][k] lambda g_k over the
-// two matrices leaving j (E rows: EE and EI with lambda g_e; I rows: IE and II with lambda
-// g_i), float4 weight loads along the rows, each lambda g value loaded once for the wave's rows,
-// FP64 sums; then lane r runs the reverse step of the wave's row r.
-constexpr uint32_t kRevRows = 4;
-__global__ __launch_bounds__(256) void k_win_reverse(WinReversePlan P) {
-    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave_e = (P.NE + kRevRows - 1) / kRevRows, wave_i = (P.NI + kRevRows - 1) / kRevRows;
-    if (w >= wave_e + wave_i) return;
-    const bool exc = w < wave_e;
-    const uint32_t l0 = (exc ? w : w - wave_e) * kRevRows;          // first group-local row
+// One block per kRevRows neurons of one group: lambda x_j = sum_k W[j][k] lambda g_k over the
+// two matrices leaving j (E rows: EE and EI with lambda g_e; I rows: IE and II with lambda g_i).
+// The block's kRevSplit waves take interleaved float4 columns of the rows (each lambda g value
+// loaded once for the block's rows), so that about 3,000 waves keep the weight reads in flight;
+// FP64 sums per lane, the lane butterfly, then the waves' sums in wave order; lane r of wave 0
+// runs the reverse step of the block's row r.
+constexpr uint32_t kRevRows = 4, kRevSplit = 4;
+__global__ __launch_bounds__(64 * kRevSplit) void k_win_reverse(WinReversePlan P) {
+    __shared__ double part[kRevSplit][kRevRows];
+    const uint32_t blk = blockIdx.x;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t blk_e = (P.NE + kRevRows - 1) / kRevRows, blk_i = (P.NI + kRevRows - 1) / kRevRows;
+    if (blk >= blk_e + blk_i) return;   // whole blocks
+    const bool exc = blk < blk_e;
+    const uint32_t l0 = (exc ? blk : blk - blk_e) * kRevRows;       // first group-local row
     const uint32_t nrows = min(kRevRows, (exc ? P.NE : P.NI) - l0);
     const uint32_t g0 = exc ? l0 : P.NE + l0;                       // first global id
     const double* lg = exc ? P.lg_in_e : P.lg_in_i;
     ReverseInputs in{};
-    if (lane < nrows) in = load_reverse_inputs(P, g0 + lane);
+    if (wave == 0 && lane < nrows) in = load_reverse_inputs(P, g0 + lane);
     double acc[kRevRows] = {};
     for (int h = 0; h < 2; ++h) {
         const int c = (exc ? 0 : 2) + h;   // to E posts, then to I posts
@@ -288,7 +303,7 @@ __global__ __launch_bounds__(256) void k_win_reverse(WinReversePlan P) {
 #pragma unroll
         for (uint32_t r = 0; r < kRevRows; ++r) wr[r] = P.W[c] + (size_t)(l0 + min(r, nrows - 1)) * P.ld[c];
 #pragma unroll 2
-        for (uint32_t q = lane; q < n4; q += 64) {
+        for (uint32_t q = wave * 64 + lane; q < n4; q += 64 * kRevSplit) {
             const double l_0 = l[4 * q], l_1 = l[4 * q + 1], l_2 = l[4 * q + 2], l_3 = l[4 * q + 3];
 #pragma unroll
             for (uint32_t r = 0; r < kRevRows; ++r) {
@@ -299,7 +314,7 @@ __global__ __launch_bounds__(256) void k_win_reverse(WinReversePlan P) {
                 acc[r] += (double)wv.w * l_3;
             }
         }
-        for (uint32_t jj = 4 * n4 + lane; jj < n_post; jj += 64)
+        for (uint32_t jj = 4 * n4 + wave * 64 + lane; jj < n_post; jj += 64 * kRevSplit)
 #pragma unroll
             for (uint32_t r = 0; r < kRevRows; ++r) acc[r] += (double)wr[r][jj] * l[jj];
     }
@@ -307,11 +322,15 @@ __global__ __launch_bounds__(256) void k_win_reverse(WinReversePlan P) {
     for (uint32_t r = 0; r < kRevRows; ++r)
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) acc[r] += __shfl_xor(acc[r], off, 64);
-    double mine = acc[0];
+    if (lane == 0)
 #pragma unroll
-    for (uint32_t r = 1; r < kRevRows; ++r)
-        if (lane == r) mine = acc[r];
-    if (lane < nrows) neuron_reverse(P, g0 + lane, mine, in);
+        for (uint32_t r = 0; r < kRevRows; ++r) part[wave][r] = acc[r];
+    __syncthreads();
+    if (wave != 0 || lane >= nrows) return;
+    double mine = part[0][lane];
+#pragma unroll
+    for (uint32_t v = 1; v < kRevSplit; ++v) mine += part[v][lane];
+    neuron_reverse(P, g0 + lane, mine, in);
 }
 
 // dW[i][j] += sum_slots x_slot[i] * lambda_slot[j], x = d - r at the start of the step: the
@@ -372,8 +391,8 @@ int launch_win_copy(const WinCopyPlan& p, bn_stream_t s) {
 }
 
 int launch_win_reverse(const WinReversePlan& p, bn_stream_t s) {
-    const uint32_t waves = (p.NE + kRevRows - 1) / kRevRows + (p.NI + kRevRows - 1) / kRevRows;
-    hipLaunchKernelGGL(k_win_reverse, dim3((waves + 3) / 4), dim3(256), 0, s, p);
+    const uint32_t blocks = (p.NE + kRevRows - 1) / kRevRows + (p.NI + kRevRows - 1) / kRevRows;
+    hipLaunchKernelGGL(k_win_reverse, dim3(blocks), dim3(64 * kRevSplit), 0, s, p);
     return (int)hipGetLastError();
 }
 
