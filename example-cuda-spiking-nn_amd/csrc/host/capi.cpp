@@ -618,13 +618,12 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
         if (sys.persistent() || sys.resident())
             throw Error(BN_ERR_RUNTIME, "kernel timing: this step is one launch per run (time bn_sim_run_timed)");
         hipStream_t s = sim->state->stream();
-        hipEvent_t ev[4];
-        for (auto& e : ev) BN_HIP_CHECK(hipEventCreate(&e), "hipEventCreate");
+        EventPair steps, kernels;   // destroyed on every exit, errors included
         // (1) num_steps whole steps launched one by one (no graph).
         BN_HIP_CHECK(hipStreamSynchronize(s), "pre-timing sync");
-        BN_HIP_CHECK(hipEventRecord(ev[0], s), "hipEventRecord");
+        BN_HIP_CHECK(hipEventRecord(steps.a, s), "hipEventRecord");
         for (uint32_t k = 0; k < num_steps; ++k) sys.launch_fused_step_eager(nullptr, nullptr);
-        BN_HIP_CHECK(hipEventRecord(ev[1], s), "hipEventRecord");
+        BN_HIP_CHECK(hipEventRecord(steps.b, s), "hipEventRecord");
         // (2) num_steps back-to-back propagate launches over the current weights.  The pending
         // partial sums belong to the next step: saved before and restored after.
         GemvPlan G = sys.gemv_plan();
@@ -633,6 +632,12 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
         for (uint32_t c = 0; c < G.n_conn; ++c) total += (size_t)G.c[c].n_chunks * G.c[c].ld;
         double* save = nullptr;
         if (total) BN_HIP_CHECK(hipMalloc(&save, total * sizeof(double)), "hipMalloc(partials copy)");
+        // Freed on every exit after the stream has drained (the copies may still read it).
+        const auto release = [s](double* p) {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        };
+        std::unique_ptr<double, decltype(release)> save_owner(save, release);
         size_t off = 0;
         for (uint32_t c = 0; c < G.n_conn; ++c) {
             const size_t n = (size_t)G.c[c].n_chunks * G.c[c].ld;
@@ -640,10 +645,10 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
                          "save partials");
             off += n;
         }
-        BN_HIP_CHECK(hipEventRecord(ev[2], s), "hipEventRecord");
+        BN_HIP_CHECK(hipEventRecord(kernels.a, s), "hipEventRecord");
         for (uint32_t k = 0; k < num_steps; ++k)
             BN_HIP_CHECK((hipError_t)launch_gemv(G, s), "propagate launch");
-        BN_HIP_CHECK(hipEventRecord(ev[3], s), "hipEventRecord");
+        BN_HIP_CHECK(hipEventRecord(kernels.b, s), "hipEventRecord");
         off = 0;
         for (uint32_t c = 0; c < G.n_conn; ++c) {
             const size_t n = (size_t)G.c[c].n_chunks * G.c[c].ld;
@@ -652,11 +657,10 @@ int bn_sim_time_propagate_kernel(bn_sim* sim, uint32_t num_steps, float* kernel_
             off += n;
         }
         BN_HIP_CHECK(hipStreamSynchronize(s), "timing sync");
-        if (save) (void)hipFree(save);
+        save_owner.reset();
         float total_ms = 0.0f, kern = 0.0f;
-        BN_HIP_CHECK(hipEventElapsedTime(&total_ms, ev[0], ev[1]), "hipEventElapsedTime");
-        BN_HIP_CHECK(hipEventElapsedTime(&kern, ev[2], ev[3]), "hipEventElapsedTime");
-        for (auto& e : ev) (void)hipEventDestroy(e);
+        BN_HIP_CHECK(hipEventElapsedTime(&total_ms, steps.a, steps.b), "hipEventElapsedTime");
+        BN_HIP_CHECK(hipEventElapsedTime(&kern, kernels.a, kernels.b), "hipEventElapsedTime");
         *kernel_ms = kern;
         *step_ms = total_ms;
         sim->state->check_status();
