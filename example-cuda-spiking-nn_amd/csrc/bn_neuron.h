@@ -156,7 +156,11 @@ __device__ __forceinline__ void range_sums(const double* const (&p)[R], const ui
 #pragma unroll
             for (uint32_t k = 0; k < kRangeBatch; ++k) {
                 const uint32_t c = c0[r] + b + k;
-                v[r][k] = c < c1[r] ? p[r][(size_t)c * ld[r]] : 0.0;
+                // Through a global-segment pointer: the partials are device memory, and a
+                // pointer that went through uni() would otherwise take flat loads.
+                const __attribute__((address_space(1))) double* g =
+                    (const __attribute__((address_space(1))) double*)p[r];
+                v[r][k] = c < c1[r] ? g[(size_t)c * ld[r]] : 0.0;
             }
 #pragma unroll
         for (int r = 0; r < R; ++r)
