@@ -429,6 +429,20 @@ void build_window(bn_window* w, bn_sim* sim, const bn_window_spec* spec) {
     // Propagate plan over the window's weights and traces, with the simulation's row blocks
     // (the same FP64 partial sums, so the window's first steps equal the plain step's).
     const float* xs[4] = {w->st.xE, w->st.xE, w->st.xI, w->st.xI};
+    // Part sums where every matrix allows them (rows padded to 16 bytes, at most
+    // kGemvPartChunksMax chunks per part): the propagate combines each part of a column's chunks
+    // itself, in the same order, so the forward reads at most kFinParts sums per source.
+    bool parts = true;
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t rb = sim->state->gemv_row_block(k);
+        const uint32_t chunks = (pres[k] + rb - 1) / rb;
+        const uint32_t per = (chunks + kFinParts - 1) / kFinParts;
+        if (w->ld[k] % 4 != 0 || (uintptr_t)w->W[k] % 16 != 0 || per > kGemvPartChunksMax) parts = false;
+    }
+    auto part_count = [](uint32_t chunks) {
+        const uint32_t per = (chunks + kFinParts - 1) / kFinParts;
+        return per ? (chunks + per - 1) / per : 0u;
+    };
     uint32_t tiles = 0;
     for (int k = 0; k < 4; ++k) {
         GemvConn& C = w->gemv.c[k];
@@ -443,16 +457,22 @@ void build_window(bn_window* w, bn_sim* sim, const bn_window_spec* spec) {
         C.partials = w->alloc<double>((size_t)C.n_chunks * C.ld, "window partials");
         C.n_coltiles = (posts[k] + kGemvTileCols - 1) / kGemvTileCols;
         C.tile_begin = tiles;
-        tiles += C.n_chunks * C.n_coltiles;
+        tiles += (parts ? part_count(C.n_chunks) : C.n_chunks) * C.n_coltiles;
         C.tile_end = tiles;
     }
     w->gemv.n_conn = 4;
+    w->gemv.parts = parts ? 1u : 0u;
     w->gemv.total_tiles = tiles;
     w->gemv.nontemporal = 0;
-    w->grp[0].fin_e = FinalizeSource{w->gemv.c[0].partials, w->gemv.c[0].n_chunks, w->ld[0]};
-    w->grp[0].fin_i = FinalizeSource{w->gemv.c[2].partials, w->gemv.c[2].n_chunks, w->ld[2]};
-    w->grp[1].fin_e = FinalizeSource{w->gemv.c[1].partials, w->gemv.c[1].n_chunks, w->ld[1]};
-    w->grp[1].fin_i = FinalizeSource{w->gemv.c[3].partials, w->gemv.c[3].n_chunks, w->ld[3]};
+    // (part sums: one partials row per part, each combined as a part of one chunk)
+    auto src = [&](int k) {
+        const GemvConn& C = w->gemv.c[k];
+        return FinalizeSource{C.partials, parts ? part_count(C.n_chunks) : C.n_chunks, w->ld[k]};
+    };
+    w->grp[0].fin_e = src(0);
+    w->grp[0].fin_i = src(2);
+    w->grp[1].fin_e = src(1);
+    w->grp[1].fin_i = src(3);
     // Records of three segments (rec_of), outputs, adjoint buffers.
     for (WinRecords& r : w->rec) {
         r.Vprev = w->alloc<float>((size_t)S * N, "records");

@@ -1,7 +1,7 @@
 // Differentiable window for CDNA4 (gfx950): forward step with per-step records, and the
-// reverse pass of the window loss (adjoint).  Forward per step: the propagate (k_gemv, shared
-// with the plain step) then k_win_forward (the plain evolve's layout: 64 neurons per block,
-// the partial sums split over 8 waves).  Reverse per step: k_win_reverse (lambda x = W lambda g
+// reverse pass of the window loss (adjoint).  Forward per step: the propagate (the plain step's
+// k_gemv, in its part-sum form where the matrices allow it) then k_win_forward (the plain
+// evolve's layout: 64 neurons per block, the partial sums split over 8 waves).  Reverse per step: k_win_reverse (lambda x = W lambda g
 // of the next step, one block per 4 pre rows, then the traces, rates and EIF adjoint of those
 // neurons); per checkpoint segment, k_win_outer adds sum_t x_{t-1} (outer) lambda g_t to dW.
 //
